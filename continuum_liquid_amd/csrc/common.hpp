@@ -1,0 +1,112 @@
+// Shared declarations of the gfx950 HIP implementation of include/cliquid.h.
+//
+// Layout in HBM (one context = one simulation grid):
+//   grid arrays      float32[C], C = nx*ny*nz, k fastest (reference helper.cl:4-6)
+//   particles        AoS float32[n][3] for position, velocity, work copies and
+//                    the previous positions (reference solver.py:161-217)
+//   bins             int32 counts/offsets[C], int32 key/rank/perm[capacity]
+//   multigrid        7 float32 arrays per coarse level (p, b, mask, diagonal,
+//                    weight x/y/z), allocated once at context creation
+//
+// Every kernel is compiled with -ffp-contract=off: a fused multiply-add
+// appears only where written as fmaf(), at exactly the sites where the
+// reference OpenCL source contracts (llvm.fmuladd), so results can be compared
+// bit for bit with the CPU oracle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cliquid.h"
+
+namespace clq {
+
+constexpr int kMaxLevels = 16;
+
+struct Level {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t C = 0;
+    float *p = nullptr, *b = nullptr, *mask = nullptr, *diag = nullptr;
+    float *wx = nullptr, *wy = nullptr, *wz = nullptr;
+};
+
+// ---- device helpers (reference helper.cl:4-23) -----------------------------
+__device__ __forceinline__ int64_t gidx(int i, int j, int k, int ny, int nz) {
+    return ((int64_t)i * ny + j) * nz + k;
+}
+__device__ __forceinline__ bool interior(int i, int j, int k, int nx, int ny, int nz) {
+    return i > 0 && j > 0 && k > 0 && i < nx - 1 && j < ny - 1 && k < nz - 1;
+}
+__device__ __forceinline__ void coords(int64_t index, int ny, int nz, int &i, int &j, int &k) {
+    const int64_t yz = (int64_t)ny * nz;
+    i = (int)(index / yz);
+    const int64_t r = index - (int64_t)i * yz;
+    j = (int)(r / nz);
+    k = (int)(r - (int64_t)j * nz);
+}
+// convert_int_rtn + clamp; NaN and out-of-range inputs clamp.
+__device__ __forceinline__ int floor_clamp(float t, int lo, int hi) {
+    t = floorf(t);
+    if (!(t >= (float)lo)) return lo;
+    if (t > (float)hi) return hi;
+    return (int)t;
+}
+// convert_int_rtz + clamp.
+__device__ __forceinline__ int trunc_clamp(float t, int lo, int hi) {
+    t = truncf(t);
+    if (!(t >= (float)lo)) return lo;
+    if (t > (float)hi) return hi;
+    return (int)t;
+}
+
+// ---- host launchers ----------------------------------------------------------
+// binning.hip
+void launch_bin_count(const float *pos, int n, float inv_dx, int nx, int ny, int nz, int32_t *key,
+                      int32_t *rank, int32_t *counts, hipStream_t s);
+int64_t scan_scratch_elems(int64_t count);
+void launch_exclusive_scan(const int32_t *in, int32_t *out, int64_t count, int32_t *scratch, hipStream_t s);
+void launch_bin_permute(const int32_t *key, const int32_t *rank, const int32_t *offsets, int n, int32_t *perm,
+                        hipStream_t s);
+void launch_bin_gather(const int32_t *perm, const int32_t *key, const int32_t *offsets, const int32_t *counts,
+                       const float *pos, const float *vel, int n, float *pos_out, float *vel_out, hipStream_t s);
+
+// p2g.hip
+void launch_p2g(const float *pos, const float *vel, const int32_t *offsets, const int32_t *counts, float *u,
+                float *v, float *w, float *ff, float inv_dx, float inv_ppc, int nx, int ny, int nz, hipStream_t s);
+
+// grid.hip
+void launch_grid_obstacle_bc(float *u, float *v, float *w, float *ff, const uint8_t *ob, int nx, int ny, int nz,
+                             hipStream_t s);
+void launch_grid_domain_bc(float *u, float *v, float *w, int nx, int ny, int nz, hipStream_t s);
+void launch_update_velocity(const float *u, const float *v, const float *w, float *ru, float *rv, float *rw,
+                            const float *ff, float ax, float ay, float az, float dt, float nudt, float inv_dx2,
+                            int nx, int ny, int nz, hipStream_t s);
+void launch_pressure_rhs(const float *u, const float *v, const float *w, const float *ff, float *rhs,
+                         float rho_over_dt, float inv_dx, int nx, int ny, int nz, hipStream_t s);
+void launch_build_finest_operator(const float *mask, float *diag, float *wx, float *wy, float *wz, int nx, int ny,
+                                  int nz, hipStream_t s);
+void launch_neumann_bc(float *p, int nx, int ny, int nz, hipStream_t s);
+void launch_pressure_correction(float *u, float *v, float *w, const float *p, float dt_over_rho, float inv_dx,
+                                int nx, int ny, int nz, hipStream_t s);
+
+// multigrid.hip
+void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s);
+void launch_coarse_operator(const Level &F, const Level &K, hipStream_t s);
+void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, hipStream_t s);
+void launch_prolongate(const Level &F, const Level &K, hipStream_t s);
+void launch_coarse_smooth(const Level &L, float delta, int iterations, hipStream_t s);
+int residual_partials_count(int64_t C);
+void launch_residual_norm(const Level &L, double inv_h2, double *partials, double *out2, hipStream_t s);
+
+// g2p.hip
+struct ParticleUpdate {
+    int flags;  // bit 0: G2P, 1: save previous, 2: advect, 3: domain BC, 4: obstacle BC
+    float inv_dx, a, b, dt, lower, upper[3];
+    int pure_flip;
+};
+enum { kG2P = 1, kSavePrev = 2, kAdvect = 4, kDomainBC = 8, kObstacleBC = 16 };
+void launch_particle_update(float *pos, float *vel, float *prev, const float *u, const float *v, const float *w,
+                            const float *uo, const float *vo, const float *wo, const uint8_t *ob, int n, int nx,
+                            int ny, int nz, const ParticleUpdate &pu, hipStream_t s);
+
+}  // namespace clq

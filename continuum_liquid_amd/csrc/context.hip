@@ -1,0 +1,679 @@
+// C-ABI of include/cliquid.h on gfx950: context, buffers, data movement and
+// the stage entry points.  Every stage enqueues on the context's stream; the
+// solve loop and the W-cycle recursion run here, natively, not in Python.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace clq;
+
+namespace {
+
+thread_local char g_error[512];
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof g_error, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess) return fail(CLQ_ERR_DEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                      \
+    } while (0)
+
+int check_launch(const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(CLQ_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    return CLQ_OK;
+}
+
+int coarse_size(int n) {  // multigrid.py:12-15
+    const int interior_size = std::max(0, n - 2);
+    return (interior_size + 1) / 2 + 2;
+}
+
+}  // namespace
+
+struct clq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    int64_t C = 0;
+    double dx = 0.0;
+    int64_t cap = 0, n = 0;
+    float *pos = nullptr, *vel = nullptr, *pos_work = nullptr, *vel_work = nullptr, *prev = nullptr;
+    int32_t *key = nullptr, *rank = nullptr, *perm = nullptr;
+    int32_t *counts = nullptr, *offsets = nullptr, *scan_scratch = nullptr;
+    float *u = nullptr, *v = nullptr, *w = nullptr, *uo = nullptr, *vo = nullptr, *wo = nullptr;
+    float *ff = nullptr, *p = nullptr, *rhs = nullptr, *diag = nullptr, *wx = nullptr, *wy = nullptr, *wz = nullptr;
+    uint8_t *obstacle = nullptr;
+    bool has_obstacle = false;
+    int nlevels = 0;
+    Level lv[kMaxLevels];
+    bool coarse_ops_valid = false;  // coarse operators match the current level-0 operator
+    double *norm_partials = nullptr, *norm_out = nullptr, *norm_host = nullptr;
+    hipEvent_t ev_start[CLQ_STAGE_COUNT] = {}, ev_stop[CLQ_STAGE_COUNT] = {};
+    bool ev_used[CLQ_STAGE_COUNT] = {};
+    std::vector<void *> allocations;
+
+    int alloc(void **ptr, size_t bytes) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes ? bytes : 16);
+        if (e != hipSuccess) return fail(CLQ_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        allocations.push_back(q);
+        *ptr = q;
+        return CLQ_OK;
+    }
+    void release(void *q) {
+        if (!q) return;
+        auto it = std::find(allocations.begin(), allocations.end(), q);
+        if (it != allocations.end()) allocations.erase(it);
+        (void)hipFree(q);
+    }
+};
+
+namespace {
+
+int alloc_particles(clq_ctx *c, int64_t cap) {
+    void *arrs[8];
+    const size_t fb = (size_t)cap * 12, ib = (size_t)cap * 4;
+    const size_t sizes[8] = {fb, fb, fb, fb, fb, ib, ib, ib};
+    for (int q = 0; q < 8; ++q) {
+        if (int rc = c->alloc(&arrs[q], sizes[q])) {
+            for (int r = 0; r < q; ++r) c->release(arrs[r]);
+            return rc;
+        }
+    }
+    c->release(c->pos); c->release(c->vel); c->release(c->pos_work); c->release(c->vel_work); c->release(c->prev);
+    c->release(c->key); c->release(c->rank); c->release(c->perm);
+    c->pos = (float *)arrs[0]; c->vel = (float *)arrs[1]; c->pos_work = (float *)arrs[2];
+    c->vel_work = (float *)arrs[3]; c->prev = (float *)arrs[4];
+    c->key = (int32_t *)arrs[5]; c->rank = (int32_t *)arrs[6]; c->perm = (int32_t *)arrs[7];
+    c->cap = cap;
+    return CLQ_OK;
+}
+
+struct DeviceGuard {
+    explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
+};
+
+void stage_begin(clq_ctx *c, int stage) {
+    (void)hipEventRecord(c->ev_start[stage], c->stream);
+}
+void stage_end(clq_ctx *c, int stage) {
+    (void)hipEventRecord(c->ev_stop[stage], c->stream);
+    c->ev_used[stage] = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *clq_last_error(void) { return g_error; }
+int clq_abi_version(void) { return CLQ_ABI_VERSION; }
+
+int clq_device_count(int32_t *count) {
+    if (!count) return fail(CLQ_ERR_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail(CLQ_ERR_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *count = n;
+    return CLQ_OK;
+}
+
+int clq_destroy(clq_ctx *c) {
+    if (!c) return CLQ_OK;
+    DeviceGuard g(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *q : c->allocations) (void)hipFree(q);
+    if (c->norm_host) (void)hipHostFree(c->norm_host);
+    for (int s = 0; s < CLQ_STAGE_COUNT; ++s) {
+        if (c->ev_start[s]) (void)hipEventDestroy(c->ev_start[s]);
+        if (c->ev_stop[s]) (void)hipEventDestroy(c->ev_stop[s]);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CLQ_OK;
+}
+
+int clq_create(int32_t nx, int32_t ny, int32_t nz, double dx, int64_t capacity, clq_ctx **out) {
+    if (!out) return fail(CLQ_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (nx < 3 || ny < 3 || nz < 3) return fail(CLQ_ERR_INVALID, "grid %dx%dx%d too small (need >= 3)", nx, ny, nz);
+    if ((int64_t)nx * ny * nz >= (1LL << 31)) return fail(CLQ_ERR_INVALID, "grid has >= 2^31 cells");
+    if (!(dx > 0.0)) return fail(CLQ_ERR_INVALID, "dx must be positive");
+    if (capacity >= (1LL << 31)) return fail(CLQ_ERR_INVALID, "capacity must be < 2^31 particles");
+    if (capacity < 1) capacity = 1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(CLQ_ERR_DEVICE, "no HIP device available (the HIP library needs a GPU)");
+    clq_ctx *c = new clq_ctx();
+    (void)hipGetDevice(&c->device);
+    c->nx = nx; c->ny = ny; c->nz = nz; c->dx = dx;
+    c->C = (int64_t)nx * ny * nz;
+    int rc = CLQ_OK;
+    auto bail = [&](int code) { clq_destroy(c); return code; };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(CLQ_ERR_DEVICE, "hipStreamCreate failed"));
+    for (int s = 0; s < CLQ_STAGE_COUNT; ++s) {
+        if (hipEventCreate(&c->ev_start[s]) != hipSuccess || hipEventCreate(&c->ev_stop[s]) != hipSuccess)
+            return bail(fail(CLQ_ERR_DEVICE, "hipEventCreate failed"));
+    }
+    if ((rc = alloc_particles(c, capacity))) return bail(rc);
+    const size_t gb = (size_t)c->C * 4;
+    void *q;
+    float **grids[] = {&c->u, &c->v, &c->w, &c->uo, &c->vo, &c->wo, &c->ff, &c->p, &c->rhs, &c->diag, &c->wx,
+                       &c->wy, &c->wz};
+    for (float **g : grids) {
+        if ((rc = c->alloc(&q, gb))) return bail(rc);
+        *g = (float *)q;
+        if (hipMemsetAsync(q, 0, gb, c->stream) != hipSuccess) return bail(fail(CLQ_ERR_DEVICE, "memset failed"));
+    }
+    if ((rc = c->alloc(&q, gb))) return bail(rc);
+    c->counts = (int32_t *)q;
+    if ((rc = c->alloc(&q, gb))) return bail(rc);
+    c->offsets = (int32_t *)q;
+    if ((rc = c->alloc(&q, (size_t)scan_scratch_elems(c->C) * 4))) return bail(rc);
+    c->scan_scratch = (int32_t *)q;
+    if ((rc = c->alloc(&q, (size_t)c->C))) return bail(rc);
+    c->obstacle = (uint8_t *)q;
+    (void)hipMemsetAsync(c->counts, 0, gb, c->stream);
+    (void)hipMemsetAsync(c->offsets, 0, gb, c->stream);
+    (void)hipMemsetAsync(c->obstacle, 0, (size_t)c->C, c->stream);
+    // level 0 aliases the simulation grid (pressure_solve.py:62-76)
+    Level &L0 = c->lv[0];
+    L0.nx = nx; L0.ny = ny; L0.nz = nz; L0.C = c->C;
+    L0.p = c->p; L0.b = c->rhs; L0.mask = c->ff; L0.diag = c->diag; L0.wx = c->wx; L0.wy = c->wy; L0.wz = c->wz;
+    // create_multigrid_levels(min_size=8): multigrid.py:18-50, solver.py:212
+    int a = coarse_size(nx), b = coarse_size(ny), d = coarse_size(nz);
+    c->nlevels = 1;
+    while (a >= 8 && b >= 8 && d >= 8 && c->nlevels < kMaxLevels) {
+        Level &L = c->lv[c->nlevels++];
+        L.nx = a; L.ny = b; L.nz = d; L.C = (int64_t)a * b * d;
+        float **arrs[] = {&L.p, &L.b, &L.mask, &L.diag, &L.wx, &L.wy, &L.wz};
+        for (float **arr : arrs) {
+            if ((rc = c->alloc(&q, (size_t)L.C * 4))) return bail(rc);
+            *arr = (float *)q;
+            (void)hipMemsetAsync(q, 0, (size_t)L.C * 4, c->stream);
+        }
+        a = coarse_size(a); b = coarse_size(b); d = coarse_size(d);
+    }
+    if ((rc = c->alloc(&q, (size_t)residual_partials_count(c->C) * 8))) return bail(rc);
+    c->norm_partials = (double *)q;
+    if ((rc = c->alloc(&q, 16))) return bail(rc);
+    c->norm_out = (double *)q;
+    if (hipHostMalloc((void **)&c->norm_host, 16, hipHostMallocDefault) != hipSuccess)
+        return bail(fail(CLQ_ERR_NOMEM, "hipHostMalloc failed"));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(CLQ_ERR_DEVICE, "initial clear failed"));
+    *out = c;
+    return CLQ_OK;
+}
+
+int clq_sync(clq_ctx *c) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CLQ_OK;
+}
+
+// ---- data movement ---------------------------------------------------------------------
+int clq_set_particles(clq_ctx *c, const float *pos, const float *vel, int64_t n) {
+    if (!c || n < 0 || (n > 0 && !pos)) return fail(CLQ_ERR_INVALID, "bad particle upload");
+    if (n >= (1LL << 31)) return fail(CLQ_ERR_INVALID, "particle count must be < 2^31");
+    DeviceGuard g(c->device);
+    if (n > c->cap) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        int64_t cap = std::max<int64_t>(n, c->cap * 3 / 2);  // solver.py:290-291
+        if (int rc = alloc_particles(c, cap)) return rc;
+    }
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(c->pos, pos, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+        if (vel) HIP_TRY(hipMemcpyAsync(c->vel, vel, (size_t)n * 12, hipMemcpyHostToDevice, c->stream));
+        else HIP_TRY(hipMemsetAsync(c->vel, 0, (size_t)n * 12, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->n = n;
+    return CLQ_OK;
+}
+
+int clq_get_particles(clq_ctx *c, float *pos, float *vel, int64_t n) {
+    if (!c || n != c->n)
+        return fail(CLQ_ERR_INVALID, "particle count mismatch (%lld requested, %lld held)", (long long)n,
+                    c ? (long long)c->n : -1LL);
+    DeviceGuard g(c->device);
+    if (n > 0 && pos) HIP_TRY(hipMemcpyAsync(pos, c->pos, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+    if (n > 0 && vel) HIP_TRY(hipMemcpyAsync(vel, c->vel, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CLQ_OK;
+}
+
+int clq_get_previous_positions(clq_ctx *c, float *pos, int64_t n) {
+    if (!c || n != c->n || !pos) return fail(CLQ_ERR_INVALID, "bad previous-position download");
+    DeviceGuard g(c->device);
+    if (n > 0) HIP_TRY(hipMemcpyAsync(pos, c->prev, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CLQ_OK;
+}
+
+int clq_get_particle_count(clq_ctx *c, int64_t *n) {
+    if (!c || !n) return fail(CLQ_ERR_INVALID, "NULL argument");
+    *n = c->n;
+    return CLQ_OK;
+}
+
+static void *field_ptr(clq_ctx *c, int f, int64_t *bytes) {
+    int64_t fb = c->C * 4;
+    void *p = nullptr;
+    switch (f) {
+    case CLQ_FIELD_U: p = c->u; break;
+    case CLQ_FIELD_V: p = c->v; break;
+    case CLQ_FIELD_W: p = c->w; break;
+    case CLQ_FIELD_U_OLD: p = c->uo; break;
+    case CLQ_FIELD_V_OLD: p = c->vo; break;
+    case CLQ_FIELD_W_OLD: p = c->wo; break;
+    case CLQ_FIELD_FLUID_FRACTION: p = c->ff; break;
+    case CLQ_FIELD_PRESSURE: p = c->p; break;
+    case CLQ_FIELD_RHS: p = c->rhs; break;
+    case CLQ_FIELD_DIAGONAL: p = c->diag; break;
+    case CLQ_FIELD_WEIGHT_X: p = c->wx; break;
+    case CLQ_FIELD_WEIGHT_Y: p = c->wy; break;
+    case CLQ_FIELD_WEIGHT_Z: p = c->wz; break;
+    case CLQ_FIELD_BIN_COUNTS: p = c->counts; break;
+    case CLQ_FIELD_BIN_OFFSETS: p = c->offsets; break;
+    case CLQ_FIELD_OBSTACLE: p = c->obstacle; fb = c->C; break;
+    default: return nullptr;
+    }
+    *bytes = fb;
+    return p;
+}
+
+int clq_set_field(clq_ctx *c, int32_t f, const void *host, int64_t bytes) {
+    int64_t fb = 0;
+    void *p = c ? field_ptr(c, f, &fb) : nullptr;
+    if (!p || !host || bytes != fb)
+        return fail(CLQ_ERR_INVALID, "bad field upload (field %d, %lld bytes)", f, (long long)bytes);
+    DeviceGuard g(c->device);
+    HIP_TRY(hipMemcpyAsync(p, host, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (f == CLQ_FIELD_OBSTACLE) {
+        const uint8_t *o = (const uint8_t *)host;
+        c->has_obstacle = std::any_of(o, o + bytes, [](uint8_t x) { return x != 0; });
+    }
+    if (f == CLQ_FIELD_FLUID_FRACTION || f == CLQ_FIELD_DIAGONAL || f == CLQ_FIELD_WEIGHT_X ||
+        f == CLQ_FIELD_WEIGHT_Y || f == CLQ_FIELD_WEIGHT_Z)
+        c->coarse_ops_valid = false;
+    return CLQ_OK;
+}
+
+int clq_get_field(clq_ctx *c, int32_t f, void *host, int64_t bytes) {
+    int64_t fb = 0;
+    void *p = c ? field_ptr(c, f, &fb) : nullptr;
+    if (!p || !host || bytes != fb)
+        return fail(CLQ_ERR_INVALID, "bad field download (field %d, %lld bytes)", f, (long long)bytes);
+    DeviceGuard g(c->device);
+    HIP_TRY(hipMemcpyAsync(host, p, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CLQ_OK;
+}
+
+int clq_level_count(clq_ctx *c, int32_t *levels) {
+    if (!c || !levels) return fail(CLQ_ERR_INVALID, "NULL argument");
+    *levels = c->nlevels;
+    return CLQ_OK;
+}
+
+int clq_level_shape(clq_ctx *c, int32_t l, int32_t *nx, int32_t *ny, int32_t *nz) {
+    if (!c || l < 0 || l >= c->nlevels || !nx || !ny || !nz) return fail(CLQ_ERR_INVALID, "bad level %d", l);
+    *nx = c->lv[l].nx; *ny = c->lv[l].ny; *nz = c->lv[l].nz;
+    return CLQ_OK;
+}
+
+static float *level_array(Level &L, int f) {
+    switch (f) {
+    case CLQ_LEVEL_P: return L.p;
+    case CLQ_LEVEL_B: return L.b;
+    case CLQ_LEVEL_MASK: return L.mask;
+    case CLQ_LEVEL_DIAGONAL: return L.diag;
+    case CLQ_LEVEL_WEIGHT_X: return L.wx;
+    case CLQ_LEVEL_WEIGHT_Y: return L.wy;
+    case CLQ_LEVEL_WEIGHT_Z: return L.wz;
+    default: return nullptr;
+    }
+}
+
+int clq_set_level_field(clq_ctx *c, int32_t l, int32_t f, const float *host, int64_t bytes) {
+    if (!c || l < 0 || l >= c->nlevels || !host) return fail(CLQ_ERR_INVALID, "bad level upload");
+    float *a = level_array(c->lv[l], f);
+    if (!a || bytes != c->lv[l].C * 4) return fail(CLQ_ERR_INVALID, "bad level field %d / size", f);
+    DeviceGuard g(c->device);
+    HIP_TRY(hipMemcpyAsync(a, host, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (f >= CLQ_LEVEL_MASK) c->coarse_ops_valid = false;
+    return CLQ_OK;
+}
+
+int clq_get_level_field(clq_ctx *c, int32_t l, int32_t f, float *host, int64_t bytes) {
+    if (!c || l < 0 || l >= c->nlevels || !host) return fail(CLQ_ERR_INVALID, "bad level download");
+    float *a = level_array(c->lv[l], f);
+    if (!a || bytes != c->lv[l].C * 4) return fail(CLQ_ERR_INVALID, "bad level field %d / size", f);
+    DeviceGuard g(c->device);
+    HIP_TRY(hipMemcpyAsync(host, a, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CLQ_OK;
+}
+
+// ---- particle_to_grid ---------------------------------------------------------------
+int clq_bin_particles(clq_ctx *c) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    DeviceGuard g(c->device);
+    const float inv_dx = (float)(1.0 / c->dx);  // FIELD_DTYPE(1.0 / dx), particles.py:103
+    const int n = (int)c->n;
+    stage_begin(c, CLQ_STAGE_BIN);
+    HIP_TRY(hipMemsetAsync(c->counts, 0, (size_t)c->C * 4, c->stream));
+    launch_bin_count(c->pos, n, inv_dx, c->nx, c->ny, c->nz, c->key, c->rank, c->counts, c->stream);
+    launch_exclusive_scan(c->counts, c->offsets, c->C, c->scan_scratch, c->stream);
+    launch_bin_permute(c->key, c->rank, c->offsets, n, c->perm, c->stream);
+    launch_bin_gather(c->perm, c->key, c->offsets, c->counts, c->pos, c->vel, n, c->pos_work, c->vel_work, c->stream);
+    stage_end(c, CLQ_STAGE_BIN);
+    std::swap(c->pos, c->pos_work);  // particles.py:136-140
+    std::swap(c->vel, c->vel_work);
+    return check_launch("bin_particles");
+}
+
+int clq_particle_to_grid(clq_ctx *c, double particles_per_cell) {
+    if (!c || !(particles_per_cell > 0)) return fail(CLQ_ERR_INVALID, "bad P2G arguments");
+    DeviceGuard g(c->device);
+    stage_begin(c, CLQ_STAGE_P2G);
+    launch_p2g(c->pos, c->vel, c->offsets, c->counts, c->u, c->v, c->w, c->ff, (float)(1.0 / c->dx),
+               (float)(1.0 / particles_per_cell), c->nx, c->ny, c->nz, c->stream);
+    stage_end(c, CLQ_STAGE_P2G);
+    c->coarse_ops_valid = false;
+    return check_launch("particle_to_grid");
+}
+
+// ---- grid glue ------------------------------------------------------------------------
+int clq_grid_obstacle_bc(clq_ctx *c) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    if (!c->has_obstacle) return CLQ_OK;  // every write of the two kernels needs a solid voxel
+    DeviceGuard g(c->device);
+    launch_grid_obstacle_bc(c->u, c->v, c->w, c->ff, c->obstacle, c->nx, c->ny, c->nz, c->stream);
+    c->coarse_ops_valid = false;
+    return check_launch("grid_obstacle_bc");
+}
+
+int clq_grid_domain_bc(clq_ctx *c) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    DeviceGuard g(c->device);
+    launch_grid_domain_bc(c->u, c->v, c->w, c->nx, c->ny, c->nz, c->stream);
+    return check_launch("grid_domain_bc");
+}
+
+int clq_update_velocity(clq_ctx *c, const double *gravity, const double *force, double rho, double nu, double dt) {
+    if (!c || !gravity || !force) return fail(CLQ_ERR_INVALID, "bad update_velocity arguments");
+    DeviceGuard g(c->device);
+    const float frho = (float)rho, fdt = (float)dt, fnu = (float)nu;
+    const float ax = (float)gravity[0] + (float)force[0] / frho;
+    const float ay = (float)gravity[1] + (float)force[1] / frho;
+    const float az = (float)gravity[2] + (float)force[2] / frho;
+    const float inv_dx2 = (float)(1.0 / (c->dx * c->dx));  // velocity_update.py:46
+    launch_update_velocity(c->u, c->v, c->w, c->uo, c->vo, c->wo, c->ff, ax, ay, az, fdt, fnu * fdt, inv_dx2, c->nx,
+                           c->ny, c->nz, c->stream);
+    std::swap(c->u, c->uo);  // solver.py:374-376
+    std::swap(c->v, c->vo);
+    std::swap(c->w, c->wo);
+    return check_launch("update_velocity");
+}
+
+// ---- pressure ----------------------------------------------------------------------------
+int clq_pressure_rhs(clq_ctx *c, double rho, double dt) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    DeviceGuard g(c->device);
+    launch_pressure_rhs(c->u, c->v, c->w, c->ff, c->rhs, (float)(rho / dt), (float)(1.0 / c->dx), c->nx, c->ny, c->nz,
+                        c->stream);
+    return check_launch("pressure_rhs");
+}
+
+int clq_build_operator(clq_ctx *c) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    DeviceGuard g(c->device);
+    launch_build_finest_operator(c->ff, c->diag, c->wx, c->wy, c->wz, c->nx, c->ny, c->nz, c->stream);
+    c->coarse_ops_valid = false;
+    return check_launch("build_operator");
+}
+
+static void ensure_coarse_operators(clq_ctx *c) {
+    if (c->coarse_ops_valid) return;
+    for (int l = 0; l + 1 < c->nlevels; ++l) launch_coarse_operator(c->lv[l], c->lv[l + 1], c->stream);
+    c->coarse_ops_valid = true;
+}
+
+static void smooth_level(clq_ctx *c, int level, int iterations) {
+    const float delta = (float)c->dx;
+    for (int it = 0; it < iterations; ++it) {
+        launch_rbgs(c->lv[level], delta, 0, c->stream);
+        launch_rbgs(c->lv[level], delta, 1, c->stream);
+    }
+}
+
+static void restrict_level(clq_ctx *c, int level) {
+    const float delta = (float)c->dx;
+    launch_restrict_residual(c->lv[level], c->lv[level + 1], 1.0f / (delta * delta), c->stream);
+}
+
+// multigrid.py:53-187
+static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra) {
+    if (level == c->nlevels - 1) {
+        launch_coarse_smooth(c->lv[level], (float)c->dx, pre + extra, c->stream);
+        return;
+    }
+    smooth_level(c, level, pre);
+    restrict_level(c, level);
+    w_cycle(c, level + 1, pre, post, extra);
+    w_cycle(c, level + 1, pre, post, extra);
+    launch_prolongate(c->lv[level], c->lv[level + 1], c->stream);
+    smooth_level(c, level, post);
+}
+
+int clq_smooth(clq_ctx *c, int32_t level, int32_t iterations) {
+    if (!c || level < 0 || level >= c->nlevels || iterations < 0) return fail(CLQ_ERR_INVALID, "bad smooth arguments");
+    DeviceGuard g(c->device);
+    smooth_level(c, level, iterations);
+    return check_launch("smooth");
+}
+
+int clq_restrict(clq_ctx *c, int32_t level) {
+    if (!c || level < 0 || level + 1 >= c->nlevels) return fail(CLQ_ERR_INVALID, "bad restrict level");
+    DeviceGuard g(c->device);
+    // The standalone call restates restrict_residual whole: operator and b.
+    launch_coarse_operator(c->lv[level], c->lv[level + 1], c->stream);
+    restrict_level(c, level);
+    if (level + 2 < c->nlevels) c->coarse_ops_valid = false;
+    return check_launch("restrict");
+}
+
+int clq_prolongate(clq_ctx *c, int32_t level) {
+    if (!c || level < 0 || level + 1 >= c->nlevels) return fail(CLQ_ERR_INVALID, "bad prolongate level");
+    DeviceGuard g(c->device);
+    launch_prolongate(c->lv[level], c->lv[level + 1], c->stream);
+    return check_launch("prolongate");
+}
+
+int clq_coarse_smooth(clq_ctx *c, int32_t iterations) {
+    if (!c || iterations < 0) return fail(CLQ_ERR_INVALID, "bad coarse_smooth arguments");
+    DeviceGuard g(c->device);
+    launch_coarse_smooth(c->lv[c->nlevels - 1], (float)c->dx, iterations, c->stream);
+    return check_launch("coarse_smooth");
+}
+
+int clq_w_cycle(clq_ctx *c, int32_t pre, int32_t post, int32_t extra) {
+    if (!c || pre < 0 || post < 0 || extra < 0) return fail(CLQ_ERR_INVALID, "bad w_cycle arguments");
+    DeviceGuard g(c->device);
+    ensure_coarse_operators(c);
+    w_cycle(c, 0, pre, post, extra);
+    return check_launch("w_cycle");
+}
+
+int clq_neumann_bc(clq_ctx *c) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    DeviceGuard g(c->device);
+    launch_neumann_bc(c->p, c->nx, c->ny, c->nz, c->stream);
+    return check_launch("neumann_bc");
+}
+
+static int residual_norm(clq_ctx *c, double *r, double *bn) {
+    const double h = (double)(float)c->dx;
+    launch_residual_norm(c->lv[0], 1.0 / (h * h), c->norm_partials, c->norm_out, c->stream);
+    HIP_TRY(hipMemcpyAsync(c->norm_host, c->norm_out, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *r = sqrt(c->norm_host[0]);
+    *bn = sqrt(c->norm_host[1]);
+    return CLQ_OK;
+}
+
+int clq_residual_norm(clq_ctx *c, double *residual, double *rhs_norm) {
+    if (!c || !residual || !rhs_norm) return fail(CLQ_ERR_INVALID, "NULL argument");
+    DeviceGuard g(c->device);
+    return residual_norm(c, residual, rhs_norm);
+}
+
+int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
+    if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
+    if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0)
+        return fail(CLQ_ERR_INVALID, "negative sweep or cycle count");
+    if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE)
+        return fail(CLQ_ERR_UNSUPPORTED, "solve mode %d is not available in this build", sp->mode);
+    DeviceGuard g(c->device);
+    clq_solve_stats s;
+    memset(&s, 0, sizeof s);
+    stage_begin(c, CLQ_STAGE_PRESSURE_SOLVE);
+    ensure_coarse_operators(c);
+    double r = 0.0, bn = 0.0;
+    if (int rc = residual_norm(c, &r, &bn)) return rc;
+    s.initial_residual = r;
+    s.rhs_norm = bn;
+    const double target = sp->tolerance * bn;
+    for (int cyc = 0; cyc < sp->max_cycles; ++cyc) {
+        if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE && r <= target) break;
+        w_cycle(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+        s.cycles++;
+        if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE)
+            if (int rc = residual_norm(c, &r, &bn)) return rc;
+    }
+    if (sp->mode == CLQ_SOLVE_FIXED_CYCLES)
+        if (int rc = residual_norm(c, &r, &bn)) return rc;
+    s.final_residual = r;
+    s.relative_residual = bn > 0 ? r / bn : 0.0;
+    launch_neumann_bc(c->p, c->nx, c->ny, c->nz, c->stream);  // pressure_solve.py:84
+    stage_end(c, CLQ_STAGE_PRESSURE_SOLVE);
+    HIP_TRY(hipEventSynchronize(c->ev_stop[CLQ_STAGE_PRESSURE_SOLVE]));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->ev_start[CLQ_STAGE_PRESSURE_SOLVE], c->ev_stop[CLQ_STAGE_PRESSURE_SOLVE]);
+    s.solve_ms = ms;
+    if (st) *st = s;
+    return check_launch("pressure_solve");
+}
+
+int clq_pressure_correction(clq_ctx *c, double rho, double dt) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    DeviceGuard g(c->device);
+    launch_pressure_correction(c->u, c->v, c->w, c->p, (float)(dt / rho), (float)(1.0 / c->dx), c->nx, c->ny, c->nz,
+                               c->stream);
+    return check_launch("pressure_correction");
+}
+
+// ---- particles -------------------------------------------------------------------------------
+static ParticleUpdate make_update(clq_ctx *c, int flags, double flip_ratio, double dt) {
+    ParticleUpdate pu;
+    pu.flags = flags;
+    pu.inv_dx = (float)(1.0 / c->dx);
+    pu.pure_flip = flip_ratio == 1.0;
+    pu.a = (float)flip_ratio;
+    pu.b = (float)(1.0 - flip_ratio);
+    pu.dt = (float)dt;
+    pu.lower = (float)(1.001 * c->dx);  // boundary_conditions.py:98-101
+    pu.upper[0] = (float)((c->nx - 1.001) * c->dx);
+    pu.upper[1] = (float)((c->ny - 1.001) * c->dx);
+    pu.upper[2] = (float)((c->nz - 1.001) * c->dx);
+    return pu;
+}
+
+static int particle_update(clq_ctx *c, int flags, double flip_ratio, double dt, const char *what) {
+    DeviceGuard g(c->device);
+    const ParticleUpdate pu = make_update(c, flags, flip_ratio, dt);
+    launch_particle_update(c->pos, c->vel, c->prev, c->u, c->v, c->w, c->uo, c->vo, c->wo, c->obstacle, (int)c->n,
+                           c->nx, c->ny, c->nz, pu, c->stream);
+    return check_launch(what);
+}
+
+int clq_grid_to_particle(clq_ctx *c, double flip_ratio) {
+    if (!c || !(flip_ratio >= 0.0 && flip_ratio <= 1.0)) return fail(CLQ_ERR_INVALID, "flip_ratio must be in [0, 1]");
+    return particle_update(c, kG2P, flip_ratio, 0.0, "grid_to_particle");
+}
+
+int clq_save_previous_positions(clq_ctx *c) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    return particle_update(c, kSavePrev, 1.0, 0.0, "save_previous_positions");
+}
+
+int clq_advect_particles(clq_ctx *c, double dt) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    return particle_update(c, kAdvect, 1.0, dt, "advect_particles");
+}
+
+int clq_particle_domain_bc(clq_ctx *c) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    return particle_update(c, kDomainBC, 1.0, 0.0, "particle_domain_bc");
+}
+
+int clq_particle_obstacle_bc(clq_ctx *c) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    if (!c->has_obstacle) return CLQ_OK;  // no solid voxel: the kernel returns before any write
+    return particle_update(c, kObstacleBC, 1.0, 0.0, "particle_obstacle_bc");
+}
+
+int clq_grid_to_particle_advect(clq_ctx *c, double flip_ratio, double dt) {
+    if (!c || !(flip_ratio >= 0.0 && flip_ratio <= 1.0)) return fail(CLQ_ERR_INVALID, "flip_ratio must be in [0, 1]");
+    // Without solids the previous positions are never read, so the copy is
+    // skipped; clq_get_previous_positions then reports the last copy made.
+    const int flags = c->has_obstacle ? (kG2P | kSavePrev | kAdvect | kDomainBC | kObstacleBC)
+                                      : (kG2P | kAdvect | kDomainBC);
+    stage_begin(c, CLQ_STAGE_G2P_ADVECT);
+    int rc = particle_update(c, flags, flip_ratio, dt, "grid_to_particle_advect");
+    stage_end(c, CLQ_STAGE_G2P_ADVECT);
+    return rc;
+}
+
+int clq_voxelise(clq_ctx *c, int32_t scale, double particles_per_cell, uint8_t *occupancy, float *density) {
+    (void)scale; (void)particles_per_cell; (void)occupancy; (void)density;
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    return fail(CLQ_ERR_UNSUPPORTED, "clq_voxelise is not available in this build");
+}
+
+int clq_stage_times(clq_ctx *c, double *ms, int32_t capacity) {
+    if (!c || !ms) return fail(CLQ_ERR_INVALID, "NULL argument");
+    DeviceGuard g(c->device);
+    const int n = std::min<int>(capacity, CLQ_STAGE_COUNT);
+    for (int s = 0; s < n; ++s) {
+        ms[s] = 0.0;
+        if (!c->ev_used[s]) continue;
+        if (hipEventSynchronize(c->ev_stop[s]) != hipSuccess) continue;
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, c->ev_start[s], c->ev_stop[s]) == hipSuccess) ms[s] = t;
+    }
+    return n;
+}
+
+}  // extern "C"

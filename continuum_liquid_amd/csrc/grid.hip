@@ -1,0 +1,253 @@
+// Grid glue kernels of the step: boundary conditions, forces and viscosity,
+// pressure right-hand side, finest operator, pressure correction.  All are
+// streaming per-cell kernels, HBM-bound, one lane per cell.
+//
+// Each restates one reference kernel (Solver/Kernel/OpenCL/...):
+//   grid_obstacle_bc + extrapolate_obstacle_velocity  boundary_conditions.cl:99-173
+//   grid_domain_bc                                    boundary_conditions.cl:33-67
+//   update_velocity                                   velocity_update.cl:3-43
+//   pressure_rhs                                      pressure_solve.cl:3-24
+//   build_finest_operator                             multigrid.cl:63-88
+//   neumann_domain_bc                                 boundary_conditions.cl:18-31
+//   pressure_correction                               pressure_solve.cl:26-47
+#include "common.hpp"
+
+namespace clq {
+
+static inline dim3 cells_grid(int64_t C) { return dim3((unsigned)((C + 255) / 256)); }
+
+__global__ __launch_bounds__(256) void grid_obstacle_kernel(float *__restrict__ u, float *__restrict__ v,
+                                                            float *__restrict__ w, float *__restrict__ ff,
+                                                            const uint8_t *__restrict__ ob, int nx, int ny, int nz) {
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= (int64_t)nx * ny * nz) return;
+    int i, j, k;
+    coords(index, ny, nz, i, j, k);
+    const bool self = ob[index];
+    if (self) ff[index] = 0.0f;
+    const bool xs = self || (i + 1 < nx && ob[gidx(i + 1, j, k, ny, nz)]);
+    const bool ys = self || (j + 1 < ny && ob[gidx(i, j + 1, k, ny, nz)]);
+    const bool zs = self || (k + 1 < nz && ob[gidx(i, j, k + 1, ny, nz)]);
+    if (xs) u[index] = 0.0f;
+    if (ys) v[index] = 0.0f;
+    if (zs) w[index] = 0.0f;
+}
+
+// Solid cells read only fluid neighbours, which this kernel never writes.
+__global__ __launch_bounds__(256) void extrapolate_obstacle_kernel(float *__restrict__ u, float *__restrict__ v,
+                                                                   float *__restrict__ w,
+                                                                   const uint8_t *__restrict__ ob, int nx, int ny,
+                                                                   int nz) {
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= (int64_t)nx * ny * nz || !ob[index]) return;
+    int i, j, k;
+    coords(index, ny, nz, i, j, k);
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+    int nf = 0;
+    bool ax = false, ay = false, az = false;
+    auto acc = [&](int a, int b, int c, int axis) {
+        const int64_t nb = gidx(a, b, c, ny, nz);
+        if (!ob[nb]) {
+            t0 = t0 + u[nb];
+            t1 = t1 + v[nb];
+            t2 = t2 + w[nb];
+            nf += 1;
+            if (axis == 0) ax = true;
+            if (axis == 1) ay = true;
+            if (axis == 2) az = true;
+        }
+    };
+    if (i > 0) acc(i - 1, j, k, 0);
+    if (i + 1 < nx) acc(i + 1, j, k, 0);
+    if (j > 0) acc(i, j - 1, k, 1);
+    if (j + 1 < ny) acc(i, j + 1, k, 1);
+    if (k > 0) acc(i, j, k - 1, 2);
+    if (k + 1 < nz) acc(i, j, k + 1, 2);
+    if (nf == 0) return;
+    const float fn = (float)nf;
+    u[index] = ax ? 0.0f : t0 / fn;
+    v[index] = ay ? 0.0f : t1 / fn;
+    w[index] = az ? 0.0f : t2 / fn;
+}
+
+void launch_grid_obstacle_bc(float *u, float *v, float *w, float *ff, const uint8_t *ob, int nx, int ny, int nz,
+                             hipStream_t s) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(grid_obstacle_kernel, cells_grid(C), dim3(256), 0, s, u, v, w, ff, ob, nx, ny, nz);
+    hipLaunchKernelGGL(extrapolate_obstacle_kernel, cells_grid(C), dim3(256), 0, s, u, v, w, ob, nx, ny, nz);
+}
+
+// A ghost cell copies only components that are not walls at its clamped
+// interior cell, and interior lanes write only wall components: race free.
+__global__ __launch_bounds__(256) void grid_domain_kernel(float *__restrict__ u, float *__restrict__ v,
+                                                          float *__restrict__ w, int nx, int ny, int nz) {
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= (int64_t)nx * ny * nz) return;
+    int i, j, k;
+    coords(index, ny, nz, i, j, k);
+    const bool ghost = !interior(i, j, k, nx, ny, nz);
+    const bool wu = i == 0 || i >= nx - 2;
+    const bool wv = j == 0 || j >= ny - 2;
+    const bool ww = k == 0 || k >= nz - 2;
+    if (!ghost && !wu && !wv && !ww) return;
+    if (ghost) {
+        const int ii = min(max(i, 1), nx - 2), jj = min(max(j, 1), ny - 2), kk = min(max(k, 1), nz - 2);
+        const int64_t in = gidx(ii, jj, kk, ny, nz);
+        u[index] = wu ? 0.0f : u[in];
+        v[index] = wv ? 0.0f : v[in];
+        w[index] = ww ? 0.0f : w[in];
+        return;
+    }
+    if (wu) u[index] = 0.0f;
+    if (wv) v[index] = 0.0f;
+    if (ww) w[index] = 0.0f;
+}
+
+void launch_grid_domain_bc(float *u, float *v, float *w, int nx, int ny, int nz, hipStream_t s) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(grid_domain_kernel, cells_grid(C), dim3(256), 0, s, u, v, w, nx, ny, nz);
+}
+
+// laplacian (helper.cl:54-66): fma(-6, centre, neighbour_sum) * inv_dx2.
+__device__ __forceinline__ float laplacian(const float *__restrict__ f, int64_t idx, int64_t sx, int64_t sy,
+                                           float inv_dx2) {
+    const float centre = f[idx];
+    float s = f[idx + sx] + f[idx - sx];
+    s = s + f[idx + sy];
+    s = s + f[idx - sy];
+    s = s + f[idx + 1];
+    s = s + f[idx - 1];
+    return fmaf(-6.0f, centre, s) * inv_dx2;
+}
+
+__global__ __launch_bounds__(256) void update_velocity_kernel(const float *__restrict__ u, const float *__restrict__ v,
+                                                              const float *__restrict__ w, float *__restrict__ ru,
+                                                              float *__restrict__ rv, float *__restrict__ rw,
+                                                              const float *__restrict__ ff, float ax, float ay,
+                                                              float az, float dt, float nudt, float inv_dx2, int nx,
+                                                              int ny, int nz) {
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= (int64_t)nx * ny * nz) return;
+    int i, j, k;
+    coords(index, ny, nz, i, j, k);
+    float r0 = u[index], r1 = v[index], r2 = w[index];
+    if (ff[index] > 0.0f) {  // velocity_update.cl:31 (fmuladd)
+        r0 = fmaf(dt, ax, r0);
+        r1 = fmaf(dt, ay, r1);
+        r2 = fmaf(dt, az, r2);
+    }
+    if (interior(i, j, k, nx, ny, nz)) {  // velocity_update.cl:35-37 (fmuladd)
+        const int64_t sx = (int64_t)ny * nz, sy = nz;
+        r0 = fmaf(nudt, laplacian(u, index, sx, sy, inv_dx2), r0);
+        r1 = fmaf(nudt, laplacian(v, index, sx, sy, inv_dx2), r1);
+        r2 = fmaf(nudt, laplacian(w, index, sx, sy, inv_dx2), r2);
+    }
+    ru[index] = r0;
+    rv[index] = r1;
+    rw[index] = r2;
+}
+
+void launch_update_velocity(const float *u, const float *v, const float *w, float *ru, float *rv, float *rw,
+                            const float *ff, float ax, float ay, float az, float dt, float nudt, float inv_dx2, int nx,
+                            int ny, int nz, hipStream_t s) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(update_velocity_kernel, cells_grid(C), dim3(256), 0, s, u, v, w, ru, rv, rw, ff, ax, ay, az,
+                       dt, nudt, inv_dx2, nx, ny, nz);
+}
+
+__global__ __launch_bounds__(256) void pressure_rhs_kernel(const float *__restrict__ u, const float *__restrict__ v,
+                                                           const float *__restrict__ w, const float *__restrict__ ff,
+                                                           float *__restrict__ rhs, float rho_over_dt, float inv_dx,
+                                                           int nx, int ny, int nz) {
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= (int64_t)nx * ny * nz) return;
+    int i, j, k;
+    coords(index, ny, nz, i, j, k);
+    if (!interior(i, j, k, nx, ny, nz) || ff[index] <= 0.0f) {
+        rhs[index] = 0.0f;
+        return;
+    }
+    const int64_t sx = (int64_t)ny * nz, sy = nz;
+    const float du = u[index] - u[index - sx];
+    const float dv = v[index] - v[index - sy];
+    const float dw = w[index] - w[index - 1];
+    rhs[index] = rho_over_dt * ((du + dv + dw) * inv_dx);
+}
+
+void launch_pressure_rhs(const float *u, const float *v, const float *w, const float *ff, float *rhs,
+                         float rho_over_dt, float inv_dx, int nx, int ny, int nz, hipStream_t s) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(pressure_rhs_kernel, cells_grid(C), dim3(256), 0, s, u, v, w, ff, rhs, rho_over_dt, inv_dx,
+                       nx, ny, nz);
+}
+
+__global__ __launch_bounds__(256) void build_finest_operator_kernel(const float *__restrict__ mask,
+                                                                    float *__restrict__ diag, float *__restrict__ wx,
+                                                                    float *__restrict__ wy, float *__restrict__ wz,
+                                                                    int nx, int ny, int nz) {
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= (int64_t)nx * ny * nz) return;
+    int i, j, k;
+    coords(index, ny, nz, i, j, k);
+    float d = 0.0f, a = 0.0f, b = 0.0f, c = 0.0f;
+    if (interior(i, j, k, nx, ny, nz) && mask[index] > 0.0f) {
+        d = 6.0f - (float)(i == 1) - (float)(i == nx - 2) - (float)(j == 1) - (float)(j == ny - 2) -
+            (float)(k == 1) - (float)(k == nz - 2);
+        if (i < nx - 2) a = mask[index + (int64_t)ny * nz] > 0.0f ? 1.0f : 0.0f;
+        if (j < ny - 2) b = mask[index + nz] > 0.0f ? 1.0f : 0.0f;
+        if (k < nz - 2) c = mask[index + 1] > 0.0f ? 1.0f : 0.0f;
+    }
+    diag[index] = d;
+    wx[index] = a;
+    wy[index] = b;
+    wz[index] = c;
+}
+
+void launch_build_finest_operator(const float *mask, float *diag, float *wx, float *wy, float *wz, int nx, int ny,
+                                  int nz, hipStream_t s) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(build_finest_operator_kernel, cells_grid(C), dim3(256), 0, s, mask, diag, wx, wy, wz, nx, ny,
+                       nz);
+}
+
+__global__ __launch_bounds__(256) void neumann_kernel(float *__restrict__ p, int nx, int ny, int nz) {
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= (int64_t)nx * ny * nz) return;
+    int i, j, k;
+    coords(index, ny, nz, i, j, k);
+    if (interior(i, j, k, nx, ny, nz)) return;
+    const int ii = min(max(i, 1), nx - 2), jj = min(max(j, 1), ny - 2), kk = min(max(k, 1), nz - 2);
+    p[index] = p[gidx(ii, jj, kk, ny, nz)];
+}
+
+void launch_neumann_bc(float *p, int nx, int ny, int nz, hipStream_t s) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(neumann_kernel, cells_grid(C), dim3(256), 0, s, p, nx, ny, nz);
+}
+
+__global__ __launch_bounds__(256) void pressure_correction_kernel(float *__restrict__ u, float *__restrict__ v,
+                                                                  float *__restrict__ w, const float *__restrict__ p,
+                                                                  float dt_over_rho, float inv_dx, int nx, int ny,
+                                                                  int nz) {
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= (int64_t)nx * ny * nz) return;
+    int i, j, k;
+    coords(index, ny, nz, i, j, k);
+    if (!interior(i, j, k, nx, ny, nz)) return;
+    const float centre = p[index];
+    const float gx = (p[index + (int64_t)ny * nz] - centre) * inv_dx;
+    const float gy = (p[index + nz] - centre) * inv_dx;
+    const float gz = (p[index + 1] - centre) * inv_dx;
+    u[index] = fmaf(-dt_over_rho, gx, u[index]);  // pressure_solve.cl:44-46 (fmuladd)
+    v[index] = fmaf(-dt_over_rho, gy, v[index]);
+    w[index] = fmaf(-dt_over_rho, gz, w[index]);
+}
+
+void launch_pressure_correction(float *u, float *v, float *w, const float *p, float dt_over_rho, float inv_dx, int nx,
+                                int ny, int nz, hipStream_t s) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(pressure_correction_kernel, cells_grid(C), dim3(256), 0, s, u, v, w, p, dt_over_rho, inv_dx,
+                       nx, ny, nz);
+}
+
+}  // namespace clq

@@ -1,0 +1,298 @@
+// Multigrid kernels of the pressure solve (reference
+// Solver/Kernel/OpenCL/multigrid.cl, driven by Solver/Kernel/multigrid.py).
+//
+//   rbgs            rbgs_step (multigrid.cl:203-230), one lane per cell of
+//                   the colour being updated
+//   coarse_operator Galerkin coarse diagonal/weights/mask of
+//                   restrict_residual (multigrid.cl:122-179), built once per
+//                   solve instead of on every restriction visit: the fine
+//                   operator does not change inside a solve, so the values are
+//                   the ones every visit of the reference recomputes
+//   restrict        coarse b = sum of child residuals, coarse p = 0
+//                   (multigrid.cl:90-180)
+//   prolongate      multigrid.cl:182-201
+//   coarse_smooth   coarse_smooth (multigrid.cl:232-264) in one work-group
+//                   with the whole coarsest level staged in LDS
+//   residual_norm   FP64 ||b - A p||^2 and ||b||^2, fixed reduction tree
+#include "common.hpp"
+
+namespace clq {
+
+// weighted_neighbor_sum (multigrid.cl:7-23) as clang contracts it.
+__device__ __forceinline__ float wsum(const float *__restrict__ p, const float *__restrict__ wx,
+                                      const float *__restrict__ wy, const float *__restrict__ wz, int64_t idx,
+                                      int64_t sx, int64_t sy) {
+    float s = wx[idx - sx] * p[idx - sx];
+    s = fmaf(wx[idx], p[idx + sx], s);
+    s = fmaf(wy[idx], p[idx + sy], s);
+    s = fmaf(wy[idx - sy], p[idx - sy], s);
+    s = fmaf(wz[idx], p[idx + 1], s);
+    s = fmaf(wz[idx - 1], p[idx - 1], s);
+    return s;
+}
+
+// One colour of red-black Gauss-Seidel.  Lane t handles the t-th cell of the
+// colour in row-major order: row (i, j), k = 2*kk + ((parity + i + j) & 1).
+// Non-fluid cells of that colour are zeroed, as rbgs_step zeroes every
+// non-fluid cell; after a full sweep both colours have been zeroed.
+__global__ __launch_bounds__(256) void rbgs_kernel(float *__restrict__ p, const float *__restrict__ b,
+                                                   const float *__restrict__ mask, const float *__restrict__ diag,
+                                                   const float *__restrict__ wx, const float *__restrict__ wy,
+                                                   const float *__restrict__ wz, float dd, int parity, int nx, int ny,
+                                                   int nz) {
+    const int half = (nz + 1) >> 1;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t rows = (int64_t)nx * ny;
+    if (t >= rows * half) return;
+    const int64_t row = t / half;
+    const int kk = (int)(t - row * half);
+    const int i = (int)(row / ny), j = (int)(row - (int64_t)i * ny);
+    const int k = 2 * kk + ((parity + i + j) & 1);
+    if (k >= nz) return;
+    const int64_t idx = row * nz + k;
+    const float m = mask[idx];
+    if (m <= 0.0f) {
+        p[idx] = 0.0f;
+        return;
+    }
+    if (!interior(i, j, k, nx, ny, nz)) return;
+    const float d = diag[idx];
+    if (d > 0.0f) {
+        const int64_t sx = (int64_t)ny * nz, sy = nz;
+        p[idx] = fmaf(-dd, b[idx], wsum(p, wx, wy, wz, idx, sx, sy)) / d;  // multigrid.cl:227 (fmuladd)
+    }
+}
+
+void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s) {
+    const int half = (L.nz + 1) >> 1;
+    const int64_t work = (int64_t)L.nx * L.ny * half;
+    hipLaunchKernelGGL(rbgs_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L.p, L.b, L.mask, L.diag,
+                       L.wx, L.wy, L.wz, delta * delta, parity, L.nx, L.ny, L.nz);
+}
+
+__global__ __launch_bounds__(256) void coarse_operator_kernel(
+    const float *__restrict__ fmask, const float *__restrict__ fdiag, const float *__restrict__ fwx,
+    const float *__restrict__ fwy, const float *__restrict__ fwz, int fnx, int fny, int fnz, float *__restrict__ cmask,
+    float *__restrict__ cdiag, float *__restrict__ cwx, float *__restrict__ cwy, float *__restrict__ cwz, int cnx,
+    int cny, int cnz) {
+    const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (ci >= (int64_t)cnx * cny * cnz) return;
+    int a, b, c;
+    coords(ci, cny, cnz, a, b, c);
+    float m = 0.0f, ds = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    if (interior(a, b, c, cnx, cny, cnz)) {
+        const int f0 = 1 + 2 * (a - 1), f1 = 1 + 2 * (b - 1), f2 = 1 + 2 * (c - 1);
+        int active = 0;
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj)
+                for (int dk = 0; dk < 2; ++dk) {
+                    const int i = f0 + di, j = f1 + dj, k = f2 + dk;
+                    if (!interior(i, j, k, fnx, fny, fnz)) continue;
+                    const int64_t idx = gidx(i, j, k, fny, fnz);
+                    if (fmask[idx] <= 0.0f) continue;
+                    ++active;
+                    ds = ds + fdiag[idx];
+                    const float x = fwx[idx], y = fwy[idx], z = fwz[idx];
+                    if (x > 0.0f) { if (di == 0) ds = fmaf(-2.0f, x, ds); else cx = cx + x; }
+                    if (y > 0.0f) { if (dj == 0) ds = fmaf(-2.0f, y, ds); else cy = cy + y; }
+                    if (z > 0.0f) { if (dk == 0) ds = fmaf(-2.0f, z, ds); else cz = cz + z; }
+                }
+        if (active > 0) m = 1.0f;
+        else ds = cx = cy = cz = 0.0f;
+    }
+    cmask[ci] = m;
+    cdiag[ci] = ds;
+    cwx[ci] = cx;
+    cwy[ci] = cy;
+    cwz[ci] = cz;
+}
+
+void launch_coarse_operator(const Level &F, const Level &K, hipStream_t s) {
+    hipLaunchKernelGGL(coarse_operator_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F.mask, F.diag,
+                       F.wx, F.wy, F.wz, F.nx, F.ny, F.nz, K.mask, K.diag, K.wx, K.wy, K.wz, K.nx, K.ny, K.nz);
+}
+
+// residual (multigrid.cl:45-61) at an interior fluid cell.
+__device__ __forceinline__ float residual_at(const float *__restrict__ p, const float *__restrict__ b,
+                                             const float *__restrict__ diag, const float *__restrict__ wx,
+                                             const float *__restrict__ wy, const float *__restrict__ wz, int64_t idx,
+                                             int64_t sx, int64_t sy, float inv_delta2) {
+    const float ap = fmaf(-diag[idx], p[idx], wsum(p, wx, wy, wz, idx, sx, sy)) * inv_delta2;  // :41 (fmuladd)
+    return b[idx] - ap;
+}
+
+__global__ __launch_bounds__(256) void restrict_kernel(const float *__restrict__ fp, const float *__restrict__ fb,
+                                                       const float *__restrict__ fmask,
+                                                       const float *__restrict__ fdiag, const float *__restrict__ fwx,
+                                                       const float *__restrict__ fwy, const float *__restrict__ fwz,
+                                                       int fnx, int fny, int fnz, float *__restrict__ cp,
+                                                       float *__restrict__ cb, int cnx, int cny, int cnz,
+                                                       float inv_delta2) {
+    const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (ci >= (int64_t)cnx * cny * cnz) return;
+    int a, b, c;
+    coords(ci, cny, cnz, a, b, c);
+    float rhs = 0.0f;
+    if (interior(a, b, c, cnx, cny, cnz)) {
+        const int64_t sx = (int64_t)fny * fnz, sy = fnz;
+        const int f0 = 1 + 2 * (a - 1), f1 = 1 + 2 * (b - 1), f2 = 1 + 2 * (c - 1);
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj)
+                for (int dk = 0; dk < 2; ++dk) {
+                    const int i = f0 + di, j = f1 + dj, k = f2 + dk;
+                    if (!interior(i, j, k, fnx, fny, fnz)) continue;
+                    const int64_t idx = gidx(i, j, k, fny, fnz);
+                    if (fmask[idx] <= 0.0f) continue;
+                    rhs = rhs + residual_at(fp, fb, fdiag, fwx, fwy, fwz, idx, sx, sy, inv_delta2);
+                }
+    }
+    cp[ci] = 0.0f;
+    cb[ci] = rhs;
+}
+
+void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, hipStream_t s) {
+    hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F.p, F.b, F.mask, F.diag,
+                       F.wx, F.wy, F.wz, F.nx, F.ny, F.nz, K.p, K.b, K.nx, K.ny, K.nz, inv_delta2);
+}
+
+__global__ __launch_bounds__(256) void prolongate_kernel(const float *__restrict__ cp, float *__restrict__ fp,
+                                                         const float *__restrict__ fmask, int cnx, int cny, int cnz,
+                                                         int fnx, int fny, int fnz) {
+    const int64_t fi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (fi >= (int64_t)fnx * fny * fnz) return;
+    int i, j, k;
+    coords(fi, fny, fnz, i, j, k);
+    if (!interior(i, j, k, fnx, fny, fnz) || fmask[fi] <= 0.0f) return;
+    const int a = min((i - 1) / 2 + 1, cnx - 2), b = min((j - 1) / 2 + 1, cny - 2), c = min((k - 1) / 2 + 1, cnz - 2);
+    fp[fi] = fp[fi] + cp[gidx(a, b, c, cny, cnz)];
+}
+
+void launch_prolongate(const Level &F, const Level &K, hipStream_t s) {
+    hipLaunchKernelGGL(prolongate_kernel, dim3((unsigned)((F.C + 255) / 256)), dim3(256), 0, s, K.p, F.p, F.mask, K.nx,
+                       K.ny, K.nz, F.nx, F.ny, F.nz);
+}
+
+// coarse_smooth: one work-group, the level's 7 arrays staged in LDS when they
+// fit (10^3 cells: 28 KB; up to 2048 cells, 56 KB).
+constexpr int kCoarseThreads = 1024;
+constexpr int kCoarseLdsCells = 2048;
+
+__global__ __launch_bounds__(kCoarseThreads) void coarse_smooth_kernel(
+    float *__restrict__ gp, const float *__restrict__ gb, const float *__restrict__ gmask,
+    const float *__restrict__ gdiag, const float *__restrict__ gwx, const float *__restrict__ gwy,
+    const float *__restrict__ gwz, float delta2, int iterations, int nx, int ny, int nz) {
+    __shared__ float sp[kCoarseLdsCells], sb[kCoarseLdsCells], sm[kCoarseLdsCells], sd[kCoarseLdsCells];
+    __shared__ float sx_[kCoarseLdsCells], sy_[kCoarseLdsCells], sz_[kCoarseLdsCells];
+    const int C = nx * ny * nz;
+    for (int q = threadIdx.x; q < C; q += kCoarseThreads) {
+        sp[q] = gp[q]; sb[q] = gb[q]; sm[q] = gmask[q]; sd[q] = gdiag[q];
+        sx_[q] = gwx[q]; sy_[q] = gwy[q]; sz_[q] = gwz[q];
+    }
+    __syncthreads();
+    const int64_t sx = (int64_t)ny * nz, sy = nz;
+    for (int it = 0; it < iterations; ++it)
+        for (int parity = 0; parity < 2; ++parity) {
+            for (int q = threadIdx.x; q < C; q += kCoarseThreads) {
+                int i, j, k;
+                coords(q, ny, nz, i, j, k);
+                if (!interior(i, j, k, nx, ny, nz) || sm[q] <= 0.0f || ((i + j + k) & 1) != parity) continue;
+                const float d = sd[q];
+                if (d > 0.0f) sp[q] = fmaf(-delta2, sb[q], wsum(sp, sx_, sy_, sz_, q, sx, sy)) / d;  // :257
+            }
+            __syncthreads();
+        }
+    for (int q = threadIdx.x; q < C; q += kCoarseThreads) gp[q] = sp[q];
+}
+
+void launch_coarse_smooth(const Level &L, float delta, int iterations, hipStream_t s) {
+    if (L.C > kCoarseLdsCells) {
+        // Larger coarsest levels (only for tiny min sizes) fall back to the
+        // colour kernels, which give the same result: fluid cells only are
+        // updated, and non-fluid cells are not touched by coarse_smooth.
+        for (int it = 0; it < iterations; ++it)
+            for (int parity = 0; parity < 2; ++parity) launch_rbgs(L, delta, parity, s);
+        return;
+    }
+    hipLaunchKernelGGL(coarse_smooth_kernel, dim3(1), dim3(kCoarseThreads), 0, s, L.p, L.b, L.mask, L.diag, L.wx, L.wy,
+                       L.wz, delta * delta, iterations, L.nx, L.ny, L.nz);
+}
+
+// ---- FP64 residual norm -------------------------------------------------------------
+constexpr int kNormThreads = 256;
+constexpr int kNormBlocks = 2048;
+
+int residual_partials_count(int64_t) { return 2 * kNormBlocks; }
+
+__global__ __launch_bounds__(kNormThreads) void residual_norm_kernel(
+    const float *__restrict__ p, const float *__restrict__ b, const float *__restrict__ mask,
+    const float *__restrict__ diag, const float *__restrict__ wx, const float *__restrict__ wy,
+    const float *__restrict__ wz, double inv_h2, int nx, int ny, int nz, double *__restrict__ partials) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    const int64_t sx = (int64_t)ny * nz, sy = nz;
+    double rr = 0.0, bb = 0.0;
+    for (int64_t idx = (int64_t)blockIdx.x * kNormThreads + threadIdx.x; idx < C;
+         idx += (int64_t)gridDim.x * kNormThreads) {
+        int i, j, k;
+        coords(idx, ny, nz, i, j, k);
+        const double bv = b[idx];
+        double ap = 0.0;
+        if (interior(i, j, k, nx, ny, nz) && mask[idx] > 0.0f) {
+            const double s = (double)wx[idx] * p[idx + sx] + (double)wx[idx - sx] * p[idx - sx] +
+                             (double)wy[idx] * p[idx + sy] + (double)wy[idx - sy] * p[idx - sy] +
+                             (double)wz[idx] * p[idx + 1] + (double)wz[idx - 1] * p[idx - 1];
+            ap = (s - (double)diag[idx] * p[idx]) * inv_h2;
+        }
+        const double r = bv - ap;
+        rr += r * r;
+        bb += bv * bv;
+    }
+    __shared__ double s_r[kNormThreads], s_b[kNormThreads];
+    s_r[threadIdx.x] = rr;
+    s_b[threadIdx.x] = bb;
+    __syncthreads();
+    for (int off = kNormThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            s_r[threadIdx.x] += s_r[threadIdx.x + off];
+            s_b[threadIdx.x] += s_b[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s_r[0];
+        partials[kNormBlocks + blockIdx.x] = s_b[0];
+    }
+}
+
+__global__ __launch_bounds__(kNormThreads) void reduce_partials_kernel(const double *__restrict__ partials, int count,
+                                                                       double *__restrict__ out) {
+    __shared__ double s_r[kNormThreads], s_b[kNormThreads];
+    double rr = 0.0, bb = 0.0;
+    for (int q = threadIdx.x; q < count; q += kNormThreads) {
+        rr += partials[q];
+        bb += partials[kNormBlocks + q];
+    }
+    s_r[threadIdx.x] = rr;
+    s_b[threadIdx.x] = bb;
+    __syncthreads();
+    for (int off = kNormThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            s_r[threadIdx.x] += s_r[threadIdx.x + off];
+            s_b[threadIdx.x] += s_b[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s_r[0];
+        out[1] = s_b[0];
+    }
+}
+
+void launch_residual_norm(const Level &L, double inv_h2, double *partials, double *out2, hipStream_t s) {
+    int blocks = (int)((L.C + kNormThreads - 1) / kNormThreads);
+    if (blocks > kNormBlocks) blocks = kNormBlocks;
+    hipLaunchKernelGGL(residual_norm_kernel, dim3(blocks), dim3(kNormThreads), 0, s, L.p, L.b, L.mask, L.diag, L.wx,
+                       L.wy, L.wz, inv_h2, L.nx, L.ny, L.nz, partials);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kNormThreads), 0, s, partials, blocks, out2);
+}
+
+}  // namespace clq

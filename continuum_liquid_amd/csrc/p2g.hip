@@ -1,0 +1,151 @@
+// P2G gather onto the MAC grid plus occupancy voxelisation.
+//
+// Replaces particle_to_grid_gather (reference
+// Solver/Kernel/OpenCL/particles.cl:135-227, launched at
+// Solver/Kernel/particles.py:142-163 with 8 lanes per cell).
+//
+// One lane owns one cell and gathers from the neighbour bins of the sorted
+// particle array; no float atomics.  The lane keeps eight partial sums, one
+// per lane of the reference's 8-lane work-group (neighbour bin n goes to
+// partial n % 8), visits each bin's particles in sorted order and combines the
+// partials with the reference's tree (0+4, 2+6, 1+5, 3+7, then pairs), so the
+// summation order is the reference's and the result is bit-identical to the
+// CPU oracle.
+//
+// Bins whose tent weight is provably zero for a component are skipped for
+// that component: with bin b = floor(fma(x, inv_dx, -0.5)) and p = x*inv_dx,
+// p - b lies in [0.5, 1.5] (also under the clamps at the domain edges), so a
+// u face (offset (1, .5, .5)) only sees bins with y and z offsets in {-1, 0},
+// v only x, z offsets in {-1, 0}, w only x, y in {-1, 0}; the occupancy cell
+// trunc(p) only sees offsets in {-1, 0}^3.  Skipped terms are exact zeros, so
+// skipping does not change a bit.  Of the 27 bins, 8 are visited for all three
+// components and 12 for one.
+#include <utility>
+
+#include "common.hpp"
+
+namespace clq {
+
+template <int NB>
+__device__ __forceinline__ void gather_bin(const float *__restrict__ pos, const float *__restrict__ vel,
+                                           const int32_t *__restrict__ offsets, const int32_t *__restrict__ counts,
+                                           int ci, int cj, int ck, int nx, int ny, int nz, float inv_dx,
+                                           const float *upos, const float *vpos, const float *wpos, float (&vs)[8][3],
+                                           float (&ws)[8][3], int (&cnt)[8]) {
+    constexpr int ox = NB / 9 - 1, oy = (NB / 3) % 3 - 1, oz = NB % 3 - 1;
+    constexpr bool doU = oy <= 0 && oz <= 0;
+    constexpr bool doV = ox <= 0 && oz <= 0;
+    constexpr bool doW = ox <= 0 && oy <= 0;
+    constexpr bool doOcc = ox <= 0 && oy <= 0 && oz <= 0;
+    constexpr int lane = NB & 7;
+    if constexpr (!(doU || doV || doW)) {
+        return;
+    } else {
+        const int bx = ci + ox, by = cj + oy, bz = ck + oz;
+        if (bx < 0 || bx >= nx - 1 || by < 0 || by >= ny - 1 || bz < 0 || bz >= nz - 1) return;
+        const int64_t bin = gidx(bx, by, bz, ny, nz);
+        const int begin = offsets[bin];
+        const int end = begin + counts[bin];
+        for (int pi = begin; pi < end; ++pi) {
+            const float x0 = pos[3LL * pi], x1 = pos[3LL * pi + 1], x2 = pos[3LL * pi + 2];
+            const float p0 = x0 * inv_dx, p1 = x1 * inv_dx, p2 = x2 * inv_dx;
+            if constexpr (doOcc) {
+                const int tx = trunc_clamp(p0, 1, nx - 2);
+                const int ty = trunc_clamp(p1, 1, ny - 2);
+                const int tz = trunc_clamp(p2, 1, nz - 2);
+                cnt[lane] += (tx == ci && ty == cj && tz == ck);
+            }
+            if constexpr (doU) {
+                const float a = fmaxf(0.0f, 1.0f - fabsf(p0 - upos[0]));
+                const float b = fmaxf(0.0f, 1.0f - fabsf(p1 - upos[1]));
+                const float c = fmaxf(0.0f, 1.0f - fabsf(p2 - upos[2]));
+                const float wgt = a * b * c;
+                vs[lane][0] = fmaf(wgt, vel[3LL * pi], vs[lane][0]);
+                ws[lane][0] = ws[lane][0] + wgt;
+            }
+            if constexpr (doV) {
+                const float a = fmaxf(0.0f, 1.0f - fabsf(p0 - vpos[0]));
+                const float b = fmaxf(0.0f, 1.0f - fabsf(p1 - vpos[1]));
+                const float c = fmaxf(0.0f, 1.0f - fabsf(p2 - vpos[2]));
+                const float wgt = a * b * c;
+                vs[lane][1] = fmaf(wgt, vel[3LL * pi + 1], vs[lane][1]);
+                ws[lane][1] = ws[lane][1] + wgt;
+            }
+            if constexpr (doW) {
+                const float a = fmaxf(0.0f, 1.0f - fabsf(p0 - wpos[0]));
+                const float b = fmaxf(0.0f, 1.0f - fabsf(p1 - wpos[1]));
+                const float c = fmaxf(0.0f, 1.0f - fabsf(p2 - wpos[2]));
+                const float wgt = a * b * c;
+                vs[lane][2] = fmaf(wgt, vel[3LL * pi + 2], vs[lane][2]);
+                ws[lane][2] = ws[lane][2] + wgt;
+            }
+        }
+    }
+}
+
+template <int... NBs>
+__device__ __forceinline__ void gather_all(std::integer_sequence<int, NBs...>, const float *pos, const float *vel,
+                                           const int32_t *offsets, const int32_t *counts, int ci, int cj, int ck,
+                                           int nx, int ny, int nz, float inv_dx, const float *upos, const float *vpos,
+                                           const float *wpos, float (&vs)[8][3], float (&ws)[8][3], int (&cnt)[8]) {
+    // Lane l of the reference visits l, l+8, l+16, l+24; the partials are
+    // independent, so visiting bins 0..26 in order adds every term to its
+    // partial in the reference order.
+    (gather_bin<NBs>(pos, vel, offsets, counts, ci, cj, ck, nx, ny, nz, inv_dx, upos, vpos, wpos, vs, ws, cnt), ...);
+}
+
+__global__ __launch_bounds__(256) void p2g_kernel(const float *__restrict__ pos, const float *__restrict__ vel,
+                                                  const int32_t *__restrict__ offsets,
+                                                  const int32_t *__restrict__ counts, float *__restrict__ u,
+                                                  float *__restrict__ v, float *__restrict__ w,
+                                                  float *__restrict__ ff, float inv_dx, float inv_ppc, int nx,
+                                                  int ny, int nz) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= C) return;
+    int ci, cj, ck;
+    coords(index, ny, nz, ci, cj, ck);
+    float vs[8][3], ws[8][3];
+    int cnt[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        cnt[l] = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vs[l][a] = 0.0f;
+            ws[l][a] = 0.0f;
+        }
+    }
+    const float gx = (float)ci, gy = (float)cj, gz = (float)ck;
+    const float upos[3] = {gx + 1.0f, gy + 0.5f, gz + 0.5f};
+    const float vpos[3] = {gx + 0.5f, gy + 1.0f, gz + 0.5f};
+    const float wpos[3] = {gx + 0.5f, gy + 0.5f, gz + 1.0f};
+    gather_all(std::make_integer_sequence<int, 27>{}, pos, vel, offsets, counts, ci, cj, ck, nx, ny, nz, inv_dx,
+               upos, vpos, wpos, vs, ws, cnt);
+    // Tree of particles.cl:204-211: offsets 4, 2, 1.
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) {
+#pragma unroll
+        for (int l = 0; l < off; ++l) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                vs[l][a] = vs[l][a] + vs[l + off][a];
+                ws[l][a] = ws[l][a] + ws[l + off][a];
+            }
+            cnt[l] += cnt[l + off];
+        }
+    }
+    u[index] = ws[0][0] > 0.0f ? vs[0][0] / ws[0][0] : 0.0f;
+    v[index] = ws[0][1] > 0.0f ? vs[0][1] / ws[0][1] : 0.0f;
+    w[index] = ws[0][2] > 0.0f ? vs[0][2] / ws[0][2] : 0.0f;
+    ff[index] = (float)cnt[0] * inv_ppc;
+}
+
+void launch_p2g(const float *pos, const float *vel, const int32_t *offsets, const int32_t *counts, float *u, float *v,
+                float *w, float *ff, float inv_dx, float inv_ppc, int nx, int ny, int nz, hipStream_t s) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(p2g_kernel, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, s, pos, vel, offsets, counts, u,
+                       v, w, ff, inv_dx, inv_ppc, nx, ny, nz);
+}
+
+}  // namespace clq

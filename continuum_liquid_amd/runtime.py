@@ -1,0 +1,392 @@
+"""ctypes binding of the C-ABI in include/cliquid.h.
+
+`load_library()` loads the gfx950 HIP library built in-tree
+(continuum_liquid_amd/libcliquid_hip.so) and fails loudly when it is missing
+or when no GPU is visible: there is no CPU fallback.  `bind_library()` sets the
+argument types on any CDLL that exports the ABI; `Context` owns one
+simulation grid on the device and raises `ClqError` for every non-zero status.
+
+The stage methods map one-to-one onto the reference's stage wrappers
+(Solver/Kernel/particles.py, multigrid.py, pressure_solve.py,
+velocity_update.py, boundary_conditions.py); see the C header for citations.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from ctypes import POINTER, byref, c_char_p, c_double, c_int32, c_int64, c_uint8, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+PACKAGE_DIR = Path(__file__).resolve().parent
+HIP_LIBRARY = PACKAGE_DIR / "libcliquid_hip.so"
+ABI_VERSION = 1
+
+# enum clq_status
+OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NOT_CONVERGED = range(6)
+
+# enum clq_field
+FIELDS = {
+    "u": 0, "v": 1, "w": 2, "u_old": 3, "v_old": 4, "w_old": 5, "fluid_fraction": 6, "pressure": 7, "rhs": 8,
+    "diagonal": 9, "weight_x": 10, "weight_y": 11, "weight_z": 12, "bin_counts": 13, "bin_offsets": 14,
+    "obstacle": 15,
+}
+FIELD_DTYPES = {"bin_counts": np.int32, "bin_offsets": np.int32, "obstacle": np.uint8}
+LEVEL_FIELDS = {"p": 0, "b": 1, "mask": 2, "diagonal": 3, "weight_x": 4, "weight_y": 5, "weight_z": 6}
+
+# enum clq_solve_mode
+SOLVE_FIXED_CYCLES, SOLVE_FP32_TOLERANCE, SOLVE_MIXED_REFINEMENT, SOLVE_LEARNED_FCG = range(4)
+SOLVE_MODES = {
+    "fixed": SOLVE_FIXED_CYCLES,
+    "fp32": SOLVE_FP32_TOLERANCE,
+    "mixed": SOLVE_MIXED_REFINEMENT,
+    "learned": SOLVE_LEARNED_FCG,
+}
+STAGES = ("bin", "p2g", "pressure_solve", "g2p_advect")
+
+
+class ClqError(RuntimeError):
+    """A non-zero status from the C-ABI, with clq_last_error()'s message."""
+
+    def __init__(self, status: int, where: str, message: str):
+        super().__init__(f"{where} failed with status {status}: {message}")
+        self.status = status
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [
+        ("mode", c_int32),
+        ("pre_smooth", c_int32),
+        ("post_smooth", c_int32),
+        ("coarse_extra", c_int32),
+        ("max_cycles", c_int32),
+        ("coarse_dtype", c_int32),
+        ("tolerance", c_double),
+    ]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [
+        ("cycles", c_int32),
+        ("fallback_cycles", c_int32),
+        ("rhs_norm", c_double),
+        ("initial_residual", c_double),
+        ("final_residual", c_double),
+        ("relative_residual", c_double),
+        ("solve_ms", c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_P = c_void_p
+_D3 = POINTER(c_double)
+_SIGNATURES = {
+    "clq_abi_version": ([], ctypes.c_int),
+    "clq_device_count": ([POINTER(c_int32)], ctypes.c_int),
+    "clq_create": ([c_int32, c_int32, c_int32, c_double, c_int64, POINTER(_P)], ctypes.c_int),
+    "clq_destroy": ([_P], ctypes.c_int),
+    "clq_sync": ([_P], ctypes.c_int),
+    "clq_set_particles": ([_P, _P, _P, c_int64], ctypes.c_int),
+    "clq_get_particles": ([_P, _P, _P, c_int64], ctypes.c_int),
+    "clq_get_previous_positions": ([_P, _P, c_int64], ctypes.c_int),
+    "clq_get_particle_count": ([_P, POINTER(c_int64)], ctypes.c_int),
+    "clq_set_field": ([_P, c_int32, _P, c_int64], ctypes.c_int),
+    "clq_get_field": ([_P, c_int32, _P, c_int64], ctypes.c_int),
+    "clq_level_count": ([_P, POINTER(c_int32)], ctypes.c_int),
+    "clq_level_shape": ([_P, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32)], ctypes.c_int),
+    "clq_set_level_field": ([_P, c_int32, c_int32, _P, c_int64], ctypes.c_int),
+    "clq_get_level_field": ([_P, c_int32, c_int32, _P, c_int64], ctypes.c_int),
+    "clq_bin_particles": ([_P], ctypes.c_int),
+    "clq_particle_to_grid": ([_P, c_double], ctypes.c_int),
+    "clq_grid_obstacle_bc": ([_P], ctypes.c_int),
+    "clq_grid_domain_bc": ([_P], ctypes.c_int),
+    "clq_update_velocity": ([_P, _D3, _D3, c_double, c_double, c_double], ctypes.c_int),
+    "clq_pressure_rhs": ([_P, c_double, c_double], ctypes.c_int),
+    "clq_build_operator": ([_P], ctypes.c_int),
+    "clq_pressure_solve": ([_P, POINTER(SolveParams), POINTER(SolveStats)], ctypes.c_int),
+    "clq_w_cycle": ([_P, c_int32, c_int32, c_int32], ctypes.c_int),
+    "clq_smooth": ([_P, c_int32, c_int32], ctypes.c_int),
+    "clq_restrict": ([_P, c_int32], ctypes.c_int),
+    "clq_prolongate": ([_P, c_int32], ctypes.c_int),
+    "clq_coarse_smooth": ([_P, c_int32], ctypes.c_int),
+    "clq_neumann_bc": ([_P], ctypes.c_int),
+    "clq_residual_norm": ([_P, POINTER(c_double), POINTER(c_double)], ctypes.c_int),
+    "clq_pressure_correction": ([_P, c_double, c_double], ctypes.c_int),
+    "clq_grid_to_particle": ([_P, c_double], ctypes.c_int),
+    "clq_save_previous_positions": ([_P], ctypes.c_int),
+    "clq_advect_particles": ([_P, c_double], ctypes.c_int),
+    "clq_particle_domain_bc": ([_P], ctypes.c_int),
+    "clq_particle_obstacle_bc": ([_P], ctypes.c_int),
+    "clq_grid_to_particle_advect": ([_P, c_double, c_double], ctypes.c_int),
+    "clq_voxelise": ([_P, c_int32, c_double, _P, _P], ctypes.c_int),
+    "clq_stage_times": ([_P, POINTER(c_double), c_int32], ctypes.c_int),
+}
+EXPORTED_SYMBOLS = ("clq_last_error", *_SIGNATURES)
+
+
+def bind_library(lib: ctypes.CDLL) -> ctypes.CDLL:
+    """Set restype/argtypes for every ABI entry point of `lib`."""
+    lib.clq_last_error.restype = c_char_p
+    lib.clq_last_error.argtypes = []
+    for name, (args, res) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    version = lib.clq_abi_version()
+    if version != ABI_VERSION:
+        raise RuntimeError(f"library ABI version {version} != expected {ABI_VERSION}")
+    return lib
+
+
+_lock = threading.Lock()
+_hip_library: ctypes.CDLL | None = None
+
+
+def load_library(path: str | Path | None = None) -> ctypes.CDLL:
+    """Load the HIP library (once per process).  Raises when it is absent."""
+    global _hip_library
+    with _lock:
+        if _hip_library is not None and path is None:
+            return _hip_library
+        lib_path = Path(path) if path is not None else HIP_LIBRARY
+        if not lib_path.exists():
+            raise RuntimeError(
+                f"{lib_path} is missing: build it with `python -m continuum_liquid_amd.build` "
+                "(the Continuum Liquid HIP step has no CPU fallback)"
+            )
+        lib = bind_library(ctypes.CDLL(str(lib_path)))
+        if path is None:
+            _hip_library = lib
+        return lib
+
+
+def _ptr(array: np.ndarray) -> c_void_p:
+    return c_void_p(array.ctypes.data)
+
+
+class Context:
+    """One simulation grid (and its particles) resident on the device."""
+
+    def __init__(self, lib: ctypes.CDLL, shape, dx: float, capacity: int = 1):
+        self.lib = lib
+        self.shape = tuple(int(n) for n in shape)
+        self.dx = float(dx)
+        handle = c_void_p()
+        self._check(lib.clq_create(*self.shape, self.dx, int(capacity), byref(handle)), "clq_create")
+        self.handle = handle
+        levels = c_int32()
+        self._check(lib.clq_level_count(self.handle, byref(levels)), "clq_level_count")
+        self.level_shapes = []
+        for level in range(levels.value):
+            a, b, d = c_int32(), c_int32(), c_int32()
+            self._check(lib.clq_level_shape(self.handle, level, byref(a), byref(b), byref(d)), "clq_level_shape")
+            self.level_shapes.append((a.value, b.value, d.value))
+
+    # -- plumbing ---------------------------------------------------------------------------------
+    def _check(self, status: int, where: str) -> None:
+        if status != OK:
+            message = self.lib.clq_last_error()
+            raise ClqError(status, where, message.decode() if message else "")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.clq_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self) -> "Context":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self) -> None:  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def cells(self) -> int:
+        return self.shape[0] * self.shape[1] * self.shape[2]
+
+    def sync(self) -> None:
+        self._check(self.lib.clq_sync(self.handle), "clq_sync")
+
+    # -- data movement ----------------------------------------------------------------------------
+    def set_particles(self, positions: np.ndarray, velocities: np.ndarray | None = None) -> None:
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        vel = None if velocities is None else np.ascontiguousarray(velocities, dtype=np.float32).reshape(-1, 3)
+        if vel is not None and vel.shape != pos.shape:
+            raise ValueError("positions and velocities must have the same shape")
+        self._check(
+            self.lib.clq_set_particles(self.handle, _ptr(pos), None if vel is None else _ptr(vel), len(pos)),
+            "clq_set_particles",
+        )
+
+    @property
+    def particle_count(self) -> int:
+        n = c_int64()
+        self._check(self.lib.clq_get_particle_count(self.handle, byref(n)), "clq_get_particle_count")
+        return n.value
+
+    def get_particles(self, out_positions: np.ndarray | None = None, out_velocities: np.ndarray | None = None):
+        n = self.particle_count
+        pos = out_positions if out_positions is not None else np.empty((n, 3), np.float32)
+        vel = out_velocities if out_velocities is not None else np.empty((n, 3), np.float32)
+        self._check(self.lib.clq_get_particles(self.handle, _ptr(pos), _ptr(vel), n), "clq_get_particles")
+        return pos, vel
+
+    def get_positions(self, out: np.ndarray | None = None) -> np.ndarray:
+        n = self.particle_count
+        pos = out if out is not None else np.empty((n, 3), np.float32)
+        self._check(self.lib.clq_get_particles(self.handle, _ptr(pos), None, n), "clq_get_particles")
+        return pos
+
+    def get_previous_positions(self) -> np.ndarray:
+        n = self.particle_count
+        pos = np.empty((n, 3), np.float32)
+        self._check(self.lib.clq_get_previous_positions(self.handle, _ptr(pos), n), "clq_get_previous_positions")
+        return pos
+
+    def set_field(self, name: str, values: np.ndarray) -> None:
+        arr = np.ascontiguousarray(values, dtype=FIELD_DTYPES.get(name, np.float32))
+        if arr.size != self.cells:
+            raise ValueError(f"field {name} needs {self.cells} values, got {arr.size}")
+        self._check(self.lib.clq_set_field(self.handle, FIELDS[name], _ptr(arr), arr.nbytes), "clq_set_field")
+
+    def get_field(self, name: str) -> np.ndarray:
+        arr = np.empty(self.shape, FIELD_DTYPES.get(name, np.float32))
+        self._check(self.lib.clq_get_field(self.handle, FIELDS[name], _ptr(arr), arr.nbytes), "clq_get_field")
+        return arr
+
+    def set_level_field(self, level: int, name: str, values: np.ndarray) -> None:
+        arr = np.ascontiguousarray(values, dtype=np.float32)
+        self._check(
+            self.lib.clq_set_level_field(self.handle, level, LEVEL_FIELDS[name], _ptr(arr), arr.nbytes),
+            "clq_set_level_field",
+        )
+
+    def get_level_field(self, level: int, name: str) -> np.ndarray:
+        arr = np.empty(self.level_shapes[level], np.float32)
+        self._check(
+            self.lib.clq_get_level_field(self.handle, level, LEVEL_FIELDS[name], _ptr(arr), arr.nbytes),
+            "clq_get_level_field",
+        )
+        return arr
+
+    # -- stages -------------------------------------------------------------------------------------
+    def bin_particles(self) -> None:
+        self._check(self.lib.clq_bin_particles(self.handle), "clq_bin_particles")
+
+    def particle_to_grid(self, particles_per_cell: float = 8.0) -> None:
+        self._check(self.lib.clq_particle_to_grid(self.handle, float(particles_per_cell)), "clq_particle_to_grid")
+
+    def grid_obstacle_bc(self) -> None:
+        self._check(self.lib.clq_grid_obstacle_bc(self.handle), "clq_grid_obstacle_bc")
+
+    def grid_domain_bc(self) -> None:
+        self._check(self.lib.clq_grid_domain_bc(self.handle), "clq_grid_domain_bc")
+
+    def update_velocity(self, gravity, force, rho: float, nu: float, dt: float) -> None:
+        g = (c_double * 3)(*[float(x) for x in gravity])
+        f = (c_double * 3)(*[float(x) for x in force])
+        self._check(
+            self.lib.clq_update_velocity(self.handle, g, f, float(rho), float(nu), float(dt)), "clq_update_velocity"
+        )
+
+    def pressure_rhs(self, rho: float, dt: float) -> None:
+        self._check(self.lib.clq_pressure_rhs(self.handle, float(rho), float(dt)), "clq_pressure_rhs")
+
+    def build_operator(self) -> None:
+        self._check(self.lib.clq_build_operator(self.handle), "clq_build_operator")
+
+    def pressure_solve(
+        self,
+        mode: int = SOLVE_FIXED_CYCLES,
+        *,
+        pre_smooth: int = 2,
+        post_smooth: int = 4,
+        coarse_extra: int = 64,
+        max_cycles: int = 1,
+        tolerance: float = 1e-6,
+        coarse_dtype: int = 0,
+    ) -> dict:
+        params = SolveParams(int(mode), pre_smooth, post_smooth, coarse_extra, max_cycles, coarse_dtype, tolerance)
+        stats = SolveStats()
+        self._check(self.lib.clq_pressure_solve(self.handle, byref(params), byref(stats)), "clq_pressure_solve")
+        return stats.as_dict()
+
+    def w_cycle(self, pre_smooth: int = 2, post_smooth: int = 4, coarse_extra: int = 64) -> None:
+        self._check(self.lib.clq_w_cycle(self.handle, pre_smooth, post_smooth, coarse_extra), "clq_w_cycle")
+
+    def smooth(self, level: int, iterations: int) -> None:
+        self._check(self.lib.clq_smooth(self.handle, level, iterations), "clq_smooth")
+
+    def restrict(self, level: int) -> None:
+        self._check(self.lib.clq_restrict(self.handle, level), "clq_restrict")
+
+    def prolongate(self, level: int) -> None:
+        self._check(self.lib.clq_prolongate(self.handle, level), "clq_prolongate")
+
+    def coarse_smooth(self, iterations: int) -> None:
+        self._check(self.lib.clq_coarse_smooth(self.handle, iterations), "clq_coarse_smooth")
+
+    def neumann_bc(self) -> None:
+        self._check(self.lib.clq_neumann_bc(self.handle), "clq_neumann_bc")
+
+    def residual_norm(self) -> tuple[float, float]:
+        r, b = c_double(), c_double()
+        self._check(self.lib.clq_residual_norm(self.handle, byref(r), byref(b)), "clq_residual_norm")
+        return r.value, b.value
+
+    def pressure_correction(self, rho: float, dt: float) -> None:
+        self._check(self.lib.clq_pressure_correction(self.handle, float(rho), float(dt)), "clq_pressure_correction")
+
+    def grid_to_particle(self, flip_ratio: float = 1.0) -> None:
+        self._check(self.lib.clq_grid_to_particle(self.handle, float(flip_ratio)), "clq_grid_to_particle")
+
+    def save_previous_positions(self) -> None:
+        self._check(self.lib.clq_save_previous_positions(self.handle), "clq_save_previous_positions")
+
+    def advect_particles(self, dt: float) -> None:
+        self._check(self.lib.clq_advect_particles(self.handle, float(dt)), "clq_advect_particles")
+
+    def particle_domain_bc(self) -> None:
+        self._check(self.lib.clq_particle_domain_bc(self.handle), "clq_particle_domain_bc")
+
+    def particle_obstacle_bc(self) -> None:
+        self._check(self.lib.clq_particle_obstacle_bc(self.handle), "clq_particle_obstacle_bc")
+
+    def grid_to_particle_advect(self, flip_ratio: float, dt: float) -> None:
+        self._check(
+            self.lib.clq_grid_to_particle_advect(self.handle, float(flip_ratio), float(dt)),
+            "clq_grid_to_particle_advect",
+        )
+
+    def voxelise(self, scale: int = 1, particles_per_cell: float = 8.0, density: bool = False):
+        dims = tuple(scale * n for n in self.shape)
+        occ = np.empty(dims, np.uint8)
+        dens = np.empty(dims, np.float32) if density else None
+        self._check(
+            self.lib.clq_voxelise(self.handle, int(scale), float(particles_per_cell), _ptr(occ),
+                                  None if dens is None else _ptr(dens)),
+            "clq_voxelise",
+        )
+        return occ, dens
+
+    def stage_times(self) -> dict:
+        ms = (c_double * len(STAGES))()
+        n = self.lib.clq_stage_times(self.handle, ms, len(STAGES))
+        if n < 0:
+            self._check(n, "clq_stage_times")
+        return {STAGES[i]: ms[i] for i in range(n)}
+
+
+def device_count(lib: ctypes.CDLL) -> int:
+    count = c_int32()
+    status = lib.clq_device_count(byref(count))
+    return count.value if status == OK else 0

@@ -1,0 +1,244 @@
+/*
+ * cliquid.h -- C-ABI of the Continuum Liquid FLIP/PIC time step.
+ *
+ * One shared library exports this ABI: libcliquid_hip.so (hand-written gfx950
+ * HIP kernels, the product).  The CPU oracle used by the tests
+ * (oracle/libcliquid_oracle.so) exports the same symbols so that both can be
+ * driven by identical call sequences and their downloads compared.
+ *
+ * What this replaces.  The reference drives 23 OpenCL kernels from Python
+ * stage wrappers of the form (kernels: dict, queue, cl.Buffer..., numpy
+ * scalars) -> buffers (Solver/Kernel/particles.py:73-226,
+ * Solver/Kernel/multigrid.py:53-249, Solver/Kernel/pressure_solve.py:10-114,
+ * Solver/Kernel/velocity_update.py:8-52,
+ * Solver/Kernel/boundary_conditions.py:8-132).  solver.py allocates every
+ * buffer (Solver/Kernel/helper.py:81-83, solver.py:161-217) and stages never
+ * allocate.  Here the library owns an opaque context that holds every device
+ * buffer of one simulation grid plus one HIP stream; each stage wrapper becomes
+ * one entry point below that enqueues on that stream.
+ *
+ * Conventions (all entry points):
+ *   - return 0 (CLQ_OK) on success, a CLQ_ERR_* code otherwise; the message of
+ *     the last failure on the calling thread is clq_last_error().  No C++
+ *     exception crosses the ABI.
+ *   - grid arrays are C-ordered (nx, ny, nz), k fastest, one ghost layer per
+ *     face (Solver/Kernel/OpenCL/helper.cl:4-6, :21-23).
+ *   - particle arrays are AoS float32 [n][3] (Solver/Kernel/solver.py:164-173).
+ *   - physical scalars are passed as double and rounded to float32 exactly as
+ *     the reference host does before each launch (e.g. FIELD_DTYPE(1.0 / dx),
+ *     Solver/Kernel/particles.py:103).
+ *   - stage calls are asynchronous on the context stream; clq_get_* downloads
+ *     and clq_sync() wait.  Calls into one context must be serialised by the
+ *     caller; distinct contexts are independent.
+ */
+#ifndef CLIQUID_H
+#define CLIQUID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLQ_ABI_VERSION 1
+
+enum clq_status {
+    CLQ_OK = 0,
+    CLQ_ERR_INVALID = 1,     /* bad argument, size or state              */
+    CLQ_ERR_DEVICE = 2,      /* HIP runtime / kernel launch failure      */
+    CLQ_ERR_NOMEM = 3,       /* allocation failed                        */
+    CLQ_ERR_UNSUPPORTED = 4, /* mode not implemented by this library     */
+    CLQ_ERR_NOT_CONVERGED = 5
+};
+
+/* Grid field identifiers for clq_set_field / clq_get_field. */
+enum clq_field {
+    CLQ_FIELD_U = 0,              /* float32[C]  MAC +x face velocity            */
+    CLQ_FIELD_V = 1,              /* float32[C]  MAC +y face velocity            */
+    CLQ_FIELD_W = 2,              /* float32[C]  MAC +z face velocity            */
+    CLQ_FIELD_U_OLD = 3,          /* float32[C]  work buffers: after
+                                     clq_update_velocity they hold the
+                                     post-BC P2G velocity (solver.py:374-376) */
+    CLQ_FIELD_V_OLD = 4,
+    CLQ_FIELD_W_OLD = 5,
+    CLQ_FIELD_FLUID_FRACTION = 6, /* float32[C]  occupancy, count/8            */
+    CLQ_FIELD_PRESSURE = 7,       /* float32[C]                                */
+    CLQ_FIELD_RHS = 8,            /* float32[C]  rho/dt * div u                */
+    CLQ_FIELD_DIAGONAL = 9,       /* float32[C]  finest operator               */
+    CLQ_FIELD_WEIGHT_X = 10,
+    CLQ_FIELD_WEIGHT_Y = 11,
+    CLQ_FIELD_WEIGHT_Z = 12,
+    CLQ_FIELD_BIN_COUNTS = 13,    /* int32[C]    particles per bin              */
+    CLQ_FIELD_BIN_OFFSETS = 14,   /* int32[C]    exclusive scan of counts       */
+    CLQ_FIELD_OBSTACLE = 15,      /* uint8[C]    static solid voxels           */
+    CLQ_FIELD_COUNT = 16
+};
+
+/* Per-level multigrid arrays for clq_get_level_field / clq_set_level_field.
+ * Level 0 is the simulation grid (p = pressure, b = rhs, mask = fluid
+ * fraction), levels 1..L-1 are the coarse grids of
+ * Solver/Kernel/multigrid.py:18-50. */
+enum clq_level_field {
+    CLQ_LEVEL_P = 0,
+    CLQ_LEVEL_B = 1,
+    CLQ_LEVEL_MASK = 2,
+    CLQ_LEVEL_DIAGONAL = 3,
+    CLQ_LEVEL_WEIGHT_X = 4,
+    CLQ_LEVEL_WEIGHT_Y = 5,
+    CLQ_LEVEL_WEIGHT_Z = 6
+};
+
+/* Pressure-solve modes. */
+enum clq_solve_mode {
+    /* Exactly `max_cycles` FP32 W-cycles, no residual test: the reference
+     * driver (Solver/Kernel/pressure_solve.py:61-84, iterations from
+     * settings.iterations, solver.py:125). */
+    CLQ_SOLVE_FIXED_CYCLES = 0,
+    /* FP32 W-cycles until the FP64-accumulated relative residual
+     * ||b - A p||_2 / ||b||_2 <= tolerance (or max_cycles). */
+    CLQ_SOLVE_FP32_TOLERANCE = 1,
+    /* FP64 iterative refinement around a W-cycle with FP32 smoothing and
+     * 16-bit coarse-level residual storage; FP64 residual, FP64 correction
+     * accumulation and FP64 coarsest solve. */
+    CLQ_SOLVE_MIXED_REFINEMENT = 2,
+    /* Flexible CG preconditioned by the learned network with the W-cycle as
+     * fallback; requires clq_set_preconditioner_weights(). */
+    CLQ_SOLVE_LEARNED_FCG = 3
+};
+
+typedef struct clq_solve_params {
+    int32_t mode;          /* enum clq_solve_mode                               */
+    int32_t pre_smooth;    /* reference: 2 (pressure_solve.py:77)               */
+    int32_t post_smooth;   /* reference: 4 (pressure_solve.py:78)               */
+    int32_t coarse_extra;  /* reference: 64 -> coarsest runs pre+64 sweeps      */
+    int32_t max_cycles;    /* fixed count (mode 0) or cap                        */
+    int32_t coarse_dtype;  /* mode 2: 0 = fp16, 1 = bf16 coarse b/r storage     */
+    double tolerance;      /* relative residual target (modes 1-3)              */
+} clq_solve_params;
+
+typedef struct clq_solve_stats {
+    int32_t cycles;            /* W-cycles (or FCG iterations) performed        */
+    int32_t fallback_cycles;   /* mode 3: W-cycle fallbacks taken               */
+    double rhs_norm;           /* ||b||_2 (FP64 accumulation)                    */
+    double initial_residual;   /* ||b - A p0||_2 before the solve               */
+    double final_residual;     /* ||b - A p||_2 after the solve                 */
+    double relative_residual;  /* final_residual / rhs_norm                     */
+    double solve_ms;           /* device time of the solve (HIP events)         */
+} clq_solve_stats;
+
+typedef struct clq_ctx clq_ctx;
+
+/* ---- library / context ---------------------------------------------------- */
+const char *clq_last_error(void);
+int clq_abi_version(void);
+/* Number of usable devices (0 for the CPU oracle is never returned: it reports 1). */
+int clq_device_count(int32_t *count);
+/* Creates a context for an (nx, ny, nz) grid of spacing dx holding up to
+ * `capacity` particles.  Allocates every buffer of solver.py:161-217,
+ * including the multigrid hierarchy with min_size 8 (solver.py:203-213). */
+int clq_create(int32_t nx, int32_t ny, int32_t nz, double dx, int64_t capacity, clq_ctx **out);
+int clq_destroy(clq_ctx *ctx);
+int clq_sync(clq_ctx *ctx);
+
+/* ---- data movement -------------------------------------------------------- */
+/* Uploads n particles (AoS float32[n][3] each); grows capacity to
+ * max(n, 3*cap/2) when needed (solver.py:290-303).  vel may be NULL (zeros). */
+int clq_set_particles(clq_ctx *ctx, const float *pos, const float *vel, int64_t n);
+/* Downloads the current n particles; either pointer may be NULL. */
+int clq_get_particles(clq_ctx *ctx, float *pos, float *vel, int64_t n);
+int clq_get_previous_positions(clq_ctx *ctx, float *pos, int64_t n);
+int clq_get_particle_count(clq_ctx *ctx, int64_t *n);
+int clq_set_field(clq_ctx *ctx, int32_t field, const void *host, int64_t bytes);
+int clq_get_field(clq_ctx *ctx, int32_t field, void *host, int64_t bytes);
+int clq_level_count(clq_ctx *ctx, int32_t *levels);
+int clq_level_shape(clq_ctx *ctx, int32_t level, int32_t *nx, int32_t *ny, int32_t *nz);
+int clq_set_level_field(clq_ctx *ctx, int32_t level, int32_t field, const float *host, int64_t bytes);
+int clq_get_level_field(clq_ctx *ctx, int32_t level, int32_t field, float *host, int64_t bytes);
+
+/* ---- particle_to_grid section (Solver/Kernel/particles.py:73-171) ---------- */
+/* Histogram, exclusive scan and counting-sort scatter of positions and
+ * velocities into bins (kernels count_particles_per_cell, scan_blocks_exclusive,
+ * add_scanned_block_offsets, sort_particles: particles.cl:44-133), then the
+ * double-buffer swap (particles.py:136-140).  The order within a bin is the
+ * stable (original-index) order. */
+int clq_bin_particles(clq_ctx *ctx);
+/* P2G gather onto the MAC grid plus occupancy voxelisation
+ * (particle_to_grid_gather, particles.cl:135-227).  Requires binned particles. */
+int clq_particle_to_grid(clq_ctx *ctx, double particles_per_cell);
+
+/* ---- grid glue ------------------------------------------------------------- */
+/* grid_obstacle_bc + extrapolate_obstacle_velocity
+ * (boundary_conditions.cl:99-173, boundary_conditions.py:53-79). */
+int clq_grid_obstacle_bc(clq_ctx *ctx);
+/* grid_domain_bc (boundary_conditions.cl:33-67). */
+int clq_grid_domain_bc(clq_ctx *ctx);
+/* update_velocity into the work buffers followed by the swap of
+ * solver.py:374-376 (velocity_update.cl:3-43). gravity/force are float[3]. */
+int clq_update_velocity(clq_ctx *ctx, const double *gravity, const double *force, double rho, double nu,
+                        double dt);
+
+/* ---- pressure_solve section (Solver/Kernel/pressure_solve.py:10-85) -------- */
+/* pressure_rhs (pressure_solve.cl:3-24). */
+int clq_pressure_rhs(clq_ctx *ctx, double rho, double dt);
+/* build_finest_operator (multigrid.cl:63-88). */
+int clq_build_operator(clq_ctx *ctx);
+/* The solve proper followed by neumann_domain_bc (pressure_solve.py:61-84). */
+int clq_pressure_solve(clq_ctx *ctx, const clq_solve_params *params, clq_solve_stats *stats);
+/* One W-cycle on the current level-0 p/b/mask/operator (multigrid.py:53-187). */
+int clq_w_cycle(clq_ctx *ctx, int32_t pre_smooth, int32_t post_smooth, int32_t coarse_extra);
+/* Red-black Gauss-Seidel sweeps on one level (multigrid.py:190-220). */
+int clq_smooth(clq_ctx *ctx, int32_t level, int32_t iterations);
+/* restrict_residual from `level` to level+1 (multigrid.cl:90-180). */
+int clq_restrict(clq_ctx *ctx, int32_t level);
+/* prolongate from level+1 into `level` (multigrid.cl:182-201). */
+int clq_prolongate(clq_ctx *ctx, int32_t level);
+/* coarse_smooth on the coarsest level (multigrid.cl:232-264). */
+int clq_coarse_smooth(clq_ctx *ctx, int32_t iterations);
+/* neumann_domain_bc on the pressure (boundary_conditions.cl:18-31). */
+int clq_neumann_bc(clq_ctx *ctx);
+/* FP64-accumulated ||b - A p||_2 and ||b||_2 on level 0. */
+int clq_residual_norm(clq_ctx *ctx, double *residual, double *rhs_norm);
+/* pressure_correction (pressure_solve.cl:26-47). */
+int clq_pressure_correction(clq_ctx *ctx, double rho, double dt);
+
+/* ---- grid_to_particle / advection section --------------------------------- */
+/* G2P with FLIP/PIC blend (grid_to_particle, particles.cl:229-262).
+ * flip_ratio 1.0 is the reference's pure FLIP update v += u(x) - u_old(x);
+ * otherwise v <- a*(v + du) + (1-a)*u(x). */
+int clq_grid_to_particle(clq_ctx *ctx, double flip_ratio);
+/* Copy of positions into the previous-position buffer (solver.py:469-474). */
+int clq_save_previous_positions(clq_ctx *ctx);
+/* advect_particles (particles.cl:264-276). */
+int clq_advect_particles(clq_ctx *ctx, double dt);
+/* particle_domain_bc (boundary_conditions.cl:69-97, bounds from
+ * boundary_conditions.py:98-101). */
+int clq_particle_domain_bc(clq_ctx *ctx);
+/* particle_obstacle_bc (boundary_conditions.cl:175-264). */
+int clq_particle_obstacle_bc(clq_ctx *ctx);
+/* The five calls above in order, as one pass over the particles. */
+int clq_grid_to_particle_advect(clq_ctx *ctx, double flip_ratio, double dt);
+
+/* ---- particle voxelisation ------------------------------------------------ */
+/* Occupancy at `scale` x grid resolution (uint8, 1 where a particle's cell is
+ * occupied) and, when density != NULL, smoothed density (float32, trilinear
+ * splat of unit mass per particle at the scaled resolution, divided by the
+ * particles-per-cell target).  Output arrays are (scale*nx, scale*ny, scale*nz). */
+int clq_voxelise(clq_ctx *ctx, int32_t scale, double particles_per_cell, uint8_t *occupancy, float *density);
+
+/* ---- timing --------------------------------------------------------------- */
+/* Device time of the most recent call of each stage, in ms, indexed by
+ * enum clq_stage; returns the number of stages written. */
+enum clq_stage {
+    CLQ_STAGE_BIN = 0,
+    CLQ_STAGE_P2G = 1,
+    CLQ_STAGE_PRESSURE_SOLVE = 2,
+    CLQ_STAGE_G2P_ADVECT = 3,
+    CLQ_STAGE_COUNT = 4
+};
+int clq_stage_times(clq_ctx *ctx, double *ms, int32_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLIQUID_H */
