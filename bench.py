@@ -1,0 +1,204 @@
+"""Benchmark of the FLIP/PIC time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 3] [--solver mixed]
+
+A step is one full time step of the path on the resident particle set:
+binning, P2G + occupancy, grid glue, the pressure solve to the relative
+residual target (FP64-checked), pressure correction, and the fused G2P/FLIP
+blend/advection/particle boundary pass.  The default workload is the 256^3
+dam break + falling sphere (dx = 1/256, 8 stratified-jitter FP32 particles
+per fluid cell, ~28.6M particles, dt = 1/240, FLIP 0.95, W(2, 2) cycles).
+
+Prints one JSON line: ms/step (`value`), particle-steps/s, pressure-solve ms,
+a `roofline` object for the dominant kernel (algorithmic bytes per launch
+divided by the launch duration measured with HIP events on the library's
+stream during the timed steps) and a `cpu_baseline` object (the CPU oracle,
+a C/OpenMP restatement of the reference's OpenCL kernels, timed on this
+host's cores on a bounded sample of the same workload).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from continuum_liquid_amd import runtime, scenes  # noqa: E402
+from continuum_liquid_amd.Solver.Kernel import solver as host_solver  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "ms/step @256^3 (~30M particles); pressure-solve ms to 1e-6 rel residual"
+
+
+def algorithmic_bytes(n_particles: int, cells: int) -> dict:
+    """Compulsory HBM bytes per launch unit in the reference FP32 layout (each
+    array counted once): binning 60N + 16C, P2G + occupancy 24N + 24C, one
+    red-black sweep of the finest level 28C, fused G2P + advection 48N + 24C."""
+    N, C = n_particles, cells
+    return {
+        "binning": 60 * N + 16 * C,
+        "p2g": 24 * N + 24 * C,
+        "rbgs_fine_sweep": 28 * C,
+        "particle_update": 48 * N + 24 * C,
+    }
+
+
+def solver_settings(name: str, tolerance: float) -> dict:
+    base = {"pre_smooth": 2, "post_smooth": 2, "coarse_extra": 64, "tolerance": tolerance}
+    if name == "fp32":
+        return {**base, "mode": "fp32", "max_cycles": 60}
+    if name == "mixed":
+        return {**base, "mode": "mixed", "max_cycles": 60, "coarse_dtype": 0}
+    if name == "fixed":
+        return {**base, "mode": "fixed", "max_cycles": 1, "post_smooth": 4}
+    raise SystemExit(f"unknown solver {name}")
+
+
+def cpu_baseline(scene, params, steps: int) -> dict:
+    """Time the CPU oracle on the same workload: one untimed step, then `steps`."""
+    lib = runtime.bind_library(ctypes.CDLL(str(ROOT / "oracle" / "libcliquid_oracle.so")))
+    cores = len(os.sched_getaffinity(0))
+    with runtime.Context(lib, scene.shape, scene.dx, scene.particle_count) as ctx:
+        ctx.set_particles(scene.positions, scene.velocities)
+        host_solver.step(ctx, params)
+        t0 = time.perf_counter()
+        cycles = []
+        for _ in range(steps):
+            cycles.append(host_solver.step(ctx, params)["cycles"])
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+    return {
+        "value": round(ms, 3),
+        "unit": "ms/step",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{steps} step(s) of the same {scene.name} workload after 1 untimed step, "
+                  f"W-cycles per step {cycles}; OpenMP over {cores} threads on {platform.processor() or 'host'}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=3, choices=(1, 2, 3, 5))
+    ap.add_argument("--solver", default="fp32", choices=("fp32", "mixed", "fixed"))
+    ap.add_argument("--tolerance", type=float, default=1e-6)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): "
+              "the step has no multi-GPU path in this build", file=sys.stderr)
+        return 2
+
+    lib = runtime.load_library()
+    if runtime.device_count(lib) < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+
+    scene = scenes.baseline_scene(args.config)
+    params = host_solver.StepParams(dt=1.0 / 240.0, gravity=scene.gravity, flip_ratio=0.95,
+                                    solver=solver_settings(args.solver, args.tolerance))
+    print(f"# {scene.name}: {scene.particle_count} particles, grid {scene.shape}, solver {args.solver}",
+          file=sys.stderr, flush=True)
+
+    with runtime.Context(lib, scene.shape, scene.dx, scene.particle_count) as ctx:
+        ctx.set_particles(scene.positions, scene.velocities)
+        for _ in range(args.warmup):
+            host_solver.step(ctx, params)
+        ctx.sync()
+        ctx.kernel_timing(True)
+        solve_ms, relres, cycles = [], [], []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            stats = host_solver.step(ctx, params)
+            solve_ms.append(stats["solve_ms"])
+            relres.append(stats["relative_residual"])
+            cycles.append(stats["cycles"])
+        ctx.sync()
+        elapsed = time.perf_counter() - t0
+        ktimes = ctx.kernel_times()
+        ctx.kernel_timing(False)
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    n, cells = scene.particle_count, int(np.prod(scene.shape))
+    nbytes = algorithmic_bytes(n, cells)
+    kernels = {}
+    for name, (total_ms, units) in ktimes.items():
+        if units == 0:
+            continue
+        avg_ms = total_ms / units
+        gbs = nbytes[name] / (avg_ms * 1e-3) / 1e9
+        kernels[name] = {
+            "avg_ms": round(avg_ms, 5),
+            "launch_units": units,
+            "share_of_step": round(total_ms / (elapsed * 1e3), 4),
+            "algorithmic_bytes": nbytes[name],
+            "achieved_GBs": round(gbs, 1),
+            "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+        }
+    dominant = max(kernels, key=lambda k: ktimes[k][0])
+    d = kernels[dominant]
+
+    result = {
+        "metric": METRIC,
+        "value": round(ms_per_step, 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.solver != "mixed" else "fp32+fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{scene.name}, dx=1/{round(1 / scene.dx)}, {n} FP32 particles (8 per fluid cell, "
+                        f"stratified jitter, seed 0), dt=1/240, FLIP 0.95, solver {args.solver} W(2,2) to "
+                        f"{args.tolerance:g} relative residual",
+            "grid": list(scene.shape),
+            "particles": n,
+            "solver": args.solver,
+        },
+        "particle_steps_per_s": round(n * args.steps / elapsed, 1),
+        "pressure_solve_ms": round(float(np.mean(solve_ms)), 3),
+        "w_cycles_per_step": round(float(np.mean(cycles)), 2),
+        "relative_residual_max": float(np.max(relres)),
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": d["achieved_GBs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": d["frac_hbm_peak"],
+            "traffic": None,
+        },
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        print("# timing the CPU oracle on the same workload", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(scene, params if args.solver != "mixed" else
+                                              host_solver.StepParams(dt=params.dt, gravity=params.gravity,
+                                                                     flip_ratio=params.flip_ratio,
+                                                                     solver=solver_settings("fp32", args.tolerance)),
+                                              args.cpu_steps)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -65,6 +65,10 @@ struct clq_ctx {
     double *norm_partials = nullptr, *norm_out = nullptr, *norm_host = nullptr;
     hipEvent_t ev_start[CLQ_STAGE_COUNT] = {}, ev_stop[CLQ_STAGE_COUNT] = {};
     bool ev_used[CLQ_STAGE_COUNT] = {};
+    // per-kernel-class event pairs while kernel timing is enabled
+    bool ktiming = false;
+    std::vector<hipEvent_t> kevents[CLQ_KERNEL_CLASS_COUNT];  // start, stop, start, stop, ...
+    std::vector<hipEvent_t> event_pool;
     std::vector<void *> allocations;
 
     int alloc(void **ptr, size_t bytes) {
@@ -108,6 +112,21 @@ struct DeviceGuard {
     explicit DeviceGuard(int dev) { (void)hipSetDevice(dev); }
 };
 
+// Kernel-class timing: a pair of events around one launch unit.
+void kernel_mark(clq_ctx *c, int cls) {
+    if (!c->ktiming) return;
+    hipEvent_t e = nullptr;
+    if (!c->event_pool.empty()) {
+        e = c->event_pool.back();
+        c->event_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        c->ktiming = false;  // out of events: stop timing rather than fail the step
+        return;
+    }
+    (void)hipEventRecord(e, c->stream);
+    c->kevents[cls].push_back(e);
+}
+
 void stage_begin(clq_ctx *c, int stage) {
     (void)hipEventRecord(c->ev_start[stage], c->stream);
 }
@@ -145,6 +164,9 @@ int clq_destroy(clq_ctx *c) {
         if (c->ev_start[s]) (void)hipEventDestroy(c->ev_start[s]);
         if (c->ev_stop[s]) (void)hipEventDestroy(c->ev_stop[s]);
     }
+    for (auto &v : c->kevents)
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CLQ_OK;
@@ -382,11 +404,13 @@ int clq_bin_particles(clq_ctx *c) {
     const float inv_dx = (float)(1.0 / c->dx);  // FIELD_DTYPE(1.0 / dx), particles.py:103
     const int n = (int)c->n;
     stage_begin(c, CLQ_STAGE_BIN);
+    kernel_mark(c, CLQ_KERNEL_BIN);
     HIP_TRY(hipMemsetAsync(c->counts, 0, (size_t)c->C * 4, c->stream));
     launch_bin_count(c->pos, n, inv_dx, c->nx, c->ny, c->nz, c->key, c->rank, c->counts, c->stream);
     launch_exclusive_scan(c->counts, c->offsets, c->C, c->scan_scratch, c->stream);
     launch_bin_permute(c->key, c->rank, c->offsets, n, c->perm, c->stream);
     launch_bin_gather(c->perm, c->key, c->offsets, c->counts, c->pos, c->vel, n, c->pos_work, c->vel_work, c->stream);
+    kernel_mark(c, CLQ_KERNEL_BIN);
     stage_end(c, CLQ_STAGE_BIN);
     std::swap(c->pos, c->pos_work);  // particles.py:136-140
     std::swap(c->vel, c->vel_work);
@@ -397,8 +421,10 @@ int clq_particle_to_grid(clq_ctx *c, double particles_per_cell) {
     if (!c || !(particles_per_cell > 0)) return fail(CLQ_ERR_INVALID, "bad P2G arguments");
     DeviceGuard g(c->device);
     stage_begin(c, CLQ_STAGE_P2G);
+    kernel_mark(c, CLQ_KERNEL_P2G);
     launch_p2g(c->pos, c->vel, c->offsets, c->counts, c->u, c->v, c->w, c->ff, (float)(1.0 / c->dx),
                (float)(1.0 / particles_per_cell), c->nx, c->ny, c->nz, c->stream);
+    kernel_mark(c, CLQ_KERNEL_P2G);
     stage_end(c, CLQ_STAGE_P2G);
     c->coarse_ops_valid = false;
     return check_launch("particle_to_grid");
@@ -463,8 +489,10 @@ static void ensure_coarse_operators(clq_ctx *c) {
 static void smooth_level(clq_ctx *c, int level, int iterations) {
     const float delta = (float)c->dx;
     for (int it = 0; it < iterations; ++it) {
+        if (level == 0) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
         launch_rbgs(c->lv[level], delta, 0, c->stream);
         launch_rbgs(c->lv[level], delta, 1, c->stream);
+        if (level == 0) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
     }
 }
 
@@ -569,8 +597,11 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
         if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE && r <= target) break;
         w_cycle(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
         s.cycles++;
-        if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE)
+        if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE) {
+            const double before = r;
             if (int rc = residual_norm(c, &r, &bn)) return rc;
+            if (r > 0.95 * before) break;  // stagnation at the FP32 floor
+        }
     }
     if (sp->mode == CLQ_SOLVE_FIXED_CYCLES)
         if (int rc = residual_norm(c, &r, &bn)) return rc;
@@ -651,7 +682,9 @@ int clq_grid_to_particle_advect(clq_ctx *c, double flip_ratio, double dt) {
     const int flags = c->has_obstacle ? (kG2P | kSavePrev | kAdvect | kDomainBC | kObstacleBC)
                                       : (kG2P | kAdvect | kDomainBC);
     stage_begin(c, CLQ_STAGE_G2P_ADVECT);
+    kernel_mark(c, CLQ_KERNEL_PARTICLE_UPDATE);
     int rc = particle_update(c, flags, flip_ratio, dt, "grid_to_particle_advect");
+    kernel_mark(c, CLQ_KERNEL_PARTICLE_UPDATE);
     stage_end(c, CLQ_STAGE_G2P_ADVECT);
     return rc;
 }
@@ -660,6 +693,37 @@ int clq_voxelise(clq_ctx *c, int32_t scale, double particles_per_cell, uint8_t *
     (void)scale; (void)particles_per_cell; (void)occupancy; (void)density;
     if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
     return fail(CLQ_ERR_UNSUPPORTED, "clq_voxelise is not available in this build");
+}
+
+int clq_kernel_timing(clq_ctx *c, int32_t enable) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    DeviceGuard g(c->device);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &v : c->kevents) {
+        c->event_pool.insert(c->event_pool.end(), v.begin(), v.end());
+        v.clear();
+    }
+    c->ktiming = enable != 0;
+    return CLQ_OK;
+}
+
+int clq_kernel_times(clq_ctx *c, double *total_ms, int64_t *launches, int32_t capacity) {
+    if (!c || !total_ms || !launches) return fail(CLQ_ERR_INVALID, "NULL argument");
+    DeviceGuard g(c->device);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int n = std::min<int>(capacity, CLQ_KERNEL_CLASS_COUNT);
+    for (int k = 0; k < n; ++k) {
+        double total = 0.0;
+        const auto &v = c->kevents[k];
+        for (size_t q = 0; q + 1 < v.size(); q += 2) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, v[q], v[q + 1]));
+            total += ms;
+        }
+        total_ms[k] = total;
+        launches[k] = (int64_t)(v.size() / 2);
+    }
+    return n;
 }
 
 int clq_stage_times(clq_ctx *c, double *ms, int32_t capacity) {

@@ -45,6 +45,7 @@ SOLVE_MODES = {
     "learned": SOLVE_LEARNED_FCG,
 }
 STAGES = ("bin", "p2g", "pressure_solve", "g2p_advect")
+KERNEL_CLASSES = ("binning", "p2g", "rbgs_fine_sweep", "particle_update")
 
 
 class ClqError(RuntimeError):
@@ -124,6 +125,8 @@ _SIGNATURES = {
     "clq_grid_to_particle_advect": ([_P, c_double, c_double], ctypes.c_int),
     "clq_voxelise": ([_P, c_int32, c_double, _P, _P], ctypes.c_int),
     "clq_stage_times": ([_P, POINTER(c_double), c_int32], ctypes.c_int),
+    "clq_kernel_timing": ([_P, c_int32], ctypes.c_int),
+    "clq_kernel_times": ([_P, POINTER(c_double), POINTER(c_int64), c_int32], ctypes.c_int),
 }
 EXPORTED_SYMBOLS = ("clq_last_error", *_SIGNATURES)
 
@@ -384,6 +387,20 @@ class Context:
         if n < 0:
             self._check(n, "clq_stage_times")
         return {STAGES[i]: ms[i] for i in range(n)}
+
+
+    def kernel_timing(self, enable: bool) -> None:
+        self._check(self.lib.clq_kernel_timing(self.handle, int(bool(enable))), "clq_kernel_timing")
+
+    def kernel_times(self) -> dict:
+        """{class: (total_ms, launch_units)} accumulated since kernel_timing(True)."""
+        k = len(KERNEL_CLASSES)
+        ms = (c_double * k)()
+        units = (c_int64 * k)()
+        n = self.lib.clq_kernel_times(self.handle, ms, units, k)
+        if n < 0:
+            self._check(n, "clq_kernel_times")
+        return {KERNEL_CLASSES[i]: (ms[i], units[i]) for i in range(n)}
 
 
 def device_count(lib: ctypes.CDLL) -> int:

@@ -95,7 +95,9 @@ enum clq_solve_mode {
      * settings.iterations, solver.py:125). */
     CLQ_SOLVE_FIXED_CYCLES = 0,
     /* FP32 W-cycles until the FP64-accumulated relative residual
-     * ||b - A p||_2 / ||b||_2 <= tolerance (or max_cycles). */
+     * ||b - A p||_2 / ||b||_2 <= tolerance, or max_cycles, or stagnation: a
+     * cycle that leaves the residual above 0.95x its previous value ends the
+     * solve (FP32 storage of p bounds the attainable residual). */
     CLQ_SOLVE_FP32_TOLERANCE = 1,
     /* FP64 iterative refinement around a W-cycle with FP32 smoothing and
      * 16-bit coarse-level residual storage; FP64 residual, FP64 correction
@@ -131,7 +133,7 @@ typedef struct clq_ctx clq_ctx;
 /* ---- library / context ---------------------------------------------------- */
 const char *clq_last_error(void);
 int clq_abi_version(void);
-/* Number of usable devices (0 for the CPU oracle is never returned: it reports 1). */
+/* Number of usable devices (the CPU oracle reports 1). */
 int clq_device_count(int32_t *count);
 /* Creates a context for an (nx, ny, nz) grid of spacing dx holding up to
  * `capacity` particles.  Allocates every buffer of solver.py:161-217,
@@ -236,6 +238,24 @@ enum clq_stage {
     CLQ_STAGE_COUNT = 4
 };
 int clq_stage_times(clq_ctx *ctx, double *ms, int32_t capacity);
+
+/* Accumulated device time per kernel class, for roofline reports.  While
+ * enabled, HIP events bracket every launch unit of each class on the context
+ * stream: one binning pass (all its kernels), one P2G gather, one full
+ * red-black sweep (both colours) of the finest level, one fused particle
+ * update.  Enabling resets the totals.  No counterpart in the reference
+ * (its debug_timing sections time whole stages on the host,
+ * Solver/Kernel/timing.py:12-29). */
+enum clq_kernel_class {
+    CLQ_KERNEL_BIN = 0,
+    CLQ_KERNEL_P2G = 1,
+    CLQ_KERNEL_RBGS_FINE = 2,
+    CLQ_KERNEL_PARTICLE_UPDATE = 3,
+    CLQ_KERNEL_CLASS_COUNT = 4
+};
+int clq_kernel_timing(clq_ctx *ctx, int32_t enable);
+/* Writes total ms and launch-unit count per class; returns classes written. */
+int clq_kernel_times(clq_ctx *ctx, double *total_ms, int64_t *launches, int32_t capacity);
 
 #ifdef __cplusplus
 }

@@ -827,7 +827,11 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
         if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE && r <= target) break;
         clq_w_cycle(c, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
         s.cycles++;
-        if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE) clq_residual_norm(c, &r, &bn);
+        if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE) {
+            const double before = r;
+            clq_residual_norm(c, &r, &bn);
+            if (r > 0.95 * before) break; /* stagnation at the FP32 floor */
+        }
     }
     if (sp->mode == CLQ_SOLVE_FIXED_CYCLES) clq_residual_norm(c, &r, &bn);
     s.final_residual = r;
@@ -1089,6 +1093,17 @@ int clq_voxelise(clq_ctx *c, int32_t s, double ppc, uint8_t *occ, float *density
         free(acc);
     }
     return CLQ_OK;
+}
+
+int clq_kernel_timing(clq_ctx *c, int32_t enable) {
+    return c ? CLQ_OK : fail(CLQ_ERR_INVALID, "ctx is NULL");
+}
+
+int clq_kernel_times(clq_ctx *c, double *total_ms, int64_t *launches, int32_t capacity) {
+    if (!c || !total_ms || !launches) return fail(CLQ_ERR_INVALID, "NULL argument");
+    int n = capacity < CLQ_KERNEL_CLASS_COUNT ? capacity : CLQ_KERNEL_CLASS_COUNT;
+    for (int i = 0; i < n; ++i) { total_ms[i] = 0.0; launches[i] = 0; }
+    return n;
 }
 
 int clq_stage_times(clq_ctx *c, double *ms, int32_t capacity) {

@@ -1,0 +1,243 @@
+"""HIP library vs CPU oracle on the same seeded inputs, through the C-ABI.
+
+Integer work (bin counts, offsets, sort order, occupancy) and every FP32 kernel
+that restates a reference kernel are compared bit for bit
+(`np.array_equal`, which treats +0 and -0 as equal): the HIP kernels and the
+oracle perform the same IEEE operations in the same order, fused exactly where
+the reference OpenCL source contracts.  FP64 reductions (residual norms) are
+compared to 1e-12 relative because their summation trees differ.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from continuum_liquid_amd import runtime, scenes
+from continuum_liquid_amd.Solver.Kernel import solver as host_solver
+
+from .helpers import bin_keys, default_params, make_context, random_particles, run_to_pressure, scene_from_arrays
+
+pytestmark = pytest.mark.gpu
+
+GRID_FIELDS = ("u", "v", "w", "u_old", "v_old", "w_old", "fluid_fraction", "pressure", "rhs", "diagonal",
+               "weight_x", "weight_y", "weight_z", "bin_counts", "bin_offsets")
+
+
+def _scene_cases():
+    rng = np.random.default_rng(7)
+    cases = [scenes.small_scene(24, seed=1), scenes.small_scene(17, seed=2, spheres=False)]
+    # uniform random particles over the whole box, ghosts and edges included
+    pos, vel = random_particles(rng, (20, 18, 22), 1.0 / 20, 30000)
+    cases.append(scene_from_arrays("random 20x18x22", (20, 18, 22), 1.0 / 20, pos, vel))
+    # adversarial: positions on bin boundaries x * inv_dx - 0.5 = integer, and a crowded bin
+    n = 16
+    dx = 1.0 / n
+    grid = np.arange(1, n - 1, dtype=np.float64)
+    bx = np.stack(np.meshgrid(grid, grid, grid, indexing="ij"), -1).reshape(-1, 3)
+    edge = ((bx + 0.5) * dx).astype(np.float32)
+    crowd = np.full((400, 3), 0.5, np.float32) + rng.uniform(-0.2, 0.2, (400, 3)).astype(np.float32) * dx
+    outside = rng.uniform(-0.5, 1.5, (500, 3)).astype(np.float32)
+    pos = np.concatenate([edge, crowd, outside])
+    vel = rng.normal(0, 1, pos.shape).astype(np.float32)
+    cases.append(scene_from_arrays("adversarial 16^3", (n, n, n), dx, pos, vel))
+    return cases
+
+
+CASES = _scene_cases()
+IDS = [c.name for c in CASES]
+
+
+def pair(oracle_lib, hip_lib, scene):
+    return make_context(oracle_lib, scene), make_context(hip_lib, scene)
+
+
+def assert_fields_equal(a, b, names=GRID_FIELDS):
+    for name in names:
+        x, y = a.get_field(name), b.get_field(name)
+        assert np.array_equal(x, y), f"{name}: {np.count_nonzero(x != y)} cells differ, max |d| = " \
+                                     f"{np.max(np.abs(x.astype(np.float64) - y)) if x.size else 0}"
+
+
+def assert_particles_equal(a, b):
+    pa, va = a.get_particles()
+    pb, vb = b.get_particles()
+    assert np.array_equal(pa, pb), f"positions differ at {np.count_nonzero(np.any(pa != pb, axis=1))} particles"
+    assert np.array_equal(va, vb), f"velocities differ at {np.count_nonzero(np.any(va != vb, axis=1))} particles"
+
+
+@pytest.mark.parametrize("scene", CASES, ids=IDS)
+def test_binning_bit_exact(oracle_lib, hip_lib, scene):
+    o, h = pair(oracle_lib, hip_lib, scene)
+    o.bin_particles()
+    h.bin_particles()
+    assert_fields_equal(o, h, ("bin_counts", "bin_offsets"))
+    assert_particles_equal(o, h)
+    # and both equal the stable counting sort of the closed-form keys
+    keys = bin_keys(scene.positions, scene.dx, scene.shape)
+    order = np.argsort(keys, kind="stable")
+    pos, vel = h.get_particles()
+    assert np.array_equal(pos, scene.positions[order])
+    assert np.array_equal(vel, scene.velocities[order])
+    counts = np.bincount(keys, minlength=int(np.prod(scene.shape)))
+    assert np.array_equal(h.get_field("bin_counts").ravel(), counts)
+
+
+@pytest.mark.parametrize("scene", CASES, ids=IDS)
+def test_p2g_bit_exact(oracle_lib, hip_lib, scene):
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        c.bin_particles()
+        c.particle_to_grid(8.0)
+    assert_fields_equal(o, h, ("u", "v", "w", "fluid_fraction"))
+
+
+def test_empty_particle_set(oracle_lib, hip_lib):
+    empty = scene_from_arrays("empty", (12, 12, 12), 1 / 12, np.zeros((0, 3), np.float32),
+                              np.zeros((0, 3), np.float32))
+    o, h = pair(oracle_lib, hip_lib, empty)
+    for c in (o, h):
+        c.bin_particles()
+        c.particle_to_grid(8.0)
+    assert_fields_equal(o, h, ("u", "v", "w", "fluid_fraction", "bin_counts", "bin_offsets"))
+    assert not h.get_field("fluid_fraction").any()
+
+
+@pytest.mark.parametrize("scene", CASES[:3], ids=IDS[:3])
+def test_grid_glue_bit_exact(oracle_lib, hip_lib, scene):
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+    assert_fields_equal(o, h)
+
+
+@pytest.mark.parametrize("scene", CASES[:2], ids=IDS[:2])
+def test_multigrid_kernels_bit_exact(oracle_lib, hip_lib, scene):
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+    for c in (o, h):
+        c.smooth(0, 3)
+    assert_fields_equal(o, h, ("pressure",))
+    levels = len(o.level_shapes)
+    for level in range(levels - 1):
+        for c in (o, h):
+            c.restrict(level)
+        for name in ("p", "b", "mask", "diagonal", "weight_x", "weight_y", "weight_z"):
+            assert np.array_equal(o.get_level_field(level + 1, name), h.get_level_field(level + 1, name)), \
+                (level + 1, name)
+        for c in (o, h):
+            c.smooth(level + 1, 2)
+        assert np.array_equal(o.get_level_field(level + 1, "p"), h.get_level_field(level + 1, "p"))
+    for c in (o, h):
+        c.coarse_smooth(66)
+    assert np.array_equal(o.get_level_field(levels - 1, "p"), h.get_level_field(levels - 1, "p"))
+    for level in range(levels - 2, -1, -1):
+        for c in (o, h):
+            c.prolongate(level)
+        assert np.array_equal(o.get_level_field(level, "p"), h.get_level_field(level, "p")), level
+
+
+@pytest.mark.parametrize("scene", CASES[:2], ids=IDS[:2])
+def test_w_cycle_and_residual(oracle_lib, hip_lib, scene):
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+        c.w_cycle(2, 4, 64)
+        c.w_cycle(2, 2, 64)
+    assert_fields_equal(o, h, ("pressure",))
+    (ro, bo), (rh, bh) = o.residual_norm(), h.residual_norm()
+    assert abs(ro - rh) <= 1e-12 * ro and abs(bo - bh) <= 1e-12 * bo
+
+
+@pytest.mark.parametrize("flip", [1.0, 0.95, 0.0])
+def test_particle_update_bit_exact(oracle_lib, hip_lib, flip):
+    scene = CASES[0]
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+        c.pressure_solve(runtime.SOLVE_FIXED_CYCLES, max_cycles=1)
+        c.pressure_correction(1000.0, 1 / 240)
+        c.grid_domain_bc()
+        c.grid_to_particle_advect(flip, 1 / 240)
+    assert_particles_equal(o, h)
+
+
+def test_fused_particle_update_equals_stages(hip_lib):
+    scene = CASES[0]
+    a, b = make_context(hip_lib, scene), make_context(hip_lib, scene)
+    for c in (a, b):
+        run_to_pressure(c, scene)
+        c.pressure_solve(runtime.SOLVE_FIXED_CYCLES, max_cycles=1)
+        c.pressure_correction(1000.0, 1 / 240)
+    a.grid_to_particle_advect(0.95, 1 / 240)
+    b.grid_to_particle(0.95)
+    b.save_previous_positions()
+    b.advect_particles(1 / 240)
+    b.particle_domain_bc()
+    b.particle_obstacle_bc()
+    assert_particles_equal(a, b)
+
+
+def _obstacle_scene():
+    base = scenes.small_scene(24, seed=3)
+    obstacle = np.zeros(base.shape, np.uint8)
+    obstacle[9:14, 1:7, 8:16] = 1
+    keep = np.ones(len(base.positions), bool)
+    cells = np.floor(base.positions / base.dx).astype(int)
+    keep &= ~obstacle[cells[:, 0], cells[:, 1], cells[:, 2]].astype(bool)
+    s = scene_from_arrays("obstacle 24^3", base.shape, base.dx, base.positions[keep], base.velocities[keep])
+    return s, obstacle
+
+
+@pytest.mark.parametrize("steps", [3])
+def test_full_step_bit_exact(oracle_lib, hip_lib, steps):
+    for scene, obstacle in [(CASES[0], None), _obstacle_scene()]:
+        o, h = pair(oracle_lib, hip_lib, scene)
+        if obstacle is not None:
+            o.set_field("obstacle", obstacle)
+            h.set_field("obstacle", obstacle)
+        params = default_params(scene)
+        for _ in range(steps):
+            so = host_solver.step(o, params)
+            sh = host_solver.step(h, params)
+            assert so["cycles"] == sh["cycles"]
+        assert_particles_equal(o, h)
+        assert_fields_equal(o, h, ("u", "v", "w", "pressure", "fluid_fraction"))
+
+
+def test_tolerance_solve_matches_oracle(oracle_lib, hip_lib):
+    scene = CASES[0]
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+    so = o.pressure_solve(runtime.SOLVE_FP32_TOLERANCE, post_smooth=2, max_cycles=12, tolerance=1e-4)
+    sh = h.pressure_solve(runtime.SOLVE_FP32_TOLERANCE, post_smooth=2, max_cycles=12, tolerance=1e-4)
+    assert so["cycles"] == sh["cycles"]
+    assert sh["relative_residual"] <= 1e-4
+    assert_fields_equal(o, h, ("pressure",))
+
+
+def test_binning_full_size_properties(hip_lib):
+    """Size-independent properties at the 256^3 workload (28.6M particles)."""
+    scene = scenes.baseline_scene(3)
+    h = make_context(hip_lib, scene)
+    h.bin_particles()
+    counts = h.get_field("bin_counts").ravel().astype(np.int64)
+    offsets = h.get_field("bin_offsets").ravel().astype(np.int64)
+    assert counts.sum() == scene.particle_count
+    assert np.array_equal(offsets[1:], np.cumsum(counts)[:-1])
+    pos, vel = h.get_particles()
+    keys = bin_keys(pos, scene.dx, scene.shape)
+    assert np.all(np.diff(keys) >= 0), "not sorted by bin"
+    assert np.array_equal(np.bincount(keys, minlength=counts.size), counts)
+    # same multiset of (position, velocity) rows
+    before = np.sort(np.ascontiguousarray(np.concatenate([scene.positions, scene.velocities], 1)).view(
+        np.dtype((np.void, 24))).ravel())
+    after = np.sort(np.ascontiguousarray(np.concatenate([pos, vel], 1)).view(np.dtype((np.void, 24))).ravel())
+    assert np.array_equal(before, after)
+    # stability: within a bin, the original order (inputs are in cell order,
+    # and each cell's 8 particles span two bins per axis) is preserved
+    keys0 = bin_keys(scene.positions, scene.dx, scene.shape)
+    order = np.argsort(keys0, kind="stable")
+    assert np.array_equal(pos, scene.positions[order])

@@ -1,0 +1,155 @@
+"""Host logic of the kept Python API, on the CPU.
+
+`Solver.General.main` and `Solver.Kernel.solver` load the HIP library in
+production; here `runtime.load_library` is replaced, inside each test only,
+by the CPU oracle so that the whole host loop (config parsing, seeding,
+output cadence, cache format, messages, cancellation, errors) runs without a
+GPU.  Reference behaviour followed: Solver/General/main.py:13-52,
+Solver/Kernel/solver.py:22-531, Solver/Kernel/output.py:19-53.
+"""
+
+from __future__ import annotations
+
+import json
+import threading
+
+import numpy as np
+import pytest
+
+from continuum_liquid_amd import runtime
+from continuum_liquid_amd.Solver.General import main as solver_main
+from continuum_liquid_amd.Solver.Kernel import output, solver as host_solver
+
+
+@pytest.fixture
+def oracle_backend(monkeypatch, oracle_lib):
+    monkeypatch.setattr(runtime, "load_library", lambda path=None: oracle_lib)
+    return oracle_lib
+
+
+def make_config(tmp_path, frames=3, substeps=1, cancel=None, **settings):
+    n = 14
+    cfg = {
+        "simulation": {
+            "output_path": str(tmp_path / "bake"),
+            "settings": {"solver_backend": "GPU", "start_frame": 1, "end_frame": 1 + frames, "fps": 24,
+                         "substeps": substeps, "iterations": 1, **settings},
+            "domain": {"resolution": 1.0 / n, "size": {"x": 1.0, "y": 1.0, "z": 1.0}},
+            "physics": {"water": {"density": 1000.0, "viscosity": 1e-6},
+                        "air": {"density": 1.225, "viscosity": 1.5e-5}},
+            "sources": [{"type": "GEOMETRY", "particle_density": 8, "velocity": [0.0, 0.0, 0.0],
+                         "shapes": [{"box": [[0.08, 0.08, 0.08], [0.45, 0.9, 0.5]]}]}],
+            "obstacles": [{"shapes": [{"box": [[0.6, 0.0, 0.0], [0.75, 1.0, 0.2]]}]}],
+        },
+        "meta": {"cancel_event": cancel},
+    }
+    return cfg
+
+
+def test_main_runs_and_writes_the_viewer_cache(tmp_path, oracle_backend, capsys):
+    messages = []
+    solver_main.set_message_callback(messages.append)
+    try:
+        solver_main.main(make_config(tmp_path, frames=3))
+    finally:
+        solver_main.clear_message_callback()
+    out = capsys.readouterr().out
+    assert "Bake runtime:" in out
+    meta = json.loads((tmp_path / "bake" / output.CACHE_FILENAME).read_text())
+    assert meta == {"version": 2, "fps": 24, "start_frame": 1, "end_frame": 4, "variable_particle_count": True,
+                    "dtype": "float32", "components": 3, "frame_directory": "particles",
+                    "frame_pattern": "particles_{frame:06d}.bin"}
+    frames = sorted((tmp_path / "bake" / "particles").glob("*.bin"))
+    assert [f.name for f in frames] == [f"particles_{i:06d}.bin" for i in range(1, 5)]
+    first = np.fromfile(frames[0], np.float32).reshape(-1, 3)
+    last = np.fromfile(frames[-1], np.float32).reshape(-1, 3)
+    assert len(first) == len(last) > 0
+    assert last[:, 2].mean() < first[:, 2].mean()  # gravity on z pulls the block down
+    assert not list((tmp_path / "bake" / "particles").glob("*.tmp"))
+    assert [m["frame"] for m in messages] == [0, 1, 2, 3]
+    assert all(m["type"] == "stats" and m["total_frames"] == 3 for m in messages)
+
+
+def test_cancel_event_stops_cleanly(tmp_path, oracle_backend):
+    cancel = threading.Event()
+    seen = []
+
+    def on_message(msg):
+        seen.append(msg)
+        if msg["frame"] == 1:
+            cancel.set()
+
+    solver_main.set_message_callback(on_message)
+    try:
+        solver_main.main(make_config(tmp_path, frames=5, cancel=cancel))
+    finally:
+        solver_main.clear_message_callback()
+    assert [m["frame"] for m in seen] == [0, 1]
+    assert len(list((tmp_path / "bake" / "particles").glob("*.bin"))) == 2
+
+
+def test_debug_timing_reports_reference_sections(tmp_path, oracle_backend, capsys):
+    solver_main.main(make_config(tmp_path, frames=1, debug_timing=True))
+    out = capsys.readouterr().out
+    for section in ("particle_to_grid", "pressure_solve", "grid_to_particle", "advect_particles",
+                    "particle_domain_bc", "particle_obstacle_bc", "particles_to_host"):
+        assert section in out
+
+
+def test_cpu_backend_is_rejected(tmp_path):
+    cfg = make_config(tmp_path)
+    cfg["simulation"]["settings"]["solver_backend"] = "CPU"
+    with pytest.raises(RuntimeError, match="CPU backend"):
+        solver_main.main(cfg)
+    cfg["simulation"]["settings"]["solver_backend"] = "VULKAN"
+    with pytest.raises(ValueError, match="Unsupported solver backend"):
+        solver_main.main(cfg)
+
+
+def test_usd_geometry_needs_a_mask(tmp_path, oracle_backend):
+    cfg = make_config(tmp_path)
+    cfg["simulation"]["sources"] = [{"type": "GEOMETRY", "geometry_inputs": [{"mesh_file": "geometry/a.usdc"}]}]
+    with pytest.raises(NotImplementedError, match="mask"):
+        solver_main.main(cfg)
+
+
+def test_stage_errors_propagate(tmp_path, oracle_backend):
+    cfg = make_config(tmp_path, pressure_solver={"mode": "learned"})
+    with pytest.raises(runtime.ClqError, match="solve mode"):
+        solver_main.main(cfg)
+
+
+def test_writer_errors_surface(tmp_path):
+    writer = output.ParticleOutput(tmp_path / "missing" / "dir")
+    writer.submit(np.zeros((4, 3), np.float32), 1)
+    with pytest.raises(RuntimeError, match="writer failed"):
+        writer.close()
+
+
+def test_message_callback_is_thread_safe():
+    got = []
+    solver_main.set_message_callback(got.append)
+    threads = [threading.Thread(target=solver_main.emit_message, args=({"i": i},)) for i in range(16)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    solver_main.clear_message_callback()
+    solver_main.emit_message({"i": -1})
+    assert sorted(m["i"] for m in got) == list(range(16))
+
+
+def test_fused_and_sectioned_steps_agree(oracle_lib):
+    from continuum_liquid_amd import scenes
+    from continuum_liquid_amd.Solver.Kernel import timing
+
+    scene = scenes.small_scene(16, seed=5)
+    params = host_solver.StepParams(dt=1 / 240, gravity=scene.gravity, flip_ratio=0.95,
+                                    solver={"mode": "fixed", "max_cycles": 1})
+    out = []
+    for timings in (None, timing.RunTimings()):
+        with runtime.Context(oracle_lib, scene.shape, scene.dx, scene.particle_count) as ctx:
+            ctx.set_particles(scene.positions, scene.velocities)
+            host_solver.step(ctx, params, timings)
+            out.append(ctx.get_particles())
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
