@@ -92,7 +92,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=3, choices=(1, 2, 3, 5))
-    ap.add_argument("--solver", default="fp32", choices=("fp32", "mixed", "fixed"))
+    ap.add_argument("--solver", default="mixed", choices=("fp32", "mixed", "fixed"))
     ap.add_argument("--tolerance", type=float, default=1e-6)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -191,11 +191,7 @@ def main() -> int:
     }
     if not args.no_cpu:
         print("# timing the CPU oracle on the same workload", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(scene, params if args.solver != "mixed" else
-                                              host_solver.StepParams(dt=params.dt, gravity=params.gravity,
-                                                                     flip_ratio=params.flip_ratio,
-                                                                     solver=solver_settings("fp32", args.tolerance)),
-                                              args.cpu_steps)
+        result["cpu_baseline"] = cpu_baseline(scene, params, args.cpu_steps)
     print(json.dumps(result), flush=True)
     return 0
 

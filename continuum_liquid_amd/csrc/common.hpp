@@ -14,6 +14,7 @@
 // bit for bit with the CPU oracle.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -28,7 +29,37 @@ struct Level {
     int64_t C = 0;
     float *p = nullptr, *b = nullptr, *mask = nullptr, *diag = nullptr;
     float *wx = nullptr, *wy = nullptr, *wz = nullptr;
+    // 16-bit storage of b for the mixed-precision solve: bkind 0 = float b,
+    // 1 = fp16, 2 = bf16, stored value b / bscale (bscale a power of two).
+    uint16_t *b16 = nullptr;
+    int bkind = 0;
+    float bscale = 1.0f;
 };
+
+// b of a level through its storage format (uniform branch per launch).
+__device__ __forceinline__ float bits_to_f16f(uint16_t h) { return __half2float(__ushort_as_half(h)); }
+__device__ __forceinline__ float bits_to_bf16f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ float load_b(const float *__restrict__ b, const uint16_t *__restrict__ b16, int kind,
+                                        float scale, int64_t i) {
+    if (kind == 0) return b[i];
+    const float v = kind == 1 ? bits_to_f16f(b16[i]) : bits_to_bf16f(b16[i]);
+    return v * scale;
+}
+__device__ __forceinline__ void store_b(float *__restrict__ b, uint16_t *__restrict__ b16, int kind, float scale,
+                                        int64_t i, float v) {
+    if (kind == 0) {
+        b[i] = v;
+        return;
+    }
+    const float x = v * (1.0f / scale);
+    if (kind == 1) {
+        b16[i] = __half_as_ushort(__float2half_rn(x));
+    } else {
+        uint32_t u = __float_as_uint(x);  // round to nearest even, NaN kept quiet
+        b16[i] = (u & 0x7fffffffu) > 0x7f800000u ? (uint16_t)((u >> 16) | 0x40u)
+                                                 : (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    }
+}
 
 // ---- device helpers (reference helper.cl:4-23) -----------------------------
 __device__ __forceinline__ int64_t gidx(int i, int j, int k, int ny, int nz) {
@@ -95,6 +126,12 @@ void launch_coarse_operator(const Level &F, const Level &K, hipStream_t s);
 void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, hipStream_t s);
 void launch_prolongate(const Level &F, const Level &K, hipStream_t s);
 void launch_coarse_smooth(const Level &L, float delta, int iterations, hipStream_t s);
+void launch_coarse_smooth_f64(const Level &L, double delta, int iterations, hipStream_t s);
+int coarse_f64_capacity();
+void launch_ir_update_residual(const Level &L, int from32, const float *p32, const double *pin, double *pout,
+                               float *r32, const float *b, double inv_h2, double *partials, double *out2,
+                               hipStream_t s);
+void launch_widen_narrow(const double *p64, float *p32, int64_t C, hipStream_t s);
 int residual_partials_count(int64_t C);
 void launch_residual_norm(const Level &L, double inv_h2, double *partials, double *out2, hipStream_t s);
 

@@ -63,6 +63,9 @@ struct clq_ctx {
     Level lv[kMaxLevels];
     bool coarse_ops_valid = false;  // coarse operators match the current level-0 operator
     double *norm_partials = nullptr, *norm_out = nullptr, *norm_host = nullptr;
+    // mixed-precision solve state, allocated on first use
+    double *p64a = nullptr, *p64b = nullptr;
+    float *r32 = nullptr;
     hipEvent_t ev_start[CLQ_STAGE_COUNT] = {}, ev_stop[CLQ_STAGE_COUNT] = {};
     bool ev_used[CLQ_STAGE_COUNT] = {};
     // per-kernel-class event pairs while kernel timing is enabled
@@ -577,17 +580,107 @@ int clq_residual_norm(clq_ctx *c, double *residual, double *rhs_norm) {
     return residual_norm(c, residual, rhs_norm);
 }
 
+
+// ---- mixed-precision iterative refinement -------------------------------------------
+static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra) {
+    if (level == c->nlevels - 1) {
+        const Level &L = c->lv[level];
+        if (L.C <= coarse_f64_capacity()) launch_coarse_smooth_f64(L, (double)(float)c->dx, pre + extra, c->stream);
+        else launch_coarse_smooth(L, (float)c->dx, pre + extra, c->stream);
+        return;
+    }
+    smooth_level(c, level, pre);
+    restrict_level(c, level);
+    w_cycle_mixed(c, level + 1, pre, post, extra);
+    w_cycle_mixed(c, level + 1, pre, post, extra);
+    launch_prolongate(c->lv[level], c->lv[level + 1], c->stream);
+    smooth_level(c, level, post);
+}
+
+static int ir_residual(clq_ctx *c, int from32, const double *pin, double *pout, double *r, double *bn) {
+    const double h = (double)(float)c->dx;
+    launch_ir_update_residual(c->lv[0], from32, c->p, pin, pout, c->r32, c->rhs, 1.0 / (h * h), c->norm_partials,
+                              c->norm_out, c->stream);
+    HIP_TRY(hipMemcpyAsync(c->norm_host, c->norm_out, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *r = sqrt(c->norm_host[0]);
+    *bn = sqrt(c->norm_host[1]);
+    return CLQ_OK;
+}
+
+// FP64 iterate p64 and residual, W-cycle on A e = r with FP32 smoothing and
+// 16-bit coarse b (power-of-two scale 2^(3l + es) per level from the RMS of
+// the fine residual), FP64 coarsest smoother, p64 += e.
+static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
+    if (!c->p64a) {
+        void *q;
+        if (int rc = c->alloc(&q, (size_t)c->C * 8)) return rc;
+        c->p64a = (double *)q;
+        if (int rc = c->alloc(&q, (size_t)c->C * 8)) return rc;
+        c->p64b = (double *)q;
+        if (int rc = c->alloc(&q, (size_t)c->C * 4)) return rc;
+        c->r32 = (float *)q;
+        for (int l = 1; l < c->nlevels; ++l) {
+            if (int rc = c->alloc(&q, (size_t)c->lv[l].C * 2)) return rc;
+            c->lv[l].b16 = (uint16_t *)q;
+            HIP_TRY(hipMemsetAsync(q, 0, (size_t)c->lv[l].C * 2, c->stream));
+        }
+    }
+    double r = 0.0, bn = 0.0;
+    if (int rc = ir_residual(c, 1, nullptr, c->p64a, &r, &bn)) return rc;
+    s->initial_residual = r;
+    s->rhs_norm = bn;
+    const int kind = sp->coarse_dtype == 1 ? 2 : 1;
+    Level &L0 = c->lv[0];
+    int rc = CLQ_OK;
+    for (int cyc = 0; cyc < sp->max_cycles && r > sp->tolerance * bn; ++cyc) {
+        const int es = (int)lround(log2(r / sqrt((double)c->C) + 1e-300));
+        for (int l = 1; l < c->nlevels; ++l) {
+            c->lv[l].bkind = kind;
+            c->lv[l].bscale = ldexpf(1.0f, 3 * l + es);
+        }
+        L0.b = c->r32;
+        HIP_TRY(hipMemsetAsync(c->p, 0, (size_t)c->C * 4, c->stream));
+        w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+        L0.b = c->rhs;
+        const double before = r;
+        if ((rc = ir_residual(c, 0, c->p64a, c->p64b, &r, &bn))) break;
+        std::swap(c->p64a, c->p64b);
+        s->cycles++;
+        if (r > 0.95 * before) break;
+    }
+    L0.b = c->rhs;
+    for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
+    if (rc) return rc;
+    launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
+    s->final_residual = r;
+    s->relative_residual = bn > 0 ? r / bn : 0.0;
+    return check_launch("solve_mixed");
+}
+
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
     if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0)
         return fail(CLQ_ERR_INVALID, "negative sweep or cycle count");
-    if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE)
+    if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE &&
+        sp->mode != CLQ_SOLVE_MIXED_REFINEMENT)
         return fail(CLQ_ERR_UNSUPPORTED, "solve mode %d is not available in this build", sp->mode);
     DeviceGuard g(c->device);
     clq_solve_stats s;
     memset(&s, 0, sizeof s);
     stage_begin(c, CLQ_STAGE_PRESSURE_SOLVE);
     ensure_coarse_operators(c);
+    if (sp->mode == CLQ_SOLVE_MIXED_REFINEMENT) {
+        if (int rc = solve_mixed(c, sp, &s)) return rc;
+        launch_neumann_bc(c->p, c->nx, c->ny, c->nz, c->stream);
+        stage_end(c, CLQ_STAGE_PRESSURE_SOLVE);
+        HIP_TRY(hipEventSynchronize(c->ev_stop[CLQ_STAGE_PRESSURE_SOLVE]));
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, c->ev_start[CLQ_STAGE_PRESSURE_SOLVE], c->ev_stop[CLQ_STAGE_PRESSURE_SOLVE]);
+        s.solve_ms = ms;
+        if (st) *st = s;
+        return check_launch("pressure_solve");
+    }
     double r = 0.0, bn = 0.0;
     if (int rc = residual_norm(c, &r, &bn)) return rc;
     s.initial_residual = r;

@@ -36,6 +36,7 @@ __device__ __forceinline__ float wsum(const float *__restrict__ p, const float *
 // Non-fluid cells of that colour are zeroed, as rbgs_step zeroes every
 // non-fluid cell; after a full sweep both colours have been zeroed.
 __global__ __launch_bounds__(256) void rbgs_kernel(float *__restrict__ p, const float *__restrict__ b,
+                                                   const uint16_t *__restrict__ b16, int bkind, float bscale,
                                                    const float *__restrict__ mask, const float *__restrict__ diag,
                                                    const float *__restrict__ wx, const float *__restrict__ wy,
                                                    const float *__restrict__ wz, float dd, int parity, int nx, int ny,
@@ -59,15 +60,15 @@ __global__ __launch_bounds__(256) void rbgs_kernel(float *__restrict__ p, const 
     const float d = diag[idx];
     if (d > 0.0f) {
         const int64_t sx = (int64_t)ny * nz, sy = nz;
-        p[idx] = fmaf(-dd, b[idx], wsum(p, wx, wy, wz, idx, sx, sy)) / d;  // multigrid.cl:227 (fmuladd)
+        p[idx] = fmaf(-dd, load_b(b, b16, bkind, bscale, idx), wsum(p, wx, wy, wz, idx, sx, sy)) / d;  // :227
     }
 }
 
 void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s) {
     const int half = (L.nz + 1) >> 1;
     const int64_t work = (int64_t)L.nx * L.ny * half;
-    hipLaunchKernelGGL(rbgs_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L.p, L.b, L.mask, L.diag,
-                       L.wx, L.wy, L.wz, delta * delta, parity, L.nx, L.ny, L.nz);
+    hipLaunchKernelGGL(rbgs_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L.p, L.b, L.b16, L.bkind,
+                       L.bscale, L.mask, L.diag, L.wx, L.wy, L.wz, delta * delta, parity, L.nx, L.ny, L.nz);
 }
 
 __global__ __launch_bounds__(256) void coarse_operator_kernel(
@@ -113,21 +114,22 @@ void launch_coarse_operator(const Level &F, const Level &K, hipStream_t s) {
 }
 
 // residual (multigrid.cl:45-61) at an interior fluid cell.
-__device__ __forceinline__ float residual_at(const float *__restrict__ p, const float *__restrict__ b,
-                                             const float *__restrict__ diag, const float *__restrict__ wx,
-                                             const float *__restrict__ wy, const float *__restrict__ wz, int64_t idx,
-                                             int64_t sx, int64_t sy, float inv_delta2) {
+__device__ __forceinline__ float residual_at(const float *__restrict__ p, float bv, const float *__restrict__ diag,
+                                             const float *__restrict__ wx, const float *__restrict__ wy,
+                                             const float *__restrict__ wz, int64_t idx, int64_t sx, int64_t sy,
+                                             float inv_delta2) {
     const float ap = fmaf(-diag[idx], p[idx], wsum(p, wx, wy, wz, idx, sx, sy)) * inv_delta2;  // :41 (fmuladd)
-    return b[idx] - ap;
+    return bv - ap;
 }
 
 __global__ __launch_bounds__(256) void restrict_kernel(const float *__restrict__ fp, const float *__restrict__ fb,
+                                                       const uint16_t *__restrict__ fb16, int fkind, float fscale,
                                                        const float *__restrict__ fmask,
                                                        const float *__restrict__ fdiag, const float *__restrict__ fwx,
                                                        const float *__restrict__ fwy, const float *__restrict__ fwz,
                                                        int fnx, int fny, int fnz, float *__restrict__ cp,
-                                                       float *__restrict__ cb, int cnx, int cny, int cnz,
-                                                       float inv_delta2) {
+                                                       float *__restrict__ cb, uint16_t *__restrict__ cb16, int ckind,
+                                                       float cscale, int cnx, int cny, int cnz, float inv_delta2) {
     const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (ci >= (int64_t)cnx * cny * cnz) return;
     int a, b, c;
@@ -143,16 +145,18 @@ __global__ __launch_bounds__(256) void restrict_kernel(const float *__restrict__
                     if (!interior(i, j, k, fnx, fny, fnz)) continue;
                     const int64_t idx = gidx(i, j, k, fny, fnz);
                     if (fmask[idx] <= 0.0f) continue;
-                    rhs = rhs + residual_at(fp, fb, fdiag, fwx, fwy, fwz, idx, sx, sy, inv_delta2);
+                    const float bv = load_b(fb, fb16, fkind, fscale, idx);
+                    rhs = rhs + residual_at(fp, bv, fdiag, fwx, fwy, fwz, idx, sx, sy, inv_delta2);
                 }
     }
     cp[ci] = 0.0f;
-    cb[ci] = rhs;
+    store_b(cb, cb16, ckind, cscale, ci, rhs);
 }
 
 void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, hipStream_t s) {
-    hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F.p, F.b, F.mask, F.diag,
-                       F.wx, F.wy, F.wz, F.nx, F.ny, F.nz, K.p, K.b, K.nx, K.ny, K.nz, inv_delta2);
+    hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F.p, F.b, F.b16, F.bkind,
+                       F.bscale, F.mask, F.diag, F.wx, F.wy, F.wz, F.nx, F.ny, F.nz, K.p, K.b, K.b16, K.bkind, K.bscale,
+                       K.nx, K.ny, K.nz, inv_delta2);
 }
 
 __global__ __launch_bounds__(256) void prolongate_kernel(const float *__restrict__ cp, float *__restrict__ fp,
@@ -178,14 +182,15 @@ constexpr int kCoarseThreads = 1024;
 constexpr int kCoarseLdsCells = 2048;
 
 __global__ __launch_bounds__(kCoarseThreads) void coarse_smooth_kernel(
-    float *__restrict__ gp, const float *__restrict__ gb, const float *__restrict__ gmask,
+    float *__restrict__ gp, const float *__restrict__ gb, const uint16_t *__restrict__ gb16, int bkind, float bscale,
+    const float *__restrict__ gmask,
     const float *__restrict__ gdiag, const float *__restrict__ gwx, const float *__restrict__ gwy,
     const float *__restrict__ gwz, float delta2, int iterations, int nx, int ny, int nz) {
     __shared__ float sp[kCoarseLdsCells], sb[kCoarseLdsCells], sm[kCoarseLdsCells], sd[kCoarseLdsCells];
     __shared__ float sx_[kCoarseLdsCells], sy_[kCoarseLdsCells], sz_[kCoarseLdsCells];
     const int C = nx * ny * nz;
     for (int q = threadIdx.x; q < C; q += kCoarseThreads) {
-        sp[q] = gp[q]; sb[q] = gb[q]; sm[q] = gmask[q]; sd[q] = gdiag[q];
+        sp[q] = gp[q]; sb[q] = load_b(gb, gb16, bkind, bscale, q); sm[q] = gmask[q]; sd[q] = gdiag[q];
         sx_[q] = gwx[q]; sy_[q] = gwy[q]; sz_[q] = gwz[q];
     }
     __syncthreads();
@@ -213,8 +218,125 @@ void launch_coarse_smooth(const Level &L, float delta, int iterations, hipStream
             for (int parity = 0; parity < 2; ++parity) launch_rbgs(L, delta, parity, s);
         return;
     }
-    hipLaunchKernelGGL(coarse_smooth_kernel, dim3(1), dim3(kCoarseThreads), 0, s, L.p, L.b, L.mask, L.diag, L.wx, L.wy,
-                       L.wz, delta * delta, iterations, L.nx, L.ny, L.nz);
+    hipLaunchKernelGGL(coarse_smooth_kernel, dim3(1), dim3(kCoarseThreads), 0, s, L.p, L.b, L.b16, L.bkind, L.bscale,
+                       L.mask, L.diag, L.wx, L.wy, L.wz, delta * delta, iterations, L.nx, L.ny, L.nz);
+}
+
+// coarse_smooth in FP64 for the mixed-precision solve: p is widened to double
+// in LDS, b (from its 16-bit storage, exact in float) and the operator (small
+// integers) stay float in LDS and are widened at use.  32 B per cell.
+constexpr int kCoarse64Cells = 2000;
+
+__global__ __launch_bounds__(kCoarseThreads) void coarse_smooth_f64_kernel(
+    float *__restrict__ gp, const float *__restrict__ gb, const uint16_t *__restrict__ gb16, int bkind, float bscale,
+    const float *__restrict__ gmask, const float *__restrict__ gdiag, const float *__restrict__ gwx,
+    const float *__restrict__ gwy, const float *__restrict__ gwz, double dd, int iterations, int nx, int ny, int nz) {
+    __shared__ double sp[kCoarse64Cells];
+    __shared__ float sb[kCoarse64Cells], sm[kCoarse64Cells], sd[kCoarse64Cells];
+    __shared__ float sx_[kCoarse64Cells], sy_[kCoarse64Cells], sz_[kCoarse64Cells];
+    const int C = nx * ny * nz;
+    for (int q = threadIdx.x; q < C; q += kCoarseThreads) {
+        sp[q] = gp[q]; sb[q] = load_b(gb, gb16, bkind, bscale, q); sm[q] = gmask[q]; sd[q] = gdiag[q];
+        sx_[q] = gwx[q]; sy_[q] = gwy[q]; sz_[q] = gwz[q];
+    }
+    __syncthreads();
+    const int sx = ny * nz, sy = nz;
+    for (int it = 0; it < iterations; ++it)
+        for (int parity = 0; parity < 2; ++parity) {
+            for (int q = threadIdx.x; q < C; q += kCoarseThreads) {
+                int i, j, k;
+                coords(q, ny, nz, i, j, k);
+                if (!interior(i, j, k, nx, ny, nz) || sm[q] <= 0.0f || ((i + j + k) & 1) != parity) continue;
+                const double d = sd[q];
+                if (!(d > 0.0)) continue;
+                double t = (double)sx_[q - sx] * sp[q - sx];
+                t = fma((double)sx_[q], sp[q + sx], t);
+                t = fma((double)sy_[q], sp[q + sy], t);
+                t = fma((double)sy_[q - sy], sp[q - sy], t);
+                t = fma((double)sz_[q], sp[q + 1], t);
+                t = fma((double)sz_[q - 1], sp[q - 1], t);
+                sp[q] = fma(-dd, (double)sb[q], t) / d;
+            }
+            __syncthreads();
+        }
+    for (int q = threadIdx.x; q < C; q += kCoarseThreads) gp[q] = (float)sp[q];
+}
+
+void launch_coarse_smooth_f64(const Level &L, double delta, int iterations, hipStream_t s) {
+    hipLaunchKernelGGL(coarse_smooth_f64_kernel, dim3(1), dim3(kCoarseThreads), 0, s, L.p, L.b, L.b16, L.bkind,
+                       L.bscale, L.mask, L.diag, L.wx, L.wy, L.wz, delta * delta, iterations, L.nx, L.ny, L.nz);
+}
+
+int coarse_f64_capacity() { return kCoarse64Cells; }
+
+// ---- FP64 iterative-refinement update -----------------------------------------------
+// p_new = p_old + e on interior fluid cells (0 elsewhere), r = b - A p_new in
+// FP64 with left-to-right sums over the non-zero weights, r32 = float(r), and
+// per-block partial sums of r^2 and b^2.  With from32 the iterate is the FP32
+// pressure (start of a solve).  Neighbour values are recomputed from p_old
+// and e, so p_new goes to a separate buffer.
+constexpr int kNormThreadsIR = 256;
+constexpr int kNormBlocksIR = 2048;
+
+__global__ __launch_bounds__(kNormThreadsIR) void ir_update_residual_kernel(
+    int from32, const float *__restrict__ p32, const double *__restrict__ pin, double *__restrict__ pout,
+    float *__restrict__ r32, const float *__restrict__ b, const float *__restrict__ mask,
+    const float *__restrict__ diag, const float *__restrict__ wx, const float *__restrict__ wy,
+    const float *__restrict__ wz, double inv_h2, int nx, int ny, int nz, double *__restrict__ partials) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    const int64_t sx = (int64_t)ny * nz, sy = nz;
+    double rr = 0.0, bb = 0.0;
+    for (int64_t q = (int64_t)blockIdx.x * kNormThreadsIR + threadIdx.x; q < C;
+         q += (int64_t)gridDim.x * kNormThreadsIR) {
+        int i, j, k;
+        coords(q, ny, nz, i, j, k);
+        const bool fluid = interior(i, j, k, nx, ny, nz) && mask[q] > 0.0f;
+        double r = 0.0;
+        const float bq = b[q];
+        if (fluid) {
+            auto pnew = [&](int64_t x) -> double { return from32 ? (double)p32[x] : pin[x] + (double)p32[x]; };
+            const double pc = pnew(q);
+            double s = 0.0;
+            if (wx[q] != 0.0f) s = s + (double)wx[q] * pnew(q + sx);
+            if (wx[q - sx] != 0.0f) s = s + (double)wx[q - sx] * pnew(q - sx);
+            if (wy[q] != 0.0f) s = s + (double)wy[q] * pnew(q + sy);
+            if (wy[q - sy] != 0.0f) s = s + (double)wy[q - sy] * pnew(q - sy);
+            if (wz[q] != 0.0f) s = s + (double)wz[q] * pnew(q + 1);
+            if (wz[q - 1] != 0.0f) s = s + (double)wz[q - 1] * pnew(q - 1);
+            pout[q] = pc;
+            r = (double)bq - (s - (double)diag[q] * pc) * inv_h2;
+        } else {
+            pout[q] = 0.0;
+        }
+        r32[q] = (float)r;
+        rr += r * r;
+        bb += (double)bq * (double)bq;
+    }
+    __shared__ double s_r[kNormThreadsIR], s_b[kNormThreadsIR];
+    s_r[threadIdx.x] = rr;
+    s_b[threadIdx.x] = bb;
+    __syncthreads();
+    for (int off = kNormThreadsIR / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            s_r[threadIdx.x] += s_r[threadIdx.x + off];
+            s_b[threadIdx.x] += s_b[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s_r[0];
+        partials[kNormBlocksIR + blockIdx.x] = s_b[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void narrow_kernel(const double *__restrict__ p64, float *__restrict__ p32,
+                                                     int64_t C) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q < C) p32[q] = (float)p64[q];
+}
+
+void launch_widen_narrow(const double *p64, float *p32, int64_t C, hipStream_t s) {
+    hipLaunchKernelGGL(narrow_kernel, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, s, p64, p32, C);
 }
 
 // ---- FP64 residual norm -------------------------------------------------------------
@@ -285,6 +407,16 @@ __global__ __launch_bounds__(kNormThreads) void reduce_partials_kernel(const dou
         out[0] = s_r[0];
         out[1] = s_b[0];
     }
+}
+
+void launch_ir_update_residual(const Level &L, int from32, const float *p32, const double *pin, double *pout,
+                               float *r32, const float *b, double inv_h2, double *partials, double *out2,
+                               hipStream_t s) {
+    int blocks = (int)((L.C + kNormThreadsIR - 1) / kNormThreadsIR);
+    if (blocks > kNormBlocksIR) blocks = kNormBlocksIR;
+    hipLaunchKernelGGL(ir_update_residual_kernel, dim3(blocks), dim3(kNormThreadsIR), 0, s, from32, p32, pin, pout, r32,
+                       b, L.mask, L.diag, L.wx, L.wy, L.wz, inv_h2, L.nx, L.ny, L.nz, partials);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kNormThreads), 0, s, partials, blocks, out2);
 }
 
 void launch_residual_norm(const Level &L, double inv_h2, double *partials, double *out2, hipStream_t s) {

@@ -36,6 +36,8 @@
 #include <string.h>
 #include <time.h>
 
+#include <immintrin.h>
+
 #include "../include/cliquid.h"
 
 #define MAX_LEVELS 16
@@ -45,7 +47,40 @@ typedef struct level_t {
     long long C;
     float *p, *b, *mask, *diag, *wx, *wy, *wz;
     int owns;
+    /* 16-bit storage of b used by the mixed-precision solve on levels >= 1:
+     * bkind 0 = float b, 1 = fp16, 2 = bf16; the stored value is
+     * b / bscale with bscale a power of two, so scaling is exact. */
+    uint16_t *b16;
+    int bkind;
+    float bscale;
 } level_t;
+
+/* IEEE round-to-nearest-even conversions (F16C for fp16). */
+static inline uint16_t f32_to_f16(float v) { return (uint16_t)_cvtss_sh(v, _MM_FROUND_TO_NEAREST_INT); }
+static inline float f16_to_f32(uint16_t h) { return _cvtsh_ss(h); }
+static inline uint16_t f32_to_bf16(float v) {
+    uint32_t x;
+    memcpy(&x, &v, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u); /* quiet NaN */
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return (uint16_t)(x >> 16);
+}
+static inline float bf16_to_f32(uint16_t h) {
+    uint32_t x = (uint32_t)h << 16;
+    float v;
+    memcpy(&v, &x, 4);
+    return v;
+}
+static inline float load_b(const level_t *L, long long i) {
+    if (L->bkind == 0) return L->b[i];
+    const float v = L->bkind == 1 ? f16_to_f32(L->b16[i]) : bf16_to_f32(L->b16[i]);
+    return v * L->bscale;
+}
+static inline void store_b(level_t *L, long long i, float v) {
+    if (L->bkind == 0) { L->b[i] = v; return; }
+    const float scaled = v * (1.0f / L->bscale);
+    L->b16[i] = L->bkind == 1 ? f32_to_f16(scaled) : f32_to_bf16(scaled);
+}
 
 struct clq_ctx {
     int nx, ny, nz;
@@ -59,6 +94,9 @@ struct clq_ctx {
     int nlevels;
     level_t lv[MAX_LEVELS];
     double stage_ms[CLQ_STAGE_COUNT];
+    /* mixed-precision solve state, allocated on first use */
+    double *p64a, *p64b;
+    float *r32;
 };
 
 static __thread char g_error[512];
@@ -143,7 +181,9 @@ int clq_destroy(clq_ctx *c) {
     for (int l = 1; l < c->nlevels; ++l) {
         level_t *L = &c->lv[l];
         free(L->p); free(L->b); free(L->mask); free(L->diag); free(L->wx); free(L->wy); free(L->wz);
+        free(L->b16);
     }
+    free(c->p64a); free(c->p64b); free(c->r32);
     free(c);
     return CLQ_OK;
 }
@@ -634,7 +674,7 @@ static void smooth_level(clq_ctx *c, level_t *L, int iterations) {
                     continue;
                 }
                 const float d = L->diag[index];
-                if (d > 0.0f) L->p[index] = fmaf(-dd, L->b[index], wsum(L, L->p, i, j, k)) / d; /* :227 fmuladd */
+                if (d > 0.0f) L->p[index] = fmaf(-dd, load_b(L, index), wsum(L, L->p, i, j, k)) / d; /* :227 fmuladd */
             }
         }
 }
@@ -651,7 +691,7 @@ static inline float residual_at(const level_t *L, int i, int j, int k, float inv
     float ap = 0.0f;
     if (interior(i, j, k, L->nx, L->ny, L->nz) && L->mask[idx] > 0.0f)
         ap = fmaf(-L->diag[idx], L->p[idx], wsum(L, L->p, i, j, k)) * inv_delta2; /* :41 fmuladd */
-    return L->b[idx] - ap;
+    return load_b(L, idx) - ap;
 }
 
 /* restrict_residual (multigrid.cl:90-180). */
@@ -663,7 +703,7 @@ int clq_restrict(clq_ctx *c, int32_t level) {
     const float inv_delta2 = 1.0f / (delta * delta);
 #pragma omp parallel for schedule(static)
     for (long long ci = 0; ci < K->C; ++ci) {
-        K->p[ci] = 0.0f; K->b[ci] = 0.0f; K->mask[ci] = 0.0f; K->diag[ci] = 0.0f;
+        K->p[ci] = 0.0f; store_b(K, ci, 0.0f); K->mask[ci] = 0.0f; K->diag[ci] = 0.0f;
         K->wx[ci] = 0.0f; K->wy[ci] = 0.0f; K->wz[ci] = 0.0f;
         int a, b, d;
         coords(ci, K->ny, K->nz, &a, &b, &d);
@@ -687,7 +727,7 @@ int clq_restrict(clq_ctx *c, int32_t level) {
                     if (wz > 0.0f) { if (dk == 0) diag_sum = fmaf(-2.0f, wz, diag_sum); else cz = cz + wz; }
                 }
         if (active > 0) {
-            K->b[ci] = rhs_sum; K->mask[ci] = 1.0f; K->diag[ci] = diag_sum;
+            store_b(K, ci, rhs_sum); K->mask[ci] = 1.0f; K->diag[ci] = diag_sum;
             K->wx[ci] = cx; K->wy[ci] = cy; K->wz[ci] = cz;
         }
     }
@@ -728,7 +768,7 @@ int clq_coarse_smooth(clq_ctx *c, int32_t iterations) {
                 coords(index, ny, nz, &i, &j, &k);
                 if (!interior(i, j, k, nx, ny, nz) || L->mask[index] <= 0.0f || ((i + j + k) & 1) != parity) continue;
                 const float d = L->diag[index];
-                if (d > 0.0f) L->p[index] = fmaf(-delta2, L->b[index], wsum(L, L->p, i, j, k)) / d; /* :257 */
+                if (d > 0.0f) L->p[index] = fmaf(-delta2, load_b(L, index), wsum(L, L->p, i, j, k)) / d; /* :257 */
             }
     return CLQ_OK;
 }
@@ -810,13 +850,153 @@ int clq_residual_norm(clq_ctx *c, double *residual, double *rhs_norm) {
     return CLQ_OK;
 }
 
+/* ---- mixed-precision iterative refinement (no reference counterpart) ------- */
+/* coarse_smooth in FP64: the coarsest level's p, b and operator are widened
+ * to double, swept with the same red-black order, and p is rounded back. */
+static void coarse_smooth_f64(clq_ctx *c, int iterations) {
+    level_t *L = &c->lv[c->nlevels - 1];
+    const int nx = L->nx, ny = L->ny, nz = L->nz;
+    const long long sx = (long long)ny * nz, sy = nz;
+    const double delta = (double)(float)c->dx;
+    const double dd = delta * delta;
+    double *p = malloc((size_t)L->C * sizeof(double));
+    for (long long q = 0; q < L->C; ++q) p[q] = L->p[q];
+    for (int it = 0; it < iterations; ++it)
+        for (int parity = 0; parity < 2; ++parity)
+            for (long long q = 0; q < L->C; ++q) {
+                int i, j, k;
+                coords(q, ny, nz, &i, &j, &k);
+                if (!interior(i, j, k, nx, ny, nz) || L->mask[q] <= 0.0f || ((i + j + k) & 1) != parity) continue;
+                const double d = L->diag[q];
+                if (!(d > 0.0)) continue;
+                double s = (double)L->wx[q - sx] * p[q - sx];
+                s = fma((double)L->wx[q], p[q + sx], s);
+                s = fma((double)L->wy[q], p[q + sy], s);
+                s = fma((double)L->wy[q - sy], p[q - sy], s);
+                s = fma((double)L->wz[q], p[q + 1], s);
+                s = fma((double)L->wz[q - 1], p[q - 1], s);
+                p[q] = fma(-dd, (double)load_b(L, q), s) / d;
+            }
+    for (long long q = 0; q < L->C; ++q) L->p[q] = (float)p[q];
+    free(p);
+}
+
+static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra) {
+    if (level == c->nlevels - 1) {
+        coarse_smooth_f64(c, pre + extra);
+        return;
+    }
+    smooth_level(c, &c->lv[level], pre);
+    clq_restrict(c, level);
+    w_cycle_mixed(c, level + 1, pre, post, extra);
+    w_cycle_mixed(c, level + 1, pre, post, extra);
+    clq_prolongate(c, level);
+    smooth_level(c, &c->lv[level], post);
+}
+
+/* One refinement update on level 0: p_new = p_old + e on interior fluid
+ * cells (0 elsewhere), r = b - A p_new in FP64 (left-to-right sums, no fused
+ * operations), r32 = float(r); returns ||r||^2 and ||b||^2.  With from32 the
+ * iterate is taken from the FP32 pressure instead (start of the solve). */
+static void ir_update_residual(clq_ctx *c, int from32, const double *pin, double *pout, double *rr_out,
+                               double *bb_out) {
+    const level_t *L = &c->lv[0];
+    const int nx = L->nx, ny = L->ny, nz = L->nz;
+    const long long sx = (long long)ny * nz, sy = nz;
+    const double h = (double)(float)c->dx;
+    const double inv_h2 = 1.0 / (h * h);
+    const float *e = c->p;
+    const float *b = c->rhs;
+    double rr = 0.0, bb = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rr, bb)
+    for (long long q = 0; q < L->C; ++q) {
+        int i, j, k;
+        coords(q, ny, nz, &i, &j, &k);
+        const int fluid = interior(i, j, k, nx, ny, nz) && L->mask[q] > 0.0f;
+        double r = 0.0;
+        if (fluid) {
+#define PNEW(x) (from32 ? (double)c->p[x] : pin[x] + (double)e[x])
+            const double pc = PNEW(q);
+            double s = 0.0;
+            if (L->wx[q] != 0.0f) s = s + (double)L->wx[q] * PNEW(q + sx);
+            if (L->wx[q - sx] != 0.0f) s = s + (double)L->wx[q - sx] * PNEW(q - sx);
+            if (L->wy[q] != 0.0f) s = s + (double)L->wy[q] * PNEW(q + sy);
+            if (L->wy[q - sy] != 0.0f) s = s + (double)L->wy[q - sy] * PNEW(q - sy);
+            if (L->wz[q] != 0.0f) s = s + (double)L->wz[q] * PNEW(q + 1);
+            if (L->wz[q - 1] != 0.0f) s = s + (double)L->wz[q - 1] * PNEW(q - 1);
+#undef PNEW
+            pout[q] = pc;
+            r = (double)b[q] - (s - (double)L->diag[q] * pc) * inv_h2;
+        } else {
+            pout[q] = 0.0;
+        }
+        c->r32[q] = (float)r;
+        rr += r * r;
+        bb += (double)b[q] * (double)b[q];
+    }
+    *rr_out = rr;
+    *bb_out = bb;
+}
+
+static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
+    if (!c->p64a) {
+        c->p64a = xcalloc(c->C, sizeof(double));
+        c->p64b = xcalloc(c->C, sizeof(double));
+        c->r32 = xcalloc(c->C, sizeof(float));
+        for (int l = 1; l < c->nlevels; ++l) c->lv[l].b16 = xcalloc(c->lv[l].C, sizeof(uint16_t));
+        if (!c->p64a || !c->p64b || !c->r32) return fail(CLQ_ERR_NOMEM, "mixed-precision buffers");
+    }
+    double rr, bb;
+    ir_update_residual(c, 1, NULL, c->p64a, &rr, &bb);
+    double r = sqrt(rr);
+    const double bn = sqrt(bb);
+    s->initial_residual = r;
+    s->rhs_norm = bn;
+    const int kind = sp->coarse_dtype == 1 ? 2 : 1;
+    level_t *L0 = &c->lv[0];
+    for (int cyc = 0; cyc < sp->max_cycles && r > sp->tolerance * bn; ++cyc) {
+        /* per-level power-of-two scale: coarse b grows ~8x per level from the
+         * RMS of the fine residual */
+        const int es = (int)lround(log2(r / sqrt((double)c->C) + 1e-300));
+        for (int l = 1; l < c->nlevels; ++l) {
+            c->lv[l].bkind = kind;
+            c->lv[l].bscale = ldexpf(1.0f, 3 * l + es);
+        }
+        L0->b = c->r32;
+        memset(c->p, 0, (size_t)c->C * sizeof(float));
+        w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+        L0->b = c->rhs;
+        ir_update_residual(c, 0, c->p64a, c->p64b, &rr, &bb);
+        double *t = c->p64a; c->p64a = c->p64b; c->p64b = t;
+        s->cycles++;
+        const double before = r;
+        r = sqrt(rr);
+        if (r > 0.95 * before) break;
+    }
+    for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
+    for (long long q = 0; q < c->C; ++q) c->p[q] = (float)c->p64a[q];
+    s->final_residual = r;
+    s->relative_residual = bn > 0 ? r / bn : 0.0;
+    return CLQ_OK;
+}
+
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
-    if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE)
-        return fail(CLQ_ERR_UNSUPPORTED, "the oracle implements solve modes 0 and 1 only (asked for %d)", sp->mode);
+    if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE &&
+        sp->mode != CLQ_SOLVE_MIXED_REFINEMENT)
+        return fail(CLQ_ERR_UNSUPPORTED, "solve mode %d is not available in the oracle", sp->mode);
     double t0 = now_ms();
     clq_solve_stats s;
     memset(&s, 0, sizeof s);
+    if (sp->mode == CLQ_SOLVE_MIXED_REFINEMENT) {
+        int rc = solve_mixed(c, sp, &s);
+        if (rc) return rc;
+        clq_neumann_bc(c);
+        s.solve_ms = now_ms() - t0;
+        c->stage_ms[CLQ_STAGE_PRESSURE_SOLVE] = s.solve_ms;
+        if (st) *st = s;
+        return CLQ_OK;
+    }
     double r0, bn;
     clq_residual_norm(c, &r0, &bn);
     s.initial_residual = r0;

@@ -241,3 +241,18 @@ def test_binning_full_size_properties(hip_lib):
     keys0 = bin_keys(scene.positions, scene.dx, scene.shape)
     order = np.argsort(keys0, kind="stable")
     assert np.array_equal(pos, scene.positions[order])
+
+
+@pytest.mark.parametrize("coarse_dtype", [0, 1], ids=["fp16", "bf16"])
+def test_mixed_refinement_bit_exact(oracle_lib, hip_lib, coarse_dtype):
+    scene = CASES[0]
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+    kw = dict(post_smooth=2, max_cycles=40, tolerance=1e-8, coarse_dtype=coarse_dtype)
+    so = o.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, **kw)
+    sh = h.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, **kw)
+    assert so["cycles"] == sh["cycles"]
+    assert sh["relative_residual"] <= 1e-8
+    assert abs(so["final_residual"] - sh["final_residual"]) <= 1e-9 * so["final_residual"]
+    assert_fields_equal(o, h, ("pressure",))

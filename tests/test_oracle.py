@@ -441,3 +441,26 @@ def test_voxelise_occupancy_and_density(oracle_lib):
     # a refined cell is occupied only inside an occupied coarse cell
     coarse = occ2.reshape(16, 2, 16, 2, 16, 2).max(axis=(1, 3, 5))
     assert not (coarse & ~occ.astype(bool)).any()
+
+
+@pytest.mark.parametrize("coarse_dtype", [0, 1], ids=["fp16", "bf16"])
+def test_mixed_refinement_reaches_1e8_against_fp64_direct_solve(oracle_lib, coarse_dtype):
+    """Mode 2 (FP64 refinement around the FP32/16-bit W-cycle) passes the FP32
+    floor: 1e-8 relative residual, FP64-checked, and the iterate equals the
+    scipy FP64 solution to that accuracy."""
+    ctx = _pressure_context(oracle_lib, 16, seed=2)
+    fp32 = _pressure_context(oracle_lib, 16, seed=2)
+    s32 = fp32.pressure_solve(runtime.SOLVE_FP32_TOLERANCE, post_smooth=2, max_cycles=80, tolerance=1e-10)
+    stats = ctx.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, post_smooth=2, max_cycles=80, tolerance=1e-8,
+                               coarse_dtype=coarse_dtype)
+    assert s32["relative_residual"] > 1e-7  # FP32 storage stalls above the target
+    assert stats["relative_residual"] <= 1e-8
+    mask = ctx.get_field("fluid_fraction")
+    h2 = np.float64(F32(ctx.dx)) ** 2
+    A, fluid, _ = operator_matrix(*[ctx.get_field(n) for n in ("diagonal", "weight_x", "weight_y", "weight_z")],
+                                  mask, h2)
+    b = ctx.get_field("rhs").astype(np.float64)[fluid]
+    exact = spla.spsolve(A.tocsc(), b)
+    p = ctx.get_field("pressure").astype(np.float64)[fluid]
+    # the returned FP32 pressure is the FP64 iterate rounded once
+    assert np.linalg.norm(p - exact) <= 1e-6 * np.linalg.norm(exact)
