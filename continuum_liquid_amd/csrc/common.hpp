@@ -68,12 +68,15 @@ __device__ __forceinline__ int64_t gidx(int i, int j, int k, int ny, int nz) {
 __device__ __forceinline__ bool interior(int i, int j, int k, int nx, int ny, int nz) {
     return i > 0 && j > 0 && k > 0 && i < nx - 1 && j < ny - 1 && k < nz - 1;
 }
+// Grids have fewer than 2^31 cells (checked at context creation), so the
+// index arithmetic runs in 32-bit unsigned integers.
 __device__ __forceinline__ void coords(int64_t index, int ny, int nz, int &i, int &j, int &k) {
-    const int64_t yz = (int64_t)ny * nz;
-    i = (int)(index / yz);
-    const int64_t r = index - (int64_t)i * yz;
-    j = (int)(r / nz);
-    k = (int)(r - (int64_t)j * nz);
+    const uint32_t q = (uint32_t)index, yz = (uint32_t)ny * (uint32_t)nz;
+    const uint32_t a = q / yz, r = q - a * yz;
+    const uint32_t b = r / (uint32_t)nz;
+    i = (int)a;
+    j = (int)b;
+    k = (int)(r - b * (uint32_t)nz);
 }
 // convert_int_rtn + clamp; NaN and out-of-range inputs clamp.
 __device__ __forceinline__ int floor_clamp(float t, int lo, int hi) {
@@ -119,6 +122,8 @@ void launch_build_finest_operator(const float *mask, float *diag, float *wx, flo
 void launch_neumann_bc(float *p, int nx, int ny, int nz, hipStream_t s);
 void launch_pressure_correction(float *u, float *v, float *w, const float *p, float dt_over_rho, float inv_dx,
                                 int nx, int ny, int nz, hipStream_t s);
+void launch_pressure_correction64(float *u, float *v, float *w, const double *p, double dt_over_rho, double inv_dx,
+                                  int nx, int ny, int nz, hipStream_t s);
 
 // multigrid.hip
 void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s);
@@ -128,12 +133,42 @@ void launch_prolongate(const Level &F, const Level &K, hipStream_t s);
 void launch_coarse_smooth(const Level &L, float delta, int iterations, hipStream_t s);
 void launch_coarse_smooth_f64(const Level &L, double delta, int iterations, hipStream_t s);
 int coarse_f64_capacity();
+// One work-group runs a list of multigrid phases on small levels.
+enum { kOpSmooth = 0, kOpRestrict = 1, kOpProlong = 2, kOpCoarse32 = 3, kOpCoarse64 = 4 };
+struct BottomOp {
+    int code, level, count;
+};
+constexpr int kMaxBottomOps = 96;
+struct BottomArgs {
+    Level lv[kMaxLevels];
+    BottomOp ops[kMaxBottomOps];
+    int nops;
+    float dd, inv_delta2;
+    double dd64;
+};
+void launch_wcycle_bottom(const BottomArgs &a, hipStream_t s);
 void launch_ir_update_residual(const Level &L, int from32, const float *p32, const double *pin, double *pout,
                                float *r32, const float *b, double inv_h2, double *partials, double *out2,
                                hipStream_t s);
 void launch_widen_narrow(const double *p64, float *p32, int64_t C, hipStream_t s);
 int residual_partials_count(int64_t C);
 void launch_residual_norm(const Level &L, double inv_h2, double *partials, double *out2, hipStream_t s);
+
+// fcg.hip
+int fcg_partials_count();
+void launch_fcg_init(const Level &L, const float *p32, const float *b, double inv_h2, double *x, double *r, float *r32,
+                     double *partials, double *s, hipStream_t st);
+void launch_fcg_dots(const Level &L, const float *z, const double *qv, const double *r, int first, double *partials,
+                     double *s, hipStream_t st);
+void launch_fcg_direction(const Level &L, const float *z, const double *d_old, double *d_new, double *qv, int first,
+                          double inv_h2, double *partials, double *s, hipStream_t st);
+void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const double *d, const double *qv,
+                       double *partials, double *s, hipStream_t st);
+
+// voxelise.hip
+int voxelise_max_scale();
+void launch_voxelise(const float *pos, int n, const int32_t *offsets, const int32_t *counts, int s, float sinv,
+                     double scale, int nx, int ny, int nz, uint8_t *occ, float *dens, hipStream_t st);
 
 // g2p.hip
 struct ParticleUpdate {

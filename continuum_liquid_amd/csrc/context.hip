@@ -59,6 +59,7 @@ struct clq_ctx {
     float *ff = nullptr, *p = nullptr, *rhs = nullptr, *diag = nullptr, *wx = nullptr, *wy = nullptr, *wz = nullptr;
     uint8_t *obstacle = nullptr;
     bool has_obstacle = false;
+    bool bins_current = false;  // counts/offsets describe the current particle order
     int nlevels = 0;
     Level lv[kMaxLevels];
     bool coarse_ops_valid = false;  // coarse operators match the current level-0 operator
@@ -66,6 +67,9 @@ struct clq_ctx {
     // mixed-precision solve state, allocated on first use
     double *p64a = nullptr, *p64b = nullptr;
     float *r32 = nullptr;
+    bool p64_valid = false;  // p64a holds the pressure of the last solve (modes 2-4)
+    // flexible-CG state, allocated on first use
+    double *r64 = nullptr, *dir[2] = {nullptr, nullptr}, *q64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
     hipEvent_t ev_start[CLQ_STAGE_COUNT] = {}, ev_stop[CLQ_STAGE_COUNT] = {};
     bool ev_used[CLQ_STAGE_COUNT] = {};
     // per-kernel-class event pairs while kernel timing is enabled
@@ -271,6 +275,7 @@ int clq_set_particles(clq_ctx *c, const float *pos, const float *vel, int64_t n)
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->n = n;
+    c->bins_current = false;
     return CLQ_OK;
 }
 
@@ -319,6 +324,7 @@ static void *field_ptr(clq_ctx *c, int f, int64_t *bytes) {
     case CLQ_FIELD_BIN_COUNTS: p = c->counts; break;
     case CLQ_FIELD_BIN_OFFSETS: p = c->offsets; break;
     case CLQ_FIELD_OBSTACLE: p = c->obstacle; fb = c->C; break;
+    case CLQ_FIELD_PRESSURE_FP64: p = c->p64a; fb = c->C * 8; break;
     default: return nullptr;
     }
     *bytes = fb;
@@ -333,6 +339,7 @@ int clq_set_field(clq_ctx *c, int32_t f, const void *host, int64_t bytes) {
     DeviceGuard g(c->device);
     HIP_TRY(hipMemcpyAsync(p, host, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (f == CLQ_FIELD_PRESSURE) c->p64_valid = false;
     if (f == CLQ_FIELD_OBSTACLE) {
         const uint8_t *o = (const uint8_t *)host;
         c->has_obstacle = std::any_of(o, o + bytes, [](uint8_t x) { return x != 0; });
@@ -417,6 +424,7 @@ int clq_bin_particles(clq_ctx *c) {
     stage_end(c, CLQ_STAGE_BIN);
     std::swap(c->pos, c->pos_work);  // particles.py:136-140
     std::swap(c->vel, c->vel_work);
+    c->bins_current = true;
     return check_launch("bin_particles");
 }
 
@@ -504,8 +512,50 @@ static void restrict_level(clq_ctx *c, int level) {
     launch_restrict_residual(c->lv[level], c->lv[level + 1], 1.0f / (delta * delta), c->stream);
 }
 
+// Levels with at most this many cells run inside one work-group launch
+// (wcycle_bottom_kernel) for the rest of the recursion below them.
+constexpr int64_t kBottomCells = 65536;
+
+static void bottom_ops(const clq_ctx *c, int level, int pre, int post, int extra, bool f64, BottomArgs &a) {
+    if (level == c->nlevels - 1) {
+        a.ops[a.nops++] = BottomOp{f64 ? kOpCoarse64 : kOpCoarse32, level, pre + extra};
+        return;
+    }
+    a.ops[a.nops++] = BottomOp{kOpSmooth, level, pre};
+    a.ops[a.nops++] = BottomOp{kOpRestrict, level, 0};
+    bottom_ops(c, level + 1, pre, post, extra, f64, a);
+    bottom_ops(c, level + 1, pre, post, extra, f64, a);
+    a.ops[a.nops++] = BottomOp{kOpProlong, level, 0};
+    a.ops[a.nops++] = BottomOp{kOpSmooth, level, post};
+}
+
+static int bottom_ops_count(int depth) { return depth == 0 ? 1 : 4 + 2 * bottom_ops_count(depth - 1); }
+
+// First level >= 1 whose remaining recursion runs in one work-group.
+static int bottom_level(const clq_ctx *c) {
+    for (int l = 1; l < c->nlevels; ++l)
+        if (c->lv[l].C <= kBottomCells && bottom_ops_count(c->nlevels - 1 - l) <= kMaxBottomOps) return l;
+    return c->nlevels;  // none
+}
+
+static void launch_bottom(clq_ctx *c, int level, int pre, int post, int extra, bool f64) {
+    BottomArgs a;
+    for (int l = 0; l < c->nlevels; ++l) a.lv[l] = c->lv[l];
+    a.nops = 0;
+    bottom_ops(c, level, pre, post, extra, f64, a);
+    const float delta = (float)c->dx;
+    a.dd = delta * delta;
+    a.inv_delta2 = 1.0f / (delta * delta);
+    a.dd64 = (double)delta * (double)delta;
+    launch_wcycle_bottom(a, c->stream);
+}
+
 // multigrid.py:53-187
 static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra) {
+    if (level > 0 && level == bottom_level(c)) {
+        launch_bottom(c, level, pre, post, extra, false);
+        return;
+    }
     if (level == c->nlevels - 1) {
         launch_coarse_smooth(c->lv[level], (float)c->dx, pre + extra, c->stream);
         return;
@@ -583,6 +633,10 @@ int clq_residual_norm(clq_ctx *c, double *residual, double *rhs_norm) {
 
 // ---- mixed-precision iterative refinement -------------------------------------------
 static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra) {
+    if (level > 0 && level == bottom_level(c)) {
+        launch_bottom(c, level, pre, post, extra, true);
+        return;
+    }
     if (level == c->nlevels - 1) {
         const Level &L = c->lv[level];
         if (L.C <= coarse_f64_capacity()) launch_coarse_smooth_f64(L, (double)(float)c->dx, pre + extra, c->stream);
@@ -611,21 +665,10 @@ static int ir_residual(clq_ctx *c, int from32, const double *pin, double *pout, 
 // FP64 iterate p64 and residual, W-cycle on A e = r with FP32 smoothing and
 // 16-bit coarse b (power-of-two scale 2^(3l + es) per level from the RMS of
 // the fine residual), FP64 coarsest smoother, p64 += e.
+static int ensure_mixed_buffers(clq_ctx *c);
+
 static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
-    if (!c->p64a) {
-        void *q;
-        if (int rc = c->alloc(&q, (size_t)c->C * 8)) return rc;
-        c->p64a = (double *)q;
-        if (int rc = c->alloc(&q, (size_t)c->C * 8)) return rc;
-        c->p64b = (double *)q;
-        if (int rc = c->alloc(&q, (size_t)c->C * 4)) return rc;
-        c->r32 = (float *)q;
-        for (int l = 1; l < c->nlevels; ++l) {
-            if (int rc = c->alloc(&q, (size_t)c->lv[l].C * 2)) return rc;
-            c->lv[l].b16 = (uint16_t *)q;
-            HIP_TRY(hipMemsetAsync(q, 0, (size_t)c->lv[l].C * 2, c->stream));
-        }
-    }
+    if (int rc = ensure_mixed_buffers(c)) return rc;
     double r = 0.0, bn = 0.0;
     if (int rc = ir_residual(c, 1, nullptr, c->p64a, &r, &bn)) return rc;
     s->initial_residual = r;
@@ -658,20 +701,119 @@ static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     return check_launch("solve_mixed");
 }
 
+// ---- flexible CG ---------------------------------------------------------------------------
+static int alloc_zeroed(clq_ctx *c, void **q, size_t bytes) {
+    if (int rc = c->alloc(q, bytes)) return rc;
+    HIP_TRY(hipMemsetAsync(*q, 0, bytes, c->stream));
+    return CLQ_OK;
+}
+
+static int ensure_mixed_buffers(clq_ctx *c) {
+    if (c->p64a) return CLQ_OK;
+    void *q;
+    if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
+    c->p64a = (double *)q;
+    if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
+    c->p64b = (double *)q;
+    if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 4)) return rc;
+    c->r32 = (float *)q;
+    for (int l = 1; l < c->nlevels; ++l) {
+        if (int rc = alloc_zeroed(c, &q, (size_t)c->lv[l].C * 2)) return rc;
+        c->lv[l].b16 = (uint16_t *)q;
+    }
+    return CLQ_OK;
+}
+
+// z = M r: the mixed-precision W-cycle on A e = r32 from e = 0 (e is the
+// level-0 pressure array), 16-bit coarse b scaled from the residual norm.
+static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rnorm) {
+    const int kind = sp->coarse_dtype == 1 ? 2 : 1;
+    const int es = (int)lround(log2(rnorm / sqrt((double)c->C) + 1e-300));
+    for (int l = 1; l < c->nlevels; ++l) {
+        c->lv[l].bkind = kind;
+        c->lv[l].bscale = ldexpf(1.0f, 3 * l + es);
+    }
+    Level &L0 = c->lv[0];
+    L0.b = c->r32;
+    HIP_TRY(hipMemsetAsync(c->p, 0, (size_t)c->C * 4, c->stream));
+    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+    L0.b = c->rhs;
+    for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
+    return CLQ_OK;
+}
+
+static int read_scalars(clq_ctx *c) {
+    HIP_TRY(hipMemcpyAsync(c->norm_host, c->fcg_s + 6, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CLQ_OK;
+}
+
+static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
+    if (int rc = ensure_mixed_buffers(c)) return rc;
+    if (!c->r64) {
+        void *q;
+        if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
+        c->r64 = (double *)q;
+        for (int k = 0; k < 2; ++k) {
+            if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
+            c->dir[k] = (double *)q;
+        }
+        if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
+        c->q64 = (double *)q;
+        if (int rc = alloc_zeroed(c, &q, 16 * 8)) return rc;
+        c->fcg_s = (double *)q;
+        if (int rc = alloc_zeroed(c, &q, (size_t)fcg_partials_count() * 8)) return rc;
+        c->fcg_partials = (double *)q;
+    }
+    const Level &L0 = c->lv[0];
+    const double h = (double)(float)c->dx;
+    const double inv_h2 = 1.0 / (h * h);
+    launch_fcg_init(L0, c->p, c->rhs, inv_h2, c->p64a, c->r64, c->r32, c->fcg_partials, c->fcg_s, c->stream);
+    if (int rc = read_scalars(c)) return rc;
+    double r = sqrt(c->norm_host[0]);
+    const double bn = sqrt(c->norm_host[1]);
+    s->initial_residual = r;
+    s->rhs_norm = bn;
+    int cur = 0;
+    if (r > sp->tolerance * bn && sp->max_cycles > 0) {
+        if (int rc = precondition_wcycle(c, sp, r)) return rc;
+        s->cycles++;
+        launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, c->fcg_partials, c->fcg_s, c->stream);
+        for (int it = 0;; ++it) {
+            launch_fcg_direction(L0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, c->fcg_partials,
+                                 c->fcg_s, c->stream);
+            cur ^= 1;
+            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, c->fcg_partials, c->fcg_s, c->stream);
+            if (int rc = read_scalars(c)) return rc;
+            r = sqrt(c->norm_host[0]);
+            if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
+            if (int rc = precondition_wcycle(c, sp, r)) return rc;
+            s->cycles++;
+            launch_fcg_dots(L0, c->p, c->q64, c->r64, 0, c->fcg_partials, c->fcg_s, c->stream);
+        }
+    }
+    launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
+    s->final_residual = r;
+    s->relative_residual = bn > 0 ? r / bn : 0.0;
+    return check_launch("solve_fcg");
+}
+
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
     if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0)
         return fail(CLQ_ERR_INVALID, "negative sweep or cycle count");
     if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE &&
-        sp->mode != CLQ_SOLVE_MIXED_REFINEMENT)
+        sp->mode != CLQ_SOLVE_MIXED_REFINEMENT && sp->mode != CLQ_SOLVE_FCG)
         return fail(CLQ_ERR_UNSUPPORTED, "solve mode %d is not available in this build", sp->mode);
     DeviceGuard g(c->device);
     clq_solve_stats s;
     memset(&s, 0, sizeof s);
+    c->p64_valid = false;
     stage_begin(c, CLQ_STAGE_PRESSURE_SOLVE);
     ensure_coarse_operators(c);
-    if (sp->mode == CLQ_SOLVE_MIXED_REFINEMENT) {
-        if (int rc = solve_mixed(c, sp, &s)) return rc;
+    if (sp->mode == CLQ_SOLVE_MIXED_REFINEMENT || sp->mode == CLQ_SOLVE_FCG) {
+        if (int rc = sp->mode == CLQ_SOLVE_FCG ? solve_fcg(c, sp, &s) : solve_mixed(c, sp, &s)) return rc;
+        c->p64_valid = true;
         launch_neumann_bc(c->p, c->nx, c->ny, c->nz, c->stream);
         stage_end(c, CLQ_STAGE_PRESSURE_SOLVE);
         HIP_TRY(hipEventSynchronize(c->ev_stop[CLQ_STAGE_PRESSURE_SOLVE]));
@@ -713,8 +855,12 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
 int clq_pressure_correction(clq_ctx *c, double rho, double dt) {
     if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
     DeviceGuard g(c->device);
-    launch_pressure_correction(c->u, c->v, c->w, c->p, (float)(dt / rho), (float)(1.0 / c->dx), c->nx, c->ny, c->nz,
-                               c->stream);
+    if (c->p64_valid)
+        launch_pressure_correction64(c->u, c->v, c->w, c->p64a, (double)(float)(dt / rho),
+                                     (double)(float)(1.0 / c->dx), c->nx, c->ny, c->nz, c->stream);
+    else
+        launch_pressure_correction(c->u, c->v, c->w, c->p, (float)(dt / rho), (float)(1.0 / c->dx), c->nx, c->ny,
+                                   c->nz, c->stream);
     return check_launch("pressure_correction");
 }
 
@@ -736,6 +882,7 @@ static ParticleUpdate make_update(clq_ctx *c, int flags, double flip_ratio, doub
 
 static int particle_update(clq_ctx *c, int flags, double flip_ratio, double dt, const char *what) {
     DeviceGuard g(c->device);
+    if (flags & (kAdvect | kDomainBC | kObstacleBC)) c->bins_current = false;
     const ParticleUpdate pu = make_update(c, flags, flip_ratio, dt);
     launch_particle_update(c->pos, c->vel, c->prev, c->u, c->v, c->w, c->uo, c->vo, c->wo, c->obstacle, (int)c->n,
                            c->nx, c->ny, c->nz, pu, c->stream);
@@ -783,9 +930,38 @@ int clq_grid_to_particle_advect(clq_ctx *c, double flip_ratio, double dt) {
 }
 
 int clq_voxelise(clq_ctx *c, int32_t scale, double particles_per_cell, uint8_t *occupancy, float *density) {
-    (void)scale; (void)particles_per_cell; (void)occupancy; (void)density;
-    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
-    return fail(CLQ_ERR_UNSUPPORTED, "clq_voxelise is not available in this build");
+    if (!c || scale < 1 || !(particles_per_cell > 0)) return fail(CLQ_ERR_INVALID, "bad voxelise arguments");
+    if (scale > voxelise_max_scale()) return fail(CLQ_ERR_UNSUPPORTED, "voxelise scale %d > %d", scale,
+                                                  voxelise_max_scale());
+    DeviceGuard g(c->device);
+    if (density && !c->bins_current)
+        if (int rc = clq_bin_particles(c)) return rc;  // the density gather walks the bins
+    const int64_t S = (int64_t)scale * scale * scale * c->C;
+    void *q = nullptr;
+    uint8_t *docc = nullptr;
+    float *ddens = nullptr;
+    // output staging lives only for this call (the outputs are host arrays)
+    if (occupancy) {
+        if (int rc = c->alloc(&q, (size_t)S)) return rc;
+        docc = (uint8_t *)q;
+        HIP_TRY(hipMemsetAsync(docc, 0, (size_t)S, c->stream));
+    }
+    if (density) {
+        if (int rc = c->alloc(&q, (size_t)S * 4)) { c->release(docc); return rc; }
+        ddens = (float *)q;
+    }
+    const float sinv = (float)scale * (float)(1.0 / c->dx);
+    launch_voxelise(c->pos, (int)c->n, c->offsets, c->counts, scale, sinv,
+                    (double)scale * scale * scale / particles_per_cell, c->nx, c->ny, c->nz, docc, ddens, c->stream);
+    int rc = check_launch("voxelise");
+    if (!rc && docc) rc = hipMemcpyAsync(occupancy, docc, (size_t)S, hipMemcpyDeviceToHost, c->stream) == hipSuccess
+                              ? CLQ_OK : fail(CLQ_ERR_DEVICE, "occupancy download failed");
+    if (!rc && ddens) rc = hipMemcpyAsync(density, ddens, (size_t)S * 4, hipMemcpyDeviceToHost, c->stream) == hipSuccess
+                               ? CLQ_OK : fail(CLQ_ERR_DEVICE, "density download failed");
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = fail(CLQ_ERR_DEVICE, "voxelise sync failed");
+    c->release(docc);
+    c->release(ddens);
+    return rc;
 }
 
 int clq_kernel_timing(clq_ctx *c, int32_t enable) {

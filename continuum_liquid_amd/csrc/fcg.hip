@@ -1,0 +1,207 @@
+// Flexible conjugate gradients in FP64 on the level-0 pressure system, with a
+// preconditioner applied by the caller (the mixed-precision W-cycle, or the
+// learned network followed by it).  No reference counterpart: the reference
+// runs a fixed number of stationary W-cycles (pressure_solve.py:61-83).
+//
+// Vectors over all C cells, FP64: x (iterate), r (residual), d (direction,
+// ping-pong), q = A d.  The preconditioned residual z arrives as FP32 in the
+// level-0 correction array e.  Scalars live in device memory so that alpha and
+// beta never round-trip through the host:
+//   s[0] rz = r.z   s[1] d.q   s[2] alpha   s[3] z.q   s[4] z.r   s[5] beta
+//   s[6] r.r        s[7] b.b
+// beta = -alpha (z_{k+1} . q_k) / (z_k . r_k), the flexible (Polak-Ribiere)
+// form written with q_k = -(r_{k+1} - r_k) / alpha, so r_k need not be kept.
+// Every pass writes all cells (0 outside the interior fluid cells), so stale
+// values from an earlier mask never leak in.
+#include "common.hpp"
+
+namespace clq {
+
+constexpr int kT = 256;
+constexpr int kB = 2048;
+
+__device__ __forceinline__ bool fluid_cell(const Level &L, int64_t q) {
+    int i, j, k;
+    coords(q, L.ny, L.nz, i, j, k);
+    return interior(i, j, k, L.nx, L.ny, L.nz) && L.mask[q] > 0.0f;
+}
+
+// A v at a fluid cell, FP64, left-to-right over the non-zero weights (the
+// order of the refinement residual).
+template <typename F>
+__device__ __forceinline__ double apply_a(const Level &L, int64_t q, F v, double inv_h2) {
+    const int64_t sx = (int64_t)L.ny * L.nz, sy = L.nz;
+    double s = 0.0;
+    if (L.wx[q] != 0.0f) s = s + (double)L.wx[q] * v(q + sx);
+    if (L.wx[q - sx] != 0.0f) s = s + (double)L.wx[q - sx] * v(q - sx);
+    if (L.wy[q] != 0.0f) s = s + (double)L.wy[q] * v(q + sy);
+    if (L.wy[q - sy] != 0.0f) s = s + (double)L.wy[q - sy] * v(q - sy);
+    if (L.wz[q] != 0.0f) s = s + (double)L.wz[q] * v(q + 1);
+    if (L.wz[q - 1] != 0.0f) s = s + (double)L.wz[q - 1] * v(q - 1);
+    return (s - (double)L.diag[q] * v(q)) * inv_h2;
+}
+
+template <int N>
+__device__ __forceinline__ void block_sums(double (&v)[N], double *__restrict__ partials) {
+    __shared__ double sh[N][kT];
+#pragma unroll
+    for (int a = 0; a < N; ++a) sh[a][threadIdx.x] = v[a];
+    __syncthreads();
+    for (int off = kT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+#pragma unroll
+            for (int a = 0; a < N; ++a) sh[a][threadIdx.x] += sh[a][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int a = 0; a < N; ++a) partials[a * kB + blockIdx.x] = sh[a][0];
+}
+
+// x = p32 on fluid cells, r = b - A x, r32 = float(r); partial r.r, b.b.
+__global__ __launch_bounds__(kT) void fcg_init_kernel(Level L, const float *__restrict__ p32,
+                                                     const float *__restrict__ b, double inv_h2, double *__restrict__ x,
+                                                     double *__restrict__ r, float *__restrict__ r32,
+                                                     double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    for (int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x; q < L.C; q += (int64_t)gridDim.x * kT) {
+        double xv = 0.0, rv = 0.0;
+        if (fluid_cell(L, q)) {
+            xv = (double)p32[q];
+            rv = (double)b[q] - apply_a(L, q, [&](int64_t t) { return (double)p32[t]; }, inv_h2);
+        }
+        x[q] = xv;
+        r[q] = rv;
+        r32[q] = (float)rv;
+        acc[0] += rv * rv;
+        acc[1] += (double)b[q] * (double)b[q];
+    }
+    block_sums<2>(acc, partials);
+}
+
+// d_new = z + beta d_old, q = A d_new; partial d.q.  first: beta = 0.
+__global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const float *__restrict__ z,
+                                                          const double *__restrict__ d_old, double *__restrict__ d_new,
+                                                          double *__restrict__ qv, const double *__restrict__ s,
+                                                          int first, double inv_h2, double *__restrict__ partials) {
+    const double beta = first ? 0.0 : s[5];
+    double acc[1] = {0.0};
+    for (int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x; q < L.C; q += (int64_t)gridDim.x * kT) {
+        double dv = 0.0, av = 0.0;
+        if (fluid_cell(L, q)) {
+            auto dn = [&](int64_t t) { return first ? (double)z[t] : (double)z[t] + beta * d_old[t]; };
+            dv = dn(q);
+            // neighbours outside the fluid carry zero weight, so dn() is only
+            // evaluated at fluid cells
+            av = apply_a(L, q, dn, inv_h2);
+        }
+        d_new[q] = dv;
+        qv[q] = av;
+        acc[0] += dv * av;
+    }
+    block_sums<1>(acc, partials);
+}
+
+// x += alpha d, r -= alpha q, r32 = float(r); partial r.r.
+__global__ __launch_bounds__(kT) void fcg_update_kernel(int64_t C, double *__restrict__ x, double *__restrict__ r,
+                                                       float *__restrict__ r32, const double *__restrict__ d,
+                                                       const double *__restrict__ qv, const double *__restrict__ s,
+                                                       double *__restrict__ partials) {
+    const double alpha = s[2];
+    double acc[1] = {0.0};
+    for (int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x; q < C; q += (int64_t)gridDim.x * kT) {
+        const double xv = x[q] + alpha * d[q];
+        const double rv = r[q] - alpha * qv[q];
+        x[q] = xv;
+        r[q] = rv;
+        r32[q] = (float)rv;
+        acc[0] += rv * rv;
+    }
+    block_sums<1>(acc, partials);
+}
+
+// partial z.q, z.r
+__global__ __launch_bounds__(kT) void fcg_dots_kernel(int64_t C, const float *__restrict__ z,
+                                                     const double *__restrict__ qv, const double *__restrict__ r,
+                                                     double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    for (int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x; q < C; q += (int64_t)gridDim.x * kT) {
+        const double zv = (double)z[q];
+        acc[0] += zv * qv[q];
+        acc[1] += zv * r[q];
+    }
+    block_sums<2>(acc, partials);
+}
+
+// Reduce `n` partial arrays of `blocks` entries and update the scalars.
+// stage 0: init (s6 = r.r, s7 = b.b); 1: d.q -> alpha; 2: r.r;
+// 3: z.q, z.r -> beta (unless first), rz.
+__global__ __launch_bounds__(kT) void fcg_scalars_kernel(const double *__restrict__ partials, int blocks, int stage,
+                                                        int first, double *__restrict__ s) {
+    __shared__ double sh[2][kT];
+    const int n = (stage == 0 || stage == 3) ? 2 : 1;
+    for (int a = 0; a < n; ++a) {
+        double v = 0.0;
+        for (int q = threadIdx.x; q < blocks; q += kT) v += partials[a * kB + q];
+        sh[a][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int off = kT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int a = 0; a < n; ++a) sh[a][threadIdx.x] += sh[a][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (stage == 0) {
+        s[6] = sh[0][0];
+        s[7] = sh[1][0];
+    } else if (stage == 1) {
+        s[1] = sh[0][0];
+        s[2] = s[1] != 0.0 ? s[0] / s[1] : 0.0;
+    } else if (stage == 2) {
+        s[6] = sh[0][0];
+    } else {
+        s[3] = sh[0][0];
+        s[4] = sh[1][0];
+        s[5] = (first || s[0] == 0.0) ? 0.0 : -s[2] * s[3] / s[0];
+        s[0] = s[4];
+    }
+}
+
+static int blocks_for(int64_t C) {
+    const int64_t b = (C + kT - 1) / kT;
+    return (int)(b < kB ? b : kB);
+}
+
+int fcg_partials_count() { return 2 * kB; }
+
+void launch_fcg_init(const Level &L, const float *p32, const float *b, double inv_h2, double *x, double *r, float *r32,
+                     double *partials, double *s, hipStream_t st) {
+    const int nb = blocks_for(L.C);
+    hipLaunchKernelGGL(fcg_init_kernel, dim3(nb), dim3(kT), 0, st, L, p32, b, inv_h2, x, r, r32, partials);
+    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 0, 0, s);
+}
+
+void launch_fcg_dots(const Level &L, const float *z, const double *qv, const double *r, int first, double *partials,
+                     double *s, hipStream_t st) {
+    const int nb = blocks_for(L.C);
+    hipLaunchKernelGGL(fcg_dots_kernel, dim3(nb), dim3(kT), 0, st, L.C, z, qv, r, partials);
+    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 3, first, s);
+}
+
+void launch_fcg_direction(const Level &L, const float *z, const double *d_old, double *d_new, double *qv, int first,
+                          double inv_h2, double *partials, double *s, hipStream_t st) {
+    const int nb = blocks_for(L.C);
+    hipLaunchKernelGGL(fcg_direction_kernel, dim3(nb), dim3(kT), 0, st, L, z, d_old, d_new, qv, s, first, inv_h2,
+                       partials);
+    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 1, 0, s);
+}
+
+void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const double *d, const double *qv,
+                       double *partials, double *s, hipStream_t st) {
+    const int nb = blocks_for(L.C);
+    hipLaunchKernelGGL(fcg_update_kernel, dim3(nb), dim3(kT), 0, st, L.C, x, r, r32, d, qv, s, partials);
+    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 2, 0, s);
+}
+
+}  // namespace clq

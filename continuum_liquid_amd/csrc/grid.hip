@@ -251,3 +251,35 @@ void launch_pressure_correction(float *u, float *v, float *w, const float *p, fl
 }
 
 }  // namespace clq
+
+namespace clq {
+
+// Pressure correction from the FP64 pressure of the refined solves: the
+// +1 neighbour index is clamped to the interior (the Neumann ghost value),
+// the gradient and update run in FP64, u is rounded once.
+__global__ __launch_bounds__(256) void pressure_correction64_kernel(float *__restrict__ u, float *__restrict__ v,
+                                                                    float *__restrict__ w, const double *__restrict__ p,
+                                                                    double dt_over_rho, double inv_dx, int nx, int ny,
+                                                                    int nz) {
+    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (index >= (int64_t)nx * ny * nz) return;
+    int i, j, k;
+    coords(index, ny, nz, i, j, k);
+    if (!interior(i, j, k, nx, ny, nz)) return;
+    const double centre = p[index];
+    const double gx = (p[gidx(min(i + 1, nx - 2), j, k, ny, nz)] - centre) * inv_dx;
+    const double gy = (p[gidx(i, min(j + 1, ny - 2), k, ny, nz)] - centre) * inv_dx;
+    const double gz = (p[gidx(i, j, min(k + 1, nz - 2), ny, nz)] - centre) * inv_dx;
+    u[index] = (float)((double)u[index] - dt_over_rho * gx);
+    v[index] = (float)((double)v[index] - dt_over_rho * gy);
+    w[index] = (float)((double)w[index] - dt_over_rho * gz);
+}
+
+void launch_pressure_correction64(float *u, float *v, float *w, const double *p, double dt_over_rho, double inv_dx,
+                                  int nx, int ny, int nz, hipStream_t s) {
+    const int64_t C = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(pressure_correction64_kernel, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, s, u, v, w, p,
+                       dt_over_rho, inv_dx, nx, ny, nz);
+}
+
+}  // namespace clq

@@ -35,40 +35,39 @@ __device__ __forceinline__ float wsum(const float *__restrict__ p, const float *
 // colour in row-major order: row (i, j), k = 2*kk + ((parity + i + j) & 1).
 // Non-fluid cells of that colour are zeroed, as rbgs_step zeroes every
 // non-fluid cell; after a full sweep both colours have been zeroed.
-__global__ __launch_bounds__(256) void rbgs_kernel(float *__restrict__ p, const float *__restrict__ b,
-                                                   const uint16_t *__restrict__ b16, int bkind, float bscale,
-                                                   const float *__restrict__ mask, const float *__restrict__ diag,
-                                                   const float *__restrict__ wx, const float *__restrict__ wy,
-                                                   const float *__restrict__ wz, float dd, int parity, int nx, int ny,
-                                                   int nz) {
-    const int half = (nz + 1) >> 1;
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t rows = (int64_t)nx * ny;
-    if (t >= rows * half) return;
-    const int64_t row = t / half;
-    const int kk = (int)(t - row * half);
-    const int i = (int)(row / ny), j = (int)(row - (int64_t)i * ny);
+__device__ __forceinline__ void rbgs_item(const Level &L, float dd, int parity, int64_t t) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const uint32_t half = (uint32_t)(nz + 1) >> 1;
+    const uint32_t row = (uint32_t)t / half;
+    const int kk = (int)((uint32_t)t - row * half);
+    const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
     const int k = 2 * kk + ((parity + i + j) & 1);
     if (k >= nz) return;
-    const int64_t idx = row * nz + k;
-    const float m = mask[idx];
-    if (m <= 0.0f) {
+    const int64_t idx = (int64_t)row * nz + k;
+    float *__restrict__ p = L.p;
+    if (L.mask[idx] <= 0.0f) {
         p[idx] = 0.0f;
         return;
     }
     if (!interior(i, j, k, nx, ny, nz)) return;
-    const float d = diag[idx];
+    const float d = L.diag[idx];
     if (d > 0.0f) {
         const int64_t sx = (int64_t)ny * nz, sy = nz;
-        p[idx] = fmaf(-dd, load_b(b, b16, bkind, bscale, idx), wsum(p, wx, wy, wz, idx, sx, sy)) / d;  // :227
+        p[idx] = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), wsum(p, L.wx, L.wy, L.wz, idx, sx, sy)) /
+                 d;  // multigrid.cl:227 (fmuladd)
     }
+}
+
+__global__ __launch_bounds__(256) void rbgs_kernel(Level L, float dd, int parity) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (int64_t)L.nx * L.ny * ((L.nz + 1) >> 1)) return;
+    rbgs_item(L, dd, parity, t);
 }
 
 void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s) {
     const int half = (L.nz + 1) >> 1;
     const int64_t work = (int64_t)L.nx * L.ny * half;
-    hipLaunchKernelGGL(rbgs_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L.p, L.b, L.b16, L.bkind,
-                       L.bscale, L.mask, L.diag, L.wx, L.wy, L.wz, delta * delta, parity, L.nx, L.ny, L.nz);
+    hipLaunchKernelGGL(rbgs_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L, delta * delta, parity);
 }
 
 __global__ __launch_bounds__(256) void coarse_operator_kernel(
@@ -122,152 +121,189 @@ __device__ __forceinline__ float residual_at(const float *__restrict__ p, float 
     return bv - ap;
 }
 
-__global__ __launch_bounds__(256) void restrict_kernel(const float *__restrict__ fp, const float *__restrict__ fb,
-                                                       const uint16_t *__restrict__ fb16, int fkind, float fscale,
-                                                       const float *__restrict__ fmask,
-                                                       const float *__restrict__ fdiag, const float *__restrict__ fwx,
-                                                       const float *__restrict__ fwy, const float *__restrict__ fwz,
-                                                       int fnx, int fny, int fnz, float *__restrict__ cp,
-                                                       float *__restrict__ cb, uint16_t *__restrict__ cb16, int ckind,
-                                                       float cscale, int cnx, int cny, int cnz, float inv_delta2) {
-    const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (ci >= (int64_t)cnx * cny * cnz) return;
+__device__ __forceinline__ void restrict_item(const Level &F, const Level &K, float inv_delta2, int64_t ci) {
     int a, b, c;
-    coords(ci, cny, cnz, a, b, c);
+    coords(ci, K.ny, K.nz, a, b, c);
     float rhs = 0.0f;
-    if (interior(a, b, c, cnx, cny, cnz)) {
-        const int64_t sx = (int64_t)fny * fnz, sy = fnz;
+    if (interior(a, b, c, K.nx, K.ny, K.nz)) {
+        const int64_t sx = (int64_t)F.ny * F.nz, sy = F.nz;
         const int f0 = 1 + 2 * (a - 1), f1 = 1 + 2 * (b - 1), f2 = 1 + 2 * (c - 1);
         for (int di = 0; di < 2; ++di)
             for (int dj = 0; dj < 2; ++dj)
                 for (int dk = 0; dk < 2; ++dk) {
                     const int i = f0 + di, j = f1 + dj, k = f2 + dk;
-                    if (!interior(i, j, k, fnx, fny, fnz)) continue;
-                    const int64_t idx = gidx(i, j, k, fny, fnz);
-                    if (fmask[idx] <= 0.0f) continue;
-                    const float bv = load_b(fb, fb16, fkind, fscale, idx);
-                    rhs = rhs + residual_at(fp, bv, fdiag, fwx, fwy, fwz, idx, sx, sy, inv_delta2);
+                    if (!interior(i, j, k, F.nx, F.ny, F.nz)) continue;
+                    const int64_t idx = gidx(i, j, k, F.ny, F.nz);
+                    if (F.mask[idx] <= 0.0f) continue;
+                    const float bv = load_b(F.b, F.b16, F.bkind, F.bscale, idx);
+                    rhs = rhs + residual_at(F.p, bv, F.diag, F.wx, F.wy, F.wz, idx, sx, sy, inv_delta2);
                 }
     }
-    cp[ci] = 0.0f;
-    store_b(cb, cb16, ckind, cscale, ci, rhs);
+    K.p[ci] = 0.0f;
+    store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
+}
+
+__global__ __launch_bounds__(256) void restrict_kernel(Level F, Level K, float inv_delta2) {
+    const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (ci >= K.C) return;
+    restrict_item(F, K, inv_delta2, ci);
 }
 
 void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, hipStream_t s) {
-    hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F.p, F.b, F.b16, F.bkind,
-                       F.bscale, F.mask, F.diag, F.wx, F.wy, F.wz, F.nx, F.ny, F.nz, K.p, K.b, K.b16, K.bkind, K.bscale,
-                       K.nx, K.ny, K.nz, inv_delta2);
+    hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2);
 }
 
-__global__ __launch_bounds__(256) void prolongate_kernel(const float *__restrict__ cp, float *__restrict__ fp,
-                                                         const float *__restrict__ fmask, int cnx, int cny, int cnz,
-                                                         int fnx, int fny, int fnz) {
-    const int64_t fi = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (fi >= (int64_t)fnx * fny * fnz) return;
+__device__ __forceinline__ void prolong_item(const Level &F, const Level &K, int64_t fi) {
     int i, j, k;
-    coords(fi, fny, fnz, i, j, k);
-    if (!interior(i, j, k, fnx, fny, fnz) || fmask[fi] <= 0.0f) return;
-    const int a = min((i - 1) / 2 + 1, cnx - 2), b = min((j - 1) / 2 + 1, cny - 2), c = min((k - 1) / 2 + 1, cnz - 2);
-    fp[fi] = fp[fi] + cp[gidx(a, b, c, cny, cnz)];
+    coords(fi, F.ny, F.nz, i, j, k);
+    if (!interior(i, j, k, F.nx, F.ny, F.nz) || F.mask[fi] <= 0.0f) return;
+    const int a = min((i - 1) / 2 + 1, K.nx - 2), b = min((j - 1) / 2 + 1, K.ny - 2), c = min((k - 1) / 2 + 1, K.nz - 2);
+    F.p[fi] = F.p[fi] + K.p[gidx(a, b, c, K.ny, K.nz)];
+}
+
+__global__ __launch_bounds__(256) void prolongate_kernel(Level F, Level K) {
+    const int64_t fi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (fi >= F.C) return;
+    prolong_item(F, K, fi);
 }
 
 void launch_prolongate(const Level &F, const Level &K, hipStream_t s) {
-    hipLaunchKernelGGL(prolongate_kernel, dim3((unsigned)((F.C + 255) / 256)), dim3(256), 0, s, K.p, F.p, F.mask, K.nx,
-                       K.ny, K.nz, F.nx, F.ny, F.nz);
+    hipLaunchKernelGGL(prolongate_kernel, dim3((unsigned)((F.C + 255) / 256)), dim3(256), 0, s, F, K);
 }
 
-// coarse_smooth: one work-group, the level's 7 arrays staged in LDS when they
-// fit (10^3 cells: 28 KB; up to 2048 cells, 56 KB).
+// coarse_smooth: one work-group of 1024 lanes owns the whole coarsest level
+// (up to 2048 cells, two per lane).  Only p lives in LDS; each lane keeps its
+// cells' weights, b, diagonal and colour in registers, so a half-sweep is six
+// LDS reads, one fused chain and one division per active cell, then a
+// barrier.  T = float restates coarse_smooth (multigrid.cl:232-264); T =
+// double is the FP64 coarsest smoother of the mixed-precision solve (the
+// same operations widened: b and the operator are exact in float).
 constexpr int kCoarseThreads = 1024;
-constexpr int kCoarseLdsCells = 2048;
+constexpr int kCoarseCells = 2 * kCoarseThreads;
 
-__global__ __launch_bounds__(kCoarseThreads) void coarse_smooth_kernel(
-    float *__restrict__ gp, const float *__restrict__ gb, const uint16_t *__restrict__ gb16, int bkind, float bscale,
-    const float *__restrict__ gmask,
-    const float *__restrict__ gdiag, const float *__restrict__ gwx, const float *__restrict__ gwy,
-    const float *__restrict__ gwz, float delta2, int iterations, int nx, int ny, int nz) {
-    __shared__ float sp[kCoarseLdsCells], sb[kCoarseLdsCells], sm[kCoarseLdsCells], sd[kCoarseLdsCells];
-    __shared__ float sx_[kCoarseLdsCells], sy_[kCoarseLdsCells], sz_[kCoarseLdsCells];
+template <typename T>
+__device__ void coarse_smooth_body(const Level &L, T dd, int iterations) {
+    __shared__ T sp[kCoarseCells];
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int C = nx * ny * nz;
-    for (int q = threadIdx.x; q < C; q += kCoarseThreads) {
-        sp[q] = gp[q]; sb[q] = load_b(gb, gb16, bkind, bscale, q); sm[q] = gmask[q]; sd[q] = gdiag[q];
-        sx_[q] = gwx[q]; sy_[q] = gwy[q]; sz_[q] = gwz[q];
+    const int sx = ny * nz, sy = nz;
+    int q[2], par[2];
+    bool act[2];
+    T w[2][6], bv[2], dv[2];
+#pragma unroll
+    for (int slot = 0; slot < 2; ++slot) {
+        const int cell = threadIdx.x + slot * kCoarseThreads;
+        q[slot] = cell;
+        act[slot] = false;
+        par[slot] = 0;
+        if (cell < C) {
+            sp[cell] = (T)L.p[cell];
+            int i, j, k;
+            coords(cell, ny, nz, i, j, k);
+            par[slot] = (i + j + k) & 1;
+            if (interior(i, j, k, nx, ny, nz) && L.mask[cell] > 0.0f && L.diag[cell] > 0.0f) {
+                act[slot] = true;
+                w[slot][0] = (T)L.wx[cell - sx];
+                w[slot][1] = (T)L.wx[cell];
+                w[slot][2] = (T)L.wy[cell];
+                w[slot][3] = (T)L.wy[cell - sy];
+                w[slot][4] = (T)L.wz[cell];
+                w[slot][5] = (T)L.wz[cell - 1];
+                bv[slot] = (T)load_b(L.b, L.b16, L.bkind, L.bscale, cell);
+                dv[slot] = (T)L.diag[cell];
+            }
+        }
     }
     __syncthreads();
-    const int64_t sx = (int64_t)ny * nz, sy = nz;
     for (int it = 0; it < iterations; ++it)
         for (int parity = 0; parity < 2; ++parity) {
-            for (int q = threadIdx.x; q < C; q += kCoarseThreads) {
-                int i, j, k;
-                coords(q, ny, nz, i, j, k);
-                if (!interior(i, j, k, nx, ny, nz) || sm[q] <= 0.0f || ((i + j + k) & 1) != parity) continue;
-                const float d = sd[q];
-                if (d > 0.0f) sp[q] = fmaf(-delta2, sb[q], wsum(sp, sx_, sy_, sz_, q, sx, sy)) / d;  // :257
+#pragma unroll
+            for (int slot = 0; slot < 2; ++slot) {
+                if (!act[slot] || par[slot] != parity) continue;
+                const int c = q[slot];
+                T t = w[slot][0] * sp[c - sx];
+                t = fma(w[slot][1], sp[c + sx], t);
+                t = fma(w[slot][2], sp[c + sy], t);
+                t = fma(w[slot][3], sp[c - sy], t);
+                t = fma(w[slot][4], sp[c + 1], t);
+                t = fma(w[slot][5], sp[c - 1], t);
+                sp[c] = fma(-dd, bv[slot], t) / dv[slot];  // multigrid.cl:257 (fmuladd)
             }
             __syncthreads();
         }
-    for (int q = threadIdx.x; q < C; q += kCoarseThreads) gp[q] = sp[q];
+#pragma unroll
+    for (int slot = 0; slot < 2; ++slot)
+        if (q[slot] < C) L.p[q[slot]] = (float)sp[q[slot]];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void coarse_smooth_kernel(Level L, T dd, int iterations) {
+    coarse_smooth_body<T>(L, dd, iterations);
 }
 
 void launch_coarse_smooth(const Level &L, float delta, int iterations, hipStream_t s) {
-    if (L.C > kCoarseLdsCells) {
-        // Larger coarsest levels (only for tiny min sizes) fall back to the
+    if (L.C > kCoarseCells) {
+        // Larger coarsest levels (only for elongated grids) fall back to the
         // colour kernels, which give the same result: fluid cells only are
         // updated, and non-fluid cells are not touched by coarse_smooth.
         for (int it = 0; it < iterations; ++it)
             for (int parity = 0; parity < 2; ++parity) launch_rbgs(L, delta, parity, s);
         return;
     }
-    hipLaunchKernelGGL(coarse_smooth_kernel, dim3(1), dim3(kCoarseThreads), 0, s, L.p, L.b, L.b16, L.bkind, L.bscale,
-                       L.mask, L.diag, L.wx, L.wy, L.wz, delta * delta, iterations, L.nx, L.ny, L.nz);
-}
-
-// coarse_smooth in FP64 for the mixed-precision solve: p is widened to double
-// in LDS, b (from its 16-bit storage, exact in float) and the operator (small
-// integers) stay float in LDS and are widened at use.  32 B per cell.
-constexpr int kCoarse64Cells = 2000;
-
-__global__ __launch_bounds__(kCoarseThreads) void coarse_smooth_f64_kernel(
-    float *__restrict__ gp, const float *__restrict__ gb, const uint16_t *__restrict__ gb16, int bkind, float bscale,
-    const float *__restrict__ gmask, const float *__restrict__ gdiag, const float *__restrict__ gwx,
-    const float *__restrict__ gwy, const float *__restrict__ gwz, double dd, int iterations, int nx, int ny, int nz) {
-    __shared__ double sp[kCoarse64Cells];
-    __shared__ float sb[kCoarse64Cells], sm[kCoarse64Cells], sd[kCoarse64Cells];
-    __shared__ float sx_[kCoarse64Cells], sy_[kCoarse64Cells], sz_[kCoarse64Cells];
-    const int C = nx * ny * nz;
-    for (int q = threadIdx.x; q < C; q += kCoarseThreads) {
-        sp[q] = gp[q]; sb[q] = load_b(gb, gb16, bkind, bscale, q); sm[q] = gmask[q]; sd[q] = gdiag[q];
-        sx_[q] = gwx[q]; sy_[q] = gwy[q]; sz_[q] = gwz[q];
-    }
-    __syncthreads();
-    const int sx = ny * nz, sy = nz;
-    for (int it = 0; it < iterations; ++it)
-        for (int parity = 0; parity < 2; ++parity) {
-            for (int q = threadIdx.x; q < C; q += kCoarseThreads) {
-                int i, j, k;
-                coords(q, ny, nz, i, j, k);
-                if (!interior(i, j, k, nx, ny, nz) || sm[q] <= 0.0f || ((i + j + k) & 1) != parity) continue;
-                const double d = sd[q];
-                if (!(d > 0.0)) continue;
-                double t = (double)sx_[q - sx] * sp[q - sx];
-                t = fma((double)sx_[q], sp[q + sx], t);
-                t = fma((double)sy_[q], sp[q + sy], t);
-                t = fma((double)sy_[q - sy], sp[q - sy], t);
-                t = fma((double)sz_[q], sp[q + 1], t);
-                t = fma((double)sz_[q - 1], sp[q - 1], t);
-                sp[q] = fma(-dd, (double)sb[q], t) / d;
-            }
-            __syncthreads();
-        }
-    for (int q = threadIdx.x; q < C; q += kCoarseThreads) gp[q] = (float)sp[q];
+    hipLaunchKernelGGL(coarse_smooth_kernel<float>, dim3(1), dim3(kCoarseThreads), 0, s, L, delta * delta, iterations);
 }
 
 void launch_coarse_smooth_f64(const Level &L, double delta, int iterations, hipStream_t s) {
-    hipLaunchKernelGGL(coarse_smooth_f64_kernel, dim3(1), dim3(kCoarseThreads), 0, s, L.p, L.b, L.b16, L.bkind,
-                       L.bscale, L.mask, L.diag, L.wx, L.wy, L.wz, delta * delta, iterations, L.nx, L.ny, L.nz);
+    hipLaunchKernelGGL(coarse_smooth_kernel<double>, dim3(1), dim3(kCoarseThreads), 0, s, L, delta * delta, iterations);
 }
 
-int coarse_f64_capacity() { return kCoarse64Cells; }
+// ---- bottom of the W-cycle in one work-group --------------------------------------
+// The levels below ~64k cells cost a few microseconds of dispatch each and
+// almost no work.  One work-group of 1024 lanes runs the whole recursion below
+// a given level (op list built by the host), each phase a strided loop over
+// the level's cells with the same per-cell code as the standalone kernels,
+// separated by work-group barriers; the data stays in L2.
+__global__ __launch_bounds__(kCoarseThreads) void wcycle_bottom_kernel(BottomArgs a) {
+    for (int o = 0; o < a.nops; ++o) {
+        const BottomOp op = a.ops[o];
+        const Level &L = a.lv[op.level];
+        if (op.code == kOpSmooth) {
+            const int64_t work = (int64_t)L.nx * L.ny * ((L.nz + 1) >> 1);
+            for (int it = 0; it < op.count; ++it)
+                for (int parity = 0; parity < 2; ++parity) {
+                    for (int64_t t = threadIdx.x; t < work; t += kCoarseThreads) rbgs_item(L, a.dd, parity, t);
+                    __syncthreads();
+                }
+        } else if (op.code == kOpRestrict) {
+            const Level &K = a.lv[op.level + 1];
+            for (int64_t ci = threadIdx.x; ci < K.C; ci += kCoarseThreads) restrict_item(L, K, a.inv_delta2, ci);
+            __syncthreads();
+        } else if (op.code == kOpProlong) {
+            const Level &K = a.lv[op.level + 1];
+            for (int64_t fi = threadIdx.x; fi < L.C; fi += kCoarseThreads) prolong_item(L, K, fi);
+            __syncthreads();
+        } else if (op.code == kOpCoarse32 || op.code == kOpCoarse64) {
+            if (L.C > kCoarseCells) {  // same result as coarse_smooth (see launch_coarse_smooth)
+                const int64_t work = (int64_t)L.nx * L.ny * ((L.nz + 1) >> 1);
+                for (int it = 0; it < op.count; ++it)
+                    for (int parity = 0; parity < 2; ++parity) {
+                        for (int64_t t = threadIdx.x; t < work; t += kCoarseThreads) rbgs_item(L, a.dd, parity, t);
+                        __syncthreads();
+                    }
+            } else {
+                if (op.code == kOpCoarse32) coarse_smooth_body<float>(L, a.dd, op.count);
+                else coarse_smooth_body<double>(L, a.dd64, op.count);
+                __syncthreads();
+            }
+        }
+    }
+}
+
+void launch_wcycle_bottom(const BottomArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(wcycle_bottom_kernel, dim3(1), dim3(kCoarseThreads), 0, s, a);
+}
+
+int coarse_f64_capacity() { return kCoarseCells; }
 
 // ---- FP64 iterative-refinement update -----------------------------------------------
 // p_new = p_old + e on interior fluid cells (0 elsewhere), r = b - A p_new in

@@ -31,18 +31,19 @@ OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NOT_CONVERGED = ran
 FIELDS = {
     "u": 0, "v": 1, "w": 2, "u_old": 3, "v_old": 4, "w_old": 5, "fluid_fraction": 6, "pressure": 7, "rhs": 8,
     "diagonal": 9, "weight_x": 10, "weight_y": 11, "weight_z": 12, "bin_counts": 13, "bin_offsets": 14,
-    "obstacle": 15,
+    "obstacle": 15, "pressure_fp64": 16,
 }
-FIELD_DTYPES = {"bin_counts": np.int32, "bin_offsets": np.int32, "obstacle": np.uint8}
+FIELD_DTYPES = {"bin_counts": np.int32, "bin_offsets": np.int32, "obstacle": np.uint8, "pressure_fp64": np.float64}
 LEVEL_FIELDS = {"p": 0, "b": 1, "mask": 2, "diagonal": 3, "weight_x": 4, "weight_y": 5, "weight_z": 6}
 
 # enum clq_solve_mode
-SOLVE_FIXED_CYCLES, SOLVE_FP32_TOLERANCE, SOLVE_MIXED_REFINEMENT, SOLVE_LEARNED_FCG = range(4)
+SOLVE_FIXED_CYCLES, SOLVE_FP32_TOLERANCE, SOLVE_MIXED_REFINEMENT, SOLVE_LEARNED_FCG, SOLVE_FCG = range(5)
 SOLVE_MODES = {
     "fixed": SOLVE_FIXED_CYCLES,
     "fp32": SOLVE_FP32_TOLERANCE,
     "mixed": SOLVE_MIXED_REFINEMENT,
     "learned": SOLVE_LEARNED_FCG,
+    "fcg": SOLVE_FCG,
 }
 STAGES = ("bin", "p2g", "pressure_solve", "g2p_advect")
 KERNEL_CLASSES = ("binning", "p2g", "rbgs_fine_sweep", "particle_update")

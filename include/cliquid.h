@@ -71,7 +71,10 @@ enum clq_field {
     CLQ_FIELD_BIN_COUNTS = 13,    /* int32[C]    particles per bin              */
     CLQ_FIELD_BIN_OFFSETS = 14,   /* int32[C]    exclusive scan of counts       */
     CLQ_FIELD_OBSTACLE = 15,      /* uint8[C]    static solid voxels           */
-    CLQ_FIELD_COUNT = 16
+    CLQ_FIELD_PRESSURE_FP64 = 16, /* float64[C]  FP64 iterate of solve modes 2-4;
+                                     while valid, clq_pressure_correction uses
+                                     it instead of the FP32 pressure            */
+    CLQ_FIELD_COUNT = 17
 };
 
 /* Per-level multigrid arrays for clq_get_level_field / clq_set_level_field.
@@ -105,7 +108,10 @@ enum clq_solve_mode {
     CLQ_SOLVE_MIXED_REFINEMENT = 2,
     /* Flexible CG preconditioned by the learned network with the W-cycle as
      * fallback; requires clq_set_preconditioner_weights(). */
-    CLQ_SOLVE_LEARNED_FCG = 3
+    CLQ_SOLVE_LEARNED_FCG = 3,
+    /* Flexible CG in FP64 preconditioned by the mixed-precision W-cycle of
+     * mode 2 (one W-cycle per iteration). */
+    CLQ_SOLVE_FCG = 4
 };
 
 typedef struct clq_solve_params {
@@ -119,7 +125,7 @@ typedef struct clq_solve_params {
 } clq_solve_params;
 
 typedef struct clq_solve_stats {
-    int32_t cycles;            /* W-cycles (or FCG iterations) performed        */
+    int32_t cycles;            /* W-cycles performed (one per FCG iteration)    */
     int32_t fallback_cycles;   /* mode 3: W-cycle fallbacks taken               */
     double rhs_norm;           /* ||b||_2 (FP64 accumulation)                    */
     double initial_residual;   /* ||b - A p0||_2 before the solve               */
@@ -200,7 +206,10 @@ int clq_coarse_smooth(clq_ctx *ctx, int32_t iterations);
 int clq_neumann_bc(clq_ctx *ctx);
 /* FP64-accumulated ||b - A p||_2 and ||b||_2 on level 0. */
 int clq_residual_norm(clq_ctx *ctx, double *residual, double *rhs_norm);
-/* pressure_correction (pressure_solve.cl:26-47). */
+/* pressure_correction (pressure_solve.cl:26-47).  After a solve in modes 2-4
+ * the FP64 pressure is used (ghost neighbours read the clamped interior value,
+ * which is what neumann_domain_bc stores): the gradient and the update run in
+ * FP64 and u is rounded once. */
 int clq_pressure_correction(clq_ctx *ctx, double rho, double dt);
 
 /* ---- grid_to_particle / advection section --------------------------------- */

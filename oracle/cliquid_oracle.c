@@ -97,6 +97,9 @@ struct clq_ctx {
     /* mixed-precision solve state, allocated on first use */
     double *p64a, *p64b;
     float *r32;
+    double *r64, *dir0, *dir1, *q64; /* flexible CG */
+    int p64_valid;                    /* p64a holds the pressure of the last solve (modes 2-4) */
+    int bins_current; /* counts/offsets describe the current particle order */
 };
 
 static __thread char g_error[512];
@@ -184,6 +187,7 @@ int clq_destroy(clq_ctx *c) {
         free(L->b16);
     }
     free(c->p64a); free(c->p64b); free(c->r32);
+    free(c->r64); free(c->dir0); free(c->dir1); free(c->q64);
     free(c);
     return CLQ_OK;
 }
@@ -244,6 +248,7 @@ int clq_set_particles(clq_ctx *c, const float *pos, const float *vel, int64_t n)
     if (vel) memcpy(c->vel, vel, (size_t)n * 12);
     else memset(c->vel, 0, (size_t)n * 12);
     c->n = n;
+    c->bins_current = 0;
     return CLQ_OK;
 }
 
@@ -287,6 +292,7 @@ static void *field_ptr(clq_ctx *c, int f, long long *bytes) {
     case CLQ_FIELD_BIN_COUNTS: p = c->counts; break;
     case CLQ_FIELD_BIN_OFFSETS: p = c->offsets; break;
     case CLQ_FIELD_OBSTACLE: p = c->obstacle; fb = c->C; break;
+    case CLQ_FIELD_PRESSURE_FP64: p = c->p64a; fb = c->C * 8; break;
     default: return NULL;
     }
     *bytes = fb;
@@ -298,6 +304,7 @@ int clq_set_field(clq_ctx *c, int32_t f, const void *host, int64_t bytes) {
     void *p = c ? field_ptr(c, f, &fb) : NULL;
     if (!p || !host || bytes != fb) return fail(CLQ_ERR_INVALID, "bad field upload (field %d, %lld bytes)", f, (long long)bytes);
     memcpy(p, host, (size_t)bytes);
+    if (f == CLQ_FIELD_PRESSURE) c->p64_valid = 0;
     return CLQ_OK;
 }
 
@@ -395,6 +402,7 @@ int clq_bin_particles(clq_ctx *c) {
     float *t = c->pos; c->pos = c->pos_work; c->pos_work = t;
     t = c->vel; c->vel = c->vel_work; c->vel_work = t;
     c->stage_ms[CLQ_STAGE_BIN] = now_ms() - t0;
+    c->bins_current = 1;
     return CLQ_OK;
 }
 
@@ -980,17 +988,135 @@ static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     return CLQ_OK;
 }
 
+
+/* ---- flexible CG (no reference counterpart) --------------------------------- */
+/* A v at a fluid cell in FP64, left-to-right over the non-zero weights. */
+static inline double apply_a64(const level_t *L, long long q, const double *v, double inv_h2) {
+    const long long sx = (long long)L->ny * L->nz, sy = L->nz;
+    double s = 0.0;
+    if (L->wx[q] != 0.0f) s = s + (double)L->wx[q] * v[q + sx];
+    if (L->wx[q - sx] != 0.0f) s = s + (double)L->wx[q - sx] * v[q - sx];
+    if (L->wy[q] != 0.0f) s = s + (double)L->wy[q] * v[q + sy];
+    if (L->wy[q - sy] != 0.0f) s = s + (double)L->wy[q - sy] * v[q - sy];
+    if (L->wz[q] != 0.0f) s = s + (double)L->wz[q] * v[q + 1];
+    if (L->wz[q - 1] != 0.0f) s = s + (double)L->wz[q - 1] * v[q - 1];
+    return (s - (double)L->diag[q] * v[q]) * inv_h2;
+}
+
+static inline int fluid_at(const level_t *L, long long q) {
+    int i, j, k;
+    coords(q, L->ny, L->nz, &i, &j, &k);
+    return interior(i, j, k, L->nx, L->ny, L->nz) && L->mask[q] > 0.0f;
+}
+
+static void precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rnorm) {
+    const int kind = sp->coarse_dtype == 1 ? 2 : 1;
+    const int es = (int)lround(log2(rnorm / sqrt((double)c->C) + 1e-300));
+    for (int l = 1; l < c->nlevels; ++l) {
+        c->lv[l].bkind = kind;
+        c->lv[l].bscale = ldexpf(1.0f, 3 * l + es);
+    }
+    c->lv[0].b = c->r32;
+    memset(c->p, 0, (size_t)c->C * sizeof(float));
+    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+    c->lv[0].b = c->rhs;
+    for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
+}
+
+static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
+    if (!c->p64a) {
+        c->p64a = xcalloc(c->C, sizeof(double));
+        c->p64b = xcalloc(c->C, sizeof(double));
+        c->r32 = xcalloc(c->C, sizeof(float));
+        for (int l = 1; l < c->nlevels; ++l) c->lv[l].b16 = xcalloc(c->lv[l].C, sizeof(uint16_t));
+    }
+    if (!c->r64) {
+        c->r64 = xcalloc(c->C, sizeof(double));
+        c->dir0 = xcalloc(c->C, sizeof(double));
+        c->dir1 = xcalloc(c->C, sizeof(double));
+        c->q64 = xcalloc(c->C, sizeof(double));
+    }
+    if (!c->p64a || !c->r64 || !c->dir0 || !c->dir1 || !c->q64) return fail(CLQ_ERR_NOMEM, "FCG buffers");
+    const level_t *L = &c->lv[0];
+    const double h = (double)(float)c->dx;
+    const double inv_h2 = 1.0 / (h * h);
+    double *x = c->p64a, *r = c->r64, *q = c->q64, *dold = c->dir0, *dnew = c->dir1;
+    /* x = p (fluid), r = b - A x */
+    double rr = 0.0, bb = 0.0;
+    for (long long t = 0; t < c->C; ++t) x[t] = fluid_at(L, t) ? (double)c->p[t] : 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rr, bb)
+    for (long long t = 0; t < c->C; ++t) {
+        double rv = 0.0;
+        if (fluid_at(L, t)) rv = (double)c->rhs[t] - apply_a64(L, t, x, inv_h2);
+        r[t] = rv;
+        c->r32[t] = (float)rv;
+        rr += rv * rv;
+        bb += (double)c->rhs[t] * (double)c->rhs[t];
+    }
+    double rn = sqrt(rr);
+    const double bn = sqrt(bb);
+    s->initial_residual = rn;
+    s->rhs_norm = bn;
+    if (rn > sp->tolerance * bn && sp->max_cycles > 0) {
+        precondition_wcycle(c, sp, rn);
+        s->cycles++;
+        double rz = 0.0, alpha = 0.0, beta = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rz)
+        for (long long t = 0; t < c->C; ++t) rz += (double)c->p[t] * r[t];
+        for (int it = 0;; ++it) {
+            /* d_new = z + beta d_old (beta = 0 first), q = A d_new */
+            for (long long t = 0; t < c->C; ++t)
+                dnew[t] = fluid_at(L, t) ? (it == 0 ? (double)c->p[t] : (double)c->p[t] + beta * dold[t]) : 0.0;
+            double dq = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : dq)
+            for (long long t = 0; t < c->C; ++t) {
+                const double av = fluid_at(L, t) ? apply_a64(L, t, dnew, inv_h2) : 0.0;
+                q[t] = av;
+                dq += dnew[t] * av;
+            }
+            alpha = dq != 0.0 ? rz / dq : 0.0;
+            rr = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rr)
+            for (long long t = 0; t < c->C; ++t) {
+                x[t] = x[t] + alpha * dnew[t];
+                r[t] = r[t] - alpha * q[t];
+                c->r32[t] = (float)r[t];
+                rr += r[t] * r[t];
+            }
+            double *tmp = dold; dold = dnew; dnew = tmp;
+            rn = sqrt(rr);
+            if (rn <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(rn == rn)) break;
+            precondition_wcycle(c, sp, rn);
+            s->cycles++;
+            double zq = 0.0, zr = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : zq, zr)
+            for (long long t = 0; t < c->C; ++t) {
+                zq += (double)c->p[t] * q[t];
+                zr += (double)c->p[t] * r[t];
+            }
+            beta = rz != 0.0 ? -alpha * zq / rz : 0.0;
+            rz = zr;
+        }
+    }
+    for (long long t = 0; t < c->C; ++t) c->p[t] = (float)x[t];
+    s->final_residual = rn;
+    s->relative_residual = bn > 0 ? rn / bn : 0.0;
+    return CLQ_OK;
+}
+
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
     if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE &&
-        sp->mode != CLQ_SOLVE_MIXED_REFINEMENT)
+        sp->mode != CLQ_SOLVE_MIXED_REFINEMENT && sp->mode != CLQ_SOLVE_FCG)
         return fail(CLQ_ERR_UNSUPPORTED, "solve mode %d is not available in the oracle", sp->mode);
     double t0 = now_ms();
     clq_solve_stats s;
     memset(&s, 0, sizeof s);
-    if (sp->mode == CLQ_SOLVE_MIXED_REFINEMENT) {
-        int rc = solve_mixed(c, sp, &s);
+    c->p64_valid = 0;
+    if (sp->mode == CLQ_SOLVE_MIXED_REFINEMENT || sp->mode == CLQ_SOLVE_FCG) {
+        int rc = sp->mode == CLQ_SOLVE_FCG ? solve_fcg(c, sp, &s) : solve_mixed(c, sp, &s);
         if (rc) return rc;
+        c->p64_valid = 1;
         clq_neumann_bc(c);
         s.solve_ms = now_ms() - t0;
         c->stage_ms[CLQ_STAGE_PRESSURE_SOLVE] = s.solve_ms;
@@ -1028,6 +1154,24 @@ int clq_pressure_correction(clq_ctx *c, double rho, double dt) {
     if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
     const int nx = c->nx, ny = c->ny, nz = c->nz;
     const float dt_over_rho = (float)(dt / rho), inv_dx = (float)(1.0 / c->dx);
+    if (c->p64_valid) { /* FP64 pressure of modes 2-4; +1 neighbour clamped to the interior */
+        const double *p = c->p64a;
+        const double k = (double)dt_over_rho, id = (double)inv_dx;
+#pragma omp parallel for schedule(static)
+        for (long long index = 0; index < c->C; ++index) {
+            int i, j, kk;
+            coords(index, ny, nz, &i, &j, &kk);
+            if (!interior(i, j, kk, nx, ny, nz)) continue;
+            const double centre = p[index];
+            const double gx = (p[gidx(i + 1 > nx - 2 ? nx - 2 : i + 1, j, kk, ny, nz)] - centre) * id;
+            const double gy = (p[gidx(i, j + 1 > ny - 2 ? ny - 2 : j + 1, kk, ny, nz)] - centre) * id;
+            const double gz = (p[gidx(i, j, kk + 1 > nz - 2 ? nz - 2 : kk + 1, ny, nz)] - centre) * id;
+            c->u[index] = (float)((double)c->u[index] - k * gx);
+            c->v[index] = (float)((double)c->v[index] - k * gy);
+            c->w[index] = (float)((double)c->w[index] - k * gz);
+        }
+        return CLQ_OK;
+    }
 #pragma omp parallel for schedule(static)
     for (long long index = 0; index < c->C; ++index) {
         int i, j, k;
@@ -1114,6 +1258,7 @@ int clq_save_previous_positions(clq_ctx *c) {
 
 int clq_advect_particles(clq_ctx *c, double dt) {
     if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    c->bins_current = 0;
     const float fdt = (float)dt;
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < 3 * c->n; ++i) c->pos[i] = fmaf(fdt, c->vel[i], c->pos[i]); /* particles.cl:275 */
@@ -1145,6 +1290,7 @@ static void domain_bounds(const clq_ctx *c, float *lower, float *upper) {
 
 int clq_particle_domain_bc(clq_ctx *c) {
     if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    c->bins_current = 0;
     float lower, upper[3];
     domain_bounds(c, &lower, upper);
 #pragma omp parallel for schedule(static)
@@ -1204,6 +1350,7 @@ static inline void obstacle_bc_one(const clq_ctx *c, long long pi, float inv_dx)
 
 int clq_particle_obstacle_bc(clq_ctx *c) {
     if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    c->bins_current = 0;
     const float inv_dx = (float)(1.0 / c->dx);
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < c->n; ++i) obstacle_bc_one(c, i, inv_dx);
@@ -1230,7 +1377,10 @@ int clq_grid_to_particle_advect(clq_ctx *c, double flip_ratio, double dt) {
  * seeded region reads 1.  Summation runs particle by particle in the current
  * (binned) order, in FP64, and is rounded once. */
 int clq_voxelise(clq_ctx *c, int32_t s, double ppc, uint8_t *occ, float *density) {
-    if (!c || s < 1 || s > 8 || !(ppc > 0)) return fail(CLQ_ERR_INVALID, "bad voxelise arguments");
+    if (!c || s < 1 || !(ppc > 0)) return fail(CLQ_ERR_INVALID, "bad voxelise arguments");
+    if (s > 2) return fail(CLQ_ERR_UNSUPPORTED, "voxelise scale %d > 2", s);
+    /* the density is accumulated in binned particle order, as the GPU gather walks bins */
+    if (density && !c->bins_current) clq_bin_particles(c);
     const long long sx = (long long)s * c->nx, sy = (long long)s * c->ny, sz = (long long)s * c->nz;
     const long long S = sx * sy * sz;
     const float sinv = (float)s * (float)(1.0 / c->dx);

@@ -255,4 +255,41 @@ def test_mixed_refinement_bit_exact(oracle_lib, hip_lib, coarse_dtype):
     assert so["cycles"] == sh["cycles"]
     assert sh["relative_residual"] <= 1e-8
     assert abs(so["final_residual"] - sh["final_residual"]) <= 1e-9 * so["final_residual"]
-    assert_fields_equal(o, h, ("pressure",))
+    assert_fields_equal(o, h, ("pressure", "pressure_fp64"))
+    for c in (o, h):  # the correction then reads the FP64 pressure
+        c.pressure_correction(1000.0, 1 / 240)
+    assert_fields_equal(o, h, ("u", "v", "w"))
+
+
+@pytest.mark.parametrize("scale", [1, 2])
+def test_voxelise_matches_oracle(oracle_lib, hip_lib, scale):
+    scene = CASES[0]
+    o, h = pair(oracle_lib, hip_lib, scene)
+    outs = []
+    for c in (o, h):
+        c.bin_particles()
+        c.particle_to_grid(8.0)
+        outs.append(c.voxelise(scale, 8.0, density=True))
+    (oo, od), (ho, hd) = outs
+    assert np.array_equal(oo, ho)
+    np.testing.assert_allclose(hd, od, rtol=1e-6, atol=1e-7)
+    if scale == 1:
+        assert np.array_equal(ho.astype(bool), h.get_field("fluid_fraction") > 0)
+
+
+def test_fcg_matches_oracle(oracle_lib, hip_lib):
+    """FP64 FCG: the dot products are summed in different trees on the two
+    sides, so iterates agree to rounding, not bit for bit."""
+    scene = CASES[0]
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+    kw = dict(post_smooth=2, coarse_extra=16, max_cycles=40, tolerance=1e-8)
+    so = o.pressure_solve(runtime.SOLVE_FCG, **kw)
+    sh = h.pressure_solve(runtime.SOLVE_FCG, **kw)
+    assert abs(so["cycles"] - sh["cycles"]) <= 1
+    assert sh["relative_residual"] <= 1e-8
+    po, ph = o.get_field("pressure"), h.get_field("pressure")
+    assert np.max(np.abs(po - ph)) <= 1e-6 * np.max(np.abs(po))
+    r, b = h.residual_norm()
+    assert r / b <= 2e-8  # FP64 recheck of the FP32-rounded result on the device

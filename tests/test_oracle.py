@@ -464,3 +464,26 @@ def test_mixed_refinement_reaches_1e8_against_fp64_direct_solve(oracle_lib, coar
     p = ctx.get_field("pressure").astype(np.float64)[fluid]
     # the returned FP32 pressure is the FP64 iterate rounded once
     assert np.linalg.norm(p - exact) <= 1e-6 * np.linalg.norm(exact)
+
+
+def test_fcg_reaches_tolerance_in_fewer_cycles(oracle_lib):
+    """Mode 4 (FP64 flexible CG around the mixed-precision W-cycle) reaches the
+    target with fewer W-cycles than refinement alone and matches the scipy
+    FP64 solution."""
+    kw = dict(pre_smooth=2, post_smooth=2, coarse_extra=16, max_cycles=80, tolerance=1e-9)
+    ir = _pressure_context(oracle_lib, 16, seed=2)
+    s_ir = ir.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, **kw)
+    ctx = _pressure_context(oracle_lib, 16, seed=2)
+    s = ctx.pressure_solve(runtime.SOLVE_FCG, **kw)
+    assert s["relative_residual"] <= 1e-9
+    assert s["cycles"] < s_ir["cycles"]
+    mask = ctx.get_field("fluid_fraction")
+    h2 = np.float64(F32(ctx.dx)) ** 2
+    A, fluid, _ = operator_matrix(*[ctx.get_field(n) for n in ("diagonal", "weight_x", "weight_y", "weight_z")],
+                                  mask, h2)
+    b = ctx.get_field("rhs").astype(np.float64)[fluid]
+    exact = spla.spsolve(A.tocsc(), b)
+    p = ctx.get_field("pressure").astype(np.float64)[fluid]
+    assert np.linalg.norm(p - exact) <= 1e-6 * np.linalg.norm(exact)
+    p64 = ctx.get_field("pressure_fp64")[fluid]
+    assert s["final_residual"] == pytest.approx(np.linalg.norm(b - A @ p64), rel=1e-6)
