@@ -514,7 +514,7 @@ static void restrict_level(clq_ctx *c, int level) {
 
 // Levels with at most this many cells run inside one work-group launch
 // (wcycle_bottom_kernel) for the rest of the recursion below them.
-constexpr int64_t kBottomCells = 65536;
+constexpr int64_t kBottomCells = 8192;
 
 static void bottom_ops(const clq_ctx *c, int level, int pre, int post, int extra, bool f64, BottomArgs &a) {
     if (level == c->nlevels - 1) {

@@ -58,6 +58,51 @@ __device__ __forceinline__ void rbgs_item(const Level &L, float dd, int parity, 
     }
 }
 
+// Four same-colour cells per lane with every load issued before any store:
+// cells of one colour never read each other, so the batch is exact and the
+// lane keeps four memory requests in flight (used by the single-work-group
+// bottom kernel, where latency, not bandwidth, bounds a half-sweep).
+__device__ __forceinline__ void rbgs_batch4(const Level &L, float dd, int parity, int64_t t0, int64_t stride,
+                                            int64_t work) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const uint32_t half = (uint32_t)(nz + 1) >> 1;
+    const int64_t sx = (int64_t)ny * nz, sy = nz;
+    int64_t idx[4];
+    int mode[4];  // 0 skip, 1 zero, 2 update
+    float num[4], den[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int64_t t = t0 + u * stride;
+        mode[u] = 0;
+        idx[u] = 0;
+        num[u] = 0.0f;
+        den[u] = 1.0f;
+        if (t >= work) continue;
+        const uint32_t row = (uint32_t)t / half;
+        const int kk = (int)((uint32_t)t - row * half);
+        const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
+        const int k = 2 * kk + ((parity + i + j) & 1);
+        if (k >= nz) continue;
+        const int64_t q = (int64_t)row * nz + k;
+        idx[u] = q;
+        if (L.mask[q] <= 0.0f) {
+            mode[u] = 1;
+            continue;
+        }
+        if (!interior(i, j, k, nx, ny, nz)) continue;
+        const float d = L.diag[q];
+        if (!(d > 0.0f)) continue;
+        mode[u] = 2;
+        num[u] = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, q), wsum(L.p, L.wx, L.wy, L.wz, q, sx, sy));
+        den[u] = d;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        if (mode[u] == 1) L.p[idx[u]] = 0.0f;
+        else if (mode[u] == 2) L.p[idx[u]] = num[u] / den[u];  // multigrid.cl:227
+    }
+}
+
 __global__ __launch_bounds__(256) void rbgs_kernel(Level L, float dd, int parity) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= (int64_t)L.nx * L.ny * ((L.nz + 1) >> 1)) return;
@@ -271,7 +316,8 @@ __global__ __launch_bounds__(kCoarseThreads) void wcycle_bottom_kernel(BottomArg
             const int64_t work = (int64_t)L.nx * L.ny * ((L.nz + 1) >> 1);
             for (int it = 0; it < op.count; ++it)
                 for (int parity = 0; parity < 2; ++parity) {
-                    for (int64_t t = threadIdx.x; t < work; t += kCoarseThreads) rbgs_item(L, a.dd, parity, t);
+                    for (int64_t t = threadIdx.x; t < work; t += 4 * kCoarseThreads)
+                        rbgs_batch4(L, a.dd, parity, t, kCoarseThreads, work);
                     __syncthreads();
                 }
         } else if (op.code == kOpRestrict) {

@@ -1156,7 +1156,11 @@ int clq_pressure_correction(clq_ctx *c, double rho, double dt) {
     const float dt_over_rho = (float)(dt / rho), inv_dx = (float)(1.0 / c->dx);
     if (c->p64_valid) { /* FP64 pressure of modes 2-4; +1 neighbour clamped to the interior */
         const double *p = c->p64a;
-        const double k = (double)dt_over_rho, id = (double)inv_dx;
+        /* rounded through float explicitly: gcc's OpenMP outlining of a
+         * const float widened to double has been seen to keep the unrounded
+         * double initialiser */
+        volatile float k32 = (float)(dt / rho), id32 = (float)(1.0 / c->dx);
+        const double k = (double)k32, id = (double)id32;
 #pragma omp parallel for schedule(static)
         for (long long index = 0; index < c->C; ++index) {
             int i, j, kk;
