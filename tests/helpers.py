@@ -3,6 +3,7 @@
 from __future__ import annotations
 
 import numpy as np
+import scipy.sparse as sp
 
 from continuum_liquid_amd import runtime, scenes
 from continuum_liquid_amd.Solver.Kernel import solver as host_solver
@@ -62,3 +63,29 @@ def random_particles(rng, shape, dx, n, margin=0.0):
 
 def scene_from_arrays(name, shape, dx, pos, vel, gravity=(0.0, -9.81, 0.0)):
     return scenes.Scene(name=name, shape=tuple(shape), dx=dx, positions=pos, velocities=vel, gravity=gravity)
+
+
+def interior(shape):
+    m = np.zeros(shape, bool)
+    m[1:-1, 1:-1, 1:-1] = True
+    return m
+
+
+def operator_matrix(diag, wx, wy, wz, mask, h2):
+    """Sparse matrix of apply_operator (multigrid.cl:25-43) on the fluid cells."""
+    shape = diag.shape
+    fluid = interior(shape) & (mask > 0)
+    ids = -np.ones(shape, int)
+    ids[fluid] = np.arange(fluid.sum())
+    rows, cols, vals = [], [], []
+    for (i, j, k) in np.argwhere(fluid):
+        r = ids[i, j, k]
+        rows.append(r); cols.append(r); vals.append(-float(diag[i, j, k]))
+        for w, nb in ((wx[i, j, k], (i + 1, j, k)), (wx[i - 1, j, k], (i - 1, j, k)),
+                      (wy[i, j, k], (i, j + 1, k)), (wy[i, j - 1, k], (i, j - 1, k)),
+                      (wz[i, j, k], (i, j, k + 1)), (wz[i, j, k - 1], (i, j, k - 1))):
+            if w != 0:
+                assert ids[nb] >= 0, "weight towards a non-fluid cell"
+                rows.append(r); cols.append(ids[nb]); vals.append(float(w))
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(fluid.sum(),) * 2) / h2
+    return A, fluid, ids

@@ -16,7 +16,8 @@ import pytest
 from continuum_liquid_amd import runtime, scenes
 from continuum_liquid_amd.Solver.Kernel import solver as host_solver
 
-from .helpers import bin_keys, default_params, make_context, random_particles, run_to_pressure, scene_from_arrays
+from .helpers import (bin_keys, default_params, make_context, operator_matrix, random_particles, run_to_pressure,
+                      scene_from_arrays)
 
 pytestmark = pytest.mark.gpu
 
@@ -291,5 +292,13 @@ def test_fcg_matches_oracle(oracle_lib, hip_lib):
     assert sh["relative_residual"] <= 1e-8
     po, ph = o.get_field("pressure"), h.get_field("pressure")
     assert np.max(np.abs(po - ph)) <= 1e-6 * np.max(np.abs(po))
-    r, b = h.residual_norm()
-    assert r / b <= 2e-8  # FP64 recheck of the FP32-rounded result on the device
+    # FP64 recheck of the device's FP64 iterate (the FP32-rounded pressure
+    # carries a rounding floor of about 1e-7 relative residual on its own)
+    mask = h.get_field("fluid_fraction")
+    h2 = np.float64(np.float32(h.dx)) ** 2
+    A, fluid, _ = operator_matrix(*[h.get_field(n) for n in ("diagonal", "weight_x", "weight_y", "weight_z")], mask, h2)
+    b = h.get_field("rhs").astype(np.float64)[fluid]
+    p64 = h.get_field("pressure_fp64")[fluid]
+    res = np.linalg.norm(b - A @ p64) / np.linalg.norm(b)
+    assert res <= 1.01e-8
+    assert sh["final_residual"] == pytest.approx(np.linalg.norm(b - A @ p64), rel=1e-6)

@@ -34,7 +34,8 @@ import scipy.sparse.linalg as spla
 
 from continuum_liquid_amd import runtime, scenes
 
-from .helpers import F32, bin_keys, fma32, make_context, random_particles, run_to_pressure, scene_from_arrays
+from .helpers import (F32, bin_keys, fma32, interior, make_context, operator_matrix, random_particles, run_to_pressure,
+                      scene_from_arrays)
 
 
 def random_scene(seed, shape=(10, 11, 9), n=4000, margin=0.0):
@@ -131,12 +132,6 @@ def _glue_context(oracle_lib, seed=5, shape=(9, 10, 8)):
     return ctx, fields, ff
 
 
-def interior(shape):
-    m = np.zeros(shape, bool)
-    m[1:-1, 1:-1, 1:-1] = True
-    return m
-
-
 def test_update_velocity_closed_form(oracle_lib):
     ctx, f, ff = _glue_context(oracle_lib)
     g, dt, nu, rho = (0.0, -9.81, 0.0), 1 / 240, 0.05, 1000.0
@@ -218,26 +213,6 @@ def test_grid_domain_bc_and_neumann_closed_form(oracle_lib):
 
 
 # ---- multigrid -------------------------------------------------------------------------------
-def operator_matrix(diag, wx, wy, wz, mask, h2):
-    """Sparse matrix of apply_operator (multigrid.cl:25-43) on the fluid cells."""
-    shape = diag.shape
-    fluid = interior(shape) & (mask > 0)
-    ids = -np.ones(shape, int)
-    ids[fluid] = np.arange(fluid.sum())
-    rows, cols, vals = [], [], []
-    for (i, j, k) in np.argwhere(fluid):
-        r = ids[i, j, k]
-        rows.append(r); cols.append(r); vals.append(-float(diag[i, j, k]))
-        for w, nb in ((wx[i, j, k], (i + 1, j, k)), (wx[i - 1, j, k], (i - 1, j, k)),
-                      (wy[i, j, k], (i, j + 1, k)), (wy[i, j - 1, k], (i, j - 1, k)),
-                      (wz[i, j, k], (i, j, k + 1)), (wz[i, j, k - 1], (i, j, k - 1))):
-            if w != 0:
-                assert ids[nb] >= 0, "weight towards a non-fluid cell"
-                rows.append(r); cols.append(ids[nb]); vals.append(float(w))
-    A = sp.csr_matrix((vals, (rows, cols)), shape=(fluid.sum(),) * 2) / h2
-    return A, fluid, ids
-
-
 def _pressure_context(oracle_lib, n=16, seed=0):
     scene = scenes.small_scene(n, seed=seed)
     ctx = make_context(oracle_lib, scene)
