@@ -65,6 +65,8 @@ def solver_settings(name: str, tolerance: float) -> dict:
         return {**base, "mode": "mixed", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16}
     if name == "fcg":
         return {**base, "mode": "fcg", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16}
+    if name == "learned":
+        return {**base, "mode": "learned", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16}
     if name == "fixed":
         return {**base, "mode": "fixed", "max_cycles": 1, "post_smooth": 4}
     raise SystemExit(f"unknown solver {name}")
@@ -98,7 +100,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=3, choices=(1, 2, 3, 5))
-    ap.add_argument("--solver", default="fcg", choices=("fp32", "mixed", "fcg", "fixed"))
+    ap.add_argument("--solver", default="fcg", choices=("fp32", "mixed", "fcg", "learned", "fixed"))
+    ap.add_argument("--weights", type=Path, default=None,
+                    help="learned solver: parameter file (default: the package's trained weights)")
+    ap.add_argument("--stall-ratio", type=float, default=0.5,
+                    help="learned solver: residual ratio above which the W-cycle takes over")
     ap.add_argument("--tolerance", type=float, default=1e-6)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -123,6 +129,12 @@ def main() -> int:
 
     with runtime.Context(lib, scene.shape, scene.dx, scene.particle_count) as ctx:
         ctx.set_particles(scene.positions, scene.velocities)
+        if args.solver == "learned":
+            from continuum_liquid_amd import learned
+
+            ctx.attach_net(learned.load_weights(lib, scene.shape, args.weights or learned.DEFAULT_WEIGHTS),
+                           args.stall_ratio)
+        fallback = []
         for _ in range(args.warmup):
             host_solver.step(ctx, params)
         ctx.sync()
@@ -134,6 +146,7 @@ def main() -> int:
             solve_ms.append(stats["solve_ms"])
             relres.append(stats["relative_residual"])
             cycles.append(stats["cycles"])
+            fallback.append(stats["fallback_cycles"])
         ctx.sync()
         elapsed = time.perf_counter() - t0
         ktimes = ctx.kernel_times()
@@ -184,6 +197,7 @@ def main() -> int:
         "pressure_solve_ms": round(float(np.mean(solve_ms)), 3),
         "w_cycles_per_step": round(float(np.mean(cycles)), 2),
         "relative_residual_max": float(np.max(relres)),
+        **({"w_cycle_fallback_per_step": round(float(np.mean(fallback)), 2)} if args.solver == "learned" else {}),
         "roofline": {
             "kernel": dominant,
             "bound": "hbm",
@@ -197,7 +211,11 @@ def main() -> int:
     }
     if not args.no_cpu:
         print("# timing the CPU oracle on the same workload", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(scene, params, args.cpu_steps)
+        cpu_params = params
+        if args.solver == "learned":  # the oracle has no network: it runs the FCG solve of the same workload
+            cpu_params = host_solver.StepParams(dt=params.dt, gravity=params.gravity, flip_ratio=params.flip_ratio,
+                                                solver=solver_settings("fcg", args.tolerance))
+        result["cpu_baseline"] = cpu_baseline(scene, cpu_params, args.cpu_steps)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -5,7 +5,7 @@ from __future__ import annotations
 from ... import runtime
 
 
-def pressure_solve(ctx, dt: float, rho: float, solver_settings: dict) -> dict:
+def pressure_solve(ctx, dt: float, rho: float, solver_settings: dict, observer=None) -> dict:
     """RHS, finest operator, the solve and the Neumann ghost update.
 
     Reference pressure_solve.py:10-85.  `solver_settings` selects the solve:
@@ -16,7 +16,9 @@ def pressure_solve(ctx, dt: float, rho: float, solver_settings: dict) -> dict:
     ctx.pressure_rhs(rho, dt)
     ctx.build_operator()
     mode = runtime.SOLVE_MODES[solver_settings.get("mode", "fixed")]
-    return ctx.pressure_solve(
+    if observer is not None:
+        observer(ctx, None)
+    stats = ctx.pressure_solve(
         mode,
         pre_smooth=int(solver_settings.get("pre_smooth", 2)),
         post_smooth=int(solver_settings.get("post_smooth", 4)),
@@ -25,6 +27,9 @@ def pressure_solve(ctx, dt: float, rho: float, solver_settings: dict) -> dict:
         tolerance=float(solver_settings.get("tolerance", 1e-6)),
         coarse_dtype=int(solver_settings.get("coarse_dtype", 0)),
     )
+    if observer is not None:
+        observer(ctx, stats)
+    return stats
 
 
 def pressure_correction(ctx, dt: float, rho: float) -> None:

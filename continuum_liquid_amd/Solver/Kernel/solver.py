@@ -19,7 +19,7 @@ With timing off, the last five run as one fused pass over the particles
 Settings added to the reference schema (all optional; defaults reproduce the
 reference): `settings.gravity` (default (0, 0, -9.81)), `settings.flip_ratio`
 (default 1.0, pure FLIP), and `settings.pressure_solver` with `mode`
-("fixed" | "fp32" | "mixed" | "learned"), `tolerance`, `max_cycles`,
+("fixed" | "fp32" | "mixed" | "fcg" | "learned"), `tolerance`, `max_cycles`,
 `pre_smooth`, `post_smooth`, `coarse_extra` (defaults: fixed, `iterations`
 W(2, 4) cycles with 64 extra coarsest sweeps, reference pressure_solve.py:61-83).
 """
@@ -49,6 +49,9 @@ class StepParams:
     flip_ratio: float = 1.0
     particles_per_cell: float = PARTICLES_PER_CELL
     solver: dict = field(default_factory=lambda: {"mode": "fixed", "max_cycles": 1})
+    # called as observer(ctx, None) before and observer(ctx, stats) after the
+    # pressure solve (the learned preconditioner's training data is dumped here)
+    observer: object = None
 
 
 def step(ctx, params: StepParams, timings=None) -> dict:
@@ -67,7 +70,7 @@ def step(ctx, params: StepParams, timings=None) -> dict:
         boundary_conditions.grid_domain_bc(ctx)
         boundary_conditions.grid_obstacle_bc(ctx)
     with timings.section("solver", "pressure_solve", ctx):
-        stats = pressure_solve.pressure_solve(ctx, params.dt, params.rho, params.solver)
+        stats = pressure_solve.pressure_solve(ctx, params.dt, params.rho, params.solver, params.observer)
     with timings.section("solver", "pressure_correction", ctx):
         pressure_solve.pressure_correction(ctx, params.dt, params.rho)
     with timings.section("solver", "projected_grid_domain_bc", ctx):

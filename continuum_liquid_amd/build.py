@@ -18,7 +18,9 @@ from pathlib import Path
 PACKAGE_DIR = Path(__file__).resolve().parent
 CSRC = PACKAGE_DIR / "csrc"
 LIBRARY = PACKAGE_DIR / "libcliquid_hip.so"
-SOURCES = ("context.hip", "binning.hip", "p2g.hip", "grid.hip", "multigrid.hip", "g2p.hip", "voxelise.hip", "fcg.hip")
+SOURCES = (
+    "context.hip", "binning.hip", "p2g.hip", "grid.hip", "multigrid.hip", "g2p.hip", "voxelise.hip", "fcg.hip", "unet.hip",
+)
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -165,6 +165,14 @@ void launch_fcg_direction(const Level &L, const float *z, const double *d_old, d
 void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const double *d, const double *qv,
                        double *partials, double *s, hipStream_t st);
 
+// context.hip: sets the message returned by clq_last_error(), returns code
+int set_error(int code, const char *fmt, ...);
+
+// unet.hip
+int unet_check_grid(const clq_net *n, int nx, int ny, int nz);
+int unet_device(const clq_net *n);
+void unet_predict(clq_net *n, const float *rhs, const float *mask, const float *prev, float *out, hipStream_t s);
+
 // voxelise.hip
 int voxelise_max_scale();
 void launch_voxelise(const float *pos, int n, const int32_t *offsets, const int32_t *counts, int s, float sinv,

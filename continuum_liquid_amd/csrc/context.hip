@@ -32,6 +32,18 @@ int fail(int code, const char *fmt, ...) {
                                           __FILE__, __LINE__);                                      \
     } while (0)
 
+}  // namespace
+
+int clq::set_error(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof g_error, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+namespace {
+
 int check_launch(const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(CLQ_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
@@ -70,6 +82,10 @@ struct clq_ctx {
     bool p64_valid = false;  // p64a holds the pressure of the last solve (modes 2-4)
     // flexible-CG state, allocated on first use
     double *r64 = nullptr, *dir[2] = {nullptr, nullptr}, *q64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
+    // learned preconditioner (not owned) and its FP32 initial-guess buffer
+    clq_net *net = nullptr;
+    double stall_ratio = 0.5;
+    float *guess32 = nullptr;
     hipEvent_t ev_start[CLQ_STAGE_COUNT] = {}, ev_stop[CLQ_STAGE_COUNT] = {};
     bool ev_used[CLQ_STAGE_COUNT] = {};
     // per-kernel-class event pairs while kernel timing is enabled
@@ -748,7 +764,7 @@ static int read_scalars(clq_ctx *c) {
     return CLQ_OK;
 }
 
-static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
+static int ensure_fcg_buffers(clq_ctx *c) {
     if (int rc = ensure_mixed_buffers(c)) return rc;
     if (!c->r64) {
         void *q;
@@ -765,6 +781,11 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
         if (int rc = alloc_zeroed(c, &q, (size_t)fcg_partials_count() * 8)) return rc;
         c->fcg_partials = (double *)q;
     }
+    return CLQ_OK;
+}
+
+static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
+    if (int rc = ensure_fcg_buffers(c)) return rc;
     const Level &L0 = c->lv[0];
     const double h = (double)(float)c->dx;
     const double inv_h2 = 1.0 / (h * h);
@@ -798,21 +819,102 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     return check_launch("solve_fcg");
 }
 
+// Mode 3: FCG with the network as initial guess and preconditioner, the
+// W-cycle taking over for good once an iteration reduces the residual by less
+// than stall_ratio.  z (network or W-cycle output) lands in the level-0
+// pressure array, as in mode 4.
+static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
+    if (!c->net) return fail(CLQ_ERR_INVALID, "CLQ_SOLVE_LEARNED_FCG needs a network: call clq_attach_net first");
+    if (int rc = ensure_fcg_buffers(c)) return rc;
+    if (!c->guess32) {
+        void *q;
+        if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 4)) return rc;
+        c->guess32 = (float *)q;
+    }
+    const Level &L0 = c->lv[0];
+    const double h = (double)(float)c->dx;
+    const double inv_h2 = 1.0 / (h * h);
+    // warm start (the previous pressure) against the network's guess
+    double r_warm = 0.0, bn = 0.0;
+    if (int rc = residual_norm(c, &r_warm, &bn)) return rc;
+    unet_predict(c->net, c->rhs, L0.mask, c->p, c->guess32, c->stream);
+    launch_fcg_init(L0, c->guess32, c->rhs, inv_h2, c->p64a, c->r64, c->r32, c->fcg_partials, c->fcg_s, c->stream);
+    if (int rc = read_scalars(c)) return rc;
+    double r = sqrt(c->norm_host[0]);
+    bn = sqrt(c->norm_host[1]);
+    if (!(r < r_warm)) {
+        launch_fcg_init(L0, c->p, c->rhs, inv_h2, c->p64a, c->r64, c->r32, c->fcg_partials, c->fcg_s, c->stream);
+        if (int rc = read_scalars(c)) return rc;
+        r = sqrt(c->norm_host[0]);
+    }
+    s->initial_residual = r;
+    s->rhs_norm = bn;
+    bool use_net = true;
+    auto precondition = [&](double rnorm) -> int {
+        if (use_net) {
+            unet_predict(c->net, c->r32, L0.mask, nullptr, c->p, c->stream);
+        } else {
+            if (int rc = precondition_wcycle(c, sp, rnorm)) return rc;
+            s->fallback_cycles++;
+        }
+        s->cycles++;
+        return CLQ_OK;
+    };
+    int cur = 0;
+    if (r > sp->tolerance * bn && sp->max_cycles > 0) {
+        if (int rc = precondition(r)) return rc;
+        launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, c->fcg_partials, c->fcg_s, c->stream);
+        for (int it = 0;; ++it) {
+            launch_fcg_direction(L0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, c->fcg_partials,
+                                 c->fcg_s, c->stream);
+            cur ^= 1;
+            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, c->fcg_partials, c->fcg_s, c->stream);
+            if (int rc = read_scalars(c)) return rc;
+            const double before = r;
+            r = sqrt(c->norm_host[0]);
+            if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
+            if (use_net && r > c->stall_ratio * before) use_net = false;
+            if (int rc = precondition(r)) return rc;
+            launch_fcg_dots(L0, c->p, c->q64, c->r64, 0, c->fcg_partials, c->fcg_s, c->stream);
+        }
+    }
+    launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
+    s->final_residual = r;
+    s->relative_residual = bn > 0 ? r / bn : 0.0;
+    return check_launch("solve_learned");
+}
+
+int clq_attach_net(clq_ctx *c, clq_net *net, double stall_ratio) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    if (net) {
+        if (int rc = unet_check_grid(net, c->nx, c->ny, c->nz)) return rc;
+        if (unet_device(net) != c->device) return fail(CLQ_ERR_INVALID, "network and context are on different devices");
+        if (!(stall_ratio > 0.0 && stall_ratio <= 1.0)) return fail(CLQ_ERR_INVALID, "stall_ratio must be in (0, 1]");
+    }
+    c->net = net;
+    c->stall_ratio = stall_ratio;
+    return CLQ_OK;
+}
+
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
     if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0)
         return fail(CLQ_ERR_INVALID, "negative sweep or cycle count");
-    if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE &&
-        sp->mode != CLQ_SOLVE_MIXED_REFINEMENT && sp->mode != CLQ_SOLVE_FCG)
-        return fail(CLQ_ERR_UNSUPPORTED, "solve mode %d is not available in this build", sp->mode);
+    if (sp->mode < CLQ_SOLVE_FIXED_CYCLES || sp->mode > CLQ_SOLVE_FCG)
+        return fail(CLQ_ERR_INVALID, "unknown solve mode %d", sp->mode);
+    if (sp->mode == CLQ_SOLVE_LEARNED_FCG && !c->net)
+        return fail(CLQ_ERR_INVALID, "CLQ_SOLVE_LEARNED_FCG needs a network: call clq_attach_net first");
     DeviceGuard g(c->device);
     clq_solve_stats s;
     memset(&s, 0, sizeof s);
     c->p64_valid = false;
     stage_begin(c, CLQ_STAGE_PRESSURE_SOLVE);
     ensure_coarse_operators(c);
-    if (sp->mode == CLQ_SOLVE_MIXED_REFINEMENT || sp->mode == CLQ_SOLVE_FCG) {
-        if (int rc = sp->mode == CLQ_SOLVE_FCG ? solve_fcg(c, sp, &s) : solve_mixed(c, sp, &s)) return rc;
+    if (sp->mode == CLQ_SOLVE_MIXED_REFINEMENT || sp->mode == CLQ_SOLVE_FCG || sp->mode == CLQ_SOLVE_LEARNED_FCG) {
+        const int rc = sp->mode == CLQ_SOLVE_FCG           ? solve_fcg(c, sp, &s)
+                       : sp->mode == CLQ_SOLVE_LEARNED_FCG ? solve_learned(c, sp, &s)
+                                                           : solve_mixed(c, sp, &s);
+        if (rc) return rc;
         c->p64_valid = true;
         launch_neumann_bc(c->p, c->nx, c->ny, c->nz, c->stream);
         stage_end(c, CLQ_STAGE_PRESSURE_SOLVE);

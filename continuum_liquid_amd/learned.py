@@ -1,0 +1,197 @@
+"""The learned preconditioner: training data from the solver itself, training
+on the GPU through the C-ABI (clq_net_train_step), and weight files.
+
+A training pair is what one pressure solve of a running simulation sees and
+returns: the right-hand side (scaled divergence), the fluid fraction (the
+solve's mask), the pressure before the solve (the previous step's, the warm
+start) and the FP64 solution of the solve rounded to FP32.  Pairs come from
+`scenes.random_scene` simulations advanced with the FP64 FCG solve.
+
+The network is fully convolutional: weights trained on one grid shape are
+loaded into a `runtime.Network` of any other shape.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import runtime, scenes
+from .Solver.Kernel import solver as host_solver
+
+WEIGHTS_DIR = Path(__file__).resolve().parent / "weights"
+DEFAULT_WEIGHTS = WEIGHTS_DIR / "unet_pressure.npy"
+
+# the solve that produces the targets
+TARGET_SOLVER = {"mode": "fcg", "tolerance": 1e-8, "max_cycles": 60, "pre_smooth": 2, "post_smooth": 2,
+                 "coarse_extra": 16}
+
+
+@dataclass
+class PairSet:
+    rhs: np.ndarray     # [n, nx, ny, nz] float32
+    mask: np.ndarray
+    prev: np.ndarray
+    target: np.ndarray
+
+    def __len__(self) -> int:
+        return len(self.rhs)
+
+    def save(self, path) -> None:
+        np.savez(path, rhs=self.rhs, mask=self.mask, prev=self.prev, target=self.target)
+
+    @classmethod
+    def load(cls, path) -> "PairSet":
+        with np.load(path, allow_pickle=False) as f:
+            return cls(f["rhs"], f["mask"], f["prev"], f["target"])
+
+    def split(self, n_train: int) -> tuple["PairSet", "PairSet"]:
+        a = PairSet(self.rhs[:n_train], self.mask[:n_train], self.prev[:n_train], self.target[:n_train])
+        b = PairSet(self.rhs[n_train:], self.mask[n_train:], self.prev[n_train:], self.target[n_train:])
+        return a, b
+
+
+def collect_pairs(lib, shape, n_scenes: int, steps: int, seed: int = 0, dt: float = 1.0 / 240.0,
+                  flip_ratio: float = 0.95, skip: int = 0, log=None) -> PairSet:
+    """Simulate `n_scenes` random scenes for `skip + steps` steps each and keep
+    one pair per solve after the first `skip` steps of each scene."""
+    out = {k: [] for k in ("rhs", "mask", "prev", "target")}
+    for s in range(n_scenes):
+        scene = scenes.random_scene(shape, seed + s)
+        if scene.fluid_cells == 0:
+            continue
+        ctx = runtime.Context(lib, scene.shape, scene.dx, capacity=scene.particle_count)
+        ctx.set_particles(scene.positions, scene.velocities)
+        pending = {}
+        step_index = [0]
+
+        def observer(c, stats):
+            if step_index[0] < skip:
+                return
+            if stats is None:
+                pending["rhs"] = c.get_field("rhs")
+                pending["mask"] = c.get_field("fluid_fraction")
+                pending["prev"] = c.get_field("pressure")
+            else:
+                if stats["relative_residual"] > 1e-6:
+                    return  # keep converged solves only
+                out["rhs"].append(pending["rhs"])
+                out["mask"].append(pending["mask"])
+                out["prev"].append(pending["prev"])
+                out["target"].append(c.get_field("pressure_fp64").astype(np.float32))
+
+        params = host_solver.StepParams(dt=dt, gravity=scene.gravity, flip_ratio=flip_ratio,
+                                        solver=dict(TARGET_SOLVER), observer=observer)
+        for k in range(skip + steps):
+            step_index[0] = k
+            host_solver.step(ctx, params)
+        ctx.close()
+        if log:
+            log(f"scene {s}: {scene.fluid_cells} fluid cells, {len(out['rhs'])} pairs so far")
+    return PairSet(*(np.stack(out[k]) if out[k] else np.zeros((0,) + tuple(shape), np.float32)
+                     for k in ("rhs", "mask", "prev", "target")))
+
+
+def train(net: "runtime.Network", data: PairSet, epochs: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
+          prev_dropout: float = 0.5, log=None) -> list[float]:
+    """Adam on the mean squared error of the scaled pressure (clq_net_train_step).
+
+    With probability `prev_dropout` a sample's previous-pressure channel is
+    zeroed: inside FCG the network is applied to residuals with no previous
+    solution, as a preconditioner, and to (rhs, previous pressure) as the
+    initial guess; both uses are trained."""
+    rng = np.random.default_rng(seed)
+    losses = []
+    n = len(data)
+    t0 = time.perf_counter()
+    for epoch in range(epochs):
+        order = rng.permutation(n)
+        epoch_loss, batches = 0.0, 0
+        for i in range(0, n - batch + 1, batch):
+            idx = np.sort(order[i:i + batch])
+            prev = data.prev[idx].copy()
+            prev[rng.random(len(idx)) < prev_dropout] = 0.0
+            loss = net.train_step(data.rhs[idx], data.mask[idx], prev, data.target[idx], lr)
+            epoch_loss += loss
+            batches += 1
+        losses.append(epoch_loss / max(batches, 1))
+        if log:
+            log(f"epoch {epoch}: loss {losses[-1]:.4e} ({time.perf_counter() - t0:.1f} s)")
+    return losses
+
+
+def relative_error(net: "runtime.Network", data: PairSet, with_prev: bool = True) -> float:
+    """Mean over pairs of ||out - target|| / ||target|| on the interior fluid cells."""
+    errs = []
+    inner = scenes.interior_mask(data.rhs.shape[1:])
+    for i in range(len(data)):
+        out = net.forward(data.rhs[i], data.mask[i], data.prev[i] if with_prev else None)
+        sel = inner & (data.mask[i] > 0)
+        t = data.target[i][sel].astype(np.float64)
+        errs.append(np.linalg.norm(out[sel] - t) / max(np.linalg.norm(t), 1e-30))
+    return float(np.mean(errs))
+
+
+def save_weights(net: "runtime.Network", path=DEFAULT_WEIGHTS) -> Path:
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    np.save(path, net.get_params())
+    return path
+
+
+def load_weights(lib, shape, path=DEFAULT_WEIGHTS) -> "runtime.Network":
+    """A network for `shape` with the parameters stored at `path`."""
+    params = np.load(path, allow_pickle=False)
+    net = runtime.Network(lib, shape, seed=None)
+    if params.size != net.param_count:
+        raise ValueError(f"{path}: {params.size} parameters, expected {net.param_count}")
+    net.set_params(params)
+    return net
+
+
+def main(argv=None) -> int:
+    """Generate pairs on the GPU, train, and write the weights file."""
+    import argparse
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--shape", type=int, nargs=3, default=(64, 64, 64))
+    ap.add_argument("--scenes", type=int, default=24)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--skip", type=int, default=2)
+    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--holdout", type=int, default=2, help="scenes kept out of training for the report")
+    ap.add_argument("--out", type=Path, default=DEFAULT_WEIGHTS)
+    ap.add_argument("--pairs", type=Path, default=None, help="also save the pairs here (.npz)")
+    args = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+
+    lib = runtime.load_library()
+    shape = tuple(args.shape)
+    t0 = time.perf_counter()
+    data = collect_pairs(lib, shape, args.scenes, args.steps, seed=args.seed, skip=args.skip, log=log)
+    log(f"{len(data)} pairs in {time.perf_counter() - t0:.1f} s")
+    if args.pairs:
+        data.save(args.pairs)
+    per_scene = len(data) // max(args.scenes, 1)
+    train_set, test_set = data.split(len(data) - args.holdout * per_scene)
+    net = runtime.Network(lib, shape, seed=args.seed)
+    before = relative_error(net, test_set) if len(test_set) else float("nan")
+    train(net, train_set, args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, log=log)
+    after = relative_error(net, test_set) if len(test_set) else float("nan")
+    after_np = relative_error(net, test_set, with_prev=False) if len(test_set) else float("nan")
+    log(f"held-out relative error: untrained {before:.3e}, trained {after:.3e} (no previous pressure: {after_np:.3e})")
+    path = save_weights(net, args.out)
+    log(f"weights written to {path}")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

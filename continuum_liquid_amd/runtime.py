@@ -22,7 +22,7 @@ import numpy as np
 
 PACKAGE_DIR = Path(__file__).resolve().parent
 HIP_LIBRARY = PACKAGE_DIR / "libcliquid_hip.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # enum clq_status
 OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NOT_CONVERGED = range(6)
@@ -129,7 +129,22 @@ _SIGNATURES = {
     "clq_kernel_timing": ([_P, c_int32], ctypes.c_int),
     "clq_kernel_times": ([_P, POINTER(c_double), POINTER(c_int64), c_int32], ctypes.c_int),
 }
-EXPORTED_SYMBOLS = ("clq_last_error", *_SIGNATURES)
+# The learned preconditioner's entry points (GPU library only: the CPU
+# oracle restates the FLIP step, the network's CPU check is a torch model).
+_NET_SIGNATURES = {
+    "clq_net_param_count": ([], c_int64),
+    "clq_net_create": ([c_int32, c_int32, c_int32, POINTER(_P)], ctypes.c_int),
+    "clq_net_destroy": ([_P], ctypes.c_int),
+    "clq_net_init": ([_P, ctypes.c_uint64], ctypes.c_int),
+    "clq_net_set_params": ([_P, _P, c_int64], ctypes.c_int),
+    "clq_net_get_params": ([_P, _P, c_int64], ctypes.c_int),
+    "clq_net_forward": ([_P, _P, _P, _P, _P], ctypes.c_int),
+    "clq_net_backward": ([_P, _P, _P, c_int64], ctypes.c_int),
+    "clq_net_train_step": ([_P, c_int32, _P, _P, _P, _P, c_double, POINTER(c_double)], ctypes.c_int),
+    "clq_attach_net": ([_P, _P, c_double], ctypes.c_int),
+}
+EXPORTED_SYMBOLS = ("clq_last_error", *_SIGNATURES, *_NET_SIGNATURES)
+NET_SYMBOLS = tuple(_NET_SIGNATURES)
 
 
 def bind_library(lib: ctypes.CDLL) -> ctypes.CDLL:
@@ -140,6 +155,11 @@ def bind_library(lib: ctypes.CDLL) -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
+    for name, (args, res) in _NET_SIGNATURES.items():
+        if hasattr(lib, name):
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
     version = lib.clq_abi_version()
     if version != ABI_VERSION:
         raise RuntimeError(f"library ABI version {version} != expected {ABI_VERSION}")
@@ -200,6 +220,7 @@ class Context:
         if getattr(self, "handle", None) is not None and self.handle.value:
             self.lib.clq_destroy(self.handle)
             self.handle = None
+        self._net = None
 
     def __enter__(self) -> "Context":
         return self
@@ -324,6 +345,15 @@ class Context:
         self._check(self.lib.clq_pressure_solve(self.handle, byref(params), byref(stats)), "clq_pressure_solve")
         return stats.as_dict()
 
+    def attach_net(self, net: "Network | None", stall_ratio: float = 0.5) -> None:
+        """Use `net` in SOLVE_LEARNED_FCG (None detaches).  The context keeps a
+        reference so the network outlives its use here."""
+        if not hasattr(self.lib, "clq_attach_net"):
+            raise ClqError(ERR_UNSUPPORTED, "clq_attach_net", "this library has no learned preconditioner")
+        self._check(self.lib.clq_attach_net(self.handle, net.handle if net is not None else None, stall_ratio),
+                    "clq_attach_net")
+        self._net = net
+
     def w_cycle(self, pre_smooth: int = 2, post_smooth: int = 4, coarse_extra: int = 64) -> None:
         self._check(self.lib.clq_w_cycle(self.handle, pre_smooth, post_smooth, coarse_extra), "clq_w_cycle")
 
@@ -408,3 +438,84 @@ def device_count(lib: ctypes.CDLL) -> int:
     count = c_int32()
     status = lib.clq_device_count(byref(count))
     return count.value if status == OK else 0
+
+
+class Network:
+    """The learned preconditioner's U-Net for one pressure-grid shape, on the
+    device (include/cliquid.h, clq_net_*).  Arrays are float32 grid-shaped."""
+
+    def __init__(self, lib: ctypes.CDLL, shape, seed: int | None = 0):
+        if not hasattr(lib, "clq_net_create"):
+            raise ClqError(ERR_UNSUPPORTED, "clq_net_create", "this library has no learned preconditioner")
+        self.lib = lib
+        self.shape = tuple(int(n) for n in shape)
+        handle = c_void_p()
+        self._check(lib.clq_net_create(*self.shape, byref(handle)), "clq_net_create")
+        self.handle = handle
+        self.param_count = int(lib.clq_net_param_count())
+        if seed is not None:
+            self._check(lib.clq_net_init(self.handle, int(seed)), "clq_net_init")
+
+    _check = Context._check
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.clq_net_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self) -> None:  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _grid(self, a, name: str) -> np.ndarray:
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.shape != self.shape:
+            raise ValueError(f"{name} has shape {a.shape}, expected {self.shape}")
+        return a
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.param_count, np.float32)
+        self._check(self.lib.clq_net_get_params(self.handle, _ptr(out), out.size), "clq_net_get_params")
+        return out
+
+    def set_params(self, params) -> None:
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.lib.clq_net_set_params(self.handle, _ptr(params), params.size), "clq_net_set_params")
+
+    def forward(self, rhs, mask, prev=None) -> np.ndarray:
+        rhs, mask = self._grid(rhs, "rhs"), self._grid(mask, "mask")
+        prev = self._grid(prev, "prev") if prev is not None else None
+        out = np.empty(self.shape, np.float32)
+        self._check(self.lib.clq_net_forward(self.handle, _ptr(rhs), _ptr(mask),
+                                             _ptr(prev) if prev is not None else None, _ptr(out)), "clq_net_forward")
+        return out
+
+    def backward(self, grad_out) -> np.ndarray:
+        grad_out = self._grid(grad_out, "grad_out")
+        grads = np.empty(self.param_count, np.float32)
+        self._check(self.lib.clq_net_backward(self.handle, _ptr(grad_out), _ptr(grads), grads.size),
+                    "clq_net_backward")
+        return grads
+
+    def train_step(self, rhs, mask, prev, target, lr: float = 1e-3) -> float:
+        """One Adam step on a batch ([batch, *shape] arrays; prev may be None)."""
+        def batch(a, name):
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            if a.ndim == 3:
+                a = a[None]
+            if a.shape[1:] != self.shape:
+                raise ValueError(f"{name} has shape {a.shape}, expected (batch, *{self.shape})")
+            return a
+        rhs, mask, target = batch(rhs, "rhs"), batch(mask, "mask"), batch(target, "target")
+        prev = batch(prev, "prev") if prev is not None else None
+        n = rhs.shape[0]
+        if mask.shape[0] != n or target.shape[0] != n or (prev is not None and prev.shape[0] != n):
+            raise ValueError("batch sizes differ")
+        loss = c_double()
+        self._check(self.lib.clq_net_train_step(self.handle, n, _ptr(rhs), _ptr(mask),
+                                                _ptr(prev) if prev is not None else None, _ptr(target), lr,
+                                                byref(loss)), "clq_net_train_step")
+        return loss.value
+

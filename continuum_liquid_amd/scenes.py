@@ -150,3 +150,43 @@ def small_scene(n: int = 24, seed: int = 0, spheres: bool = True) -> Scene:
     """A small dam break (+ falling sphere) for parity tests."""
     sph = [Sphere((0.65, 0.7, 0.5), 0.18, (0.3, -2.0, 0.1))] if spheres else []
     return dam_break(f"dam-break {n}^3", (n, n, n), 1.0 / n, (0.4, 0.5), sph, seed)
+
+
+def random_scene(shape, seed: int, dx: float | None = None) -> Scene:
+    """Training scene of the learned preconditioner: 1-3 fluid boxes and 0-2
+    spheres of random sizes and positions, each body moving with its own
+    velocity drawn uniformly from [-3, 3] m/s per axis; gravity -9.81 on y."""
+    shape = tuple(int(n) for n in shape)
+    dx = dx if dx is not None else 1.0 / max(shape)
+    rng = np.random.default_rng(seed)
+    X = cell_centres(shape, dx)
+    inside = interior_mask(shape)
+    fluid = np.zeros(shape, bool)
+    cell_vel = np.zeros(shape + (3,), np.float32)
+    bodies = []
+    for _ in range(int(rng.integers(1, 4))):
+        lo = rng.uniform(0.0, 0.7, 3)
+        hi = np.minimum(lo + rng.uniform(0.15, 0.6, 3), 1.0)
+        lo[1] = 0.0 if rng.random() < 0.5 else lo[1]  # half of the boxes rest on the floor
+        m = inside & ~fluid
+        for a in range(3):
+            m &= (X[a] >= lo[a]) & (X[a] < hi[a])
+        vel = rng.uniform(-3.0, 3.0, 3).astype(np.float32)
+        fluid |= m
+        cell_vel[m] = vel
+        bodies.append({"box": (lo.tolist(), hi.tolist()), "velocity": vel.tolist()})
+    extent = np.array(shape, np.float64) * dx
+    for _ in range(int(rng.integers(0, 3))):
+        r = float(rng.uniform(0.08, 0.2)) * float(extent.min())
+        c = rng.uniform(0.2, 0.8, 3) * extent
+        d2 = sum((X[a] * extent[a] - c[a]) ** 2 for a in range(3))
+        m = inside & ~fluid & (d2 < r * r)
+        vel = rng.uniform(-3.0, 3.0, 3).astype(np.float32)
+        fluid |= m
+        cell_vel[m] = vel
+        bodies.append({"sphere": (c.tolist(), r), "velocity": vel.tolist()})
+    positions = stratified_fill(fluid, dx, rng)
+    velocities = np.ascontiguousarray(np.repeat(cell_vel[fluid], 8, axis=0))
+    return Scene(name=f"random scene {seed}", shape=shape, dx=dx, positions=positions, velocities=velocities,
+                 gravity=(0.0, -9.81, 0.0), fluid_cells=int(fluid.sum()), notes={"seed": seed, "bodies": bodies})
+

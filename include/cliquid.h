@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define CLQ_ABI_VERSION 1
+#define CLQ_ABI_VERSION 2
 
 enum clq_status {
     CLQ_OK = 0,
@@ -106,8 +106,12 @@ enum clq_solve_mode {
      * 16-bit coarse-level residual storage; FP64 residual, FP64 correction
      * accumulation and FP64 coarsest solve. */
     CLQ_SOLVE_MIXED_REFINEMENT = 2,
-    /* Flexible CG preconditioned by the learned network with the W-cycle as
-     * fallback; requires clq_set_preconditioner_weights(). */
+    /* Flexible CG in FP64 with the learned network (clq_attach_net) as the
+     * initial guess and as the preconditioner; once an iteration leaves the
+     * residual above stall_ratio x its previous value, the mixed-precision
+     * W-cycle of mode 4 preconditions every later iteration.  The network's
+     * initial guess is kept only when its FP64 residual is below the warm
+     * start's (the previous pressure, as the reference starts). */
     CLQ_SOLVE_LEARNED_FCG = 3,
     /* Flexible CG in FP64 preconditioned by the mixed-precision W-cycle of
      * mode 2 (one W-cycle per iteration). */
@@ -125,8 +129,9 @@ typedef struct clq_solve_params {
 } clq_solve_params;
 
 typedef struct clq_solve_stats {
-    int32_t cycles;            /* W-cycles performed (one per FCG iteration)    */
-    int32_t fallback_cycles;   /* mode 3: W-cycle fallbacks taken               */
+    int32_t cycles;            /* cycles (modes 0-2) or FCG iterations (3, 4)   */
+    int32_t fallback_cycles;   /* mode 3: iterations preconditioned by the
+                                  W-cycle after the network stalled          */
     double rhs_norm;           /* ||b||_2 (FP64 accumulation)                    */
     double initial_residual;   /* ||b - A p0||_2 before the solve               */
     double final_residual;     /* ||b - A p||_2 after the solve                 */
@@ -265,6 +270,41 @@ enum clq_kernel_class {
 int clq_kernel_timing(clq_ctx *ctx, int32_t enable);
 /* Writes total ms and launch-unit count per class; returns classes written. */
 int clq_kernel_times(clq_ctx *ctx, double *total_ms, int64_t *launches, int32_t capacity);
+
+/* ---- learned preconditioner --------------------------------------------------
+ * No counterpart in the reference.  A 3-level 3D U-Net (16/32/64 channels,
+ * 3x3x3 convolutions, ReLU, BF16 weights and activations with FP32
+ * accumulation on the matrix cores) mapping (rhs, fluid mask, previous
+ * pressure) on a pressure grid to the pressure; its output is scaled by the
+ * RMS of rhs over the interior fluid cells (mask > 0), its input divided by it.
+ * Parameter vector (FP32), in order: for each of the 10 conv layers
+ * (16->16, 16->16, 16->32, 32->32, 32->64, 64->64, 96->32, 32->32, 48->16,
+ * 16->16) the weights W[tap][c_in][c_out] with tap = 9(dx+1) + 3(dy+1) + (dz+1)
+ * then the c_out biases; then the 1x1 head's 16 weights and its bias.
+ * Host arrays are float32 [nx][ny][nz] of the pressure grid. */
+typedef struct clq_net clq_net;
+int64_t clq_net_param_count(void);
+/* A network for an (nx, ny, nz) pressure grid on the current device. */
+int clq_net_create(int32_t nx, int32_t ny, int32_t nz, clq_net **out);
+int clq_net_destroy(clq_net *net);
+/* He-normal weights from a fixed generator seeded with `seed`, zero biases. */
+int clq_net_init(clq_net *net, uint64_t seed);
+int clq_net_set_params(clq_net *net, const float *params, int64_t count);
+int clq_net_get_params(clq_net *net, float *params, int64_t count);
+/* out = network(rhs, mask, prev); prev may be NULL (zero channel). */
+int clq_net_forward(clq_net *net, const float *rhs, const float *mask, const float *prev, float *out);
+/* Parameter gradients of sum(grad_out * out) for the last clq_net_forward. */
+int clq_net_backward(clq_net *net, const float *grad_out, float *grads, int64_t count);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on the mean over `batch` samples of
+ * the mean squared error between out / s and target / s over the interior
+ * cells with mask > 0 (s = the input scale above).  Arrays are [batch][grid];
+ * prev may be NULL.  *loss receives the mean loss before the step. */
+int clq_net_train_step(clq_net *net, int32_t batch, const float *rhs, const float *mask, const float *prev,
+                       const float *target, double lr, double *loss);
+/* Use `net` (created for the context's grid, on its device) in
+ * CLQ_SOLVE_LEARNED_FCG; NULL detaches.  The context does not own the net:
+ * detach before destroying it. */
+int clq_attach_net(clq_ctx *ctx, clq_net *net, double stall_ratio);
 
 #ifdef __cplusplus
 }

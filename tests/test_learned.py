@@ -1,0 +1,227 @@
+"""The learned preconditioner.
+
+CPU tests pin the torch reference (tests/unet_reference.py) itself: its
+parameter layout against a direct loop over taps and channels, its scale
+invariance, and the random training scenes.  GPU tests compare the HIP
+network with that reference (forward within BF16 rounding, gradients within
+the rounding of dL/dY to BF16 on the device), train it with the device's own
+Adam step, and run the learned FCG solve to an FP64-checked residual.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from continuum_liquid_amd import runtime, scenes
+
+from . import unet_reference as ref
+from .helpers import make_context, operator_matrix, run_to_pressure
+
+SHAPE = (14, 11, 17)  # the network pads it to (16, 12, 20)
+
+
+def random_params(seed=0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    p = np.zeros(ref.param_count(), np.float32)
+    for name, sl in ref.layer_slices():
+        n = sl.stop - sl.start
+        if name.endswith("weight") and name.startswith("c"):
+            ci = ref.LAYERS[int(name[1:name.index(".")]) - 1][0]
+            p[sl] = rng.normal(0.0, np.sqrt(2.0 / (27 * ci)), n)
+        elif name == "head.weight":
+            p[sl] = rng.normal(0.0, 0.25, n)
+        else:
+            p[sl] = rng.normal(0.0, 0.05, n)
+    return p
+
+
+def random_inputs(seed=0, shape=SHAPE):
+    rng = np.random.default_rng(seed)
+    mask = np.zeros(shape, np.float32)
+    mask[1:-1, 1:-1, 1:-1] = (rng.random(tuple(n - 2 for n in shape)) < 0.7).astype(np.float32)
+    rhs = (rng.normal(size=shape) * 3e3 * (mask > 0)).astype(np.float32)
+    prev = (rng.normal(size=shape) * 40.0 * (mask > 0)).astype(np.float32)
+    return rhs, mask, prev
+
+
+# ---- CPU: the reference ------------------------------------------------------------------
+def test_reference_layout_matches_direct_loops():
+    """conv3d with the unpacked weight equals sum_{tap,ci} x[v + d(tap)] W[tap][ci][co]
+    evaluated directly from the parameter vector."""
+    p = random_params(1)
+    layers, _, _ = ref.unpack(p)
+    w, b = layers[2]  # 16 -> 32
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(16, 5, 4, 6))
+    got = F.conv3d(torch.as_tensor(x)[None], w, b, padding=1)[0].numpy()
+    W = p[ref.layer_slices()[4][1]].astype(np.float64).reshape(27, 16, 32)
+    bias = p[ref.layer_slices()[5][1]].astype(np.float64)
+    xp = np.pad(x, ((0, 0), (1, 1), (1, 1), (1, 1)))
+    want = np.zeros((32, 5, 4, 6))
+    for tap in range(27):
+        dx, dy, dz = tap // 9 - 1, (tap // 3) % 3 - 1, tap % 3 - 1
+        shifted = xp[:, 1 + dx:6 + dx, 1 + dy:5 + dy, 1 + dz:7 + dz]
+        want += np.einsum("cxyz,co->oxyz", shifted, W[tap])
+    want += bias[:, None, None, None]
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_reference_is_scale_invariant():
+    """out(k rhs, mask, k prev) = k out(rhs, mask, prev) exactly for k = 2^m
+    (the input is divided by the RMS of rhs, the output multiplied by it)."""
+    p = random_params(3)
+    rhs, mask, prev = random_inputs(4)
+    a = ref.forward(p, rhs, mask, prev).numpy()
+    b = ref.forward(p, rhs * 8.0, mask, prev * 8.0).numpy()
+    np.testing.assert_array_equal(b, 8.0 * a)
+
+
+def test_reference_gradient_matches_finite_differences():
+    """Autograd of the unrounded reference against central differences."""
+    p = random_params(5).astype(np.float64)
+    rhs, mask, prev = random_inputs(6, (8, 8, 8))
+    g_out = np.random.default_rng(7).normal(size=rhs.shape)
+    grads = ref.gradients(p, rhs, mask, prev, g_out, emulate=False)
+    rng = np.random.default_rng(8)
+    for name, sl in ref.layer_slices():
+        i = sl.start + int(rng.integers(sl.stop - sl.start))
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fp = (ref.forward(pp, rhs, mask, prev, emulate=False).numpy() * g_out).sum()
+        fm = (ref.forward(pm, rhs, mask, prev, emulate=False).numpy() * g_out).sum()
+        fd = (fp - fm) / (2 * h)
+        assert fd == pytest.approx(grads[i], rel=1e-4, abs=1e-6 * max(1.0, abs(fd))), name
+
+
+def test_random_scenes_are_seeded_and_bounded():
+    a, b = scenes.random_scene((24, 24, 24), 3), scenes.random_scene((24, 24, 24), 3)
+    assert a.digest() == b.digest()
+    assert 1 <= len(a.notes["bodies"]) <= 5
+    assert np.all(np.abs(a.velocities) <= 3.0)
+    inner = a.positions / a.dx
+    assert inner.min() >= 1.0 and inner.max() <= 23.0
+
+
+# ---- GPU: the HIP network ------------------------------------------------------------------
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("with_prev", [True, False], ids=["prev", "no_prev"])
+def test_forward_matches_reference(hip_lib, with_prev):
+    net = runtime.Network(hip_lib, SHAPE, seed=11)
+    p = net.get_params()
+    rhs, mask, prev = random_inputs(12)
+    prev = prev if with_prev else None
+    out = net.forward(rhs, mask, prev)
+    want = ref.forward(p, rhs, mask, prev).detach().numpy()
+    assert _rel(out, want) <= 2e-3
+    assert np.max(np.abs(out - want)) <= 2e-2 * np.max(np.abs(want))
+
+
+@pytest.mark.gpu
+def test_init_is_seeded(hip_lib):
+    a = runtime.Network(hip_lib, SHAPE, seed=5).get_params()
+    b = runtime.Network(hip_lib, (8, 8, 8), seed=5).get_params()
+    c = runtime.Network(hip_lib, SHAPE, seed=6).get_params()
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    for name, sl in ref.layer_slices():
+        if name.startswith("c") and name.endswith("weight"):
+            ci = ref.LAYERS[int(name[1:name.index(".")]) - 1][0]
+            assert np.std(a[sl]) == pytest.approx(np.sqrt(2.0 / (27 * ci)), rel=0.1)
+
+
+@pytest.mark.gpu
+def test_gradients_match_reference(hip_lib):
+    """Every parameter block's gradient within 3% (relative L2) of the FP64
+    autograd gradient of the BF16-emulating reference, cosine >= 0.999."""
+    net = runtime.Network(hip_lib, SHAPE, seed=21)
+    p = random_params(22)
+    net.set_params(p)
+    rhs, mask, prev = random_inputs(23)
+    net.forward(rhs, mask, prev)
+    g_out = (np.random.default_rng(24).normal(size=SHAPE) * (mask > 0)).astype(np.float32)
+    got = net.backward(g_out)
+    want = ref.gradients(net.get_params(), rhs, mask, prev, g_out)
+    for name, sl in ref.layer_slices():
+        a, b = got[sl].astype(np.float64), want[sl]
+        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300)
+        assert _rel(a, b) <= 3e-2, name
+        assert cos >= 0.999, name
+
+
+@pytest.mark.gpu
+def test_training_step_follows_adam_and_reduces_loss(hip_lib):
+    """The device's Adam step equals Adam on the reference gradient (first
+    step: w - lr * sign-like g / (|g| + eps)), and repeated steps on a fixed
+    batch drive the loss down."""
+    rhs, mask, prev = random_inputs(31)
+    target = (np.random.default_rng(32).normal(size=SHAPE) * 30.0 * (mask > 0)).astype(np.float32)
+    net = runtime.Network(hip_lib, SHAPE, seed=33)
+    p0 = net.get_params()
+    lr = 1e-3
+    losses = [net.train_step(rhs, mask, prev, target, lr)]
+    p1 = net.get_params()
+    # first Adam step with bias correction: delta = -lr * g / (|g| + eps)
+    # (|delta| = lr wherever |g| >> eps); compare the sign pattern with the
+    # reference gradient of the same loss
+    inv_s, s, count = ref.input_scale(rhs, mask)
+    inner = np.zeros(SHAPE, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    sel = inner & (mask > 0)
+    out = ref.forward(p0, rhs, mask, prev).detach().numpy()
+    # L = sum_sel (out / s - target / s)^2 / count  ->  dL/dout = 2 (out - target) / s^2 / count on sel
+    g_out = np.where(sel, 2.0 * (out / s - target.astype(np.float64) * inv_s) / s / count, 0.0)
+    g_ref = ref.gradients(p0, rhs, mask, prev, g_out)
+    delta = (p1 - p0).astype(np.float64)
+    big = np.abs(g_ref) > 1e-3 * np.max(np.abs(g_ref))
+    assert np.mean(np.sign(delta[big]) == -np.sign(g_ref[big])) >= 0.99
+    assert np.allclose(np.abs(delta[big]), lr, rtol=0.02)
+    for _ in range(60):
+        losses.append(net.train_step(rhs, mask, prev, target, lr))
+    assert losses[-1] < 0.2 * losses[0]
+
+
+@pytest.mark.gpu
+def test_attach_rejects_a_network_of_another_grid(hip_lib):
+    scene = scenes.small_scene(16)
+    ctx = make_context(hip_lib, scene)
+    net = runtime.Network(hip_lib, (16, 16, 12))
+    with pytest.raises(runtime.ClqError, match="does not match"):
+        ctx.attach_net(net)
+    with pytest.raises(runtime.ClqError, match="clq_attach_net"):
+        ctx.pressure_solve(runtime.SOLVE_LEARNED_FCG, max_cycles=5)
+
+
+@pytest.mark.gpu
+def test_learned_fcg_reaches_fp64_checked_tolerance(hip_lib, oracle_lib):
+    """Mode 3 with an untrained network: the W-cycle takes over once the
+    network stalls, and the solve reaches 1e-8 relative residual, checked in
+    FP64 against the numpy-assembled operator, on the oracle FCG's answer."""
+    scene = scenes.small_scene(24, seed=1)
+    h = make_context(hip_lib, scene)
+    o = make_context(oracle_lib, scene)
+    for c in (h, o):
+        run_to_pressure(c, scene)
+    net = runtime.Network(hip_lib, scene.shape, seed=41)
+    h.attach_net(net, stall_ratio=0.5)
+    kw = dict(pre_smooth=2, post_smooth=2, coarse_extra=16, max_cycles=60, tolerance=1e-8)
+    sh = h.pressure_solve(runtime.SOLVE_LEARNED_FCG, **kw)
+    so = o.pressure_solve(runtime.SOLVE_FCG, **kw)
+    assert sh["relative_residual"] <= 1e-8
+    assert sh["fallback_cycles"] >= 1
+    mask = h.get_field("fluid_fraction")
+    h2 = np.float64(np.float32(h.dx)) ** 2
+    A, fluid, _ = operator_matrix(*[h.get_field(n) for n in ("diagonal", "weight_x", "weight_y", "weight_z")], mask, h2)
+    b = h.get_field("rhs").astype(np.float64)[fluid]
+    p64 = h.get_field("pressure_fp64")[fluid]
+    assert np.linalg.norm(b - A @ p64) / np.linalg.norm(b) <= 1.01e-8
+    po = o.get_field("pressure_fp64")[fluid]
+    assert np.linalg.norm(p64 - po) <= 1e-6 * np.linalg.norm(po)
+    h.attach_net(None)
