@@ -106,6 +106,8 @@ def main() -> int:
     ap.add_argument("--stall-ratio", type=float, default=0.5,
                     help="learned solver: residual ratio above which the W-cycle takes over")
     ap.add_argument("--tolerance", type=float, default=1e-6)
+    ap.add_argument("--vcycle-from", type=int, default=None,
+                    help="refined solves: W-cycle above this level, V-cycle from it down (0: W everywhere)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -122,8 +124,10 @@ def main() -> int:
         return 1
 
     scene = scenes.baseline_scene(args.config)
-    params = host_solver.StepParams(dt=1.0 / 240.0, gravity=scene.gravity, flip_ratio=0.95,
-                                    solver=solver_settings(args.solver, args.tolerance))
+    settings = solver_settings(args.solver, args.tolerance)
+    if args.vcycle_from is not None:
+        settings["vcycle_from"] = args.vcycle_from
+    params = host_solver.StepParams(dt=1.0 / 240.0, gravity=scene.gravity, flip_ratio=0.95, solver=settings)
     print(f"# {scene.name}: {scene.particle_count} particles, grid {scene.shape}, solver {args.solver}",
           file=sys.stderr, flush=True)
 
@@ -132,7 +136,7 @@ def main() -> int:
         if args.solver == "learned":
             from continuum_liquid_amd import learned
 
-            ctx.attach_net(learned.load_weights(lib, scene.shape, args.weights or learned.DEFAULT_WEIGHTS),
+            ctx.attach_net(learned.load_weights(lib, scene.shape, scene.dx, args.weights or learned.DEFAULT_WEIGHTS),
                            args.stall_ratio)
         fallback = []
         for _ in range(args.warmup):
@@ -192,6 +196,7 @@ def main() -> int:
             "grid": list(scene.shape),
             "particles": n,
             "solver": args.solver,
+            "vcycle_from": settings.get("vcycle_from", 0),
         },
         "particle_steps_per_s": round(n * args.steps / elapsed, 1),
         "pressure_solve_ms": round(float(np.mean(solve_ms)), 3),

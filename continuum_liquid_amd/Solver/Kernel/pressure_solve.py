@@ -26,6 +26,7 @@ def pressure_solve(ctx, dt: float, rho: float, solver_settings: dict, observer=N
         max_cycles=int(solver_settings.get("max_cycles", 1)),
         tolerance=float(solver_settings.get("tolerance", 1e-6)),
         coarse_dtype=int(solver_settings.get("coarse_dtype", 0)),
+        vcycle_from=int(solver_settings.get("vcycle_from", 0)),
     )
     if observer is not None:
         observer(ctx, stats)

@@ -169,7 +169,7 @@ void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const d
 int set_error(int code, const char *fmt, ...);
 
 // unet.hip
-int unet_check_grid(const clq_net *n, int nx, int ny, int nz);
+int unet_check_grid(const clq_net *n, int nx, int ny, int nz, double dx);
 int unet_device(const clq_net *n);
 void unet_predict(clq_net *n, const float *rhs, const float *mask, const float *prev, float *out, hipStream_t s);
 

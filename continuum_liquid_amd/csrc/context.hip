@@ -532,15 +532,16 @@ static void restrict_level(clq_ctx *c, int level) {
 // (wcycle_bottom_kernel) for the rest of the recursion below them.
 constexpr int64_t kBottomCells = 8192;
 
-static void bottom_ops(const clq_ctx *c, int level, int pre, int post, int extra, bool f64, BottomArgs &a) {
+static void bottom_ops(const clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool f64,
+                       BottomArgs &a) {
     if (level == c->nlevels - 1) {
         a.ops[a.nops++] = BottomOp{f64 ? kOpCoarse64 : kOpCoarse32, level, pre + extra};
         return;
     }
     a.ops[a.nops++] = BottomOp{kOpSmooth, level, pre};
     a.ops[a.nops++] = BottomOp{kOpRestrict, level, 0};
-    bottom_ops(c, level + 1, pre, post, extra, f64, a);
-    bottom_ops(c, level + 1, pre, post, extra, f64, a);
+    bottom_ops(c, level + 1, pre, post, extra, vfrom, f64, a);
+    if (vfrom <= 0 || level < vfrom) bottom_ops(c, level + 1, pre, post, extra, vfrom, f64, a);
     a.ops[a.nops++] = BottomOp{kOpProlong, level, 0};
     a.ops[a.nops++] = BottomOp{kOpSmooth, level, post};
 }
@@ -554,11 +555,11 @@ static int bottom_level(const clq_ctx *c) {
     return c->nlevels;  // none
 }
 
-static void launch_bottom(clq_ctx *c, int level, int pre, int post, int extra, bool f64) {
+static void launch_bottom(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool f64) {
     BottomArgs a;
     for (int l = 0; l < c->nlevels; ++l) a.lv[l] = c->lv[l];
     a.nops = 0;
-    bottom_ops(c, level, pre, post, extra, f64, a);
+    bottom_ops(c, level, pre, post, extra, vfrom, f64, a);
     const float delta = (float)c->dx;
     a.dd = delta * delta;
     a.inv_delta2 = 1.0f / (delta * delta);
@@ -567,9 +568,9 @@ static void launch_bottom(clq_ctx *c, int level, int pre, int post, int extra, b
 }
 
 // multigrid.py:53-187
-static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra) {
+static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfrom) {
     if (level > 0 && level == bottom_level(c)) {
-        launch_bottom(c, level, pre, post, extra, false);
+        launch_bottom(c, level, pre, post, extra, vfrom, false);
         return;
     }
     if (level == c->nlevels - 1) {
@@ -578,8 +579,8 @@ static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra) {
     }
     smooth_level(c, level, pre);
     restrict_level(c, level);
-    w_cycle(c, level + 1, pre, post, extra);
-    w_cycle(c, level + 1, pre, post, extra);
+    w_cycle(c, level + 1, pre, post, extra, vfrom);
+    if (vfrom <= 0 || level < vfrom) w_cycle(c, level + 1, pre, post, extra, vfrom);
     launch_prolongate(c->lv[level], c->lv[level + 1], c->stream);
     smooth_level(c, level, post);
 }
@@ -619,7 +620,7 @@ int clq_w_cycle(clq_ctx *c, int32_t pre, int32_t post, int32_t extra) {
     if (!c || pre < 0 || post < 0 || extra < 0) return fail(CLQ_ERR_INVALID, "bad w_cycle arguments");
     DeviceGuard g(c->device);
     ensure_coarse_operators(c);
-    w_cycle(c, 0, pre, post, extra);
+    w_cycle(c, 0, pre, post, extra, 0);
     return check_launch("w_cycle");
 }
 
@@ -648,9 +649,9 @@ int clq_residual_norm(clq_ctx *c, double *residual, double *rhs_norm) {
 
 
 // ---- mixed-precision iterative refinement -------------------------------------------
-static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra) {
+static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, int vfrom) {
     if (level > 0 && level == bottom_level(c)) {
-        launch_bottom(c, level, pre, post, extra, true);
+        launch_bottom(c, level, pre, post, extra, vfrom, true);
         return;
     }
     if (level == c->nlevels - 1) {
@@ -661,8 +662,8 @@ static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra) {
     }
     smooth_level(c, level, pre);
     restrict_level(c, level);
-    w_cycle_mixed(c, level + 1, pre, post, extra);
-    w_cycle_mixed(c, level + 1, pre, post, extra);
+    w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
+    if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
     launch_prolongate(c->lv[level], c->lv[level + 1], c->stream);
     smooth_level(c, level, post);
 }
@@ -700,7 +701,7 @@ static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
         }
         L0.b = c->r32;
         HIP_TRY(hipMemsetAsync(c->p, 0, (size_t)c->C * 4, c->stream));
-        w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+        w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
         L0.b = c->rhs;
         const double before = r;
         if ((rc = ir_residual(c, 0, c->p64a, c->p64b, &r, &bn))) break;
@@ -752,7 +753,7 @@ static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rn
     Level &L0 = c->lv[0];
     L0.b = c->r32;
     HIP_TRY(hipMemsetAsync(c->p, 0, (size_t)c->C * 4, c->stream));
-    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
     L0.b = c->rhs;
     for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
     return CLQ_OK;
@@ -887,7 +888,7 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
 int clq_attach_net(clq_ctx *c, clq_net *net, double stall_ratio) {
     if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
     if (net) {
-        if (int rc = unet_check_grid(net, c->nx, c->ny, c->nz)) return rc;
+        if (int rc = unet_check_grid(net, c->nx, c->ny, c->nz, c->dx)) return rc;
         if (unet_device(net) != c->device) return fail(CLQ_ERR_INVALID, "network and context are on different devices");
         if (!(stall_ratio > 0.0 && stall_ratio <= 1.0)) return fail(CLQ_ERR_INVALID, "stall_ratio must be in (0, 1]");
     }
@@ -898,8 +899,9 @@ int clq_attach_net(clq_ctx *c, clq_net *net, double stall_ratio) {
 
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
-    if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0)
-        return fail(CLQ_ERR_INVALID, "negative sweep or cycle count");
+    if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0 ||
+        sp->vcycle_from < 0)
+        return fail(CLQ_ERR_INVALID, "negative sweep, cycle count or level");
     if (sp->mode < CLQ_SOLVE_FIXED_CYCLES || sp->mode > CLQ_SOLVE_FCG)
         return fail(CLQ_ERR_INVALID, "unknown solve mode %d", sp->mode);
     if (sp->mode == CLQ_SOLVE_LEARNED_FCG && !c->net)
@@ -932,7 +934,7 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     const double target = sp->tolerance * bn;
     for (int cyc = 0; cyc < sp->max_cycles; ++cyc) {
         if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE && r <= target) break;
-        w_cycle(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+        w_cycle(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
         s.cycles++;
         if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE) {
             const double before = r;

@@ -3,14 +3,16 @@
 // solve is the W-cycle alone, Solver/Kernel/pressure_solve.py:61-83).
 //
 // Architecture (channels; every conv is 3x3x3, zero padding, bias, ReLU):
-//   input 3 (rhs / s, fluid indicator, previous pressure / s), zero-padded to 16
+//   input 3 (rhs / s, fluid indicator, previous pressure / (s h^2)), zero-padded to 16
 //   L0: c1 16->16, c2 16->16 = s0        pool 2x2x2 (average)
 //   L1: c3 16->32, c4 32->32 = s1        pool
 //   L2: c5 32->64, c6 64->64             nearest upsample, concat [up, s1] -> 96
 //   L1: c7 96->32, c8 32->32             nearest upsample, concat [up, s0] -> 48
 //   L0: c9 48->16, c10 16->16            head: 1x1x1 16->1, linear, FP32
-// out = s * head, s = rms of rhs over the interior fluid cells, so the map is
-// invariant to the scale of the right-hand side.
+// With A = L / h^2 (L the stencil in grid units, h the cell size), the network
+// works on the grid-unit problem: inputs rhs / s and prev / (s h^2), output
+// p = s h^2 * head, s = rms of rhs over the interior fluid cells.  The map is
+// invariant to the scale of the right-hand side, and the prediction is O(1).
 // The network grid is the pressure grid rounded up to multiples of 4 in each
 // dimension (zero inputs in the padding).  Activations are BF16 with channels
 // fastest ([voxel][channel]); gradients are FP32.  Convolutions are implicit
@@ -346,8 +348,9 @@ __global__ __launch_bounds__(256) void norm_partials_kernel(const float *__restr
     }
 }
 
-// scal[0] = 1/s, scal[1] = s, scal[2] = 1/count (0 when there is no fluid cell)
-__global__ __launch_bounds__(256) void norm_final_kernel(const double *__restrict__ partials, int blocks,
+// scal[0] = 1/s, scal[1] = s h^2, scal[2] = 1/count, scal[3] = 1/(s h^2)
+// (all 0 when there is no fluid cell or rhs vanishes there)
+__global__ __launch_bounds__(256) void norm_final_kernel(const double *__restrict__ partials, int blocks, double h2,
                                                          float *__restrict__ scal) {
     __shared__ double sh[2][256];
     double s = 0.0, n = 0.0;
@@ -369,12 +372,13 @@ __global__ __launch_bounds__(256) void norm_final_kernel(const double *__restric
         const double cnt = sh[1][0];
         const double rms = cnt > 0.0 ? sqrt(sh[0][0] / cnt) : 0.0;
         scal[0] = rms > 0.0 ? (float)(1.0 / rms) : 0.0f;
-        scal[1] = (float)rms;
+        scal[1] = (float)(rms * h2);
         scal[2] = cnt > 0.0 ? (float)(1.0 / cnt) : 0.0f;
+        scal[3] = rms > 0.0 ? (float)(1.0 / (rms * h2)) : 0.0f;
     }
 }
 
-// in16[v] = bf16 [rhs / s, mask > 0, prev / s, 0 ... 0]
+// in16[v] = bf16 [rhs / s, mask > 0, prev / (s h^2), 0 ... 0]
 __global__ void pack_input_kernel(const float *__restrict__ rhs, const float *__restrict__ mask,
                                   const float *__restrict__ prev, const float *__restrict__ scal, Dims G, Dims d0,
                                   uint16_t *__restrict__ out) {
@@ -386,7 +390,7 @@ __global__ void pack_input_kernel(const float *__restrict__ rhs, const float *__
     if (grid_cell(G, d0, v, g, inner)) {
         const float is = scal[0];
         const uint16_t c0 = f2bf(rhs[g] * is), c1 = mask[g] > 0.0f ? (uint16_t)0x3f80 : (uint16_t)0;
-        const uint16_t c2 = prev ? f2bf(prev[g] * is) : (uint16_t)0;
+        const uint16_t c2 = prev ? f2bf(prev[g] * scal[3]) : (uint16_t)0;
         w0 = (uint32_t)c0 | ((uint32_t)c1 << 16);
         w1 = c2;
     }
@@ -405,7 +409,7 @@ __global__ void head_fwd_kernel(const uint16_t *__restrict__ y, const float *__r
     pred[v] = s;
 }
 
-// out[g] = s * pred[v(g)] on the pressure grid
+// out[g] = s h^2 * pred[v(g)] on the pressure grid
 __global__ void unpack_output_kernel(const float *__restrict__ pred, const float *__restrict__ scal, Dims G, Dims d0,
                                      float *__restrict__ out) {
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -415,7 +419,7 @@ __global__ void unpack_output_kernel(const float *__restrict__ pred, const float
     out[g] = scal[1] * pred[((int64_t)x * d0.Y + y) * d0.Z + z];
 }
 
-// gpred[v] = s * grad_out[g(v)] (dL/d pred from dL/d out), zero in the padding
+// gpred[v] = s h^2 * grad_out[g(v)] (dL/d pred from dL/d out), zero in the padding
 __global__ void pack_grad_kernel(const float *__restrict__ gout, const float *__restrict__ scal, Dims G, Dims d0,
                                  float *__restrict__ gpred) {
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -426,13 +430,13 @@ __global__ void pack_grad_kernel(const float *__restrict__ gout, const float *__
 }
 
 // Mean squared error of the normalised prediction over the interior fluid
-// cells: e = pred - target / s, L = sum e^2 / count, gpred = 2 e / count / batch.
+// cells: e = pred - target / (s h^2), L = sum e^2 / count, gpred = 2 e / count / batch.
 __global__ __launch_bounds__(256) void mse_kernel(const float *__restrict__ pred, const float *__restrict__ target,
                                                   const float *__restrict__ mask, const float *__restrict__ scal,
                                                   Dims G, Dims d0, float inv_batch, float *__restrict__ gpred,
                                                   float *__restrict__ partial) {
     float s = 0.0f;
-    const float is = scal[0], inv_n = scal[2];
+    const float is = scal[3], inv_n = scal[2];
     for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < d0.V; v += (int64_t)gridDim.x * 256) {
         int64_t g = 0;
         bool inner = false;
@@ -521,6 +525,7 @@ struct clq_net {
     int device = 0;
     hipStream_t stream = nullptr;
     Dims G{}, d0{}, d1{}, d2{};
+    double dx = 0.0;
     int64_t nparams = 0;
     int64_t woff[kLayers] = {}, boff[kLayers] = {}, head_off = 0;
     float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
@@ -594,7 +599,8 @@ void conv_wgrad(const uint16_t *x, const float *gy, const uint16_t *y, float *pa
 
 void normalise_and_pack(clq_net *n, const float *rhs, const float *mask, const float *prev, hipStream_t s) {
     hipLaunchKernelGGL(norm_partials_kernel, dim3(kNormBlocks), dim3(256), 0, s, rhs, mask, n->G, n->norm_partials);
-    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, s, n->norm_partials, kNormBlocks, n->scal);
+    const double h = (double)(float)n->dx;
+    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, s, n->norm_partials, kNormBlocks, h * h, n->scal);
     hipLaunchKernelGGL(pack_input_kernel, g1(n->d0.V), dim3(256), 0, s, rhs, mask, prev, n->scal, n->G, n->d0,
                        n->in16);
 }
@@ -714,11 +720,13 @@ struct InitRng {
 
 }  // namespace
 
-int unet_check_grid(const clq_net *n, int nx, int ny, int nz) {
+int unet_check_grid(const clq_net *n, int nx, int ny, int nz, double dx) {
     if (!n) return set_error(CLQ_ERR_INVALID, "no network attached");
     if (n->G.X != nx || n->G.Y != ny || n->G.Z != nz)
         return set_error(CLQ_ERR_INVALID, "network grid %dx%dx%d does not match the context grid %dx%dx%d", n->G.X,
                          n->G.Y, n->G.Z, nx, ny, nz);
+    if ((float)n->dx != (float)dx)
+        return set_error(CLQ_ERR_INVALID, "network cell size %g does not match the context's %g", n->dx, dx);
     return CLQ_OK;
 }
 
@@ -750,9 +758,10 @@ int clq_net_destroy(clq_net *n) {
     return CLQ_OK;
 }
 
-int clq_net_create(int32_t nx, int32_t ny, int32_t nz, clq_net **out) {
+int clq_net_create(int32_t nx, int32_t ny, int32_t nz, double dx, clq_net **out) {
     if (!out) return set_error(CLQ_ERR_INVALID, "out is NULL");
     *out = nullptr;
+    if (!(dx > 0.0)) return set_error(CLQ_ERR_INVALID, "dx must be positive");
     if (nx < 4 || ny < 4 || nz < 4) return set_error(CLQ_ERR_INVALID, "grid must be at least 4 cells in each dimension");
     const int64_t vol = (int64_t)((nx + 3) / 4 * 4) * ((ny + 3) / 4 * 4) * ((nz + 3) / 4 * 4);
     if (vol >= ((int64_t)1 << 31) / 48) return set_error(CLQ_ERR_INVALID, "grid too large for the network");
@@ -763,6 +772,7 @@ int clq_net_create(int32_t nx, int32_t ny, int32_t nz, clq_net **out) {
         return set_error(CLQ_ERR_DEVICE, "hipStreamCreate failed");
     }
     n->G = make_dims(nx, ny, nz);
+    n->dx = dx;
     n->d0 = make_dims((nx + 3) / 4 * 4, (ny + 3) / 4 * 4, (nz + 3) / 4 * 4);
     n->d1 = make_dims(n->d0.X / 2, n->d0.Y / 2, n->d0.Z / 2);
     n->d2 = make_dims(n->d0.X / 4, n->d0.Y / 4, n->d0.Z / 4);

@@ -142,10 +142,10 @@ def save_weights(net: "runtime.Network", path=DEFAULT_WEIGHTS) -> Path:
     return path
 
 
-def load_weights(lib, shape, path=DEFAULT_WEIGHTS) -> "runtime.Network":
-    """A network for `shape` with the parameters stored at `path`."""
+def load_weights(lib, shape, dx: float, path=DEFAULT_WEIGHTS) -> "runtime.Network":
+    """A network for the (shape, dx) pressure grid with the parameters stored at `path`."""
     params = np.load(path, allow_pickle=False)
-    net = runtime.Network(lib, shape, seed=None)
+    net = runtime.Network(lib, shape, dx, seed=None)
     if params.size != net.param_count:
         raise ValueError(f"{path}: {params.size} parameters, expected {net.param_count}")
     net.set_params(params)
@@ -182,7 +182,7 @@ def main(argv=None) -> int:
         data.save(args.pairs)
     per_scene = len(data) // max(args.scenes, 1)
     train_set, test_set = data.split(len(data) - args.holdout * per_scene)
-    net = runtime.Network(lib, shape, seed=args.seed)
+    net = runtime.Network(lib, shape, 1.0 / max(shape), seed=args.seed)
     before = relative_error(net, test_set) if len(test_set) else float("nan")
     train(net, train_set, args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, log=log)
     after = relative_error(net, test_set) if len(test_set) else float("nan")

@@ -65,6 +65,7 @@ class SolveParams(ctypes.Structure):
         ("coarse_extra", c_int32),
         ("max_cycles", c_int32),
         ("coarse_dtype", c_int32),
+        ("vcycle_from", c_int32),
         ("tolerance", c_double),
     ]
 
@@ -133,7 +134,7 @@ _SIGNATURES = {
 # oracle restates the FLIP step, the network's CPU check is a torch model).
 _NET_SIGNATURES = {
     "clq_net_param_count": ([], c_int64),
-    "clq_net_create": ([c_int32, c_int32, c_int32, POINTER(_P)], ctypes.c_int),
+    "clq_net_create": ([c_int32, c_int32, c_int32, c_double, POINTER(_P)], ctypes.c_int),
     "clq_net_destroy": ([_P], ctypes.c_int),
     "clq_net_init": ([_P, ctypes.c_uint64], ctypes.c_int),
     "clq_net_set_params": ([_P, _P, c_int64], ctypes.c_int),
@@ -339,8 +340,10 @@ class Context:
         max_cycles: int = 1,
         tolerance: float = 1e-6,
         coarse_dtype: int = 0,
+        vcycle_from: int = 0,
     ) -> dict:
-        params = SolveParams(int(mode), pre_smooth, post_smooth, coarse_extra, max_cycles, coarse_dtype, tolerance)
+        params = SolveParams(int(mode), pre_smooth, post_smooth, coarse_extra, max_cycles, coarse_dtype, vcycle_from,
+                             tolerance)
         stats = SolveStats()
         self._check(self.lib.clq_pressure_solve(self.handle, byref(params), byref(stats)), "clq_pressure_solve")
         return stats.as_dict()
@@ -441,16 +444,18 @@ def device_count(lib: ctypes.CDLL) -> int:
 
 
 class Network:
-    """The learned preconditioner's U-Net for one pressure-grid shape, on the
-    device (include/cliquid.h, clq_net_*).  Arrays are float32 grid-shaped."""
+    """The learned preconditioner's U-Net for one pressure grid (shape and cell
+    size dx), on the device (include/cliquid.h, clq_net_*).  Arrays are float32
+    grid-shaped."""
 
-    def __init__(self, lib: ctypes.CDLL, shape, seed: int | None = 0):
+    def __init__(self, lib: ctypes.CDLL, shape, dx: float, seed: int | None = 0):
         if not hasattr(lib, "clq_net_create"):
             raise ClqError(ERR_UNSUPPORTED, "clq_net_create", "this library has no learned preconditioner")
         self.lib = lib
         self.shape = tuple(int(n) for n in shape)
+        self.dx = float(dx)
         handle = c_void_p()
-        self._check(lib.clq_net_create(*self.shape, byref(handle)), "clq_net_create")
+        self._check(lib.clq_net_create(*self.shape, self.dx, byref(handle)), "clq_net_create")
         self.handle = handle
         self.param_count = int(lib.clq_net_param_count())
         if seed is not None:

@@ -124,8 +124,11 @@ typedef struct clq_solve_params {
     int32_t post_smooth;   /* reference: 4 (pressure_solve.py:78)               */
     int32_t coarse_extra;  /* reference: 64 -> coarsest runs pre+64 sweeps      */
     int32_t max_cycles;    /* fixed count (mode 0) or cap                        */
-    int32_t coarse_dtype;  /* mode 2: 0 = fp16, 1 = bf16 coarse b/r storage     */
-    double tolerance;      /* relative residual target (modes 1-3)              */
+    int32_t coarse_dtype;  /* modes 2-4: 0 = fp16, 1 = bf16 coarse b storage    */
+    int32_t vcycle_from;   /* 0: W-cycle at every level (multigrid.py:53-187);
+                              k > 0: the coarse-grid correction runs twice
+                              below levels < k and once from level k down  */
+    double tolerance;      /* relative residual target (modes 1-4)              */
 } clq_solve_params;
 
 typedef struct clq_solve_stats {
@@ -275,8 +278,9 @@ int clq_kernel_times(clq_ctx *ctx, double *total_ms, int64_t *launches, int32_t 
  * No counterpart in the reference.  A 3-level 3D U-Net (16/32/64 channels,
  * 3x3x3 convolutions, ReLU, BF16 weights and activations with FP32
  * accumulation on the matrix cores) mapping (rhs, fluid mask, previous
- * pressure) on a pressure grid to the pressure; its output is scaled by the
- * RMS of rhs over the interior fluid cells (mask > 0), its input divided by it.
+ * pressure) on a pressure grid of cell size h to the pressure.  With s the RMS
+ * of rhs over the interior cells with mask > 0, its inputs are rhs / s and
+ * prev / (s h^2) and its output is s h^2 times the network's head.
  * Parameter vector (FP32), in order: for each of the 10 conv layers
  * (16->16, 16->16, 16->32, 32->32, 32->64, 64->64, 96->32, 32->32, 48->16,
  * 16->16) the weights W[tap][c_in][c_out] with tap = 9(dx+1) + 3(dy+1) + (dz+1)
@@ -284,8 +288,9 @@ int clq_kernel_times(clq_ctx *ctx, double *total_ms, int64_t *launches, int32_t 
  * Host arrays are float32 [nx][ny][nz] of the pressure grid. */
 typedef struct clq_net clq_net;
 int64_t clq_net_param_count(void);
-/* A network for an (nx, ny, nz) pressure grid on the current device. */
-int clq_net_create(int32_t nx, int32_t ny, int32_t nz, clq_net **out);
+/* A network for an (nx, ny, nz) pressure grid of cell size dx on the current
+ * device (parameters are independent of the grid: copy them between nets). */
+int clq_net_create(int32_t nx, int32_t ny, int32_t nz, double dx, clq_net **out);
 int clq_net_destroy(clq_net *net);
 /* He-normal weights from a fixed generator seeded with `seed`, zero biases. */
 int clq_net_init(clq_net *net, uint64_t seed);
@@ -296,12 +301,12 @@ int clq_net_forward(clq_net *net, const float *rhs, const float *mask, const flo
 /* Parameter gradients of sum(grad_out * out) for the last clq_net_forward. */
 int clq_net_backward(clq_net *net, const float *grad_out, float *grads, int64_t count);
 /* One Adam step (beta 0.9/0.999, eps 1e-8) on the mean over `batch` samples of
- * the mean squared error between out / s and target / s over the interior
- * cells with mask > 0 (s = the input scale above).  Arrays are [batch][grid];
+ * the mean squared error between out / (s h^2) and target / (s h^2) over the
+ * interior cells with mask > 0 (s as above).  Arrays are [batch][grid];
  * prev may be NULL.  *loss receives the mean loss before the step. */
 int clq_net_train_step(clq_net *net, int32_t batch, const float *rhs, const float *mask, const float *prev,
                        const float *target, double lr, double *loss);
-/* Use `net` (created for the context's grid, on its device) in
+/* Use `net` (created for the context's grid and cell size, on its device) in
  * CLQ_SOLVE_LEARNED_FCG; NULL detaches.  The context does not own the net:
  * detach before destroying it. */
 int clq_attach_net(clq_ctx *ctx, clq_net *net, double stall_ratio);

