@@ -40,15 +40,17 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "ms/step @256^3 (~30M particles); pressure-solve ms to 1e-6 rel residual"
 
 
-def algorithmic_bytes(n_particles: int, cells: int) -> dict:
+def algorithmic_bytes(n_particles: int, cells: int, swept_cells: float) -> dict:
     """Compulsory HBM bytes per launch unit in the reference FP32 layout (each
     array counted once): binning 60N + 16C, P2G + occupancy 24N + 24C, one
-    red-black sweep of the finest level 28C, fused G2P + advection 48N + 24C."""
+    red-black sweep of the finest level 28 bytes per swept cell (p, b, mask,
+    diagonal and three weights; the sweep visits the rows holding an interior
+    fluid cell), fused G2P + advection 48N + 24C."""
     N, C = n_particles, cells
     return {
         "binning": 60 * N + 16 * C,
         "p2g": 24 * N + 24 * C,
-        "rbgs_fine_sweep": 28 * C,
+        "rbgs_fine_sweep": int(28 * swept_cells),
         "particle_update": 48 * N + 24 * C,
     }
 
@@ -138,7 +140,7 @@ def main() -> int:
 
             ctx.attach_net(learned.load_weights(lib, scene.shape, scene.dx, args.weights or learned.DEFAULT_WEIGHTS),
                            args.stall_ratio)
-        fallback = []
+        fallback, swept = [], []
         for _ in range(args.warmup):
             host_solver.step(ctx, params)
         ctx.sync()
@@ -151,6 +153,7 @@ def main() -> int:
             relres.append(stats["relative_residual"])
             cycles.append(stats["cycles"])
             fallback.append(stats["fallback_cycles"])
+            swept.append(stats["active_cells"])
         ctx.sync()
         elapsed = time.perf_counter() - t0
         ktimes = ctx.kernel_times()
@@ -158,7 +161,7 @@ def main() -> int:
 
     ms_per_step = elapsed * 1e3 / args.steps
     n, cells = scene.particle_count, int(np.prod(scene.shape))
-    nbytes = algorithmic_bytes(n, cells)
+    nbytes = algorithmic_bytes(n, cells, float(np.mean(swept)))
     kernels = {}
     for name, (total_ms, units) in ktimes.items():
         if units == 0:
@@ -202,6 +205,7 @@ def main() -> int:
         "pressure_solve_ms": round(float(np.mean(solve_ms)), 3),
         "w_cycles_per_step": round(float(np.mean(cycles)), 2),
         "relative_residual_max": float(np.max(relres)),
+        "swept_cells_per_sweep": int(np.mean(swept)),
         **({"w_cycle_fallback_per_step": round(float(np.mean(fallback)), 2)} if args.solver == "learned" else {}),
         "roofline": {
             "kernel": dominant,

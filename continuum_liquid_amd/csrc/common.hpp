@@ -147,6 +147,16 @@ struct BottomArgs {
     double dd64;
 };
 void launch_wcycle_bottom(const BottomArgs &a, hipStream_t s);
+// active rows (cells (i, j, *) holding an interior cell with mask > 0)
+void launch_row_flags(const Level &L, int32_t *flags, hipStream_t s);
+void launch_coarse_row_flags(const int32_t *fine, const Level &F, const Level &K, int32_t *flags, hipStream_t s);
+void launch_compact_rows(const int32_t *flags, const int32_t *pos, int64_t rows, int32_t *list, int32_t *count,
+                         hipStream_t s);
+void launch_zero_inactive_rows(const Level &L, const int32_t *flags, int coarse, hipStream_t s);
+void launch_rbgs_rows(const Level &L, float delta, int parity, const int32_t *rows, int64_t nrows, hipStream_t s);
+void launch_restrict_rows(const Level &F, const Level &K, float inv_delta2, const int32_t *krows, int64_t nkrows,
+                          hipStream_t s);
+void launch_prolongate_rows(const Level &F, const Level &K, const int32_t *frows, int64_t nfrows, hipStream_t s);
 void launch_ir_update_residual(const Level &L, int from32, const float *p32, const double *pin, double *pout,
                                float *r32, const float *b, double inv_h2, double *partials, double *out2,
                                hipStream_t s);
@@ -158,12 +168,13 @@ void launch_residual_norm(const Level &L, double inv_h2, double *partials, doubl
 int fcg_partials_count();
 void launch_fcg_init(const Level &L, const float *p32, const float *b, double inv_h2, double *x, double *r, float *r32,
                      double *partials, double *s, hipStream_t st);
-void launch_fcg_dots(const Level &L, const float *z, const double *qv, const double *r, int first, double *partials,
-                     double *s, hipStream_t st);
+void launch_fcg_dots(const Level &L, const float *z, const double *qv, const double *r, int first,
+                     const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st);
 void launch_fcg_direction(const Level &L, const float *z, const double *d_old, double *d_new, double *qv, int first,
-                          double inv_h2, double *partials, double *s, hipStream_t st);
+                          double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s,
+                          hipStream_t st);
 void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const double *d, const double *qv,
-                       double *partials, double *s, hipStream_t st);
+                       const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st);
 
 // context.hip: sets the message returned by clq_last_error(), returns code
 int set_error(int code, const char *fmt, ...);

@@ -82,6 +82,12 @@ struct clq_ctx {
     bool p64_valid = false;  // p64a holds the pressure of the last solve (modes 2-4)
     // flexible-CG state, allocated on first use
     double *r64 = nullptr, *dir[2] = {nullptr, nullptr}, *q64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
+    // active rows of levels 0 and 1 (rows holding an interior fluid cell),
+    // used by the smoother, transfers and FCG vectors inside clq_pressure_solve
+    int32_t *row_flags[2] = {nullptr, nullptr}, *rows[2] = {nullptr, nullptr}, *row_pos = nullptr;
+    int32_t *row_count = nullptr, *row_count_host = nullptr;
+    int64_t nrows[2] = {0, 0};
+    bool rows_valid = false, use_rows = false;
     // learned preconditioner (not owned) and its FP32 initial-guess buffer
     clq_net *net = nullptr;
     double stall_ratio = 0.5;
@@ -511,21 +517,82 @@ static void ensure_coarse_operators(clq_ctx *c) {
     if (c->coarse_ops_valid) return;
     for (int l = 0; l + 1 < c->nlevels; ++l) launch_coarse_operator(c->lv[l], c->lv[l + 1], c->stream);
     c->coarse_ops_valid = true;
+    c->rows_valid = false;
+}
+
+// Active-row lists of levels 0 and 1 for the current operator (one host
+// read-back of the two counts per operator build).
+static int ensure_active_rows(clq_ctx *c) {
+    if (c->rows_valid) return CLQ_OK;
+    if (c->nlevels < 2) return CLQ_OK;
+    if (!c->row_pos) {
+        const int64_t r0 = (int64_t)c->nx * c->ny;
+        void *q;
+        for (int l = 0; l < 2; ++l) {
+            const int64_t r = (int64_t)c->lv[l].nx * c->lv[l].ny;
+            if (int rc = c->alloc(&q, (size_t)r * 4)) return rc;
+            c->row_flags[l] = (int32_t *)q;
+            if (int rc = c->alloc(&q, (size_t)r * 4)) return rc;
+            c->rows[l] = (int32_t *)q;
+        }
+        if (int rc = c->alloc(&q, (size_t)r0 * 4)) return rc;
+        c->row_pos = (int32_t *)q;
+        if (int rc = c->alloc(&q, 16)) return rc;
+        c->row_count = (int32_t *)q;
+        HIP_TRY(hipHostMalloc((void **)&c->row_count_host, 16, hipHostMallocDefault));
+    }
+    launch_row_flags(c->lv[0], c->row_flags[0], c->stream);
+    launch_coarse_row_flags(c->row_flags[0], c->lv[0], c->lv[1], c->row_flags[1], c->stream);
+    for (int l = 0; l < 2; ++l) {
+        const int64_t r = (int64_t)c->lv[l].nx * c->lv[l].ny;
+        launch_exclusive_scan(c->row_flags[l], c->row_pos, r, c->scan_scratch, c->stream);
+        launch_compact_rows(c->row_flags[l], c->row_pos, r, c->rows[l], c->row_count + l, c->stream);
+    }
+    HIP_TRY(hipMemcpyAsync(c->row_count_host, c->row_count, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->nrows[0] = c->row_count_host[0];
+    c->nrows[1] = c->row_count_host[1];
+    c->rows_valid = true;
+    return check_launch("active rows");
+}
+
+// Inside a pressure solve: the row lists, and the writes the dense kernels
+// would make on the inactive rows.
+static int begin_sparse_solve(clq_ctx *c) {
+    if (int rc = ensure_active_rows(c)) return rc;
+    if (c->nlevels < 2) return CLQ_OK;
+    launch_zero_inactive_rows(c->lv[0], c->row_flags[0], 0, c->stream);
+    launch_zero_inactive_rows(c->lv[1], c->row_flags[1], 1, c->stream);
+    c->use_rows = true;
+    return CLQ_OK;
 }
 
 static void smooth_level(clq_ctx *c, int level, int iterations) {
     const float delta = (float)c->dx;
+    const bool sparse = c->use_rows && level < 2;
     for (int it = 0; it < iterations; ++it) {
         if (level == 0) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
-        launch_rbgs(c->lv[level], delta, 0, c->stream);
-        launch_rbgs(c->lv[level], delta, 1, c->stream);
+        for (int parity = 0; parity < 2; ++parity) {
+            if (sparse) launch_rbgs_rows(c->lv[level], delta, parity, c->rows[level], c->nrows[level], c->stream);
+            else launch_rbgs(c->lv[level], delta, parity, c->stream);
+        }
         if (level == 0) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
     }
 }
 
 static void restrict_level(clq_ctx *c, int level) {
     const float delta = (float)c->dx;
-    launch_restrict_residual(c->lv[level], c->lv[level + 1], 1.0f / (delta * delta), c->stream);
+    if (c->use_rows && level == 0)
+        launch_restrict_rows(c->lv[0], c->lv[1], 1.0f / (delta * delta), c->rows[1], c->nrows[1], c->stream);
+    else
+        launch_restrict_residual(c->lv[level], c->lv[level + 1], 1.0f / (delta * delta), c->stream);
+}
+
+static void prolongate_level(clq_ctx *c, int level) {
+    if (c->use_rows && level == 0)
+        launch_prolongate_rows(c->lv[0], c->lv[1], c->rows[0], c->nrows[0], c->stream);
+    else
+        launch_prolongate(c->lv[level], c->lv[level + 1], c->stream);
 }
 
 // Levels with at most this many cells run inside one work-group launch
@@ -581,7 +648,7 @@ static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfr
     restrict_level(c, level);
     w_cycle(c, level + 1, pre, post, extra, vfrom);
     if (vfrom <= 0 || level < vfrom) w_cycle(c, level + 1, pre, post, extra, vfrom);
-    launch_prolongate(c->lv[level], c->lv[level + 1], c->stream);
+    prolongate_level(c, level);
     smooth_level(c, level, post);
 }
 
@@ -664,7 +731,7 @@ static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, i
     restrict_level(c, level);
     w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
     if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
-    launch_prolongate(c->lv[level], c->lv[level + 1], c->stream);
+    prolongate_level(c, level);
     smooth_level(c, level, post);
 }
 
@@ -785,6 +852,9 @@ static int ensure_fcg_buffers(clq_ctx *c) {
     return CLQ_OK;
 }
 
+// level-0 active-row list of the FCG vector kernels (all cells without one)
+#define ROWS0 (c->use_rows ? c->rows[0] : nullptr), c->nrows[0]
+
 static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
     if (int rc = ensure_fcg_buffers(c)) return rc;
     const Level &L0 = c->lv[0];
@@ -800,18 +870,19 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     if (r > sp->tolerance * bn && sp->max_cycles > 0) {
         if (int rc = precondition_wcycle(c, sp, r)) return rc;
         s->cycles++;
-        launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, c->fcg_partials, c->fcg_s, c->stream);
+        launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         for (int it = 0;; ++it) {
-            launch_fcg_direction(L0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, c->fcg_partials,
-                                 c->fcg_s, c->stream);
+            launch_fcg_direction(L0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, ROWS0,
+                                 c->fcg_partials, c->fcg_s, c->stream);
             cur ^= 1;
-            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, c->fcg_partials, c->fcg_s, c->stream);
+            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, ROWS0, c->fcg_partials, c->fcg_s,
+                              c->stream);
             if (int rc = read_scalars(c)) return rc;
             r = sqrt(c->norm_host[0]);
             if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
             if (int rc = precondition_wcycle(c, sp, r)) return rc;
             s->cycles++;
-            launch_fcg_dots(L0, c->p, c->q64, c->r64, 0, c->fcg_partials, c->fcg_s, c->stream);
+            launch_fcg_dots(L0, c->p, c->q64, c->r64, 0, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         }
     }
     launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
@@ -864,19 +935,20 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
     int cur = 0;
     if (r > sp->tolerance * bn && sp->max_cycles > 0) {
         if (int rc = precondition(r)) return rc;
-        launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, c->fcg_partials, c->fcg_s, c->stream);
+        launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         for (int it = 0;; ++it) {
-            launch_fcg_direction(L0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, c->fcg_partials,
-                                 c->fcg_s, c->stream);
+            launch_fcg_direction(L0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, ROWS0,
+                                 c->fcg_partials, c->fcg_s, c->stream);
             cur ^= 1;
-            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, c->fcg_partials, c->fcg_s, c->stream);
+            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, ROWS0, c->fcg_partials, c->fcg_s,
+                              c->stream);
             if (int rc = read_scalars(c)) return rc;
             const double before = r;
             r = sqrt(c->norm_host[0]);
             if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
             if (use_net && r > c->stall_ratio * before) use_net = false;
             if (int rc = precondition(r)) return rc;
-            launch_fcg_dots(L0, c->p, c->q64, c->r64, 0, c->fcg_partials, c->fcg_s, c->stream);
+            launch_fcg_dots(L0, c->p, c->q64, c->r64, 0, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         }
     }
     launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
@@ -897,8 +969,17 @@ int clq_attach_net(clq_ctx *c, clq_net *net, double stall_ratio) {
     return CLQ_OK;
 }
 
+static int pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st);
+
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
+    const int rc = pressure_solve(c, sp, st);
+    if (!rc && st) st->active_cells = (c->nlevels < 2 ? c->C : c->nrows[0] * c->nz);
+    c->use_rows = false;
+    return rc;
+}
+
+static int pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0 ||
         sp->vcycle_from < 0)
         return fail(CLQ_ERR_INVALID, "negative sweep, cycle count or level");
@@ -912,6 +993,7 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     c->p64_valid = false;
     stage_begin(c, CLQ_STAGE_PRESSURE_SOLVE);
     ensure_coarse_operators(c);
+    if (int rc = begin_sparse_solve(c)) return rc;
     if (sp->mode == CLQ_SOLVE_MIXED_REFINEMENT || sp->mode == CLQ_SOLVE_FCG || sp->mode == CLQ_SOLVE_LEARNED_FCG) {
         const int rc = sp->mode == CLQ_SOLVE_FCG           ? solve_fcg(c, sp, &s)
                        : sp->mode == CLQ_SOLVE_LEARNED_FCG ? solve_learned(c, sp, &s)

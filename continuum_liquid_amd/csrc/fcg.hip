@@ -11,8 +11,10 @@
 //   s[6] r.r        s[7] b.b
 // beta = -alpha (z_{k+1} . q_k) / (z_k . r_k), the flexible (Polak-Ribiere)
 // form written with q_k = -(r_{k+1} - r_k) / alpha, so r_k need not be kept.
-// Every pass writes all cells (0 outside the interior fluid cells), so stale
-// values from an earlier mask never leak in.
+// The initialisation writes all cells (0 outside the interior fluid cells);
+// the per-iteration kernels visit the active rows only (rows holding an
+// interior fluid cell, or all cells when no row list is given): outside them
+// x, r, z, d and q are zero or never read.
 #include "common.hpp"
 
 namespace clq {
@@ -39,6 +41,13 @@ __device__ __forceinline__ double apply_a(const Level &L, int64_t q, F v, double
     if (L.wz[q] != 0.0f) s = s + (double)L.wz[q] * v(q + 1);
     if (L.wz[q - 1] != 0.0f) s = s + (double)L.wz[q - 1] * v(q - 1);
     return (s - (double)L.diag[q] * v(q)) * inv_h2;
+}
+
+// cell of work item t: row-major over the listed rows, or t itself
+__device__ __forceinline__ int64_t cell_of(const int32_t *__restrict__ rows, int nz, int64_t t) {
+    if (!rows) return t;
+    const int64_t ar = t / nz;
+    return (int64_t)rows[ar] * nz + (t - ar * nz);
 }
 
 template <int N>
@@ -83,10 +92,12 @@ __global__ __launch_bounds__(kT) void fcg_init_kernel(Level L, const float *__re
 __global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const float *__restrict__ z,
                                                           const double *__restrict__ d_old, double *__restrict__ d_new,
                                                           double *__restrict__ qv, const double *__restrict__ s,
-                                                          int first, double inv_h2, double *__restrict__ partials) {
+                                                          int first, double inv_h2, const int32_t *__restrict__ rows,
+                                                          int64_t work, double *__restrict__ partials) {
     const double beta = first ? 0.0 : s[5];
     double acc[1] = {0.0};
-    for (int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x; q < L.C; q += (int64_t)gridDim.x * kT) {
+    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
+        const int64_t q = cell_of(rows, L.nz, t);
         double dv = 0.0, av = 0.0;
         if (fluid_cell(L, q)) {
             auto dn = [&](int64_t t) { return first ? (double)z[t] : (double)z[t] + beta * d_old[t]; };
@@ -103,13 +114,15 @@ __global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const float 
 }
 
 // x += alpha d, r -= alpha q, r32 = float(r); partial r.r.
-__global__ __launch_bounds__(kT) void fcg_update_kernel(int64_t C, double *__restrict__ x, double *__restrict__ r,
+__global__ __launch_bounds__(kT) void fcg_update_kernel(double *__restrict__ x, double *__restrict__ r,
                                                        float *__restrict__ r32, const double *__restrict__ d,
                                                        const double *__restrict__ qv, const double *__restrict__ s,
+                                                       const int32_t *__restrict__ rows, int nz, int64_t work,
                                                        double *__restrict__ partials) {
     const double alpha = s[2];
     double acc[1] = {0.0};
-    for (int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x; q < C; q += (int64_t)gridDim.x * kT) {
+    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
+        const int64_t q = cell_of(rows, nz, t);
         const double xv = x[q] + alpha * d[q];
         const double rv = r[q] - alpha * qv[q];
         x[q] = xv;
@@ -121,11 +134,12 @@ __global__ __launch_bounds__(kT) void fcg_update_kernel(int64_t C, double *__res
 }
 
 // partial z.q, z.r
-__global__ __launch_bounds__(kT) void fcg_dots_kernel(int64_t C, const float *__restrict__ z,
-                                                     const double *__restrict__ qv, const double *__restrict__ r,
-                                                     double *__restrict__ partials) {
+__global__ __launch_bounds__(kT) void fcg_dots_kernel(const float *__restrict__ z, const double *__restrict__ qv,
+                                                     const double *__restrict__ r, const int32_t *__restrict__ rows,
+                                                     int nz, int64_t work, double *__restrict__ partials) {
     double acc[2] = {0.0, 0.0};
-    for (int64_t q = (int64_t)blockIdx.x * kT + threadIdx.x; q < C; q += (int64_t)gridDim.x * kT) {
+    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
+        const int64_t q = cell_of(rows, nz, t);
         const double zv = (double)z[q];
         acc[0] += zv * qv[q];
         acc[1] += zv * r[q];
@@ -182,25 +196,31 @@ void launch_fcg_init(const Level &L, const float *p32, const float *b, double in
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 0, 0, s);
 }
 
-void launch_fcg_dots(const Level &L, const float *z, const double *qv, const double *r, int first, double *partials,
-                     double *s, hipStream_t st) {
-    const int nb = blocks_for(L.C);
-    hipLaunchKernelGGL(fcg_dots_kernel, dim3(nb), dim3(kT), 0, st, L.C, z, qv, r, partials);
+static int64_t work_of(const Level &L, const int32_t *rows, int64_t nrows) { return rows ? nrows * L.nz : L.C; }
+
+void launch_fcg_dots(const Level &L, const float *z, const double *qv, const double *r, int first,
+                     const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st) {
+    const int64_t work = work_of(L, rows, nrows);
+    const int nb = blocks_for(work);
+    hipLaunchKernelGGL(fcg_dots_kernel, dim3(nb), dim3(kT), 0, st, z, qv, r, rows, L.nz, work, partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 3, first, s);
 }
 
 void launch_fcg_direction(const Level &L, const float *z, const double *d_old, double *d_new, double *qv, int first,
-                          double inv_h2, double *partials, double *s, hipStream_t st) {
-    const int nb = blocks_for(L.C);
+                          double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s,
+                          hipStream_t st) {
+    const int64_t work = work_of(L, rows, nrows);
+    const int nb = blocks_for(work);
     hipLaunchKernelGGL(fcg_direction_kernel, dim3(nb), dim3(kT), 0, st, L, z, d_old, d_new, qv, s, first, inv_h2,
-                       partials);
+                       rows, work, partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 1, 0, s);
 }
 
 void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const double *d, const double *qv,
-                       double *partials, double *s, hipStream_t st) {
-    const int nb = blocks_for(L.C);
-    hipLaunchKernelGGL(fcg_update_kernel, dim3(nb), dim3(kT), 0, st, L.C, x, r, r32, d, qv, s, partials);
+                       const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st) {
+    const int64_t work = work_of(L, rows, nrows);
+    const int nb = blocks_for(work);
+    hipLaunchKernelGGL(fcg_update_kernel, dim3(nb), dim3(kT), 0, st, x, r, r32, d, qv, s, rows, L.nz, work, partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 2, 0, s);
 }
 

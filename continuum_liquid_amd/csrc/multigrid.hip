@@ -35,11 +35,8 @@ __device__ __forceinline__ float wsum(const float *__restrict__ p, const float *
 // colour in row-major order: row (i, j), k = 2*kk + ((parity + i + j) & 1).
 // Non-fluid cells of that colour are zeroed, as rbgs_step zeroes every
 // non-fluid cell; after a full sweep both colours have been zeroed.
-__device__ __forceinline__ void rbgs_item(const Level &L, float dd, int parity, int64_t t) {
+__device__ __forceinline__ void rbgs_cell(const Level &L, float dd, int parity, uint32_t row, int kk) {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
-    const uint32_t half = (uint32_t)(nz + 1) >> 1;
-    const uint32_t row = (uint32_t)t / half;
-    const int kk = (int)((uint32_t)t - row * half);
     const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
     const int k = 2 * kk + ((parity + i + j) & 1);
     if (k >= nz) return;
@@ -56,6 +53,36 @@ __device__ __forceinline__ void rbgs_item(const Level &L, float dd, int parity, 
         p[idx] = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), wsum(p, L.wx, L.wy, L.wz, idx, sx, sy)) /
                  d;  // multigrid.cl:227 (fmuladd)
     }
+}
+
+// The same update with every load issued before the first branch on loaded
+// data (one memory round trip instead of three); the extra loads at air
+// cells are cheap on the coarse levels, whose sweeps are latency-bound.
+__device__ __forceinline__ void rbgs_cell_eager(const Level &L, float dd, int parity, uint32_t row, int kk) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
+    const int k = 2 * kk + ((parity + i + j) & 1);
+    if (k >= nz) return;
+    const int64_t idx = (int64_t)row * nz + k;
+    const bool inner = interior(i, j, k, nx, ny, nz);
+    const float m = L.mask[idx];
+    float d = 0.0f, num = 0.0f;
+    if (inner) {
+        const int64_t sx = (int64_t)ny * nz, sy = nz;
+        d = L.diag[idx];
+        num = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), wsum(L.p, L.wx, L.wy, L.wz, idx, sx, sy));
+    }
+    if (m <= 0.0f) {
+        L.p[idx] = 0.0f;
+        return;
+    }
+    if (inner && d > 0.0f) L.p[idx] = num / d;  // multigrid.cl:227
+}
+
+__device__ __forceinline__ void rbgs_item(const Level &L, float dd, int parity, int64_t t) {
+    const uint32_t half = (uint32_t)(L.nz + 1) >> 1;
+    const uint32_t row = (uint32_t)t / half;
+    rbgs_cell(L, dd, parity, row, (int)((uint32_t)t - row * half));
 }
 
 // Four same-colour cells per lane with every load issued before any store:
@@ -103,16 +130,135 @@ __device__ __forceinline__ void rbgs_batch4(const Level &L, float dd, int parity
     }
 }
 
+template <bool EAGER>
 __global__ __launch_bounds__(256) void rbgs_kernel(Level L, float dd, int parity) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= (int64_t)L.nx * L.ny * ((L.nz + 1) >> 1)) return;
-    rbgs_item(L, dd, parity, t);
+    if (EAGER) {
+        const uint32_t half = (uint32_t)(L.nz + 1) >> 1;
+        const uint32_t row = (uint32_t)t / half;
+        rbgs_cell_eager(L, dd, parity, row, (int)((uint32_t)t - row * half));
+    } else {
+        rbgs_item(L, dd, parity, t);
+    }
 }
+
+// Levels below this many cells take the eager (latency-bound) variants.
+constexpr int64_t kEagerCells = (int64_t)1 << 20;
 
 void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s) {
     const int half = (L.nz + 1) >> 1;
     const int64_t work = (int64_t)L.nx * L.ny * half;
-    hipLaunchKernelGGL(rbgs_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L, delta * delta, parity);
+    if (L.C < kEagerCells)
+        hipLaunchKernelGGL(rbgs_kernel<true>, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L, delta * delta,
+                           parity);
+    else
+        hipLaunchKernelGGL(rbgs_kernel<false>, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L,
+                           delta * delta, parity);
+}
+
+// ---- active rows ----------------------------------------------------------------------
+// A row is the run of cells (i, j, 0..nz-1).  It is active when it holds an
+// interior cell with mask > 0.  Every kernel below restricted to active rows
+// computes exactly what its dense version computes on those rows; on the
+// other rows the dense smoother only writes p = 0 (mask <= 0 there), which
+// zero_inactive_rows does once per solve, and no fluid cell reads them with a
+// non-zero weight (the operator's weight towards a cell with mask <= 0 is 0).
+
+// flags[row] = 1 when active; one wave per row.
+__global__ __launch_bounds__(256) void row_flags_kernel(Level L, int32_t *__restrict__ flags) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int64_t rows = (int64_t)L.nx * L.ny;
+    if (row >= rows) return;
+    const int i = (int)(row / L.ny), j = (int)(row - (int64_t)i * L.ny);
+    bool any = false;
+    if (i > 0 && i < L.nx - 1 && j > 0 && j < L.ny - 1)
+        for (int k = 1 + lane; k < L.nz - 1; k += 64) any |= L.mask[row * L.nz + k] > 0.0f;
+    const bool active = __ballot(any) != 0;
+    if (lane == 0) flags[row] = active ? 1 : 0;
+}
+
+// Coarse row (a, b) is active when one of the fine rows it restricts from is.
+__global__ __launch_bounds__(256) void coarse_row_flags_kernel(const int32_t *__restrict__ fine, int fnx, int fny,
+                                                               int cnx, int cny, int32_t *__restrict__ flags) {
+    const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (row >= (int64_t)cnx * cny) return;
+    const int a = (int)(row / cny), b = (int)(row - (int64_t)a * cny);
+    int32_t f = 0;
+    if (a > 0 && a < cnx - 1 && b > 0 && b < cny - 1) {
+        const int f0 = 1 + 2 * (a - 1), f1 = 1 + 2 * (b - 1);
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj) {
+                const int i = f0 + di, j = f1 + dj;
+                if (i < fnx - 1 && j < fny - 1) f |= fine[(int64_t)i * fny + j];
+            }
+    }
+    flags[row] = f;
+}
+
+__global__ __launch_bounds__(256) void compact_rows_kernel(const int32_t *__restrict__ flags,
+                                                           const int32_t *__restrict__ pos, int64_t rows,
+                                                           int32_t *__restrict__ list, int32_t *__restrict__ count) {
+    const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (row >= rows) return;
+    if (flags[row]) list[pos[row]] = (int32_t)row;
+    if (row == rows - 1) *count = pos[row] + flags[row];
+}
+
+// On the inactive rows: level 0 (coarse = 0) p = 0 where mask <= 0, what the
+// first dense sweep does there; a coarse level p = 0 and b = 0 everywhere,
+// what every dense restriction writes there (no child is fluid).
+__global__ __launch_bounds__(256) void zero_inactive_rows_kernel(Level L, const int32_t *__restrict__ flags,
+                                                                 int coarse) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= L.C) return;
+    if (flags[q / L.nz]) return;
+    if (coarse) {
+        L.p[q] = 0.0f;
+        L.b[q] = 0.0f;
+        if (L.b16) L.b16[q] = 0;
+    } else if (L.mask[q] <= 0.0f) {
+        L.p[q] = 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void rbgs_rows_kernel(Level L, float dd, int parity, const int32_t *__restrict__ rows,
+                                                        int64_t work) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= work) return;
+    const uint32_t half = (uint32_t)(L.nz + 1) >> 1;
+    const uint32_t ar = (uint32_t)t / half;
+    rbgs_cell(L, dd, parity, (uint32_t)rows[ar], (int)((uint32_t)t - ar * half));
+}
+
+void launch_row_flags(const Level &L, int32_t *flags, hipStream_t s) {
+    const int64_t rows = (int64_t)L.nx * L.ny;
+    hipLaunchKernelGGL(row_flags_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, L, flags);
+}
+
+void launch_coarse_row_flags(const int32_t *fine, const Level &F, const Level &K, int32_t *flags, hipStream_t s) {
+    const int64_t rows = (int64_t)K.nx * K.ny;
+    hipLaunchKernelGGL(coarse_row_flags_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, fine, F.nx,
+                       F.ny, K.nx, K.ny, flags);
+}
+
+void launch_compact_rows(const int32_t *flags, const int32_t *pos, int64_t rows, int32_t *list, int32_t *count,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(compact_rows_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, flags, pos, rows,
+                       list, count);
+}
+
+void launch_zero_inactive_rows(const Level &L, const int32_t *flags, int coarse, hipStream_t s) {
+    hipLaunchKernelGGL(zero_inactive_rows_kernel, dim3((unsigned)((L.C + 255) / 256)), dim3(256), 0, s, L, flags,
+                       coarse);
+}
+
+void launch_rbgs_rows(const Level &L, float delta, int parity, const int32_t *rows, int64_t nrows, hipStream_t s) {
+    const int64_t work = nrows * ((L.nz + 1) >> 1);
+    if (work == 0) return;
+    hipLaunchKernelGGL(rbgs_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L, delta * delta,
+                       parity, rows, work);
 }
 
 __global__ __launch_bounds__(256) void coarse_operator_kernel(
@@ -188,14 +334,70 @@ __device__ __forceinline__ void restrict_item(const Level &F, const Level &K, fl
     store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
 }
 
+// restrict_item with the eight children's loads issued together; the sum
+// takes the fluid children in the same order.
+__device__ __forceinline__ void restrict_item_eager(const Level &F, const Level &K, float inv_delta2, int64_t ci) {
+    int a, b, c;
+    coords(ci, K.ny, K.nz, a, b, c);
+    float rhs = 0.0f;
+    if (interior(a, b, c, K.nx, K.ny, K.nz)) {
+        const int64_t sx = (int64_t)F.ny * F.nz, sy = F.nz;
+        const int f0 = 1 + 2 * (a - 1), f1 = 1 + 2 * (b - 1), f2 = 1 + 2 * (c - 1);
+        float m[8], res[8];
+        bool in[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = f0 + (q >> 2), j = f1 + ((q >> 1) & 1), k = f2 + (q & 1);
+            in[q] = interior(i, j, k, F.nx, F.ny, F.nz);
+            m[q] = 0.0f;
+            res[q] = 0.0f;
+            if (in[q]) {
+                const int64_t idx = gidx(i, j, k, F.ny, F.nz);
+                m[q] = F.mask[idx];
+                res[q] = residual_at(F.p, load_b(F.b, F.b16, F.bkind, F.bscale, idx), F.diag, F.wx, F.wy, F.wz, idx,
+                                     sx, sy, inv_delta2);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (in[q] && m[q] > 0.0f) rhs = rhs + res[q];
+    }
+    K.p[ci] = 0.0f;
+    store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
+}
+
+template <bool EAGER>
 __global__ __launch_bounds__(256) void restrict_kernel(Level F, Level K, float inv_delta2) {
     const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (ci >= K.C) return;
-    restrict_item(F, K, inv_delta2, ci);
+    if (EAGER) restrict_item_eager(F, K, inv_delta2, ci);
+    else restrict_item(F, K, inv_delta2, ci);
 }
 
 void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, hipStream_t s) {
-    hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2);
+    if (F.C < kEagerCells)
+        hipLaunchKernelGGL(restrict_kernel<true>, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K,
+                           inv_delta2);
+    else
+        hipLaunchKernelGGL(restrict_kernel<false>, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K,
+                           inv_delta2);
+}
+
+// restriction onto the active coarse rows only
+__global__ __launch_bounds__(256) void restrict_rows_kernel(Level F, Level K, float inv_delta2,
+                                                            const int32_t *__restrict__ krows, int64_t work) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= work) return;
+    const int64_t ar = t / K.nz;
+    restrict_item(F, K, inv_delta2, (int64_t)krows[ar] * K.nz + (t - ar * K.nz));
+}
+
+void launch_restrict_rows(const Level &F, const Level &K, float inv_delta2, const int32_t *krows, int64_t nkrows,
+                          hipStream_t s) {
+    const int64_t work = nkrows * K.nz;
+    if (work == 0) return;
+    hipLaunchKernelGGL(restrict_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2,
+                       krows, work);
 }
 
 __device__ __forceinline__ void prolong_item(const Level &F, const Level &K, int64_t fi) {
@@ -206,14 +408,40 @@ __device__ __forceinline__ void prolong_item(const Level &F, const Level &K, int
     F.p[fi] = F.p[fi] + K.p[gidx(a, b, c, K.ny, K.nz)];
 }
 
+// prolong_item with its three loads issued together
+__device__ __forceinline__ void prolong_item_eager(const Level &F, const Level &K, int64_t fi) {
+    int i, j, k;
+    coords(fi, F.ny, F.nz, i, j, k);
+    if (!interior(i, j, k, F.nx, F.ny, F.nz)) return;
+    const int a = min((i - 1) / 2 + 1, K.nx - 2), b = min((j - 1) / 2 + 1, K.ny - 2), c = min((k - 1) / 2 + 1, K.nz - 2);
+    const float m = F.mask[fi], pf = F.p[fi], pc = K.p[gidx(a, b, c, K.ny, K.nz)];
+    if (m > 0.0f) F.p[fi] = pf + pc;
+}
+
 __global__ __launch_bounds__(256) void prolongate_kernel(Level F, Level K) {
     const int64_t fi = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (fi >= F.C) return;
-    prolong_item(F, K, fi);
+    if (F.C < kEagerCells) prolong_item_eager(F, K, fi);
+    else prolong_item(F, K, fi);
 }
 
 void launch_prolongate(const Level &F, const Level &K, hipStream_t s) {
     hipLaunchKernelGGL(prolongate_kernel, dim3((unsigned)((F.C + 255) / 256)), dim3(256), 0, s, F, K);
+}
+
+__global__ __launch_bounds__(256) void prolongate_rows_kernel(Level F, Level K, const int32_t *__restrict__ frows,
+                                                              int64_t work) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= work) return;
+    const int64_t ar = t / F.nz;
+    prolong_item(F, K, (int64_t)frows[ar] * F.nz + (t - ar * F.nz));
+}
+
+void launch_prolongate_rows(const Level &F, const Level &K, const int32_t *frows, int64_t nfrows, hipStream_t s) {
+    const int64_t work = nfrows * F.nz;
+    if (work == 0) return;
+    hipLaunchKernelGGL(prolongate_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, K, frows,
+                       work);
 }
 
 // coarse_smooth: one work-group of 1024 lanes owns the whole coarsest level
@@ -322,11 +550,11 @@ __global__ __launch_bounds__(kCoarseThreads) void wcycle_bottom_kernel(BottomArg
                 }
         } else if (op.code == kOpRestrict) {
             const Level &K = a.lv[op.level + 1];
-            for (int64_t ci = threadIdx.x; ci < K.C; ci += kCoarseThreads) restrict_item(L, K, a.inv_delta2, ci);
+            for (int64_t ci = threadIdx.x; ci < K.C; ci += kCoarseThreads) restrict_item_eager(L, K, a.inv_delta2, ci);
             __syncthreads();
         } else if (op.code == kOpProlong) {
             const Level &K = a.lv[op.level + 1];
-            for (int64_t fi = threadIdx.x; fi < L.C; fi += kCoarseThreads) prolong_item(L, K, fi);
+            for (int64_t fi = threadIdx.x; fi < L.C; fi += kCoarseThreads) prolong_item_eager(L, K, fi);
             __syncthreads();
         } else if (op.code == kOpCoarse32 || op.code == kOpCoarse64) {
             if (L.C > kCoarseCells) {  // same result as coarse_smooth (see launch_coarse_smooth)

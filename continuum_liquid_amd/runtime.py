@@ -79,6 +79,7 @@ class SolveStats(ctypes.Structure):
         ("final_residual", c_double),
         ("relative_residual", c_double),
         ("solve_ms", c_double),
+        ("active_cells", c_int64),
     ]
 
     def as_dict(self) -> dict:

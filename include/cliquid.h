@@ -140,6 +140,9 @@ typedef struct clq_solve_stats {
     double final_residual;     /* ||b - A p||_2 after the solve                 */
     double relative_residual;  /* final_residual / rhs_norm                     */
     double solve_ms;           /* device time of the solve (HIP events)         */
+    int64_t active_cells;      /* cells of the level-0 rows (i, j, *) holding an
+                                  interior fluid cell: what the finest-level
+                                  smoother and FCG vectors visit             */
 } clq_solve_stats;
 
 typedef struct clq_ctx clq_ctx;

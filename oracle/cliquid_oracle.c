@@ -1104,6 +1104,24 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     return CLQ_OK;
 }
 
+/* Cells of the level-0 rows (i, j, *) holding an interior cell with mask > 0
+ * (the statistic the device reports for its row-restricted sweeps). */
+static long long active_cells(const clq_ctx *c) {
+    if (c->nlevels < 2) return c->C;
+    const level_t *L = &c->lv[0];
+    long long rows = 0;
+    for (int i = 1; i < L->nx - 1; ++i)
+        for (int j = 1; j < L->ny - 1; ++j) {
+            const long long base = ((long long)i * L->ny + j) * L->nz;
+            for (int k = 1; k < L->nz - 1; ++k)
+                if (L->mask[base + k] > 0.0f) {
+                    ++rows;
+                    break;
+                }
+        }
+    return rows * L->nz;
+}
+
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
     if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0 ||
@@ -1122,6 +1140,7 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
         c->p64_valid = 1;
         clq_neumann_bc(c);
         s.solve_ms = now_ms() - t0;
+        s.active_cells = active_cells(c);
         c->stage_ms[CLQ_STAGE_PRESSURE_SOLVE] = s.solve_ms;
         if (st) *st = s;
         return CLQ_OK;
@@ -1147,6 +1166,7 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     s.relative_residual = bn > 0 ? r / bn : 0.0;
     clq_neumann_bc(c); /* pressure_solve.py:84 */
     s.solve_ms = now_ms() - t0;
+    s.active_cells = active_cells(c);
     c->stage_ms[CLQ_STAGE_PRESSURE_SOLVE] = s.solve_ms;
     if (st) *st = s;
     return CLQ_OK;

@@ -244,16 +244,21 @@ def test_binning_full_size_properties(hip_lib):
     assert np.array_equal(pos, scene.positions[order])
 
 
-@pytest.mark.parametrize("coarse_dtype", [0, 1], ids=["fp16", "bf16"])
-def test_mixed_refinement_bit_exact(oracle_lib, hip_lib, coarse_dtype):
+@pytest.mark.parametrize("coarse_dtype,vcycle_from", [(0, 0), (1, 0), (0, 1), (1, 2)],
+                         ids=["fp16-W", "bf16-W", "fp16-V1", "bf16-V2"])
+def test_mixed_refinement_bit_exact(oracle_lib, hip_lib, coarse_dtype, vcycle_from):
+    """Mode 2, W-cycle or the hybrid W/V cycle: the device (row-restricted
+    finest levels, single-work-group bottom levels) and the oracle (dense,
+    recursive) agree bit for bit."""
     scene = CASES[0]
     o, h = pair(oracle_lib, hip_lib, scene)
     for c in (o, h):
         run_to_pressure(c, scene)
-    kw = dict(post_smooth=2, max_cycles=40, tolerance=1e-8, coarse_dtype=coarse_dtype)
+    kw = dict(post_smooth=2, max_cycles=40, tolerance=1e-8, coarse_dtype=coarse_dtype, vcycle_from=vcycle_from)
     so = o.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, **kw)
     sh = h.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, **kw)
     assert so["cycles"] == sh["cycles"]
+    assert so["active_cells"] == sh["active_cells"] < np.prod(scene.shape)
     assert sh["relative_residual"] <= 1e-8
     assert abs(so["final_residual"] - sh["final_residual"]) <= 1e-9 * so["final_residual"]
     assert_fields_equal(o, h, ("pressure", "pressure_fp64"))
