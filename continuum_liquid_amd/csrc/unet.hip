@@ -157,6 +157,105 @@ __global__ __launch_bounds__(256) void conv3_kernel(const void *__restrict__ in,
             }
 }
 
+// ---- LDS-tiled convolution ------------------------------------------------------------------
+// A block owns a 4 x 4 x 16 brick of output voxels: wave w the x-slice x0 + w,
+// its M-tile mt the row y0 + mt, lane m of a tile the voxel z0 + m.  For each
+// chunk of 16 input channels the block stages the 6 x 6 x 18 input brick with
+// its halo in LDS once (zeros outside the grid; for the data gradient the ReLU
+// mask and the BF16 rounding of dL/dY are applied while staging), and the 27
+// taps read their A fragments from LDS: 32 contiguous bytes per voxel, so a
+// wave's fragment read is one conflict-free 512-byte run.  B fragments (the
+// weights of one tap and 16 output channels) come from global memory and
+// serve all four M-tiles.
+constexpr int kBX = 4, kBY = 4, kBZ = 16;
+constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2;
+constexpr int kHalo = kHX * kHY * kHZ;
+
+template <int CI, int CO, int MODE>
+__global__ __launch_bounds__(256) void conv3_lds_kernel(const void *__restrict__ in, const uint16_t *__restrict__ ymask,
+                                                        const uint16_t *__restrict__ wt,
+                                                        const float *__restrict__ bias, void *__restrict__ out,
+                                                        Dims d) {
+    constexpr int NT = CO / 16;
+    __shared__ __align__(16) uint16_t tile[kHalo * 16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m = lane & 15, kg = lane >> 4;
+    const int z0 = blockIdx.x * kBZ, y0 = blockIdx.y * kBY, x0 = blockIdx.z * kBX;
+    float4_t acc[kBY][NT];
+#pragma unroll
+    for (int mt = 0; mt < kBY; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[mt][t] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c0 = 0; c0 < CI; c0 += 16) {
+        __syncthreads();  // the previous chunk's fragments have been read
+        for (int item = threadIdx.x; item < kHalo * 2; item += 256) {
+            const int hv = item >> 1, half = item & 1;
+            const int hz = hv % kHZ, hxy = hv / kHZ, hy = hxy % kHY, hx = hxy / kHY;
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (gx >= 0 && gx < d.X && gy >= 0 && gy < d.Y && gz >= 0 && gz < d.Z) {
+                const int64_t off = (((int64_t)gx * d.Y + gy) * d.Z + gz) * CI + c0 + half * 8;
+                if (MODE == 0) {
+                    val = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(in) + off);
+                } else {
+                    const float4 g0 = *reinterpret_cast<const float4 *>(static_cast<const float *>(in) + off);
+                    const float4 g1 = *reinterpret_cast<const float4 *>(static_cast<const float *>(in) + off + 4);
+                    const uint4 y = *reinterpret_cast<const uint4 *>(ymask + off);
+                    auto z = [](uint32_t ypair, float ga, float gb) {
+                        const uint32_t lo = bf2f((uint16_t)(ypair & 0xffffu)) > 0.0f ? f2bf(ga) : 0u;
+                        const uint32_t hi = bf2f((uint16_t)(ypair >> 16)) > 0.0f ? f2bf(gb) : 0u;
+                        return lo | (hi << 16);
+                    };
+                    val = make_uint4(z(y.x, g0.x, g0.y), z(y.y, g0.z, g0.w), z(y.z, g1.x, g1.y), z(y.w, g1.z, g1.w));
+                }
+            }
+            *reinterpret_cast<uint4 *>(tile + hv * 16 + half * 8) = val;
+        }
+        __syncthreads();
+        for (int tap = 0; tap < 27; ++tap) {
+            int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+            if (MODE == 1) {
+                dx = -dx;
+                dy = -dy;
+                dz = -dz;
+            }
+            short4_t b[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                b[t] = *reinterpret_cast<const short4_t *>(wt + ((int64_t)tap * CO + t * 16 + m) * CI + c0 + kg * 4);
+            const int base = ((wave + 1 + dx) * kHY + 1 + dy) * kHZ + m + 1 + dz;
+#pragma unroll
+            for (int mt = 0; mt < kBY; ++mt) {
+                const short4_t a = *reinterpret_cast<const short4_t *>(tile + (base + mt * kHZ) * 16 + kg * 4);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[mt][t], 0, 0, 0);
+            }
+        }
+    }
+    // D[row = voxel z0 + kg*4 + j][col = channel t*16 + m]
+    const int ox = x0 + wave;
+    if (ox >= d.X) return;
+#pragma unroll
+    for (int mt = 0; mt < kBY; ++mt) {
+        const int oy = y0 + mt;
+        if (oy >= d.Y) continue;
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int oz = z0 + kg * 4 + j;
+                if (oz >= d.Z) continue;
+                const int64_t v = ((int64_t)ox * d.Y + oy) * d.Z + oz;
+                const int co = t * 16 + m;
+                if (MODE == 0)
+                    static_cast<uint16_t *>(out)[v * CO + co] = f2bf(fmaxf(acc[mt][t][j] + bias[co], 0.0f));
+                else
+                    static_cast<float *>(out)[v * CO + co] = acc[mt][t][j];
+            }
+    }
+}
+
 // ---- weight gradient ------------------------------------------------------------------------
 // Block b covers voxels [b*kWChunk, (b+1)*kWChunk) and writes its partial
 // dW[tap][ci][co] (FP32).  GEMM: A = X^T (ci x voxels), B = Z (voxels x co).
@@ -576,16 +675,17 @@ int net_check(const char *what) {
     return CLQ_OK;
 }
 
+inline dim3 conv_grid(const Dims &d) {
+    return dim3((unsigned)((d.Z + kBZ - 1) / kBZ), (unsigned)((d.Y + kBY - 1) / kBY), (unsigned)((d.X + kBX - 1) / kBX));
+}
 template <int CI, int CO>
 void conv_fwd(const uint16_t *x, const uint16_t *wf, const float *b, uint16_t *y, Dims d, hipStream_t s) {
-    hipLaunchKernelGGL((conv3_kernel<CI, CO, 0>), dim3((unsigned)((d.V + kVoxPerBlock - 1) / kVoxPerBlock)),
-                       dim3(256), 0, s, x, nullptr, wf, b, y, d);
+    hipLaunchKernelGGL((conv3_lds_kernel<CI, CO, 0>), conv_grid(d), dim3(256), 0, s, x, nullptr, wf, b, y, d);
 }
 // data gradient of a CI -> CO layer: maps dL/dY (CO channels) to dL/dX (CI)
 template <int CI, int CO>
 void conv_dgrad(const float *gy, const uint16_t *y, const uint16_t *wb, float *gx, Dims d, hipStream_t s) {
-    hipLaunchKernelGGL((conv3_kernel<CO, CI, 1>), dim3((unsigned)((d.V + kVoxPerBlock - 1) / kVoxPerBlock)),
-                       dim3(256), 0, s, gy, y, wb, nullptr, gx, d);
+    hipLaunchKernelGGL((conv3_lds_kernel<CO, CI, 1>), conv_grid(d), dim3(256), 0, s, gy, y, wb, nullptr, gx, d);
 }
 template <int CI, int CO>
 void conv_wgrad(const uint16_t *x, const float *gy, const uint16_t *y, float *partials, float *dw, float *db, Dims d,
