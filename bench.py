@@ -108,6 +108,9 @@ def main() -> int:
     ap.add_argument("--stall-ratio", type=float, default=0.5,
                     help="learned solver: residual ratio above which the W-cycle takes over")
     ap.add_argument("--tolerance", type=float, default=1e-6)
+    ap.add_argument("--coarse-extra", type=int, default=None, help="extra sweeps on the coarsest level")
+    ap.add_argument("--smooth", type=int, nargs=2, default=None, metavar=("PRE", "POST"),
+                    help="smoothing sweeps before and after the coarse-grid correction")
     ap.add_argument("--vcycle-from", type=int, default=None,
                     help="refined solves: W-cycle above this level, V-cycle from it down (0: W everywhere)")
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -129,6 +132,10 @@ def main() -> int:
     settings = solver_settings(args.solver, args.tolerance)
     if args.vcycle_from is not None:
         settings["vcycle_from"] = args.vcycle_from
+    if args.coarse_extra is not None:
+        settings["coarse_extra"] = args.coarse_extra
+    if args.smooth is not None:
+        settings["pre_smooth"], settings["post_smooth"] = args.smooth
     params = host_solver.StepParams(dt=1.0 / 240.0, gravity=scene.gravity, flip_ratio=0.95, solver=settings)
     print(f"# {scene.name}: {scene.particle_count} particles, grid {scene.shape}, solver {args.solver}",
           file=sys.stderr, flush=True)
@@ -194,12 +201,14 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{scene.name}, dx=1/{round(1 / scene.dx)}, {n} FP32 particles (8 per fluid cell, "
-                        f"stratified jitter, seed 0), dt=1/240, FLIP 0.95, solver {args.solver} with W(2,2) cycles to "
+                        f"stratified jitter, seed 0), dt=1/240, FLIP 0.95, solver {args.solver} with W({settings['pre_smooth']},{settings['post_smooth']}) cycles to "
                         f"{args.tolerance:g} relative residual",
             "grid": list(scene.shape),
             "particles": n,
             "solver": args.solver,
             "vcycle_from": settings.get("vcycle_from", 0),
+            "smoothing": [settings["pre_smooth"], settings["post_smooth"]],
+            "coarse_extra": settings["coarse_extra"],
         },
         "particle_steps_per_s": round(n * args.steps / elapsed, 1),
         "pressure_solve_ms": round(float(np.mean(solve_ms)), 3),

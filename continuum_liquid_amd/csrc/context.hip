@@ -819,7 +819,9 @@ static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rn
     }
     Level &L0 = c->lv[0];
     L0.b = c->r32;
-    HIP_TRY(hipMemsetAsync(c->p, 0, (size_t)c->C * 4, c->stream));
+    // e = 0 where the cycle reads it with a non-zero weight: the active rows
+    if (c->use_rows) launch_zero_rows(c->p, c->rows[0], c->nrows[0], c->nz, c->stream);
+    else HIP_TRY(hipMemsetAsync(c->p, 0, (size_t)c->C * 4, c->stream));
     w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
     L0.b = c->rhs;
     for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;

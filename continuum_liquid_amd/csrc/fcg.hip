@@ -28,19 +28,23 @@ __device__ __forceinline__ bool fluid_cell(const Level &L, int64_t q) {
     return interior(i, j, k, L.nx, L.ny, L.nz) && L.mask[q] > 0.0f;
 }
 
-// A v at a fluid cell, FP64, left-to-right over the non-zero weights (the
-// order of the refinement residual).
+// A v at an interior cell, FP64, left-to-right over the non-zero weights (the
+// order of the refinement residual).  Every load is issued before the weight
+// tests (all seven cells are in bounds at an interior cell).
 template <typename F>
 __device__ __forceinline__ double apply_a(const Level &L, int64_t q, F v, double inv_h2) {
     const int64_t sx = (int64_t)L.ny * L.nz, sy = L.nz;
+    const float w0 = L.wx[q], w1 = L.wx[q - sx], w2 = L.wy[q], w3 = L.wy[q - sy], w4 = L.wz[q], w5 = L.wz[q - 1];
+    const double v0 = v(q + sx), v1 = v(q - sx), v2 = v(q + sy), v3 = v(q - sy), v4 = v(q + 1), v5 = v(q - 1);
+    const double vc = v(q), d = (double)L.diag[q];
     double s = 0.0;
-    if (L.wx[q] != 0.0f) s = s + (double)L.wx[q] * v(q + sx);
-    if (L.wx[q - sx] != 0.0f) s = s + (double)L.wx[q - sx] * v(q - sx);
-    if (L.wy[q] != 0.0f) s = s + (double)L.wy[q] * v(q + sy);
-    if (L.wy[q - sy] != 0.0f) s = s + (double)L.wy[q - sy] * v(q - sy);
-    if (L.wz[q] != 0.0f) s = s + (double)L.wz[q] * v(q + 1);
-    if (L.wz[q - 1] != 0.0f) s = s + (double)L.wz[q - 1] * v(q - 1);
-    return (s - (double)L.diag[q] * v(q)) * inv_h2;
+    if (w0 != 0.0f) s = s + (double)w0 * v0;
+    if (w1 != 0.0f) s = s + (double)w1 * v1;
+    if (w2 != 0.0f) s = s + (double)w2 * v2;
+    if (w3 != 0.0f) s = s + (double)w3 * v3;
+    if (w4 != 0.0f) s = s + (double)w4 * v4;
+    if (w5 != 0.0f) s = s + (double)w5 * v5;
+    return (s - d * vc) * inv_h2;
 }
 
 // cell of work item t: row-major over the listed rows, or t itself
@@ -99,12 +103,18 @@ __global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const float 
     for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
         const int64_t q = cell_of(rows, L.nz, t);
         double dv = 0.0, av = 0.0;
-        if (fluid_cell(L, q)) {
-            auto dn = [&](int64_t t) { return first ? (double)z[t] : (double)z[t] + beta * d_old[t]; };
-            dv = dn(q);
-            // neighbours outside the fluid carry zero weight, so dn() is only
-            // evaluated at fluid cells
-            av = apply_a(L, q, dn, inv_h2);
+        int i, j, k;
+        coords(q, L.ny, L.nz, i, j, k);
+        if (interior(i, j, k, L.nx, L.ny, L.nz)) {
+            // neighbours outside the fluid carry zero weight: their values are
+            // loaded with the rest but never enter the sum
+            auto dn = [&](int64_t u) { return first ? (double)z[u] : (double)z[u] + beta * d_old[u]; };
+            const float m = L.mask[q];
+            const double dq = dn(q), aq = apply_a(L, q, dn, inv_h2);
+            if (m > 0.0f) {
+                dv = dq;
+                av = aq;
+            }
         }
         d_new[q] = dv;
         qv[q] = av;

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void rbgs_kernel(Level L, float dd, int parity
 }
 
 // Levels below this many cells take the eager (latency-bound) variants.
-constexpr int64_t kEagerCells = (int64_t)1 << 20;
+constexpr int64_t kEagerCells = (int64_t)1 << 22;
 
 void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s) {
     const int half = (L.nz + 1) >> 1;
@@ -223,13 +223,22 @@ __global__ __launch_bounds__(256) void zero_inactive_rows_kernel(Level L, const 
     }
 }
 
-__global__ __launch_bounds__(256) void rbgs_rows_kernel(Level L, float dd, int parity, const int32_t *__restrict__ rows,
-                                                        int64_t work) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= work) return;
-    const uint32_t half = (uint32_t)(L.nz + 1) >> 1;
-    const uint32_t ar = (uint32_t)t / half;
-    rbgs_cell(L, dd, parity, (uint32_t)rows[ar], (int)((uint32_t)t - ar * half));
+// One block per active row and colour half: lane t takes the row's same-colour
+// cells t and t + 256 (rows of up to 1024 cells), all loads issued first.
+__global__ __launch_bounds__(256) void rbgs_rows_kernel(Level L, float dd, int parity, const int32_t *__restrict__ rows) {
+    const uint32_t row = (uint32_t)rows[blockIdx.x];
+    const int half = (L.nz + 1) >> 1;
+    for (int kk = threadIdx.x; kk < half; kk += 256) rbgs_cell_eager(L, dd, parity, row, kk);
+}
+
+__global__ __launch_bounds__(256) void zero_rows_kernel(float *__restrict__ p, const int32_t *__restrict__ rows, int nz) {
+    const int64_t base = (int64_t)rows[blockIdx.x] * nz;
+    for (int k = threadIdx.x; k < nz; k += 256) p[base + k] = 0.0f;
+}
+
+void launch_zero_rows(float *p, const int32_t *rows, int64_t nrows, int nz, hipStream_t s) {
+    if (nrows == 0) return;
+    hipLaunchKernelGGL(zero_rows_kernel, dim3((unsigned)nrows), dim3(256), 0, s, p, rows, nz);
 }
 
 void launch_row_flags(const Level &L, int32_t *flags, hipStream_t s) {
@@ -255,10 +264,8 @@ void launch_zero_inactive_rows(const Level &L, const int32_t *flags, int coarse,
 }
 
 void launch_rbgs_rows(const Level &L, float delta, int parity, const int32_t *rows, int64_t nrows, hipStream_t s) {
-    const int64_t work = nrows * ((L.nz + 1) >> 1);
-    if (work == 0) return;
-    hipLaunchKernelGGL(rbgs_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L, delta * delta,
-                       parity, rows, work);
+    if (nrows == 0) return;
+    hipLaunchKernelGGL(rbgs_rows_kernel, dim3((unsigned)nrows), dim3(256), 0, s, L, delta * delta, parity, rows);
 }
 
 __global__ __launch_bounds__(256) void coarse_operator_kernel(
@@ -389,7 +396,7 @@ __global__ __launch_bounds__(256) void restrict_rows_kernel(Level F, Level K, fl
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= work) return;
     const int64_t ar = t / K.nz;
-    restrict_item(F, K, inv_delta2, (int64_t)krows[ar] * K.nz + (t - ar * K.nz));
+    restrict_item_eager(F, K, inv_delta2, (int64_t)krows[ar] * K.nz + (t - ar * K.nz));
 }
 
 void launch_restrict_rows(const Level &F, const Level &K, float inv_delta2, const int32_t *krows, int64_t nkrows,
@@ -434,7 +441,7 @@ __global__ __launch_bounds__(256) void prolongate_rows_kernel(Level F, Level K, 
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= work) return;
     const int64_t ar = t / F.nz;
-    prolong_item(F, K, (int64_t)frows[ar] * F.nz + (t - ar * F.nz));
+    prolong_item_eager(F, K, (int64_t)frows[ar] * F.nz + (t - ar * F.nz));
 }
 
 void launch_prolongate_rows(const Level &F, const Level &K, const int32_t *frows, int64_t nfrows, hipStream_t s) {
