@@ -147,6 +147,9 @@ struct BottomArgs {
     double dd64;
 };
 void launch_wcycle_bottom(const BottomArgs &a, hipStream_t s);
+int resident_bottom_fits(const Level &B, const Level &K);
+void launch_bottom_resident(const Level &B, const Level &K, float delta, bool f64, int pre, int post, int coarse_iters,
+                            int coarse_calls, hipStream_t s);
 // active rows (cells (i, j, *) holding an interior cell with mask > 0)
 void launch_row_flags(const Level &L, int32_t *flags, hipStream_t s);
 void launch_coarse_row_flags(const int32_t *fine, const Level &F, const Level &K, int32_t *flags, hipStream_t s);

@@ -623,6 +623,12 @@ static int bottom_level(const clq_ctx *c) {
 }
 
 static void launch_bottom(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool f64) {
+    if (level == c->nlevels - 2 && resident_bottom_fits(c->lv[level], c->lv[level + 1])) {
+        const int calls = (vfrom <= 0 || level < vfrom) ? 2 : 1;
+        launch_bottom_resident(c->lv[level], c->lv[level + 1], (float)c->dx, f64, pre, post, pre + extra, calls,
+                               c->stream);
+        return;
+    }
     BottomArgs a;
     for (int l = 0; l < c->nlevels; ++l) a.lv[l] = c->lv[l];
     a.nops = 0;

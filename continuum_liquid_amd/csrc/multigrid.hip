@@ -584,6 +584,193 @@ void launch_wcycle_bottom(const BottomArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(wcycle_bottom_kernel, dim3(1), dim3(kCoarseThreads), 0, s, a);
 }
 
+// ---- the last two levels, resident in one work-group ----------------------------------
+// When the bottom is level B (at most kResB cells) above the coarsest level K
+// (at most kResK cells), the cycle below B is: smooth B, restrict to K,
+// coarse-smooth K once or twice, prolong, smooth B.  Here B's pressure lives
+// in LDS and its operator and b in registers (six cells per lane), K's cell
+// tid in lane tid; B's residuals reach the restriction through LDS.  Every
+// value is computed by the same operations in the same order as the kernels
+// above (rbgs_cell, restrict_item, coarse_smooth_body, prolong_item), so the
+// result is bit-identical; between two coarse calls the FP64 iterate is
+// rounded to FP32, as the level array stores it.
+constexpr int kResB = 6 * kCoarseThreads, kResK = kCoarseThreads;
+
+int resident_bottom_fits(const Level &B, const Level &K) { return B.C <= kResB && K.C <= kResK; }
+
+template <typename T>
+__global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B, Level K, float dd, T ddk,
+                                                                        float inv_delta2, int pre, int post,
+                                                                        int coarse_iters, int coarse_calls) {
+    __shared__ float pB[kResB];
+    __shared__ float rB[kResB];
+    __shared__ T pK[kResK];
+    const int tid = threadIdx.x;
+    const int bnx = B.nx, bny = B.ny, bnz = B.nz, BC = (int)B.C;
+    const int bsx = bny * bnz, bsy = bnz;
+    // level B: cell tid + s * 1024, flags: bit 0 parity, 1 zero (mask <= 0),
+    // 2 update (interior, fluid, d > 0), 3 fluid (interior, mask > 0)
+    float w[6][6], bv[6], dv[6];
+    int fl[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int cell = tid + s * kCoarseThreads;
+        fl[s] = 0;
+        bv[s] = 0.0f;
+        dv[s] = 1.0f;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) w[s][q] = 0.0f;
+        if (cell < BC) {
+            pB[cell] = B.p[cell];
+            int i, j, k;
+            coords(cell, bny, bnz, i, j, k);
+            const float m = B.mask[cell];
+            fl[s] = (i + j + k) & 1;
+            if (m <= 0.0f) {
+                fl[s] |= 2;
+            } else if (interior(i, j, k, bnx, bny, bnz)) {
+                fl[s] |= 8;
+                dv[s] = B.diag[cell];
+                if (dv[s] > 0.0f) fl[s] |= 4;
+                bv[s] = load_b(B.b, B.b16, B.bkind, B.bscale, cell);
+                w[s][0] = B.wx[cell - bsx];
+                w[s][1] = B.wx[cell];
+                w[s][2] = B.wy[cell];
+                w[s][3] = B.wy[cell - bsy];
+                w[s][4] = B.wz[cell];
+                w[s][5] = B.wz[cell - 1];
+            }
+        }
+    }
+    // level K: cell tid
+    const int knx = K.nx, kny = K.ny, knz = K.nz, KC = (int)K.C;
+    const int ksx = kny * knz, ksy = knz;
+    int ka, kb, kc;
+    coords(tid, kny, knz, ka, kb, kc);
+    const bool kin = tid < KC && interior(ka, kb, kc, knx, kny, knz);
+    bool kact = false;
+    T kw[6] = {T(0), T(0), T(0), T(0), T(0), T(0)}, kd = T(1);
+    if (kin && K.mask[tid] > 0.0f && K.diag[tid] > 0.0f) {
+        kact = true;
+        kw[0] = (T)K.wx[tid - ksx];
+        kw[1] = (T)K.wx[tid];
+        kw[2] = (T)K.wy[tid];
+        kw[3] = (T)K.wy[tid - ksy];
+        kw[4] = (T)K.wz[tid];
+        kw[5] = (T)K.wz[tid - 1];
+        kd = (T)K.diag[tid];
+    }
+    const int kpar = (ka + kb + kc) & 1;
+    __syncthreads();
+
+    auto smooth_b = [&](int iterations) {
+        for (int it = 0; it < iterations; ++it)
+            for (int parity = 0; parity < 2; ++parity) {
+#pragma unroll
+                for (int s = 0; s < 6; ++s) {
+                    const int c = tid + s * kCoarseThreads;
+                    if (c >= BC || (fl[s] & 1) != parity) continue;
+                    if (fl[s] & 2) {
+                        pB[c] = 0.0f;
+                    } else if (fl[s] & 4) {  // wsum (multigrid.cl:7-23) and :227
+                        float t = w[s][0] * pB[c - bsx];
+                        t = fmaf(w[s][1], pB[c + bsx], t);
+                        t = fmaf(w[s][2], pB[c + bsy], t);
+                        t = fmaf(w[s][3], pB[c - bsy], t);
+                        t = fmaf(w[s][4], pB[c + 1], t);
+                        t = fmaf(w[s][5], pB[c - 1], t);
+                        pB[c] = fmaf(-dd, bv[s], t) / dv[s];
+                    }
+                }
+                __syncthreads();
+            }
+    };
+
+    smooth_b(pre);
+    // restriction: residuals of B's interior fluid cells (+0 elsewhere, which
+    // leaves the child sums unchanged), then the sum over the eight children
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int c = tid + s * kCoarseThreads;
+        if (c >= BC) continue;
+        float r = 0.0f;
+        if (fl[s] & 8) {
+            float t = w[s][0] * pB[c - bsx];
+            t = fmaf(w[s][1], pB[c + bsx], t);
+            t = fmaf(w[s][2], pB[c + bsy], t);
+            t = fmaf(w[s][3], pB[c - bsy], t);
+            t = fmaf(w[s][4], pB[c + 1], t);
+            t = fmaf(w[s][5], pB[c - 1], t);
+            r = bv[s] - fmaf(-dv[s], pB[c], t) * inv_delta2;
+        }
+        rB[c] = r;
+    }
+    __syncthreads();
+    T kbv = T(0);
+    if (tid < KC) {
+        float rhs = 0.0f;
+        if (kin) {
+            const int f0 = 1 + 2 * (ka - 1), f1 = 1 + 2 * (kb - 1), f2 = 1 + 2 * (kc - 1);
+            for (int q = 0; q < 8; ++q) {
+                const int i = f0 + (q >> 2), j = f1 + ((q >> 1) & 1), k = f2 + (q & 1);
+                if (interior(i, j, k, bnx, bny, bnz)) rhs = rhs + rB[gidx(i, j, k, bny, bnz)];
+            }
+        }
+        store_b(K.b, K.b16, K.bkind, K.bscale, tid, rhs);
+        kbv = (T)load_b(K.b, K.b16, K.bkind, K.bscale, tid);
+        pK[tid] = T(0);
+    }
+    __syncthreads();
+    // coarsest level (coarse_smooth_body), coarse_calls calls of coarse_iters sweeps
+    for (int call = 0; call < coarse_calls; ++call) {
+        for (int it = 0; it < coarse_iters; ++it)
+            for (int parity = 0; parity < 2; ++parity) {
+                if (kact && kpar == parity) {
+                    const int c = tid;
+                    T t = kw[0] * pK[c - ksx];
+                    t = fma(kw[1], pK[c + ksx], t);
+                    t = fma(kw[2], pK[c + ksy], t);
+                    t = fma(kw[3], pK[c - ksy], t);
+                    t = fma(kw[4], pK[c + 1], t);
+                    t = fma(kw[5], pK[c - 1], t);
+                    pK[c] = fma(-ddk, kbv, t) / kd;  // multigrid.cl:257
+                }
+                __syncthreads();
+            }
+        if (tid < KC) pK[tid] = (T)(float)pK[tid];  // the level array holds FP32
+        __syncthreads();
+    }
+    // prolongation (prolong_item)
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int c = tid + s * kCoarseThreads;
+        if (c >= BC || !(fl[s] & 8)) continue;
+        int i, j, k;
+        coords(c, bny, bnz, i, j, k);
+        const int a = min((i - 1) / 2 + 1, knx - 2), b = min((j - 1) / 2 + 1, kny - 2), cc = min((k - 1) / 2 + 1, knz - 2);
+        pB[c] = pB[c] + (float)pK[gidx(a, b, cc, kny, knz)];
+    }
+    __syncthreads();
+    smooth_b(post);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int c = tid + s * kCoarseThreads;
+        if (c < BC) B.p[c] = pB[c];
+    }
+    if (tid < KC) K.p[tid] = (float)pK[tid];
+}
+
+void launch_bottom_resident(const Level &B, const Level &K, float delta, bool f64, int pre, int post, int coarse_iters,
+                            int coarse_calls, hipStream_t s) {
+    const float dd = delta * delta;
+    if (f64)
+        hipLaunchKernelGGL(bottom_resident_kernel<double>, dim3(1), dim3(kCoarseThreads), 0, s, B, K, dd,
+                           (double)delta * (double)delta, 1.0f / dd, pre, post, coarse_iters, coarse_calls);
+    else
+        hipLaunchKernelGGL(bottom_resident_kernel<float>, dim3(1), dim3(kCoarseThreads), 0, s, B, K, dd, dd, 1.0f / dd,
+                           pre, post, coarse_iters, coarse_calls);
+}
+
 int coarse_f64_capacity() { return kCoarseCells; }
 
 // ---- FP64 iterative-refinement update -----------------------------------------------

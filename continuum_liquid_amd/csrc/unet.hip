@@ -171,11 +171,15 @@ constexpr int kBX = 4, kBY = 4, kBZ = 16;
 constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
 
-template <int CI, int CO, int MODE>
+// CA > 0 (forward only): the input is the concatenation [up(coarse), in] with
+// CA channels from the coarse tensor (dims dc, read at the parent voxel) and
+// CI - CA from `in`, built while staging instead of materialised.
+template <int CI, int CO, int MODE, int CA = 0>
 __global__ __launch_bounds__(256) void conv3_lds_kernel(const void *__restrict__ in, const uint16_t *__restrict__ ymask,
                                                         const uint16_t *__restrict__ wt,
                                                         const float *__restrict__ bias, void *__restrict__ out,
-                                                        Dims d) {
+                                                        Dims d, const uint16_t *__restrict__ coarse = nullptr,
+                                                        Dims dc = Dims{0, 0, 0, 0}) {
     constexpr int NT = CO / 16;
     __shared__ __align__(16) uint16_t tile[kHalo * 16];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -194,8 +198,15 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(const void *__restrict__
             const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
             uint4 val = make_uint4(0u, 0u, 0u, 0u);
             if (gx >= 0 && gx < d.X && gy >= 0 && gy < d.Y && gz >= 0 && gz < d.Z) {
-                const int64_t off = (((int64_t)gx * d.Y + gy) * d.Z + gz) * CI + c0 + half * 8;
-                if (MODE == 0) {
+                const int64_t vox_index = ((int64_t)gx * d.Y + gy) * d.Z + gz;
+                const int64_t off = vox_index * CI + c0 + half * 8;
+                if (CA > 0 && c0 < CA) {
+                    const int64_t pv = ((int64_t)(gx >> 1) * dc.Y + (gy >> 1)) * dc.Z + (gz >> 1);
+                    val = *reinterpret_cast<const uint4 *>(coarse + pv * CA + c0 + half * 8);
+                } else if (CA > 0) {
+                    val = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(in) + vox_index * (CI - CA) +
+                                                           (c0 - CA) + half * 8);
+                } else if (MODE == 0) {
                     val = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(in) + off);
                 } else {
                     const float4 g0 = *reinterpret_cast<const float4 *>(static_cast<const float *>(in) + off);
@@ -680,12 +691,21 @@ inline dim3 conv_grid(const Dims &d) {
 }
 template <int CI, int CO>
 void conv_fwd(const uint16_t *x, const uint16_t *wf, const float *b, uint16_t *y, Dims d, hipStream_t s) {
-    hipLaunchKernelGGL((conv3_lds_kernel<CI, CO, 0>), conv_grid(d), dim3(256), 0, s, x, nullptr, wf, b, y, d);
+    hipLaunchKernelGGL((conv3_lds_kernel<CI, CO, 0>), conv_grid(d), dim3(256), 0, s, x, nullptr, wf, b, y, d, nullptr,
+                       Dims{0, 0, 0, 0});
+}
+// input [up(coarse) (CA channels, dims dc), skip (CI - CA channels)]
+template <int CI, int CO, int CA>
+void conv_fwd_cat(const uint16_t *coarse, Dims dc, const uint16_t *skip, const uint16_t *wf, const float *b,
+                  uint16_t *y, Dims d, hipStream_t s) {
+    hipLaunchKernelGGL((conv3_lds_kernel<CI, CO, 0, CA>), conv_grid(d), dim3(256), 0, s, skip, nullptr, wf, b, y, d,
+                       coarse, dc);
 }
 // data gradient of a CI -> CO layer: maps dL/dY (CO channels) to dL/dX (CI)
 template <int CI, int CO>
 void conv_dgrad(const float *gy, const uint16_t *y, const uint16_t *wb, float *gx, Dims d, hipStream_t s) {
-    hipLaunchKernelGGL((conv3_lds_kernel<CO, CI, 1>), conv_grid(d), dim3(256), 0, s, gy, y, wb, nullptr, gx, d);
+    hipLaunchKernelGGL((conv3_lds_kernel<CO, CI, 1>), conv_grid(d), dim3(256), 0, s, gy, y, wb, nullptr, gx, d,
+                       nullptr, Dims{0, 0, 0, 0});
 }
 template <int CI, int CO>
 void conv_wgrad(const uint16_t *x, const float *gy, const uint16_t *y, float *partials, float *dw, float *db, Dims d,
@@ -717,11 +737,9 @@ void forward(clq_net *n, hipStream_t s) {
     hipLaunchKernelGGL(pool_fwd_kernel, g1(d2.V * 32), dim3(256), 0, s, n->s1, n->p1, d1, d2, 32);
     conv_fwd<32, 64>(n->p1, n->wf[4], w + n->boff[4], n->a5, d2, s);
     conv_fwd<64, 64>(n->a5, n->wf[5], w + n->boff[5], n->a6, d2, s);
-    hipLaunchKernelGGL(upcat_fwd_kernel, g1(d1.V * 96), dim3(256), 0, s, n->a6, n->s1, n->u1, d1, d2, 64, 32);
-    conv_fwd<96, 32>(n->u1, n->wf[6], w + n->boff[6], n->a7, d1, s);
+    conv_fwd_cat<96, 32, 64>(n->a6, d2, n->s1, n->wf[6], w + n->boff[6], n->a7, d1, s);
     conv_fwd<32, 32>(n->a7, n->wf[7], w + n->boff[7], n->a8, d1, s);
-    hipLaunchKernelGGL(upcat_fwd_kernel, g1(d0.V * 48), dim3(256), 0, s, n->a8, n->s0, n->u0, d0, d1, 32, 16);
-    conv_fwd<48, 16>(n->u0, n->wf[8], w + n->boff[8], n->a9, d0, s);
+    conv_fwd_cat<48, 16, 32>(n->a8, d1, n->s0, n->wf[8], w + n->boff[8], n->a9, d0, s);
     conv_fwd<16, 16>(n->a9, n->wf[9], w + n->boff[9], n->a10, d0, s);
     hipLaunchKernelGGL(head_fwd_kernel, g1(d0.V), dim3(256), 0, s, n->a10, w + n->head_off, d0.V, n->pred);
 }
@@ -732,6 +750,9 @@ void forward(clq_net *n, hipStream_t s) {
 void backward(clq_net *n, int acc, hipStream_t s) {
     const Dims d0 = n->d0, d1 = n->d1, d2 = n->d2;
     float *gw = n->g, *A = n->gbuf[0], *B = n->gbuf[1], *Cb = n->gbuf[2], *P = n->partials;
+    // the weight gradients of c7 and c9 read their concatenated inputs
+    hipLaunchKernelGGL(upcat_fwd_kernel, g1(d1.V * 96), dim3(256), 0, s, n->a6, n->s1, n->u1, d1, d2, 64, 32);
+    hipLaunchKernelGGL(upcat_fwd_kernel, g1(d0.V * 48), dim3(256), 0, s, n->a8, n->s0, n->u0, d0, d1, 32, 16);
     hipLaunchKernelGGL(head_bwd_weight_kernel, dim3(17), dim3(256), 0, s, n->gpred, n->a10, d0.V, gw + n->head_off,
                        acc);
     hipLaunchKernelGGL(head_bwd_data_kernel, g1(d0.V * 16), dim3(256), 0, s, n->gpred, n->w + n->head_off, d0.V, A);
@@ -793,6 +814,8 @@ int ensure_training(clq_net *n) {
         if (int rc = n->alloc_n(&b, gmax)) return rc;
     if (int rc = n->alloc_n(&n->gs0, V0 * 16)) return rc;
     if (int rc = n->alloc_n(&n->gs1, V1 * 32)) return rc;
+    if (int rc = n->alloc_n(&n->u0, V0 * 48)) return rc;  // concatenated inputs of c9 and c7
+    if (int rc = n->alloc_n(&n->u1, V1 * 96)) return rc;
     if (int rc = n->alloc_n(&n->partials, pmax)) return rc;
     if (int rc = n->alloc_n(&n->loss_partial, kLossBlocks)) return rc;
     for (auto &b : n->stage)
@@ -896,8 +919,8 @@ int clq_net_create(int32_t nx, int32_t ny, int32_t nz, double dx, clq_net **out)
         uint16_t **p;
         int64_t count;
     } acts[] = {{&n->in16, V0 * 16}, {&n->a1, V0 * 16}, {&n->s0, V0 * 16}, {&n->p0, V1 * 16}, {&n->a3, V1 * 32},
-                {&n->s1, V1 * 32},   {&n->p1, V2 * 32}, {&n->a5, V2 * 64}, {&n->a6, V2 * 64}, {&n->u1, V1 * 96},
-                {&n->a7, V1 * 32},   {&n->a8, V1 * 32}, {&n->u0, V0 * 48}, {&n->a9, V0 * 16}, {&n->a10, V0 * 16}};
+                {&n->s1, V1 * 32},   {&n->p1, V2 * 32}, {&n->a5, V2 * 64}, {&n->a6, V2 * 64},
+                {&n->a7, V1 * 32},   {&n->a8, V1 * 32}, {&n->a9, V0 * 16}, {&n->a10, V0 * 16}};
     for (auto &a : acts)
         if (!rc) rc = n->alloc_n(a.p, a.count);
     if (!rc) rc = n->alloc_n(&n->pred, V0);
