@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void rbgs_kernel(Level L, float dd, int parity
 }
 
 // Levels below this many cells take the eager (latency-bound) variants.
-constexpr int64_t kEagerCells = (int64_t)1 << 22;
+constexpr int64_t kEagerCells = (int64_t)1 << 21;
 
 void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s) {
     const int half = (L.nz + 1) >> 1;
