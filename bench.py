@@ -96,6 +96,21 @@ def cpu_baseline(scene, params, steps: int) -> dict:
     }
 
 
+def calibrate_copy(nbytes: int) -> int:
+    """Copy nbytes between two device buffers three times (torch's copy
+    kernel: nbytes read and nbytes written per launch)."""
+    import torch
+
+    n = nbytes // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": n * 4, "launches": 3}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,7 +130,12 @@ def main() -> int:
                     help="refined solves: W-cycle above this level, V-cycle from it down (0: W everywhere)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -11,8 +11,10 @@
 //   L0: c9 48->16, c10 16->16            head: 1x1x1 16->1, linear, FP32
 // With A = L / h^2 (L the stencil in grid units, h the cell size), the network
 // works on the grid-unit problem: inputs rhs / s and prev / (s h^2), output
-// p = s h^2 * head, s = rms of rhs over the interior fluid cells.  The map is
-// invariant to the scale of the right-hand side, and the prediction is O(1).
+// p = prev + s h^2 * head (prev = 0 when absent), s = rms of rhs over the
+// interior fluid cells.  The head predicts the change from the previous
+// pressure, the map is invariant to the scale of the right-hand side, and the
+// prediction is O(1).
 // The network grid is the pressure grid rounded up to multiples of 4 in each
 // dimension (zero inputs in the padding).  Activations are BF16 with channels
 // fastest ([voxel][channel]); gradients are FP32.  Convolutions are implicit
@@ -519,14 +521,15 @@ __global__ void head_fwd_kernel(const uint16_t *__restrict__ y, const float *__r
     pred[v] = s;
 }
 
-// out[g] = s h^2 * pred[v(g)] on the pressure grid
-__global__ void unpack_output_kernel(const float *__restrict__ pred, const float *__restrict__ scal, Dims G, Dims d0,
-                                     float *__restrict__ out) {
+// out[g] = prev[g] + s h^2 * pred[v(g)] on the pressure grid (prev may be NULL)
+__global__ void unpack_output_kernel(const float *__restrict__ pred, const float *__restrict__ scal,
+                                     const float *__restrict__ prev, Dims G, Dims d0, float *__restrict__ out) {
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (g >= G.V) return;
     int x, y, z;
     vox(G, g, x, y, z);
-    out[g] = scal[1] * pred[((int64_t)x * d0.Y + y) * d0.Z + z];
+    const float change = scal[1] * pred[((int64_t)x * d0.Y + y) * d0.Z + z];
+    out[g] = prev ? prev[g] + change : change;
 }
 
 // gpred[v] = s h^2 * grad_out[g(v)] (dL/d pred from dL/d out), zero in the padding
@@ -540,11 +543,12 @@ __global__ void pack_grad_kernel(const float *__restrict__ gout, const float *__
 }
 
 // Mean squared error of the normalised prediction over the interior fluid
-// cells: e = pred - target / (s h^2), L = sum e^2 / count, gpred = 2 e / count / batch.
+// cells: e = pred - (target - prev) / (s h^2), L = sum e^2 / count,
+// gpred = 2 e / count / batch (prev may be NULL).
 __global__ __launch_bounds__(256) void mse_kernel(const float *__restrict__ pred, const float *__restrict__ target,
-                                                  const float *__restrict__ mask, const float *__restrict__ scal,
-                                                  Dims G, Dims d0, float inv_batch, float *__restrict__ gpred,
-                                                  float *__restrict__ partial) {
+                                                  const float *__restrict__ prev, const float *__restrict__ mask,
+                                                  const float *__restrict__ scal, Dims G, Dims d0, float inv_batch,
+                                                  float *__restrict__ gpred, float *__restrict__ partial) {
     float s = 0.0f;
     const float is = scal[3], inv_n = scal[2];
     for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < d0.V; v += (int64_t)gridDim.x * 256) {
@@ -552,7 +556,7 @@ __global__ __launch_bounds__(256) void mse_kernel(const float *__restrict__ pred
         bool inner = false;
         float gv = 0.0f;
         if (grid_cell(G, d0, v, g, inner) && inner && mask[g] > 0.0f) {
-            const float e = pred[v] - target[g] * is;
+            const float e = pred[v] - (prev ? target[g] - prev[g] : target[g]) * is;
             gv = 2.0f * e * inv_n * inv_batch;
             s += e * e * inv_n;
         }
@@ -860,7 +864,7 @@ int unet_device(const clq_net *n) { return n->device; }
 void unet_predict(clq_net *n, const float *rhs, const float *mask, const float *prev, float *out, hipStream_t s) {
     normalise_and_pack(n, rhs, mask, prev, s);
     forward(n, s);
-    hipLaunchKernelGGL(unpack_output_kernel, g1(n->G.V), dim3(256), 0, s, n->pred, n->scal, n->G, n->d0, out);
+    hipLaunchKernelGGL(unpack_output_kernel, g1(n->G.V), dim3(256), 0, s, n->pred, n->scal, prev, n->G, n->d0, out);
 }
 
 }  // namespace clq
@@ -1018,8 +1022,9 @@ int clq_net_train_step(clq_net *n, int32_t batch, const float *rhs, const float 
         NET_TRY(hipMemcpyAsync(n->stage[3], target + b * cells, bytes, hipMemcpyHostToDevice, n->stream));
         normalise_and_pack(n, n->stage[0], n->stage[1], prev ? n->stage[2] : nullptr, n->stream);
         forward(n, n->stream);
-        hipLaunchKernelGGL(mse_kernel, dim3(kLossBlocks), dim3(256), 0, n->stream, n->pred, n->stage[3], n->stage[1],
-                           n->scal, n->G, n->d0, 1.0f / (float)batch, n->gpred, n->loss_partial);
+        hipLaunchKernelGGL(mse_kernel, dim3(kLossBlocks), dim3(256), 0, n->stream, n->pred, n->stage[3],
+                           prev ? n->stage[2] : nullptr, n->stage[1], n->scal, n->G, n->d0, 1.0f / (float)batch,
+                           n->gpred, n->loss_partial);
         backward(n, b > 0, n->stream);
         NET_TRY(hipMemcpyAsync(n->host_loss, n->loss_partial, kLossBlocks * 4, hipMemcpyDeviceToHost, n->stream));
         NET_TRY(hipStreamSynchronize(n->stream));

@@ -9,7 +9,8 @@ vector (layout in include/cliquid.h, clq_net_*) and computes the same map:
             grid rounded up to multiples of 4 (zero padding)
     c1..c10 = 3x3x3 conv, zero padding, bias, ReLU, then rounded to bf16;
             average pooling 2x2x2 and nearest upsampling + concat as listed
-    out   = s h^2 * (head . a10 + head_bias), cropped to the grid
+    out   = prev + s h^2 * (head . a10 + head_bias), cropped to the grid
+            (prev = 0 when absent)
 
 With `emulate=True` weights and every stored activation are rounded to bf16
 (round to nearest even), as on the device; the rounding passes gradients
@@ -108,7 +109,10 @@ def forward(params, rhs, mask, prev=None, dx: float = 1.0, *, emulate: bool = Tr
     a9 = conv(torch.cat([up(a8), s0], 1), 8)
     a10 = conv(a9, 9)
     pred = torch.einsum("c,cxyz->xyz", hw, a10[0]) + hb
-    return float(out_scale) * pred[:shape[0], :shape[1], :shape[2]]
+    out = float(out_scale) * pred[:shape[0], :shape[1], :shape[2]]
+    if prev is not None:
+        out = out + torch.as_tensor(prev.astype(np.float64))
+    return out
 
 
 def gradients(params, rhs, mask, prev, grad_out, dx: float = 1.0, *, emulate: bool = True) -> np.ndarray:
