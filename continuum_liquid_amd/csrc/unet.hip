@@ -3,18 +3,19 @@
 // solve is the W-cycle alone, Solver/Kernel/pressure_solve.py:61-83).
 //
 // Architecture (channels; every conv is 3x3x3, zero padding, bias, ReLU):
-//   input 3 (rhs / s, fluid indicator, previous pressure / (s h^2)), zero-padded to 16
+//   input 3 (rhs / s, fluid indicator, previous pressure / k), zero-padded to 16
 //   L0: c1 16->16, c2 16->16 = s0        pool 2x2x2 (average)
 //   L1: c3 16->32, c4 32->32 = s1        pool
 //   L2: c5 32->64, c6 64->64             nearest upsample, concat [up, s1] -> 96
 //   L1: c7 96->32, c8 32->32             nearest upsample, concat [up, s0] -> 48
 //   L0: c9 48->16, c10 16->16            head: 1x1x1 16->1, linear, FP32
-// With A = L / h^2 (L the stencil in grid units, h the cell size), the network
-// works on the grid-unit problem: inputs rhs / s and prev / (s h^2), output
-// p = prev + s h^2 * head (prev = 0 when absent), s = rms of rhs over the
-// interior fluid cells.  The head predicts the change from the previous
-// pressure, the map is invariant to the scale of the right-hand side, and the
-// prediction is O(1).
+// Scales: s = rms of rhs over the interior fluid cells, k = s (N h)^2 / 1024
+// with N the largest grid dimension (for A = L / h^2, p = A^-1 b has the
+// magnitude s h^2 on local modes and up to s (N h)^2 on the domain-wide ones;
+// k is the 32^3 grid-unit scale carried to every resolution).  Inputs rhs / s
+// and prev / k, output p = prev + k * head (prev = 0 when absent): the head
+// predicts the change from the previous pressure, O(1) at every resolution,
+// and the map is invariant to the scale of the right-hand side.
 // The network grid is the pressure grid rounded up to multiples of 4 in each
 // dimension (zero inputs in the padding).  Activations are BF16 with channels
 // fastest ([voxel][channel]); gradients are FP32.  Convolutions are implicit
@@ -460,9 +461,9 @@ __global__ __launch_bounds__(256) void norm_partials_kernel(const float *__restr
     }
 }
 
-// scal[0] = 1/s, scal[1] = s h^2, scal[2] = 1/count, scal[3] = 1/(s h^2)
+// scal[0] = 1/s, scal[1] = k = s * len2, scal[2] = 1/count, scal[3] = 1/k
 // (all 0 when there is no fluid cell or rhs vanishes there)
-__global__ __launch_bounds__(256) void norm_final_kernel(const double *__restrict__ partials, int blocks, double h2,
+__global__ __launch_bounds__(256) void norm_final_kernel(const double *__restrict__ partials, int blocks, double len2,
                                                          float *__restrict__ scal) {
     __shared__ double sh[2][256];
     double s = 0.0, n = 0.0;
@@ -484,13 +485,13 @@ __global__ __launch_bounds__(256) void norm_final_kernel(const double *__restric
         const double cnt = sh[1][0];
         const double rms = cnt > 0.0 ? sqrt(sh[0][0] / cnt) : 0.0;
         scal[0] = rms > 0.0 ? (float)(1.0 / rms) : 0.0f;
-        scal[1] = (float)(rms * h2);
+        scal[1] = (float)(rms * len2);
         scal[2] = cnt > 0.0 ? (float)(1.0 / cnt) : 0.0f;
-        scal[3] = rms > 0.0 ? (float)(1.0 / (rms * h2)) : 0.0f;
+        scal[3] = rms > 0.0 ? (float)(1.0 / (rms * len2)) : 0.0f;
     }
 }
 
-// in16[v] = bf16 [rhs / s, mask > 0, prev / (s h^2), 0 ... 0]
+// in16[v] = bf16 [rhs / s, mask > 0, prev / k, 0 ... 0]
 __global__ void pack_input_kernel(const float *__restrict__ rhs, const float *__restrict__ mask,
                                   const float *__restrict__ prev, const float *__restrict__ scal, Dims G, Dims d0,
                                   uint16_t *__restrict__ out) {
@@ -521,7 +522,7 @@ __global__ void head_fwd_kernel(const uint16_t *__restrict__ y, const float *__r
     pred[v] = s;
 }
 
-// out[g] = prev[g] + s h^2 * pred[v(g)] on the pressure grid (prev may be NULL)
+// out[g] = prev[g] + k * pred[v(g)] on the pressure grid (prev may be NULL)
 __global__ void unpack_output_kernel(const float *__restrict__ pred, const float *__restrict__ scal,
                                      const float *__restrict__ prev, Dims G, Dims d0, float *__restrict__ out) {
     const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -532,7 +533,7 @@ __global__ void unpack_output_kernel(const float *__restrict__ pred, const float
     out[g] = prev ? prev[g] + change : change;
 }
 
-// gpred[v] = s h^2 * grad_out[g(v)] (dL/d pred from dL/d out), zero in the padding
+// gpred[v] = k * grad_out[g(v)] (dL/d pred from dL/d out), zero in the padding
 __global__ void pack_grad_kernel(const float *__restrict__ gout, const float *__restrict__ scal, Dims G, Dims d0,
                                  float *__restrict__ gpred) {
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -543,7 +544,7 @@ __global__ void pack_grad_kernel(const float *__restrict__ gout, const float *__
 }
 
 // Mean squared error of the normalised prediction over the interior fluid
-// cells: e = pred - (target - prev) / (s h^2), L = sum e^2 / count,
+// cells: e = pred - (target - prev) / k, L = sum e^2 / count,
 // gpred = 2 e / count / batch (prev may be NULL).
 __global__ __launch_bounds__(256) void mse_kernel(const float *__restrict__ pred, const float *__restrict__ target,
                                                   const float *__restrict__ prev, const float *__restrict__ mask,
@@ -724,7 +725,9 @@ void conv_wgrad(const uint16_t *x, const float *gy, const uint16_t *y, float *pa
 void normalise_and_pack(clq_net *n, const float *rhs, const float *mask, const float *prev, hipStream_t s) {
     hipLaunchKernelGGL(norm_partials_kernel, dim3(kNormBlocks), dim3(256), 0, s, rhs, mask, n->G, n->norm_partials);
     const double h = (double)(float)n->dx;
-    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, s, n->norm_partials, kNormBlocks, h * h, n->scal);
+    const double extent = h * (double)std::max(n->G.X, std::max(n->G.Y, n->G.Z));
+    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, s, n->norm_partials, kNormBlocks,
+                       extent * extent / 1024.0, n->scal);
     hipLaunchKernelGGL(pack_input_kernel, g1(n->d0.V), dim3(256), 0, s, rhs, mask, prev, n->scal, n->G, n->d0,
                        n->in16);
 }

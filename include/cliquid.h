@@ -282,9 +282,10 @@ int clq_kernel_times(clq_ctx *ctx, double *total_ms, int64_t *launches, int32_t 
  * 3x3x3 convolutions, ReLU, BF16 weights and activations with FP32
  * accumulation on the matrix cores) mapping (rhs, fluid mask, previous
  * pressure) on a pressure grid of cell size h to the pressure.  With s the RMS
- * of rhs over the interior cells with mask > 0, its inputs are rhs / s and
- * prev / (s h^2) and its output is prev + s h^2 times the network's head
- * (prev = 0 when absent): the head predicts the change from prev.
+ * of rhs over the interior cells with mask > 0 and k = s (N h)^2 / 1024 (N the
+ * largest grid dimension), its inputs are rhs / s and prev / k and its output
+ * is prev + k times the network's head (prev = 0 when absent): the head
+ * predicts the change from prev.
  * Parameter vector (FP32), in order: for each of the 10 conv layers
  * (16->16, 16->16, 16->32, 32->32, 32->64, 64->64, 96->32, 32->32, 48->16,
  * 16->16) the weights W[tap][c_in][c_out] with tap = 9(dx+1) + 3(dy+1) + (dz+1)
@@ -305,8 +306,8 @@ int clq_net_forward(clq_net *net, const float *rhs, const float *mask, const flo
 /* Parameter gradients of sum(grad_out * out) for the last clq_net_forward. */
 int clq_net_backward(clq_net *net, const float *grad_out, float *grads, int64_t count);
 /* One Adam step (beta 0.9/0.999, eps 1e-8) on the mean over `batch` samples of
- * the mean squared error between (out - prev) / (s h^2) and
- * (target - prev) / (s h^2) over the interior cells with mask > 0 (s as above).  Arrays are [batch][grid];
+ * the mean squared error between (out - prev) / k and (target - prev) / k
+ * over the interior cells with mask > 0 (k as above).  Arrays are [batch][grid];
  * prev may be NULL.  *loss receives the mean loss before the step. */
 int clq_net_train_step(clq_net *net, int32_t batch, const float *rhs, const float *mask, const float *prev,
                        const float *target, double lr, double *loss);

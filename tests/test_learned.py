@@ -242,7 +242,7 @@ def test_training_step_follows_adam_and_reduces_loss(hip_lib):
     inner[1:-1, 1:-1, 1:-1] = True
     sel = inner & (mask > 0)
     out = ref.forward(p0, rhs, mask, prev, DX).detach().numpy()
-    # L = sum_sel ((out - prev) / k - (target - prev) / k)^2 / count with k = s h^2
+    # L = sum_sel ((out - prev) / k - (target - prev) / k)^2 / count
     # -> dL/dout = 2 ((out - prev) / k - (target - prev) / k) / k / count on sel
     prev64, tgt64 = prev.astype(np.float64), target.astype(np.float64)
     g_out = np.where(sel, 2.0 * ((out - prev64) / k - (tgt64 - prev64) * inv_k) / k / count, 0.0)

@@ -4,12 +4,13 @@ Test infrastructure only: the product runs the hand-written HIP kernels of
 continuum_liquid_amd/csrc/unet.hip.  The reference reads the same parameter
 vector (layout in include/cliquid.h, clq_net_*) and computes the same map:
 
-    s     = sqrt(mean over interior cells with mask > 0 of rhs^2), h = dx
-    input = [bf16(rhs / s), mask > 0, bf16(prev / (s h^2)), 0 x 13] on the
-            grid rounded up to multiples of 4 (zero padding)
+    s     = sqrt(mean over interior cells with mask > 0 of rhs^2),
+    k     = s (N h)^2 / 1024 with h = dx and N the largest grid dimension
+    input = [bf16(rhs / s), mask > 0, bf16(prev / k), 0 x 13] on the grid
+            rounded up to multiples of 4 (zero padding)
     c1..c10 = 3x3x3 conv, zero padding, bias, ReLU, then rounded to bf16;
             average pooling 2x2x2 and nearest upsampling + concat as listed
-    out   = prev + s h^2 * (head . a10 + head_bias), cropped to the grid
+    out   = prev + k * (head . a10 + head_bias), cropped to the grid
             (prev = 0 when absent)
 
 With `emulate=True` weights and every stored activation are rounded to bf16
@@ -58,8 +59,8 @@ def padded_shape(shape):
 
 
 def input_scale(rhs: np.ndarray, mask: np.ndarray, dx: float) -> tuple:
-    """(1/s, s h^2, 1/(s h^2), count) as the device computes them (FP64 sums
-    and products, FP32 results; h is dx rounded to FP32)."""
+    """(1/s, k, 1/k, count) as the device computes them (FP64 sums and
+    products, FP32 results; h is dx rounded to FP32)."""
     inner = np.zeros(rhs.shape, bool)
     inner[1:-1, 1:-1, 1:-1] = True
     sel = inner & (mask > 0)
@@ -67,8 +68,9 @@ def input_scale(rhs: np.ndarray, mask: np.ndarray, dx: float) -> tuple:
     rms = np.sqrt(np.sum(rhs[sel].astype(np.float64) ** 2) / n) if n else 0.0
     if rms == 0.0:
         return np.float32(0), np.float32(0), np.float32(0), n
-    h2 = float(np.float32(dx)) ** 2
-    return np.float32(1.0 / rms), np.float32(rms * h2), np.float32(1.0 / (rms * h2)), n
+    extent = float(np.float32(dx)) * max(rhs.shape)
+    len2 = extent * extent / 1024.0
+    return np.float32(1.0 / rms), np.float32(rms * len2), np.float32(1.0 / (rms * len2)), n
 
 
 def forward(params, rhs, mask, prev=None, dx: float = 1.0, *, emulate: bool = True, weights=None):
