@@ -308,23 +308,11 @@ __global__ __launch_bounds__(256) void p2g_staged_kernel(const float *__restrict
 // bit-identical to the per-cell kernel.  Only the component/bin pairs that can
 // carry weight are evaluated (the oz = +1 bin only for w).
 constexpr int kSeg = 62;
-// Wave-private LDS staging of a row segment's particles, structure of arrays,
-// element j stored at j + j/8 so that lanes reading bins ~8 particles apart
-// hit different banks; rows longer than kRowChunk particles go in chunks.
-constexpr int kRowChunk = 256;
-constexpr int kRowSlots = kRowChunk + kRowChunk / 8;
-
-struct RowStage {
-    float c[6][kRowSlots];  // p0, p1, p2 (position / dx), v0, v1, v2
-};
-
-__device__ __forceinline__ int skew(int j) { return j + (j >> 3); }
 
 template <int OX, int OY>
-__device__ __forceinline__ void rowscan_row(RowStage &st, const float *__restrict__ pos, const float *__restrict__ vel,
-                                            int row_lo, int row_hi, int start, int cnt, int lane, int ci, int cj,
-                                            int ck, int nx, int ny, int nz, float inv_dx, float (&vs)[8][3],
-                                            float (&ws)[8][3], int (&occ)[8]) {
+__device__ __forceinline__ void rowscan_row(const float *__restrict__ pos, const float *__restrict__ vel, int start,
+                                            int cnt, int lane, int ci, int cj, int ck, int nx, int ny, int nz,
+                                            float inv_dx, float (&vs)[8][3], float (&ws)[8][3], int (&occ)[8]) {
     constexpr int base = (OX + 1) * 9 + (OY + 1) * 3;
     constexpr int L_UP = (base + 0) & 7;    // this lane's bin is oz = -1 for cell ck + 1
     constexpr int L_OWN = (base + 1) & 7;   // oz = 0
@@ -335,108 +323,89 @@ __device__ __forceinline__ void rowscan_row(RowStage &st, const float *__restric
     } else {
         const float gx = (float)ci, gy = (float)cj;
         const float gz_own = (float)ck, gz_up = (float)(ck + 1), gz_dn = (float)(ck - 1);
-        for (int c0 = row_lo; c0 < row_hi; c0 += kRowChunk) {
-            const int c1 = min(c0 + kRowChunk, row_hi);
-            __builtin_amdgcn_wave_barrier();
-            for (int t = c0 + lane; t < c1; t += 64) {
-                const int j = skew(t - c0);
-                st.c[0][j] = pos[3LL * t] * inv_dx;
-                st.c[1][j] = pos[3LL * t + 1] * inv_dx;
-                st.c[2][j] = pos[3LL * t + 2] * inv_dx;
-                st.c[3][j] = vel[3LL * t];
-                st.c[4][j] = vel[3LL * t + 1];
-                st.c[5][j] = vel[3LL * t + 2];
+        for (int sidx = 0; __any(sidx < cnt); ++sidx) {
+            const bool has = sidx < cnt;
+            float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f, v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
+            if (has) {
+                const int64_t t = (int64_t)start + sidx;
+                p0 = pos[3 * t] * inv_dx;
+                p1 = pos[3 * t + 1] * inv_dx;
+                p2 = pos[3 * t + 2] * inv_dx;
+                v0 = vel[3 * t];
+                v1 = vel[3 * t + 1];
+                v2 = vel[3 * t + 2];
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // this lane's particles inside the chunk, in sorted order
-            const int lo = max(start, c0), hi = min(start + cnt, c1);
-            const int n = hi > lo ? hi - lo : 0;
-            for (int sidx = 0; __any(sidx < n); ++sidx) {
-                const bool has = sidx < n;
-                float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f, v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
-                if (has) {
-                    const int j = skew(lo + sidx - c0);
-                    p0 = st.c[0][j];
-                    p1 = st.c[1][j];
-                    p2 = st.c[2][j];
-                    v0 = st.c[3][j];
-                    v1 = st.c[4][j];
-                    v2 = st.c[5][j];
-                }
-                // x and y factors are shared by the three cells of the column
-                const float ax = fmaxf(0.0f, 1.0f - fabsf(p0 - (gx + 1.0f)));  // u face x
-                const float ay = fmaxf(0.0f, 1.0f - fabsf(p1 - (gy + 0.5f)));
-                const float bx = fmaxf(0.0f, 1.0f - fabsf(p0 - (gx + 0.5f)));  // v, w faces x
-                const float by = fmaxf(0.0f, 1.0f - fabsf(p1 - (gy + 1.0f)));  // v face y
-                const float uxy = ax * ay, vxy = bx * by, wxy = bx * ay;
-                int tocc = 0;
-                if constexpr (OCC) {
-                    const int tx = trunc_clamp(p0, 1, nx - 2);
-                    const int ty = trunc_clamp(p1, 1, ny - 2);
-                    const int tz = trunc_clamp(p2, 1, nz - 2);
-                    tocc = (tx == ci && ty == cj) ? tz : -1000;
-                }
-                // own cell, oz = 0
-                if (has) {
-                    if constexpr (U) {
-                        const float wg = uxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_own + 0.5f)));
-                        vs[L_OWN][0] = fmaf(wg, v0, vs[L_OWN][0]);
-                        ws[L_OWN][0] = ws[L_OWN][0] + wg;
-                    }
-                    if constexpr (V) {
-                        const float wg = vxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_own + 0.5f)));
-                        vs[L_OWN][1] = fmaf(wg, v1, vs[L_OWN][1]);
-                        ws[L_OWN][1] = ws[L_OWN][1] + wg;
-                    }
-                    if constexpr (W) {
-                        const float wg = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_own + 1.0f)));
-                        vs[L_OWN][2] = fmaf(wg, v2, vs[L_OWN][2]);
-                        ws[L_OWN][2] = ws[L_OWN][2] + wg;
-                    }
-                    if constexpr (OCC) occ[L_OWN] += (tocc == ck);
-                }
-                // for the cell above (ck + 1): this bin is its oz = -1 neighbour
-                float su = 0.0f, sv = 0.0f, sw = 0.0f;
-                if constexpr (U) su = uxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_up + 0.5f)));
-                if constexpr (V) sv = vxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_up + 0.5f)));
-                if constexpr (W) sw = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_up + 1.0f)));
-                const int up_flags = has ? (1 | ((OCC && tocc == ck + 1) ? 2 : 0)) : 0;
-                // for the cell below (ck - 1): this bin is its oz = +1 neighbour (w only)
-                float sd = 0.0f;
-                if constexpr (W) sd = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_dn + 1.0f)));
-                const int rf = __shfl_up(up_flags, 1);
-                const int rd = __shfl_down(has ? 1 : 0, 1);
+            // x and y factors are shared by the three cells of the column
+            const float ax = fmaxf(0.0f, 1.0f - fabsf(p0 - (gx + 1.0f)));  // u face x
+            const float ay = fmaxf(0.0f, 1.0f - fabsf(p1 - (gy + 0.5f)));
+            const float bx = fmaxf(0.0f, 1.0f - fabsf(p0 - (gx + 0.5f)));  // v, w faces x
+            const float by = fmaxf(0.0f, 1.0f - fabsf(p1 - (gy + 1.0f)));  // v face y
+            const float uxy = ax * ay, vxy = bx * by, wxy = bx * ay;
+            int tocc = 0;
+            if constexpr (OCC) {
+                const int tx = trunc_clamp(p0, 1, nx - 2);
+                const int ty = trunc_clamp(p1, 1, ny - 2);
+                const int tz = trunc_clamp(p2, 1, nz - 2);
+                tocc = (tx == ci && ty == cj) ? tz : -1000;
+            }
+            // own cell, oz = 0
+            if (has) {
                 if constexpr (U) {
-                    const float wg = __shfl_up(su, 1), vv = __shfl_up(v0, 1);
-                    if (lane > 0 && (rf & 1)) {
-                        vs[L_UP][0] = fmaf(wg, vv, vs[L_UP][0]);
-                        ws[L_UP][0] = ws[L_UP][0] + wg;
-                    }
+                    const float wg = uxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_own + 0.5f)));
+                    vs[L_OWN][0] = fmaf(wg, v0, vs[L_OWN][0]);
+                    ws[L_OWN][0] = ws[L_OWN][0] + wg;
                 }
                 if constexpr (V) {
-                    const float wg = __shfl_up(sv, 1), vv = __shfl_up(v1, 1);
-                    if (lane > 0 && (rf & 1)) {
-                        vs[L_UP][1] = fmaf(wg, vv, vs[L_UP][1]);
-                        ws[L_UP][1] = ws[L_UP][1] + wg;
-                    }
+                    const float wg = vxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_own + 0.5f)));
+                    vs[L_OWN][1] = fmaf(wg, v1, vs[L_OWN][1]);
+                    ws[L_OWN][1] = ws[L_OWN][1] + wg;
                 }
                 if constexpr (W) {
-                    const float wg = __shfl_up(sw, 1), vv = __shfl_up(v2, 1);
-                    if (lane > 0 && (rf & 1)) {
-                        vs[L_UP][2] = fmaf(wg, vv, vs[L_UP][2]);
-                        ws[L_UP][2] = ws[L_UP][2] + wg;
-                    }
-                    const float wd = __shfl_down(sd, 1), vd = __shfl_down(v2, 1);
-                    if (lane < 63 && rd) {
-                        vs[L_DOWN][2] = fmaf(wd, vd, vs[L_DOWN][2]);
-                        ws[L_DOWN][2] = ws[L_DOWN][2] + wd;
-                    }
+                    const float wg = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_own + 1.0f)));
+                    vs[L_OWN][2] = fmaf(wg, v2, vs[L_OWN][2]);
+                    ws[L_OWN][2] = ws[L_OWN][2] + wg;
                 }
-                if constexpr (OCC)
-                    if (lane > 0 && (rf & 2)) occ[L_UP] += 1;
+                if constexpr (OCC) occ[L_OWN] += (tocc == ck);
             }
+            // for the cell above (ck + 1): this bin is its oz = -1 neighbour
+            float su = 0.0f, sv = 0.0f, sw = 0.0f;
+            if constexpr (U) su = uxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_up + 0.5f)));
+            if constexpr (V) sv = vxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_up + 0.5f)));
+            if constexpr (W) sw = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_up + 1.0f)));
+            const int up_flags = has ? (1 | ((OCC && tocc == ck + 1) ? 2 : 0)) : 0;
+            // for the cell below (ck - 1): this bin is its oz = +1 neighbour (w only)
+            float sd = 0.0f;
+            if constexpr (W) sd = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_dn + 1.0f)));
+            const int rf = __shfl_up(up_flags, 1);
+            const int rd = __shfl_down(has ? 1 : 0, 1);
+            if constexpr (U) {
+                const float wg = __shfl_up(su, 1), vv = __shfl_up(v0, 1);
+                if (lane > 0 && (rf & 1)) {
+                    vs[L_UP][0] = fmaf(wg, vv, vs[L_UP][0]);
+                    ws[L_UP][0] = ws[L_UP][0] + wg;
+                }
+            }
+            if constexpr (V) {
+                const float wg = __shfl_up(sv, 1), vv = __shfl_up(v1, 1);
+                if (lane > 0 && (rf & 1)) {
+                    vs[L_UP][1] = fmaf(wg, vv, vs[L_UP][1]);
+                    ws[L_UP][1] = ws[L_UP][1] + wg;
+                }
+            }
+            if constexpr (W) {
+                const float wg = __shfl_up(sw, 1), vv = __shfl_up(v2, 1);
+                if (lane > 0 && (rf & 1)) {
+                    vs[L_UP][2] = fmaf(wg, vv, vs[L_UP][2]);
+                    ws[L_UP][2] = ws[L_UP][2] + wg;
+                }
+                const float wd = __shfl_down(sd, 1), vd = __shfl_down(v2, 1);
+                if (lane < 63 && rd) {
+                    vs[L_DOWN][2] = fmaf(wd, vd, vs[L_DOWN][2]);
+                    ws[L_DOWN][2] = ws[L_DOWN][2] + wd;
+                }
+            }
+            if constexpr (OCC)
+                if (lane > 0 && (rf & 2)) occ[L_UP] += 1;
         }
     }
 }
@@ -447,12 +416,10 @@ __global__ __launch_bounds__(256) void p2g_rowscan_kernel(const float *__restric
                                                           float *__restrict__ v, float *__restrict__ w,
                                                           float *__restrict__ ff, float inv_dx, float inv_ppc, int nx,
                                                           int ny, int nz) {
-    __shared__ RowStage stage[4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ci = blockIdx.z, cj = blockIdx.y * 4 + wave;
-    if (cj >= ny) return;  // whole wave; waves never wait on each other
-    const int k0 = blockIdx.x * kSeg - 1;
-    const int ck = k0 + lane;  // this lane's bin; also its cell
+    if (cj >= ny) return;  // whole wave
+    const int ck = blockIdx.x * kSeg - 1 + lane;  // this lane's bin; also its cell
     float vs[8][3], ws[8][3];
     int occ[8];
 #pragma unroll
@@ -474,14 +441,7 @@ __global__ __launch_bounds__(256) void p2g_rowscan_kernel(const float *__restric
             start = offsets[bin];
             cnt = counts[bin];
         }
-        // the segment's particles: bins max(k0, 0) .. min(k0 + 63, nz - 2) of the row
-        const int zlo = max(k0, 0), zhi = min(k0 + 63, nz - 2);
-        if (zlo > zhi) return;
-        const int row_lo = offsets[gidx(bx, by, zlo, ny, nz)];
-        const int64_t last = gidx(bx, by, zhi, ny, nz);
-        const int row_hi = offsets[last] + counts[last];
-        rowscan_row<OX, OY>(stage[wave], pos, vel, row_lo, row_hi, start, cnt, lane, ci, cj, ck, nx, ny, nz, inv_dx,
-                            vs, ws, occ);
+        rowscan_row<OX, OY>(pos, vel, start, cnt, lane, ci, cj, ck, nx, ny, nz, inv_dx, vs, ws, occ);
     };
     using m1 = std::integral_constant<int, -1>;
     using z0 = std::integral_constant<int, 0>;
