@@ -67,99 +67,6 @@ __device__ __forceinline__ void vox(const Dims &d, int64_t v, int &x, int &y, in
     z = (int)(r - b * (uint32_t)d.Z);
 }
 
-// ---- convolution forward / data gradient (implicit GEMM on MFMA) -----------------------
-// A wave computes kMT tiles of 16 consecutive voxels; a block has 4 waves.
-constexpr int kMT = 4;
-constexpr int kVoxPerBlock = 4 * 16 * kMT;
-
-// MODE 0: forward.  in = X (bf16 [V][CI]), wt = [tap][CO][CI] (bf16),
-//         out = relu(D + bias) as bf16 [V][CO].
-// MODE 1: data gradient.  in = dL/dY (fp32 [V][CI], CI = the layer's C_out),
-//         masked by ymask > 0 (bf16 [V][CI]); wt = [tap][CO][CI] with CO the
-//         layer's C_in (the master layout); taps mirrored; out = fp32 [V][CO].
-template <int CI, int CO, int MODE>
-__global__ __launch_bounds__(256) void conv3_kernel(const void *__restrict__ in, const uint16_t *__restrict__ ymask,
-                                                    const uint16_t *__restrict__ wt, const float *__restrict__ bias,
-                                                    void *__restrict__ out, Dims d) {
-    constexpr int NT = CO / 16;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int m = lane & 15, kg = lane >> 4;
-    const int64_t vbase = ((int64_t)blockIdx.x * 4 + wave) * (16 * kMT);
-    if (vbase >= d.V) return;
-    int xs[kMT], ys[kMT], zs[kMT];
-    bool inr[kMT];
-#pragma unroll
-    for (int mt = 0; mt < kMT; ++mt) {
-        const int64_t v = vbase + mt * 16 + m;
-        inr[mt] = v < d.V;
-        vox(d, inr[mt] ? v : 0, xs[mt], ys[mt], zs[mt]);
-    }
-    float4_t acc[kMT][NT];
-#pragma unroll
-    for (int mt = 0; mt < kMT; ++mt)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[mt][t] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int tap = 0; tap < 27; ++tap) {
-        int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-        if (MODE == 1) {
-            dx = -dx;
-            dy = -dy;
-            dz = -dz;
-        }
-        int64_t nb[kMT];
-        bool ok[kMT];
-#pragma unroll
-        for (int mt = 0; mt < kMT; ++mt) {
-            const int nx = xs[mt] + dx, ny = ys[mt] + dy, nz = zs[mt] + dz;
-            ok[mt] = inr[mt] && nx >= 0 && nx < d.X && ny >= 0 && ny < d.Y && nz >= 0 && nz < d.Z;
-            nb[mt] = ok[mt] ? ((int64_t)nx * d.Y + ny) * d.Z + nz : 0;
-        }
-#pragma unroll
-        for (int c0 = 0; c0 < CI; c0 += 16) {
-            short4_t b[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-                b[t] = *reinterpret_cast<const short4_t *>(wt + ((int64_t)tap * CO + t * 16 + m) * CI + c0 + kg * 4);
-#pragma unroll
-            for (int mt = 0; mt < kMT; ++mt) {
-                short4_t a = short4_t{0, 0, 0, 0};
-                if (ok[mt]) {
-                    if (MODE == 0) {
-                        a = *reinterpret_cast<const short4_t *>(static_cast<const uint16_t *>(in) + nb[mt] * CI + c0 +
-                                                                kg * 4);
-                    } else {
-                        const float4 g = *reinterpret_cast<const float4 *>(static_cast<const float *>(in) +
-                                                                           nb[mt] * CI + c0 + kg * 4);
-                        const short4_t y = *reinterpret_cast<const short4_t *>(ymask + nb[mt] * CI + c0 + kg * 4);
-                        a[0] = (short)(bf2f((uint16_t)y[0]) > 0.0f ? f2bf(g.x) : (uint16_t)0);
-                        a[1] = (short)(bf2f((uint16_t)y[1]) > 0.0f ? f2bf(g.y) : (uint16_t)0);
-                        a[2] = (short)(bf2f((uint16_t)y[2]) > 0.0f ? f2bf(g.z) : (uint16_t)0);
-                        a[3] = (short)(bf2f((uint16_t)y[3]) > 0.0f ? f2bf(g.w) : (uint16_t)0);
-                    }
-                }
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[mt][t], 0, 0, 0);
-            }
-        }
-    }
-    // D[row = voxel][col = channel]: the lane holds column m, rows kg*4 + j
-#pragma unroll
-    for (int mt = 0; mt < kMT; ++mt)
-#pragma unroll
-        for (int t = 0; t < NT; ++t)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int64_t v = vbase + mt * 16 + kg * 4 + j;
-                if (v >= d.V) continue;
-                const int co = t * 16 + m;
-                if (MODE == 0)
-                    static_cast<uint16_t *>(out)[v * CO + co] = f2bf(fmaxf(acc[mt][t][j] + bias[co], 0.0f));
-                else
-                    static_cast<float *>(out)[v * CO + co] = acc[mt][t][j];
-            }
-}
-
 // ---- LDS-tiled convolution ------------------------------------------------------------------
 // A block owns a 4 x 4 x 16 brick of output voxels: wave w the x-slice x0 + w,
 // its M-tile mt the row y0 + mt, lane m of a tile the voxel z0 + m.  For each
