@@ -22,6 +22,15 @@
 
 namespace clq {
 
+// Work-groups are dealt round-robin to the 8 XCDs, each with its own L2.
+// Block b of nb takes item xcd_item(b, nb), a bijection of [0, nb) that gives
+// every XCD one contiguous range of items, so a stencil finds the neighbouring
+// rows and planes it reads in the L2 of its own XCD.
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb >> 3, rem = nb & 7, x = b & 7, y = b >> 3;
+    return x * q + (x < rem ? x : rem) + y;
+}
+
 constexpr int kMaxLevels = 16;
 
 struct Level {

@@ -100,7 +100,10 @@ __global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const float 
                                                           int64_t work, double *__restrict__ partials) {
     const double beta = first ? 0.0 : s[5];
     double acc[1] = {0.0};
-    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
+    // one contiguous chunk of work items per block, chunks laid out per XCD
+    const int64_t chunk = (work + (int64_t)gridDim.x * kT - 1) / ((int64_t)gridDim.x * kT) * kT;
+    const int64_t t0 = (int64_t)xcd_item(blockIdx.x, gridDim.x) * chunk, t1 = min(t0 + chunk, work);
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += kT) {
         const int64_t q = cell_of(rows, L.nz, t);
         double dv = 0.0, av = 0.0;
         int i, j, k;

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void zero_inactive_rows_kernel(Level L, const 
 // One block per active row and colour half: lane t takes the row's same-colour
 // cells t and t + 256 (rows of up to 1024 cells), all loads issued first.
 __global__ __launch_bounds__(256) void rbgs_rows_kernel(Level L, float dd, int parity, const int32_t *__restrict__ rows) {
-    const uint32_t row = (uint32_t)rows[blockIdx.x];
+    const uint32_t row = (uint32_t)rows[xcd_item(blockIdx.x, gridDim.x)];
     const int half = (L.nz + 1) >> 1;
     for (int kk = threadIdx.x; kk < half; kk += 256) rbgs_cell_eager(L, dd, parity, row, kk);
 }
@@ -393,7 +393,7 @@ void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, 
 // restriction onto the active coarse rows only
 __global__ __launch_bounds__(256) void restrict_rows_kernel(Level F, Level K, float inv_delta2,
                                                             const int32_t *__restrict__ krows, int64_t work) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t t = (int64_t)xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= work) return;
     const int64_t ar = t / K.nz;
     restrict_item_eager(F, K, inv_delta2, (int64_t)krows[ar] * K.nz + (t - ar * K.nz));
@@ -438,7 +438,7 @@ void launch_prolongate(const Level &F, const Level &K, hipStream_t s) {
 
 __global__ __launch_bounds__(256) void prolongate_rows_kernel(Level F, Level K, const int32_t *__restrict__ frows,
                                                               int64_t work) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t t = (int64_t)xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= work) return;
     const int64_t ar = t / F.nz;
     prolong_item_eager(F, K, (int64_t)frows[ar] * F.nz + (t - ar * F.nz));

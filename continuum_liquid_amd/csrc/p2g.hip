@@ -150,9 +150,13 @@ __global__ __launch_bounds__(256) void p2g_rowscan_kernel(const float *__restric
                                                           float *__restrict__ ff, float inv_dx, float inv_ppc, int nx,
                                                           int ny, int nz) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ci = blockIdx.z, cj = blockIdx.y * 4 + wave;
+    // blocks in (segment, row group, plane) order, each XCD one contiguous range
+    const uint32_t nb = gridDim.x * gridDim.y * gridDim.z;
+    const uint32_t b = xcd_item((blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, nb);
+    const int bxs = (int)(b % gridDim.x), bys = (int)((b / gridDim.x) % gridDim.y), bzs = (int)(b / (gridDim.x * gridDim.y));
+    const int ci = bzs, cj = bys * 4 + wave;
     if (cj >= ny) return;  // whole wave
-    const int ck = blockIdx.x * kSeg - 1 + lane;  // this lane's bin; also its cell
+    const int ck = bxs * kSeg - 1 + lane;  // this lane's bin; also its cell
     float vs[8][3], ws[8][3];
     int occ[8];
 #pragma unroll
