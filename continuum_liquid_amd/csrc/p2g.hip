@@ -150,22 +150,12 @@ __global__ __launch_bounds__(256) void p2g_rowscan_kernel(const float *__restric
                                                           float *__restrict__ ff, float inv_dx, float inv_ppc, int nx,
                                                           int ny, int nz) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // Block L runs on XCD L % 8.  A column is a (row group, segment) pair;
-    // XCD x takes every eighth column (the work stays spread over all XCDs) and,
-    // block after block, the planes of one column in order, so a block finds two
-    // of the three bin planes it reads in the L2 its predecessor filled.  When
-    // the column count is not a multiple of 8, blocks keep the launch order.
-    const uint32_t nseg = gridDim.x, ngrp = gridDim.y, nplane = gridDim.z, ncol = nseg * ngrp;
-    const uint32_t L = (blockIdx.z * ngrp + blockIdx.y) * nseg + blockIdx.x;
-    uint32_t plane = blockIdx.z, col = blockIdx.y * nseg + blockIdx.x;
-    if ((ncol & 7) == 0) {
-        const uint32_t m = L >> 3;
-        plane = m % nplane;
-        col = (m / nplane) * 8 + (L & 7);
-    }
-    const int ci = (int)plane, cj = (int)(col / nseg) * 4 + wave;
+    // Blocks keep the launch order (segment, row group, plane): the blocks
+    // resident together on an XCD then cover a compact region, which measured
+    // fewer fetched bytes than per-XCD plane ranges or plane-ordered columns.
+    const int ci = blockIdx.z, cj = blockIdx.y * 4 + wave;
     if (cj >= ny) return;  // whole wave
-    const int ck = (int)(col % nseg) * kSeg - 1 + lane;  // this lane's bin; also its cell
+    const int ck = blockIdx.x * kSeg - 1 + lane;  // this lane's bin; also its cell
     float vs[8][3], ws[8][3];
     int occ[8];
 #pragma unroll
