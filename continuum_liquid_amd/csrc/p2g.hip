@@ -56,17 +56,29 @@ __device__ __forceinline__ void rowscan_row(const float *__restrict__ pos, const
     } else {
         const float gx = (float)ci, gy = (float)cj;
         const float gz_own = (float)ck, gz_up = (float)(ck + 1), gz_dn = (float)(ck - 1);
+        // particle sidx + 1 is loaded while particle sidx is processed
+        float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
+        if (cnt > 0) {
+            const int64_t t = start;
+            n0 = pos[3 * t];
+            n1 = pos[3 * t + 1];
+            n2 = pos[3 * t + 2];
+            m0 = vel[3 * t];
+            m1 = vel[3 * t + 1];
+            m2 = vel[3 * t + 2];
+        }
         for (int sidx = 0; __any(sidx < cnt); ++sidx) {
             const bool has = sidx < cnt;
-            float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f, v0 = 0.0f, v1 = 0.0f, v2 = 0.0f;
-            if (has) {
-                const int64_t t = (int64_t)start + sidx;
-                p0 = pos[3 * t] * inv_dx;
-                p1 = pos[3 * t + 1] * inv_dx;
-                p2 = pos[3 * t + 2] * inv_dx;
-                v0 = vel[3 * t];
-                v1 = vel[3 * t + 1];
-                v2 = vel[3 * t + 2];
+            const float p0 = n0 * inv_dx, p1 = n1 * inv_dx, p2 = n2 * inv_dx;
+            const float v0 = m0, v1 = m1, v2 = m2;
+            if (sidx + 1 < cnt) {
+                const int64_t t = (int64_t)start + sidx + 1;
+                n0 = pos[3 * t];
+                n1 = pos[3 * t + 1];
+                n2 = pos[3 * t + 2];
+                m0 = vel[3 * t];
+                m1 = vel[3 * t + 1];
+                m2 = vel[3 * t + 2];
             }
             // x and y factors are shared by the three cells of the column
             const float ax = fmaxf(0.0f, 1.0f - fabsf(p0 - (gx + 1.0f)));  // u face x
