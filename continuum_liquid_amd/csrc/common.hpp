@@ -167,6 +167,9 @@ void launch_compact_rows(const int32_t *flags, const int32_t *pos, int64_t rows,
 void launch_zero_inactive_rows(const Level &L, const int32_t *flags, int coarse, hipStream_t s);
 void launch_rbgs_rows(const Level &L, float delta, int parity, const int32_t *rows, int64_t nrows, hipStream_t s);
 void launch_zero_rows(float *p, const int32_t *rows, int64_t nrows, int nz, hipStream_t s);
+void launch_pack_operator(const Level &L, uint8_t *op, int *bad, hipStream_t s);
+void launch_rbgs_rows_packed(const Level &L, const uint8_t *op, float delta, int parity, const int32_t *rows,
+                             int64_t nrows, hipStream_t s);
 void launch_restrict_rows(const Level &F, const Level &K, float inv_delta2, const int32_t *krows, int64_t nkrows,
                           hipStream_t s);
 void launch_prolongate_rows(const Level &F, const Level &K, const int32_t *frows, int64_t nfrows, hipStream_t s);

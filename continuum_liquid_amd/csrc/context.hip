@@ -88,6 +88,8 @@ struct clq_ctx {
     int32_t *row_count = nullptr, *row_count_host = nullptr;
     int64_t nrows[2] = {0, 0};
     bool rows_valid = false, use_rows = false;
+    uint8_t *op0 = nullptr;  // packed level-0 operator (valid when op0_ok)
+    bool op0_ok = false;
     // learned preconditioner (not owned) and its FP32 initial-guess buffer
     clq_net *net = nullptr;
     double stall_ratio = 0.5;
@@ -540,7 +542,11 @@ static int ensure_active_rows(clq_ctx *c) {
         if (int rc = c->alloc(&q, 16)) return rc;
         c->row_count = (int32_t *)q;
         HIP_TRY(hipHostMalloc((void **)&c->row_count_host, 16, hipHostMallocDefault));
+        if (int rc = c->alloc(&q, (size_t)c->C)) return rc;
+        c->op0 = (uint8_t *)q;
     }
+    HIP_TRY(hipMemsetAsync(c->row_count + 2, 0, 4, c->stream));
+    launch_pack_operator(c->lv[0], c->op0, c->row_count + 2, c->stream);
     launch_row_flags(c->lv[0], c->row_flags[0], c->stream);
     launch_coarse_row_flags(c->row_flags[0], c->lv[0], c->lv[1], c->row_flags[1], c->stream);
     for (int l = 0; l < 2; ++l) {
@@ -548,10 +554,11 @@ static int ensure_active_rows(clq_ctx *c) {
         launch_exclusive_scan(c->row_flags[l], c->row_pos, r, c->scan_scratch, c->stream);
         launch_compact_rows(c->row_flags[l], c->row_pos, r, c->rows[l], c->row_count + l, c->stream);
     }
-    HIP_TRY(hipMemcpyAsync(c->row_count_host, c->row_count, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->row_count_host, c->row_count, 12, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->nrows[0] = c->row_count_host[0];
     c->nrows[1] = c->row_count_host[1];
+    c->op0_ok = c->row_count_host[2] == 0;
     c->rows_valid = true;
     return check_launch("active rows");
 }
@@ -573,8 +580,12 @@ static void smooth_level(clq_ctx *c, int level, int iterations) {
     for (int it = 0; it < iterations; ++it) {
         if (level == 0) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
         for (int parity = 0; parity < 2; ++parity) {
-            if (sparse) launch_rbgs_rows(c->lv[level], delta, parity, c->rows[level], c->nrows[level], c->stream);
-            else launch_rbgs(c->lv[level], delta, parity, c->stream);
+            if (sparse && level == 0 && c->op0_ok)
+                launch_rbgs_rows_packed(c->lv[0], c->op0, delta, parity, c->rows[0], c->nrows[0], c->stream);
+            else if (sparse)
+                launch_rbgs_rows(c->lv[level], delta, parity, c->rows[level], c->nrows[level], c->stream);
+            else
+                launch_rbgs(c->lv[level], delta, parity, c->stream);
         }
         if (level == 0) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
     }

@@ -263,6 +263,72 @@ void launch_zero_inactive_rows(const Level &L, const int32_t *flags, int coarse,
                        coarse);
 }
 
+// ---- packed finest-level operator -----------------------------------------------------
+// The finest operator (build_finest_operator) has weights 0 or 1 and an
+// integer diagonal 0..6, and the sweep only tests mask > 0, so one byte per
+// cell holds it exactly: bit 0-2 wx, wy, wz != 0, bit 3 mask > 0, bits 4-7 the
+// diagonal.  Decoding gives the same floats, so the packed sweep is
+// bit-identical; pack_operator_kernel raises *bad for any cell whose values
+// are not of that form (the caller then keeps the float arrays).
+__global__ __launch_bounds__(256) void pack_operator_kernel(Level L, uint8_t *__restrict__ op, int *__restrict__ bad) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= L.C) return;
+    const float x = L.wx[q], y = L.wy[q], z = L.wz[q], d = L.diag[q];
+    const bool ok = (x == 0.0f || x == 1.0f) && (y == 0.0f || y == 1.0f) && (z == 0.0f || z == 1.0f) && d >= 0.0f &&
+                    d <= 15.0f && d == (float)(int)d;
+    if (!ok) atomicOr(bad, 1);
+    op[q] = (uint8_t)((x != 0.0f ? 1 : 0) | (y != 0.0f ? 2 : 0) | (z != 0.0f ? 4 : 0) | (L.mask[q] > 0.0f ? 8 : 0) |
+                      ((ok ? (int)d : 0) << 4));
+}
+
+void launch_pack_operator(const Level &L, uint8_t *op, int *bad, hipStream_t s) {
+    hipLaunchKernelGGL(pack_operator_kernel, dim3((unsigned)((L.C + 255) / 256)), dim3(256), 0, s, L, op, bad);
+}
+
+__device__ __forceinline__ float bitw(uint32_t o, uint32_t bit) { return (o & bit) ? 1.0f : 0.0f; }
+
+// rbgs_cell on the packed operator: same loads of p and b, the operator from
+// op (own byte and the -x, -y, -z neighbours' bytes), same arithmetic.
+__global__ __launch_bounds__(256) void rbgs_rows_packed_kernel(Level L, const uint8_t *__restrict__ op, float dd,
+                                                               int parity, const int32_t *__restrict__ rows) {
+    const uint32_t row = (uint32_t)rows[xcd_item(blockIdx.x, gridDim.x)];
+    const int nx = L.nx, ny = L.ny, nz = L.nz, half = (nz + 1) >> 1;
+    const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
+    const int64_t sx = (int64_t)ny * nz, sy = nz;
+    float *__restrict__ p = L.p;
+    for (int kk = threadIdx.x; kk < half; kk += 256) {
+        const int k = 2 * kk + ((parity + i + j) & 1);
+        if (k >= nz) continue;
+        const int64_t idx = (int64_t)row * nz + k;
+        const bool inner = interior(i, j, k, nx, ny, nz);
+        const uint32_t o = op[idx];
+        float num = 0.0f;
+        if (inner) {
+            const uint32_t ox = op[idx - sx], oy = op[idx - sy], oz = op[idx - 1];
+            float t = bitw(ox, 1) * p[idx - sx];
+            t = fmaf(bitw(o, 1), p[idx + sx], t);
+            t = fmaf(bitw(o, 2), p[idx + sy], t);
+            t = fmaf(bitw(oy, 2), p[idx - sy], t);
+            t = fmaf(bitw(o, 4), p[idx + 1], t);
+            t = fmaf(bitw(oz, 4), p[idx - 1], t);
+            num = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), t);
+        }
+        if (!(o & 8)) {
+            p[idx] = 0.0f;
+            continue;
+        }
+        const float d = (float)(o >> 4);
+        if (inner && d > 0.0f) p[idx] = num / d;  // multigrid.cl:227
+    }
+}
+
+void launch_rbgs_rows_packed(const Level &L, const uint8_t *op, float delta, int parity, const int32_t *rows,
+                             int64_t nrows, hipStream_t s) {
+    if (nrows == 0) return;
+    hipLaunchKernelGGL(rbgs_rows_packed_kernel, dim3((unsigned)nrows), dim3(256), 0, s, L, op, delta * delta, parity,
+                       rows);
+}
+
 void launch_rbgs_rows(const Level &L, float delta, int parity, const int32_t *rows, int64_t nrows, hipStream_t s) {
     if (nrows == 0) return;
     hipLaunchKernelGGL(rbgs_rows_kernel, dim3((unsigned)nrows), dim3(256), 0, s, L, delta * delta, parity, rows);
