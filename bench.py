@@ -55,6 +55,15 @@ def algorithmic_bytes(n_particles: int, cells: int, swept_cells: float) -> dict:
     }
 
 
+# HBM bytes per swept cell of one finest-level red-black sweep (both colour
+# launches), from rocprofv3 FETCH_SIZE/WRITE_SIZE on the 256^3 dam break +
+# sphere with the FCG solve (profiles/step_256cubed_fcg_pmc_traffic.md:
+# (20213.8 KiB x 2.0 + 16583.0 KiB) x 1024 per launch over 4,245,248 swept
+# cells, two launches per sweep).  Reported as `roofline.traffic` for that
+# workload only; the counters cannot be read from inside this process.
+MEASURED_SWEEP_BYTES_PER_CELL = {(3, "fcg"): 27.50}
+
+
 def solver_settings(name: str, tolerance: float) -> dict:
     """W(2, 2) cycles.  The plain FP32 baseline keeps the reference's 64 extra
     coarsest sweeps and the W-cycle at every level; the refined solves use 16
@@ -210,6 +219,10 @@ def main() -> int:
         }
     dominant = max(kernels, key=lambda k: ktimes[k][0])
     d = kernels[dominant]
+    traffic = None
+    per_cell = MEASURED_SWEEP_BYTES_PER_CELL.get((args.config, args.solver))
+    if dominant == "rbgs_fine_sweep" and per_cell is not None:
+        traffic = int(per_cell * float(np.mean(swept)))
 
     result = {
         "metric": METRIC,
@@ -248,7 +261,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": d["frac_hbm_peak"],
-            "traffic": None,
+            "traffic": traffic,
         },
         "kernels": kernels,
     }

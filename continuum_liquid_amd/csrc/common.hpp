@@ -26,6 +26,9 @@ namespace clq {
 // Block b of nb takes item xcd_item(b, nb), a bijection of [0, nb) that gives
 // every XCD one contiguous range of items, so a stencil finds the neighbouring
 // rows and planes it reads in the L2 of its own XCD.
+// decoded weight bit of a packed level-0 operator byte
+__device__ __forceinline__ float bitw(uint32_t o, uint32_t bit) { return (o & bit) ? 1.0f : 0.0f; }
+
 __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t nb) {
     const uint32_t q = nb >> 3, rem = nb & 7, x = b & 7, y = b >> 3;
     return x * q + (x < rem ? x : rem) + y;
@@ -170,8 +173,8 @@ void launch_zero_rows(float *p, const int32_t *rows, int64_t nrows, int nz, hipS
 void launch_pack_operator(const Level &L, uint8_t *op, int *bad, hipStream_t s);
 void launch_rbgs_rows_packed(const Level &L, const uint8_t *op, float delta, int parity, const int32_t *rows,
                              int64_t nrows, hipStream_t s);
-void launch_restrict_rows(const Level &F, const Level &K, float inv_delta2, const int32_t *krows, int64_t nkrows,
-                          hipStream_t s);
+void launch_restrict_rows(const Level &F, const uint8_t *op, const Level &K, float inv_delta2, const int32_t *krows,
+                          int64_t nkrows, hipStream_t s);
 void launch_prolongate_rows(const Level &F, const Level &K, const int32_t *frows, int64_t nfrows, hipStream_t s);
 void launch_ir_update_residual(const Level &L, int from32, const float *p32, const double *pin, double *pout,
                                float *r32, const float *b, double inv_h2, double *partials, double *out2,
@@ -186,9 +189,9 @@ void launch_fcg_init(const Level &L, const float *p32, const float *b, double in
                      double *partials, double *s, hipStream_t st);
 void launch_fcg_dots(const Level &L, const float *z, const double *qv, const double *r, int first,
                      const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st);
-void launch_fcg_direction(const Level &L, const float *z, const double *d_old, double *d_new, double *qv, int first,
-                          double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s,
-                          hipStream_t st);
+void launch_fcg_direction(const Level &L, const uint8_t *op, const float *z, const double *d_old, double *d_new,
+                          double *qv, int first, double inv_h2, const int32_t *rows, int64_t nrows, double *partials,
+                          double *s, hipStream_t st);
 void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const double *d, const double *qv,
                        const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st);
 

@@ -594,7 +594,8 @@ static void smooth_level(clq_ctx *c, int level, int iterations) {
 static void restrict_level(clq_ctx *c, int level) {
     const float delta = (float)c->dx;
     if (c->use_rows && level == 0)
-        launch_restrict_rows(c->lv[0], c->lv[1], 1.0f / (delta * delta), c->rows[1], c->nrows[1], c->stream);
+        launch_restrict_rows(c->lv[0], c->op0_ok ? c->op0 : nullptr, c->lv[1], 1.0f / (delta * delta), c->rows[1],
+                             c->nrows[1], c->stream);
     else
         launch_restrict_residual(c->lv[level], c->lv[level + 1], 1.0f / (delta * delta), c->stream);
 }
@@ -873,6 +874,8 @@ static int ensure_fcg_buffers(clq_ctx *c) {
 
 // level-0 active-row list of the FCG vector kernels (all cells without one)
 #define ROWS0 (c->use_rows ? c->rows[0] : nullptr), c->nrows[0]
+// packed level-0 operator when the row lists are in use and it is exact
+#define OP0 (c->use_rows && c->op0_ok ? c->op0 : nullptr)
 
 static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
     if (int rc = ensure_fcg_buffers(c)) return rc;
@@ -891,7 +894,7 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
         s->cycles++;
         launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         for (int it = 0;; ++it) {
-            launch_fcg_direction(L0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, ROWS0,
+            launch_fcg_direction(L0, OP0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, ROWS0,
                                  c->fcg_partials, c->fcg_s, c->stream);
             cur ^= 1;
             launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, ROWS0, c->fcg_partials, c->fcg_s,
@@ -956,7 +959,7 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
         if (int rc = precondition(r)) return rc;
         launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         for (int it = 0;; ++it) {
-            launch_fcg_direction(L0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, ROWS0,
+            launch_fcg_direction(L0, OP0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, ROWS0,
                                  c->fcg_partials, c->fcg_s, c->stream);
             cur ^= 1;
             launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, ROWS0, c->fcg_partials, c->fcg_s,

@@ -54,6 +54,24 @@ __device__ __forceinline__ int64_t cell_of(const int32_t *__restrict__ rows, int
     return (int64_t)rows[ar] * nz + (t - ar * nz);
 }
 
+// apply_a with the operator decoded from the packed level-0 bytes
+template <typename F>
+__device__ __forceinline__ double apply_a_packed(const Level &L, const uint8_t *__restrict__ op, int64_t q, F v,
+                                                 double inv_h2) {
+    const int64_t sx = (int64_t)L.ny * L.nz, sy = L.nz;
+    const uint32_t o = op[q], ox = op[q - sx], oy = op[q - sy], oz = op[q - 1];
+    const double v0 = v(q + sx), v1 = v(q - sx), v2 = v(q + sy), v3 = v(q - sy), v4 = v(q + 1), v5 = v(q - 1);
+    const double vc = v(q), d = (double)(o >> 4);
+    double s = 0.0;
+    if (o & 1) s = s + v0;
+    if (ox & 1) s = s + v1;
+    if (o & 2) s = s + v2;
+    if (oy & 2) s = s + v3;
+    if (o & 4) s = s + v4;
+    if (oz & 4) s = s + v5;
+    return (s - d * vc) * inv_h2;
+}
+
 template <int N>
 __device__ __forceinline__ void block_sums(double (&v)[N], double *__restrict__ partials) {
     __shared__ double sh[N][kT];
@@ -93,7 +111,8 @@ __global__ __launch_bounds__(kT) void fcg_init_kernel(Level L, const float *__re
 }
 
 // d_new = z + beta d_old, q = A d_new; partial d.q.  first: beta = 0.
-__global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const float *__restrict__ z,
+__global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const uint8_t *__restrict__ op,
+                                                          const float *__restrict__ z,
                                                           const double *__restrict__ d_old, double *__restrict__ d_new,
                                                           double *__restrict__ qv, const double *__restrict__ s,
                                                           int first, double inv_h2, const int32_t *__restrict__ rows,
@@ -112,9 +131,9 @@ __global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const float 
             // neighbours outside the fluid carry zero weight: their values are
             // loaded with the rest but never enter the sum
             auto dn = [&](int64_t u) { return first ? (double)z[u] : (double)z[u] + beta * d_old[u]; };
-            const float m = L.mask[q];
-            const double dq = dn(q), aq = apply_a(L, q, dn, inv_h2);
-            if (m > 0.0f) {
+            const bool fluid = op ? (op[q] & 8) != 0 : L.mask[q] > 0.0f;
+            const double dq = dn(q), aq = op ? apply_a_packed(L, op, q, dn, inv_h2) : apply_a(L, q, dn, inv_h2);
+            if (fluid) {
                 dv = dq;
                 av = aq;
             }
@@ -219,12 +238,12 @@ void launch_fcg_dots(const Level &L, const float *z, const double *qv, const dou
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 3, first, s);
 }
 
-void launch_fcg_direction(const Level &L, const float *z, const double *d_old, double *d_new, double *qv, int first,
-                          double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s,
-                          hipStream_t st) {
+void launch_fcg_direction(const Level &L, const uint8_t *op, const float *z, const double *d_old, double *d_new,
+                          double *qv, int first, double inv_h2, const int32_t *rows, int64_t nrows, double *partials,
+                          double *s, hipStream_t st) {
     const int64_t work = work_of(L, rows, nrows);
     const int nb = blocks_for(work);
-    hipLaunchKernelGGL(fcg_direction_kernel, dim3(nb), dim3(kT), 0, st, L, z, d_old, d_new, qv, s, first, inv_h2,
+    hipLaunchKernelGGL(fcg_direction_kernel, dim3(nb), dim3(kT), 0, st, L, op, z, d_old, d_new, qv, s, first, inv_h2,
                        rows, work, partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 1, 0, s);
 }

@@ -285,8 +285,6 @@ void launch_pack_operator(const Level &L, uint8_t *op, int *bad, hipStream_t s) 
     hipLaunchKernelGGL(pack_operator_kernel, dim3((unsigned)((L.C + 255) / 256)), dim3(256), 0, s, L, op, bad);
 }
 
-__device__ __forceinline__ float bitw(uint32_t o, uint32_t bit) { return (o & bit) ? 1.0f : 0.0f; }
-
 // rbgs_cell on the packed operator: same loads of p and b, the operator from
 // op (own byte and the -x, -y, -z neighbours' bytes), same arithmetic.
 __global__ __launch_bounds__(256) void rbgs_rows_packed_kernel(Level L, const uint8_t *__restrict__ op, float dd,
@@ -456,21 +454,64 @@ void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, 
                            inv_delta2);
 }
 
-// restriction onto the active coarse rows only
-__global__ __launch_bounds__(256) void restrict_rows_kernel(Level F, Level K, float inv_delta2,
-                                                            const int32_t *__restrict__ krows, int64_t work) {
+// restrict_item_eager with the fine operator read from the packed bytes
+// (same decoded values, same operations).
+__device__ __forceinline__ void restrict_item_packed(const Level &F, const uint8_t *__restrict__ op, const Level &K,
+                                                     float inv_delta2, int64_t ci) {
+    int a, b, c;
+    coords(ci, K.ny, K.nz, a, b, c);
+    float rhs = 0.0f;
+    if (interior(a, b, c, K.nx, K.ny, K.nz)) {
+        const int64_t sx = (int64_t)F.ny * F.nz, sy = F.nz;
+        const int f0 = 1 + 2 * (a - 1), f1 = 1 + 2 * (b - 1), f2 = 1 + 2 * (c - 1);
+        float res[8];
+        bool use[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int i = f0 + (q >> 2), j = f1 + ((q >> 1) & 1), k = f2 + (q & 1);
+            use[q] = false;
+            res[q] = 0.0f;
+            if (interior(i, j, k, F.nx, F.ny, F.nz)) {
+                const int64_t idx = gidx(i, j, k, F.ny, F.nz);
+                const uint32_t o = op[idx], ox = op[idx - sx], oy = op[idx - sy], oz = op[idx - 1];
+                const float *__restrict__ p = F.p;
+                float t = bitw(ox, 1) * p[idx - sx];
+                t = fmaf(bitw(o, 1), p[idx + sx], t);
+                t = fmaf(bitw(o, 2), p[idx + sy], t);
+                t = fmaf(bitw(oy, 2), p[idx - sy], t);
+                t = fmaf(bitw(o, 4), p[idx + 1], t);
+                t = fmaf(bitw(oz, 4), p[idx - 1], t);
+                const float ap = fmaf(-(float)(o >> 4), p[idx], t) * inv_delta2;  // residual_at
+                res[q] = load_b(F.b, F.b16, F.bkind, F.bscale, idx) - ap;
+                use[q] = (o & 8) != 0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (use[q]) rhs = rhs + res[q];
+    }
+    K.p[ci] = 0.0f;
+    store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
+}
+
+// restriction onto the active coarse rows only (packed fine operator when op != NULL)
+__global__ __launch_bounds__(256) void restrict_rows_kernel(Level F, const uint8_t *__restrict__ op, Level K,
+                                                            float inv_delta2, const int32_t *__restrict__ krows,
+                                                            int64_t work) {
     const int64_t t = (int64_t)xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= work) return;
     const int64_t ar = t / K.nz;
-    restrict_item_eager(F, K, inv_delta2, (int64_t)krows[ar] * K.nz + (t - ar * K.nz));
+    const int64_t ci = (int64_t)krows[ar] * K.nz + (t - ar * K.nz);
+    if (op) restrict_item_packed(F, op, K, inv_delta2, ci);
+    else restrict_item_eager(F, K, inv_delta2, ci);
 }
 
-void launch_restrict_rows(const Level &F, const Level &K, float inv_delta2, const int32_t *krows, int64_t nkrows,
-                          hipStream_t s) {
+void launch_restrict_rows(const Level &F, const uint8_t *op, const Level &K, float inv_delta2, const int32_t *krows,
+                          int64_t nkrows, hipStream_t s) {
     const int64_t work = nkrows * K.nz;
     if (work == 0) return;
-    hipLaunchKernelGGL(restrict_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2,
-                       krows, work);
+    hipLaunchKernelGGL(restrict_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, op, K,
+                       inv_delta2, krows, work);
 }
 
 __device__ __forceinline__ void prolong_item(const Level &F, const Level &K, int64_t fi) {
