@@ -7,7 +7,8 @@ binning, P2G + occupancy, grid glue, the pressure solve to the relative
 residual target (FP64-checked), pressure correction, and the fused G2P/FLIP
 blend/advection/particle boundary pass.  The default workload is the 256^3
 dam break + falling sphere (dx = 1/256, 8 stratified-jitter FP32 particles
-per fluid cell, ~28.6M particles, dt = 1/240, FLIP 0.95, W(2, 2) cycles).
+per fluid cell, ~28.6M particles, dt = 1/240, FLIP 0.95, pressure solve by
+FP64 flexible CG preconditioned with one mixed-precision W(1, 1) cycle).
 
 Prints one JSON line: ms/step (`value`), particle-steps/s, pressure-solve ms,
 a `roofline` object for the dominant kernel (algorithmic bytes per launch
@@ -65,21 +66,26 @@ MEASURED_SWEEP_BYTES_PER_CELL = {(3, "fcg"): 27.50}
 
 
 def solver_settings(name: str, tolerance: float) -> dict:
-    """W(2, 2) cycles.  The plain FP32 baseline keeps the reference's 64 extra
-    coarsest sweeps and the W-cycle at every level; the refined solves use 16
-    extra sweeps, which converge in the same number of cycles (measured on the
-    64^3 dam break: 38 cycles to 1e-8 with 64 or 16), and the W-cycle on the
-    two finest levels with the V-cycle below (14.2 FCG iterations per step
-    against 11.3 with the full W-cycle at 256^3, 21.8 against 24.9 ms/step)."""
+    """W(2, 2) cycles for the plain FP32 and mixed-refinement solves.  The
+    plain FP32 baseline keeps the reference's 64 extra coarsest sweeps and the
+    W-cycle at every level; the refined solves use 16 extra sweeps, which
+    converge in the same number of cycles (measured on the 64^3 dam break: 38
+    cycles to 1e-8 with 64 or 16), and the W-cycle on the two finest levels
+    with the V-cycle below.  The FCG solves precondition with W(1, 1) cycles:
+    at 256^3 they take 16.8 iterations per step against 14.2 with W(2, 2), and
+    15.5 against 17.2 ms/step (sweep over smoothing 0-3, V/W level and
+    coarsest sweeps on MI355X)."""
     base = {"pre_smooth": 2, "post_smooth": 2, "coarse_extra": 64, "tolerance": tolerance}
     if name == "fp32":
         return {**base, "mode": "fp32", "max_cycles": 60}
     if name == "mixed":
         return {**base, "mode": "mixed", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16, "vcycle_from": 2}
     if name == "fcg":
-        return {**base, "mode": "fcg", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16, "vcycle_from": 2}
+        return {**base, "mode": "fcg", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16, "vcycle_from": 2,
+                "pre_smooth": 1, "post_smooth": 1}
     if name == "learned":
-        return {**base, "mode": "learned", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16, "vcycle_from": 2}
+        return {**base, "mode": "learned", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16, "vcycle_from": 2,
+                "pre_smooth": 1, "post_smooth": 1}
     if name == "fixed":
         return {**base, "mode": "fixed", "max_cycles": 1, "post_smooth": 4}
     raise SystemExit(f"unknown solver {name}")
