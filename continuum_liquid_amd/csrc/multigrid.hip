@@ -437,21 +437,68 @@ __device__ __forceinline__ void restrict_item_eager(const Level &F, const Level 
     store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
 }
 
-template <bool EAGER>
+// One lane per coarse cell (restrict_item): the finer levels, where the
+// eight-lane form below measured slower (its lanes repeat the index work and
+// the neighbour loads that one lane shares between its children).
 __global__ __launch_bounds__(256) void restrict_kernel(Level F, Level K, float inv_delta2) {
     const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (ci >= K.C) return;
-    if (EAGER) restrict_item_eager(F, K, inv_delta2, ci);
-    else restrict_item(F, K, inv_delta2, ci);
+    restrict_item(F, K, inv_delta2, ci);
+}
+
+// Eight lanes per coarse cell, lane q computing child q's residual (0 where
+// the child is outside the interior or not fluid); lane 0 of the eight sums
+// them in restrict_item's order.  Adding the +0 of a skipped child leaves
+// the sum unchanged: it starts at +0 and a round-to-nearest sum is -0 only
+// when both terms are.  Eight times the lanes of restrict_kernel, so the
+// coarse levels' restrictions (a few thousand coarse cells) are no longer
+// bound by one lane's 128 loads: 5.1 and 4.1 us instead of 9.8 and 9.3 us
+// for the 66^3 -> 34^3 and 34^3 -> 18^3 restrictions on MI355X.
+__device__ __forceinline__ float sum_children(float res) {
+    const int base = (int)(threadIdx.x & 63u) & ~7;
+    float v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = __shfl(res, base + q, 64);
+    float rhs = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) rhs = rhs + v[q];
+    return rhs;
+}
+
+__global__ __launch_bounds__(256) void restrict8_kernel(Level F, Level K, float inv_delta2) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t ci = t >> 3;
+    const int q = (int)(t & 7);
+    const bool live = ci < K.C;  // the same for the eight lanes of a coarse cell
+    bool kin = false;
+    float res = 0.0f;
+    if (live) {
+        int a, b, c;
+        coords(ci, K.ny, K.nz, a, b, c);
+        kin = interior(a, b, c, K.nx, K.ny, K.nz);
+        const int i = 1 + 2 * (a - 1) + (q >> 2), j = 1 + 2 * (b - 1) + ((q >> 1) & 1), k = 1 + 2 * (c - 1) + (q & 1);
+        if (kin && interior(i, j, k, F.nx, F.ny, F.nz)) {
+            const int64_t idx = gidx(i, j, k, F.ny, F.nz), sx = (int64_t)F.ny * F.nz, sy = F.nz;
+            const float m = F.mask[idx];
+            const float r = residual_at(F.p, load_b(F.b, F.b16, F.bkind, F.bscale, idx), F.diag, F.wx, F.wy, F.wz,
+                                        idx, sx, sy, inv_delta2);
+            res = m > 0.0f ? r : 0.0f;
+        }
+    }
+    const float rhs = sum_children(res);  // every lane takes part in the shuffles
+    if (live && q == 0) {
+        K.p[ci] = 0.0f;
+        store_b(K.b, K.b16, K.bkind, K.bscale, ci, kin ? rhs : 0.0f);
+    }
 }
 
 void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, hipStream_t s) {
-    if (F.C < kEagerCells)
-        hipLaunchKernelGGL(restrict_kernel<true>, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K,
-                           inv_delta2);
-    else
-        hipLaunchKernelGGL(restrict_kernel<false>, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K,
-                           inv_delta2);
+    if (F.C >= kEagerCells) {
+        hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2);
+        return;
+    }
+    const int64_t lanes = 8 * K.C;
+    hipLaunchKernelGGL(restrict8_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2);
 }
 
 // restrict_item_eager with the fine operator read from the packed bytes
