@@ -140,7 +140,8 @@ void launch_pressure_correction64(float *u, float *v, float *w, const double *p,
 // multigrid.hip
 void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s);
 void launch_coarse_operator(const Level &F, const Level &K, hipStream_t s);
-void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, hipStream_t s);
+// red: also make the first (red) half-sweep of the coarse level's pre-smoothing
+void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, float delta, int red, hipStream_t s);
 void launch_prolongate(const Level &F, const Level &K, hipStream_t s);
 void launch_coarse_smooth(const Level &L, float delta, int iterations, hipStream_t s);
 void launch_coarse_smooth_f64(const Level &L, double delta, int iterations, hipStream_t s);
@@ -174,7 +175,12 @@ void launch_pack_operator(const Level &L, uint8_t *op, int *bad, hipStream_t s);
 void launch_rbgs_rows_packed(const Level &L, const uint8_t *op, float delta, int parity, const int32_t *rows,
                              int64_t nrows, hipStream_t s);
 void launch_restrict_rows(const Level &F, const uint8_t *op, const Level &K, float inv_delta2, const int32_t *krows,
-                          int64_t nkrows, hipStream_t s);
+                          int64_t nkrows, float delta, int red, hipStream_t s);
+// One half-sweep of L with the prolongation from K folded in (first: the red
+// half-sweep, run before the black one); rows / op as for the row sweeps,
+// rows == NULL for the dense form.
+void launch_rbgs_prolong(const Level &L, const Level &K, const uint8_t *op, float delta, int parity, int first,
+                         const int32_t *rows, int64_t nrows, hipStream_t s);
 void launch_prolongate_rows(const Level &F, const Level &K, const int32_t *frows, int64_t nfrows, hipStream_t s);
 void launch_ir_update_residual(const Level &L, int from32, const float *p32, const double *pin, double *pout,
                                float *r32, const float *b, double inv_h2, double *partials, double *out2,

@@ -574,12 +574,15 @@ static int begin_sparse_solve(clq_ctx *c) {
     return CLQ_OK;
 }
 
-static void smooth_level(clq_ctx *c, int level, int iterations) {
+// red_done: the restriction onto this level already made the first (red)
+// half-sweep (restrict_level with red).
+static void smooth_level(clq_ctx *c, int level, int iterations, bool red_done = false) {
     const float delta = (float)c->dx;
     const bool sparse = c->use_rows && level < 2;
     for (int it = 0; it < iterations; ++it) {
-        if (level == 0) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
-        for (int parity = 0; parity < 2; ++parity) {
+        const bool whole = !(it == 0 && red_done);
+        if (level == 0 && whole) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
+        for (int parity = whole ? 0 : 1; parity < 2; ++parity) {
             if (sparse && level == 0 && c->op0_ok)
                 launch_rbgs_rows_packed(c->lv[0], c->op0, delta, parity, c->rows[0], c->nrows[0], c->stream);
             else if (sparse)
@@ -587,17 +590,32 @@ static void smooth_level(clq_ctx *c, int level, int iterations) {
             else
                 launch_rbgs(c->lv[level], delta, parity, c->stream);
         }
-        if (level == 0) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
+        if (level == 0 && whole) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
     }
 }
 
-static void restrict_level(clq_ctx *c, int level) {
+// Inside a solve (row lists set up) the transfers are folded into the
+// neighbouring half-sweeps: the restriction also makes the coarse level's
+// first red half-sweep, and the prolongation rides on the first
+// post-smoothing sweep.  Same values as the separate kernels.
+static bool fold_transfers(const clq_ctx *c) { return c->use_rows; }
+
+// the first post-smoothing sweep of `level` with the prolongation from level + 1
+static void smooth_prolong(clq_ctx *c, int level) {
+    const bool sparse = level < 2;
+    const uint8_t *op = level == 0 && c->op0_ok ? c->op0 : nullptr;
+    for (int parity = 0; parity < 2; ++parity)
+        launch_rbgs_prolong(c->lv[level], c->lv[level + 1], op, (float)c->dx, parity, parity == 0,
+                            sparse ? c->rows[level] : nullptr, sparse ? c->nrows[level] : 0, c->stream);
+}
+
+static void restrict_level(clq_ctx *c, int level, bool red = false) {
     const float delta = (float)c->dx;
     if (c->use_rows && level == 0)
         launch_restrict_rows(c->lv[0], c->op0_ok ? c->op0 : nullptr, c->lv[1], 1.0f / (delta * delta), c->rows[1],
-                             c->nrows[1], c->stream);
+                             c->nrows[1], delta, red, c->stream);
     else
-        launch_restrict_residual(c->lv[level], c->lv[level + 1], 1.0f / (delta * delta), c->stream);
+        launch_restrict_residual(c->lv[level], c->lv[level + 1], 1.0f / (delta * delta), delta, red, c->stream);
 }
 
 static void prolongate_level(clq_ctx *c, int level) {
@@ -653,7 +671,34 @@ static void launch_bottom(clq_ctx *c, int level, int pre, int post, int extra, i
 }
 
 // multigrid.py:53-187
-static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfrom) {
+// Whether a visit of `level` smooths it with the colour kernels (and not in
+// the bottom kernel or the coarsest smoother).
+static bool swept_level(const clq_ctx *c, int level) {
+    return level < c->nlevels - 1 && !(level > 0 && level == bottom_level(c));
+}
+
+// The steps of one level visit shared by both cycles: pre-smoothing, the
+// restriction, then (after the caller's coarse visits) the prolongation and
+// post-smoothing.  Returns whether the coarse level's first red half-sweep
+// was made by the restriction.
+static bool visit_down(clq_ctx *c, int level, int pre, bool red_done) {
+    smooth_level(c, level, pre, red_done);
+    const bool red = fold_transfers(c) && pre > 0 && swept_level(c, level + 1);
+    restrict_level(c, level, red);
+    return red;
+}
+
+static void visit_up(clq_ctx *c, int level, int post) {
+    if (fold_transfers(c) && post > 0) {
+        smooth_prolong(c, level);
+        smooth_level(c, level, post - 1);
+    } else {
+        prolongate_level(c, level);
+        smooth_level(c, level, post);
+    }
+}
+
+static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool red_done = false) {
     if (level > 0 && level == bottom_level(c)) {
         launch_bottom(c, level, pre, post, extra, vfrom, false);
         return;
@@ -662,12 +707,10 @@ static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfr
         launch_coarse_smooth(c->lv[level], (float)c->dx, pre + extra, c->stream);
         return;
     }
-    smooth_level(c, level, pre);
-    restrict_level(c, level);
-    w_cycle(c, level + 1, pre, post, extra, vfrom);
+    const bool red = visit_down(c, level, pre, red_done);
+    w_cycle(c, level + 1, pre, post, extra, vfrom, red);
     if (vfrom <= 0 || level < vfrom) w_cycle(c, level + 1, pre, post, extra, vfrom);
-    prolongate_level(c, level);
-    smooth_level(c, level, post);
+    visit_up(c, level, post);
 }
 
 int clq_smooth(clq_ctx *c, int32_t level, int32_t iterations) {
@@ -734,7 +777,7 @@ int clq_residual_norm(clq_ctx *c, double *residual, double *rhs_norm) {
 
 
 // ---- mixed-precision iterative refinement -------------------------------------------
-static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, int vfrom) {
+static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool red_done = false) {
     if (level > 0 && level == bottom_level(c)) {
         launch_bottom(c, level, pre, post, extra, vfrom, true);
         return;
@@ -745,12 +788,10 @@ static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, i
         else launch_coarse_smooth(L, (float)c->dx, pre + extra, c->stream);
         return;
     }
-    smooth_level(c, level, pre);
-    restrict_level(c, level);
-    w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
+    const bool red = visit_down(c, level, pre, red_done);
+    w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, red);
     if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
-    prolongate_level(c, level);
-    smooth_level(c, level, post);
+    visit_up(c, level, post);
 }
 
 static int ir_residual(clq_ctx *c, int from32, const double *pin, double *pout, double *r, double *bn) {

@@ -383,7 +383,26 @@ __device__ __forceinline__ float residual_at(const float *__restrict__ p, float 
     return bv - ap;
 }
 
-__device__ __forceinline__ void restrict_item(const Level &F, const Level &K, float inv_delta2, int64_t ci) {
+// Coarse cell ci's b and pressure after the restriction (p = 0) and, with
+// red, after the first (red) half-sweep of the coarse pre-smoothing too: on
+// p = 0 every neighbour term w * 0 is +0, so the red update is
+// fmaf(-dd, b, +0) / d with b as the smoother reads it back from storage.
+__device__ __forceinline__ void restrict_store(const Level &K, int64_t ci, bool kin, float rhs, float dd, bool red) {
+    store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
+    float pv = 0.0f;
+    if (red && kin) {
+        int a, b, c;
+        coords(ci, K.ny, K.nz, a, b, c);
+        if (((a + b + c) & 1) == 0) {
+            const float m = K.mask[ci], d = K.diag[ci];
+            if (m > 0.0f && d > 0.0f) pv = fmaf(-dd, load_b(K.b, K.b16, K.bkind, K.bscale, ci), 0.0f) / d;
+        }
+    }
+    K.p[ci] = pv;
+}
+
+__device__ __forceinline__ void restrict_item(const Level &F, const Level &K, float inv_delta2, int64_t ci,
+                                              float dd = 0.0f, bool red = false) {
     int a, b, c;
     coords(ci, K.ny, K.nz, a, b, c);
     float rhs = 0.0f;
@@ -401,13 +420,13 @@ __device__ __forceinline__ void restrict_item(const Level &F, const Level &K, fl
                     rhs = rhs + residual_at(F.p, bv, F.diag, F.wx, F.wy, F.wz, idx, sx, sy, inv_delta2);
                 }
     }
-    K.p[ci] = 0.0f;
-    store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
+    restrict_store(K, ci, interior(a, b, c, K.nx, K.ny, K.nz), rhs, dd, red);
 }
 
 // restrict_item with the eight children's loads issued together; the sum
 // takes the fluid children in the same order.
-__device__ __forceinline__ void restrict_item_eager(const Level &F, const Level &K, float inv_delta2, int64_t ci) {
+__device__ __forceinline__ void restrict_item_eager(const Level &F, const Level &K, float inv_delta2, int64_t ci,
+                                                    float dd = 0.0f, bool red = false) {
     int a, b, c;
     coords(ci, K.ny, K.nz, a, b, c);
     float rhs = 0.0f;
@@ -433,17 +452,16 @@ __device__ __forceinline__ void restrict_item_eager(const Level &F, const Level 
         for (int q = 0; q < 8; ++q)
             if (in[q] && m[q] > 0.0f) rhs = rhs + res[q];
     }
-    K.p[ci] = 0.0f;
-    store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
+    restrict_store(K, ci, interior(a, b, c, K.nx, K.ny, K.nz), rhs, dd, red);
 }
 
 // One lane per coarse cell (restrict_item): the finer levels, where the
 // eight-lane form below measured slower (its lanes repeat the index work and
 // the neighbour loads that one lane shares between its children).
-__global__ __launch_bounds__(256) void restrict_kernel(Level F, Level K, float inv_delta2) {
+__global__ __launch_bounds__(256) void restrict_kernel(Level F, Level K, float inv_delta2, float dd, int red) {
     const int64_t ci = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (ci >= K.C) return;
-    restrict_item(F, K, inv_delta2, ci);
+    restrict_item(F, K, inv_delta2, ci, dd, red != 0);
 }
 
 // Eight lanes per coarse cell, lane q computing child q's residual (0 where
@@ -465,7 +483,7 @@ __device__ __forceinline__ float sum_children(float res) {
     return rhs;
 }
 
-__global__ __launch_bounds__(256) void restrict8_kernel(Level F, Level K, float inv_delta2) {
+__global__ __launch_bounds__(256) void restrict8_kernel(Level F, Level K, float inv_delta2, float dd, int red) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t ci = t >> 3;
     const int q = (int)(t & 7);
@@ -487,24 +505,26 @@ __global__ __launch_bounds__(256) void restrict8_kernel(Level F, Level K, float 
     }
     const float rhs = sum_children(res);  // every lane takes part in the shuffles
     if (live && q == 0) {
-        K.p[ci] = 0.0f;
-        store_b(K.b, K.b16, K.bkind, K.bscale, ci, kin ? rhs : 0.0f);
+        restrict_store(K, ci, kin, kin ? rhs : 0.0f, dd, red != 0);
     }
 }
 
-void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, hipStream_t s) {
+void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, float delta, int red, hipStream_t s) {
+    const float dd = delta * delta;
     if (F.C >= kEagerCells) {
-        hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2);
+        hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2, dd,
+                           red);
         return;
     }
     const int64_t lanes = 8 * K.C;
-    hipLaunchKernelGGL(restrict8_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2);
+    hipLaunchKernelGGL(restrict8_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2, dd,
+                       red);
 }
 
 // restrict_item_eager with the fine operator read from the packed bytes
 // (same decoded values, same operations).
 __device__ __forceinline__ void restrict_item_packed(const Level &F, const uint8_t *__restrict__ op, const Level &K,
-                                                     float inv_delta2, int64_t ci) {
+                                                     float inv_delta2, int64_t ci, float dd, bool red) {
     int a, b, c;
     coords(ci, K.ny, K.nz, a, b, c);
     float rhs = 0.0f;
@@ -537,28 +557,27 @@ __device__ __forceinline__ void restrict_item_packed(const Level &F, const uint8
         for (int q = 0; q < 8; ++q)
             if (use[q]) rhs = rhs + res[q];
     }
-    K.p[ci] = 0.0f;
-    store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
+    restrict_store(K, ci, interior(a, b, c, K.nx, K.ny, K.nz), rhs, dd, red);
 }
 
 // restriction onto the active coarse rows only (packed fine operator when op != NULL)
 __global__ __launch_bounds__(256) void restrict_rows_kernel(Level F, const uint8_t *__restrict__ op, Level K,
                                                             float inv_delta2, const int32_t *__restrict__ krows,
-                                                            int64_t work) {
+                                                            int64_t work, float dd, int red) {
     const int64_t t = (int64_t)xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= work) return;
     const int64_t ar = t / K.nz;
     const int64_t ci = (int64_t)krows[ar] * K.nz + (t - ar * K.nz);
-    if (op) restrict_item_packed(F, op, K, inv_delta2, ci);
-    else restrict_item_eager(F, K, inv_delta2, ci);
+    if (op) restrict_item_packed(F, op, K, inv_delta2, ci, dd, red != 0);
+    else restrict_item_eager(F, K, inv_delta2, ci, dd, red != 0);
 }
 
 void launch_restrict_rows(const Level &F, const uint8_t *op, const Level &K, float inv_delta2, const int32_t *krows,
-                          int64_t nkrows, hipStream_t s) {
+                          int64_t nkrows, float delta, int red, hipStream_t s) {
     const int64_t work = nkrows * K.nz;
     if (work == 0) return;
     hipLaunchKernelGGL(restrict_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, op, K,
-                       inv_delta2, krows, work);
+                       inv_delta2, krows, work, delta * delta, red);
 }
 
 __device__ __forceinline__ void prolong_item(const Level &F, const Level &K, int64_t fi) {
@@ -603,6 +622,161 @@ void launch_prolongate_rows(const Level &F, const Level &K, const int32_t *frows
     if (work == 0) return;
     hipLaunchKernelGGL(prolongate_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, K, frows,
                        work);
+}
+
+// ---- transfers folded into the neighbouring half-sweeps --------------------------------
+// A level visit runs restrict -> [coarse visit] -> prolong -> red sweep ->
+// black sweep.  The prolongation (prolong_item: p += p_coarse[parent] on the
+// interior fluid cells) is folded into the two post-smoothing half-sweeps:
+// the red one reads its black neighbours as they would be after the
+// prolongation (p + p_coarse[parent] where the neighbour is fluid), and each
+// colour adds the prolongation to its own fluid cells that the sweep does not
+// overwrite (d <= 0).  Every value is the one the separate kernels produce.
+__device__ __forceinline__ int64_t parent_of(const Level &K, int i, int j, int k) {
+    return gidx(min((i - 1) / 2 + 1, K.nx - 2), min((j - 1) / 2 + 1, K.ny - 2), min((k - 1) / 2 + 1, K.nz - 2), K.ny,
+                K.nz);
+}
+
+// the value of cell q = (i, j, k) after the prolongation
+__device__ __forceinline__ float prolonged(const Level &L, const Level &K, int64_t q, int i, int j, int k) {
+    const float m = L.mask[q], pv = L.p[q], pc = K.p[parent_of(K, i, j, k)];
+    return m > 0.0f ? pv + pc : pv;
+}
+
+// rbgs_cell_eager with the prolongation from K folded in (first: the red
+// half-sweep, whose neighbours still lack it)
+__device__ __forceinline__ void rbgs_cell_prolong(const Level &L, const Level &K, float dd, int parity, uint32_t row,
+                                                  int kk, bool first) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
+    const int k = 2 * kk + ((parity + i + j) & 1);
+    if (k >= nz) return;
+    const int64_t idx = (int64_t)row * nz + k;
+    const bool inner = interior(i, j, k, nx, ny, nz);
+    const float m = L.mask[idx];
+    float d = 0.0f, num = 0.0f, own = 0.0f;
+    if (inner) {
+        const int64_t sx = (int64_t)ny * nz, sy = nz;
+        d = L.diag[idx];
+        own = L.p[idx] + K.p[parent_of(K, i, j, k)];  // prolong_item
+        float t;
+        if (first) {  // wsum over the prolonged neighbours
+            const float vxm = prolonged(L, K, idx - sx, i - 1, j, k), vxp = prolonged(L, K, idx + sx, i + 1, j, k);
+            const float vyp = prolonged(L, K, idx + sy, i, j + 1, k), vym = prolonged(L, K, idx - sy, i, j - 1, k);
+            const float vzp = prolonged(L, K, idx + 1, i, j, k + 1), vzm = prolonged(L, K, idx - 1, i, j, k - 1);
+            t = L.wx[idx - sx] * vxm;
+            t = fmaf(L.wx[idx], vxp, t);
+            t = fmaf(L.wy[idx], vyp, t);
+            t = fmaf(L.wy[idx - sy], vym, t);
+            t = fmaf(L.wz[idx], vzp, t);
+            t = fmaf(L.wz[idx - 1], vzm, t);
+        } else {
+            t = wsum(L.p, L.wx, L.wy, L.wz, idx, sx, sy);
+        }
+        num = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), t);
+    }
+    if (m <= 0.0f) {
+        L.p[idx] = 0.0f;
+        return;
+    }
+    if (inner) L.p[idx] = d > 0.0f ? num / d : own;  // multigrid.cl:227 or the prolongation
+}
+
+__global__ __launch_bounds__(256) void rbgs_prolong_kernel(Level L, Level K, float dd, int parity, int first) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t half = (uint32_t)(L.nz + 1) >> 1;
+    if (t >= (int64_t)L.nx * L.ny * half) return;
+    const uint32_t row = (uint32_t)t / half;
+    rbgs_cell_prolong(L, K, dd, parity, row, (int)((uint32_t)t - row * half), first != 0);
+}
+
+__global__ __launch_bounds__(256) void rbgs_rows_prolong_kernel(Level L, Level K, float dd, int parity, int first,
+                                                                const int32_t *__restrict__ rows) {
+    const uint32_t row = (uint32_t)rows[xcd_item(blockIdx.x, gridDim.x)];
+    const int half = (L.nz + 1) >> 1;
+    for (int kk = threadIdx.x; kk < half; kk += 256) rbgs_cell_prolong(L, K, dd, parity, row, kk, first != 0);
+}
+
+// the packed level-0 form (rbgs_rows_packed_kernel): a neighbour is fluid when
+// bit 3 of its operator byte is set
+__global__ __launch_bounds__(256) void rbgs_rows_packed_prolong_kernel(Level L, const uint8_t *__restrict__ op,
+                                                                       Level K, float dd, int parity, int first,
+                                                                       const int32_t *__restrict__ rows) {
+    const uint32_t row = (uint32_t)rows[xcd_item(blockIdx.x, gridDim.x)];
+    const int nx = L.nx, ny = L.ny, nz = L.nz, half = (nz + 1) >> 1;
+    const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
+    const int64_t sx = (int64_t)ny * nz, sy = nz;
+    const float *__restrict__ pc = K.p;
+    float *__restrict__ p = L.p;
+    for (int kk = threadIdx.x; kk < half; kk += 256) {
+        const int k = 2 * kk + ((parity + i + j) & 1);
+        if (k >= nz) continue;
+        const int64_t idx = (int64_t)row * nz + k;
+        const bool inner = interior(i, j, k, nx, ny, nz);
+        const uint32_t o = op[idx];
+        float num = 0.0f, own = 0.0f;
+        if (inner) {
+            const uint32_t ox = op[idx - sx], oy = op[idx - sy], oz = op[idx - 1];
+            own = p[idx] + pc[parent_of(K, i, j, k)];  // prolong_item
+            float vxm, vxp, vyp, vym, vzp, vzm;
+            if (first) {
+                const uint32_t oxp = op[idx + sx], oyp = op[idx + sy], ozp = op[idx + 1];
+                vxm = p[idx - sx];
+                vxp = p[idx + sx];
+                vyp = p[idx + sy];
+                vym = p[idx - sy];
+                vzp = p[idx + 1];
+                vzm = p[idx - 1];
+                const float cxm = pc[parent_of(K, i - 1, j, k)], cxp = pc[parent_of(K, i + 1, j, k)];
+                const float cyp = pc[parent_of(K, i, j + 1, k)], cym = pc[parent_of(K, i, j - 1, k)];
+                const float czp = pc[parent_of(K, i, j, k + 1)], czm = pc[parent_of(K, i, j, k - 1)];
+                if (ox & 8) vxm = vxm + cxm;
+                if (oxp & 8) vxp = vxp + cxp;
+                if (oyp & 8) vyp = vyp + cyp;
+                if (oy & 8) vym = vym + cym;
+                if (ozp & 8) vzp = vzp + czp;
+                if (oz & 8) vzm = vzm + czm;
+            } else {
+                vxm = p[idx - sx];
+                vxp = p[idx + sx];
+                vyp = p[idx + sy];
+                vym = p[idx - sy];
+                vzp = p[idx + 1];
+                vzm = p[idx - 1];
+            }
+            float t = bitw(ox, 1) * vxm;
+            t = fmaf(bitw(o, 1), vxp, t);
+            t = fmaf(bitw(o, 2), vyp, t);
+            t = fmaf(bitw(oy, 2), vym, t);
+            t = fmaf(bitw(o, 4), vzp, t);
+            t = fmaf(bitw(oz, 4), vzm, t);
+            num = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), t);
+        }
+        if (!(o & 8)) {
+            p[idx] = 0.0f;
+            continue;
+        }
+        const float d = (float)(o >> 4);
+        if (inner) p[idx] = d > 0.0f ? num / d : own;  // multigrid.cl:227 or the prolongation
+    }
+}
+
+void launch_rbgs_prolong(const Level &L, const Level &K, const uint8_t *op, float delta, int parity, int first,
+                         const int32_t *rows, int64_t nrows, hipStream_t s) {
+    const float dd = delta * delta;
+    if (rows) {
+        if (nrows == 0) return;
+        if (op)
+            hipLaunchKernelGGL(rbgs_rows_packed_prolong_kernel, dim3((unsigned)nrows), dim3(256), 0, s, L, op, K, dd,
+                               parity, first, rows);
+        else
+            hipLaunchKernelGGL(rbgs_rows_prolong_kernel, dim3((unsigned)nrows), dim3(256), 0, s, L, K, dd, parity,
+                               first, rows);
+        return;
+    }
+    const int64_t work = (int64_t)L.nx * L.ny * ((L.nz + 1) >> 1);
+    hipLaunchKernelGGL(rbgs_prolong_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, L, K, dd, parity,
+                       first);
 }
 
 // coarse_smooth: one work-group of 1024 lanes owns the whole coarsest level

@@ -267,6 +267,44 @@ def test_mixed_refinement_bit_exact(oracle_lib, hip_lib, coarse_dtype, vcycle_fr
     assert_fields_equal(o, h, ("u", "v", "w"))
 
 
+DEEP_SCENES = {"96^3": lambda: scenes.small_scene(96, seed=3),
+               "96x80x72": lambda: scenes.dam_break("dam-break 96x80x72", (96, 80, 72), 1.0 / 96, (0.45, 0.55), [], 5)}
+
+
+@pytest.mark.parametrize("coarse_dtype,vcycle_from,smooth", [(0, 0, (2, 2)), (1, 1, (1, 1)), (0, 2, (1, 2)),
+                                                              (1, 0, (3, 1))],
+                         ids=["fp16-W22", "bf16-V1-11", "fp16-V2-12", "bf16-W31"])
+@pytest.mark.parametrize("shape", list(DEEP_SCENES), ids=list(DEEP_SCENES))
+def test_deep_cycles_bit_exact(oracle_lib, hip_lib, shape, coarse_dtype, vcycle_from, smooth):
+    """Hierarchies of five and four levels: the row kernels on levels 0-1, the
+    dense colour kernels on level 2, the single-work-group bottom below, and
+    the transfers folded into the neighbouring half-sweeps, against the
+    oracle's plain recursion, bit for bit."""
+    scene = DEEP_SCENES[shape]()
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+    kw = dict(pre_smooth=smooth[0], post_smooth=smooth[1], coarse_extra=16, max_cycles=3, tolerance=1e-12,
+              coarse_dtype=coarse_dtype, vcycle_from=vcycle_from)
+    so = o.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, **kw)
+    sh = h.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, **kw)
+    print(f"{shape} {kw}: cycles {sh['cycles']}, relres {sh['relative_residual']:.3g}", flush=True)
+    assert so["cycles"] == sh["cycles"]
+    assert_fields_equal(o, h, ("pressure", "pressure_fp64"))
+
+
+def test_deep_fixed_cycle_bit_exact(oracle_lib, hip_lib):
+    """The reference's fixed FP32 W(2, 4) cycle on a five-level hierarchy."""
+    scene = DEEP_SCENES["96^3"]()
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+    so = o.pressure_solve(runtime.SOLVE_FIXED_CYCLES, post_smooth=4, max_cycles=2)
+    sh = h.pressure_solve(runtime.SOLVE_FIXED_CYCLES, post_smooth=4, max_cycles=2)
+    assert so["cycles"] == sh["cycles"] == 2
+    assert_fields_equal(o, h, ("pressure",))
+
+
 @pytest.mark.parametrize("scale", [1, 2])
 def test_voxelise_matches_oracle(oracle_lib, hip_lib, scale):
     scene = CASES[0]
