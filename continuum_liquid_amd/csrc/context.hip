@@ -81,6 +81,7 @@ struct clq_ctx {
     float *r32 = nullptr;
     bool p64_valid = false;  // p64a holds the pressure of the last solve (modes 2-4)
     // flexible-CG state, allocated on first use
+    // FCG: dir[0] the direction d, dir[1] w = A z
     double *r64 = nullptr, *dir[2] = {nullptr, nullptr}, *q64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
     // active rows of levels 0 and 1 (rows holding an interior fluid cell),
     // used by the smoother, transfers and FCG vectors inside clq_pressure_solve
@@ -869,7 +870,9 @@ static int ensure_mixed_buffers(clq_ctx *c) {
 
 // z = M r: the mixed-precision W-cycle on A e = r32 from e = 0 (e is the
 // level-0 pressure array), 16-bit coarse b scaled from the residual norm.
-static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rnorm) {
+// red_done: the FCG update already left e after the cycle's first red
+// half-sweep on level 0 (fcg_update with e), in place of zeroing it.
+static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rnorm, bool red_done = false) {
     const int kind = sp->coarse_dtype == 1 ? 2 : 1;
     const int es = (int)lround(log2(rnorm / sqrt((double)c->C) + 1e-300));
     for (int l = 1; l < c->nlevels; ++l) {
@@ -879,9 +882,11 @@ static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rn
     Level &L0 = c->lv[0];
     L0.b = c->r32;
     // e = 0 where the cycle reads it with a non-zero weight: the active rows
-    if (c->use_rows) launch_zero_rows(c->p, c->rows[0], c->nrows[0], c->nz, c->stream);
-    else HIP_TRY(hipMemsetAsync(c->p, 0, (size_t)c->C * 4, c->stream));
-    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
+    if (!red_done) {
+        if (c->use_rows) launch_zero_rows(c->p, c->rows[0], c->nrows[0], c->nz, c->stream);
+        else HIP_TRY(hipMemsetAsync(c->p, 0, (size_t)c->C * 4, c->stream));
+    }
+    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, red_done);
     L0.b = c->rhs;
     for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
     return CLQ_OK;
@@ -929,23 +934,25 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     const double bn = sqrt(c->norm_host[1]);
     s->initial_residual = r;
     s->rhs_norm = bn;
-    int cur = 0;
+    // the update makes the next cycle's first level-0 red half-sweep
+    const bool fold = fold_transfers(c) && c->op0_ok && sp->pre_smooth > 0 && swept_level(c, 0);
     if (r > sp->tolerance * bn && sp->max_cycles > 0) {
         if (int rc = precondition_wcycle(c, sp, r)) return rc;
         s->cycles++;
-        launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
+        launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
+                        c->stream);
         for (int it = 0;; ++it) {
-            launch_fcg_direction(L0, OP0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, ROWS0,
-                                 c->fcg_partials, c->fcg_s, c->stream);
-            cur ^= 1;
-            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, ROWS0, c->fcg_partials, c->fcg_s,
-                              c->stream);
+            launch_fcg_direction(L0, OP0, c->p, c->dir[0], c->dir[1], c->q64, it == 0, ROWS0, c->fcg_partials,
+                                 c->fcg_s, c->stream);
+            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[0], c->q64, ROWS0, OP0, fold ? c->p : nullptr,
+                              (float)c->dx, c->fcg_partials, c->fcg_s, c->stream);
             if (int rc = read_scalars(c)) return rc;
             r = sqrt(c->norm_host[0]);
             if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
-            if (int rc = precondition_wcycle(c, sp, r)) return rc;
+            if (int rc = precondition_wcycle(c, sp, r, fold)) return rc;
             s->cycles++;
-            launch_fcg_dots(L0, c->p, c->q64, c->r64, 0, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
+            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 0, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
+                            c->stream);
         }
     }
     launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
@@ -995,23 +1002,23 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
         s->cycles++;
         return CLQ_OK;
     };
-    int cur = 0;
     if (r > sp->tolerance * bn && sp->max_cycles > 0) {
         if (int rc = precondition(r)) return rc;
-        launch_fcg_dots(L0, c->p, c->q64, c->r64, 1, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
+        launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
+                        c->stream);
         for (int it = 0;; ++it) {
-            launch_fcg_direction(L0, OP0, c->p, c->dir[cur], c->dir[cur ^ 1], c->q64, it == 0, inv_h2, ROWS0,
-                                 c->fcg_partials, c->fcg_s, c->stream);
-            cur ^= 1;
-            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[cur], c->q64, ROWS0, c->fcg_partials, c->fcg_s,
-                              c->stream);
+            launch_fcg_direction(L0, OP0, c->p, c->dir[0], c->dir[1], c->q64, it == 0, ROWS0, c->fcg_partials,
+                                 c->fcg_s, c->stream);
+            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[0], c->q64, ROWS0, OP0, nullptr, (float)c->dx,
+                              c->fcg_partials, c->fcg_s, c->stream);
             if (int rc = read_scalars(c)) return rc;
             const double before = r;
             r = sqrt(c->norm_host[0]);
             if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
             if (use_net && r > c->stall_ratio * before) use_net = false;
             if (int rc = precondition(r)) return rc;
-            launch_fcg_dots(L0, c->p, c->q64, c->r64, 0, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
+            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 0, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
+                            c->stream);
         }
     }
     launch_widen_narrow(c->p64a, c->p, c->C, c->stream);

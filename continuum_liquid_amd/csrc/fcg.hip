@@ -3,10 +3,12 @@
 // learned network followed by it).  No reference counterpart: the reference
 // runs a fixed number of stationary W-cycles (pressure_solve.py:61-83).
 //
-// Vectors over all C cells, FP64: x (iterate), r (residual), d (direction,
-// ping-pong), q = A d.  The preconditioned residual z arrives as FP32 in the
-// level-0 correction array e.  Scalars live in device memory so that alpha and
-// beta never round-trip through the host:
+// Vectors over all C cells, FP64: x (iterate), r (residual), d (direction),
+// q = A d, w = A z.  The preconditioned residual z arrives as FP32 in the
+// level-0 correction array e.  The matrix-vector product is taken of z, right
+// after the preconditioner and before beta is known, so the direction update
+// streams: d = z + beta d, q = A z + beta q (= A d).  Scalars live in device
+// memory so that alpha and beta never round-trip through the host:
 //   s[0] rz = r.z   s[1] d.q   s[2] alpha   s[3] z.q   s[4] z.r   s[5] beta
 //   s[6] r.r        s[7] b.b
 // beta = -alpha (z_{k+1} . q_k) / (z_k . r_k), the flexible (Polak-Ribiere)
@@ -110,71 +112,99 @@ __global__ __launch_bounds__(kT) void fcg_init_kernel(Level L, const float *__re
     block_sums<2>(acc, partials);
 }
 
-// d_new = z + beta d_old, q = A d_new; partial d.q.  first: beta = 0.
+// d = z + beta d, q = w + beta q on the interior fluid cells (0 elsewhere);
+// partial d.q.  first: beta = 0.
 __global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const uint8_t *__restrict__ op,
-                                                          const float *__restrict__ z,
-                                                          const double *__restrict__ d_old, double *__restrict__ d_new,
-                                                          double *__restrict__ qv, const double *__restrict__ s,
-                                                          int first, double inv_h2, const int32_t *__restrict__ rows,
-                                                          int64_t work, double *__restrict__ partials) {
+                                                          const float *__restrict__ z, double *__restrict__ d,
+                                                          const double *__restrict__ w, double *__restrict__ qv,
+                                                          const double *__restrict__ s, int first,
+                                                          const int32_t *__restrict__ rows, int64_t work,
+                                                          double *__restrict__ partials) {
     const double beta = first ? 0.0 : s[5];
     double acc[1] = {0.0};
-    // one contiguous chunk of work items per block, chunks laid out per XCD
+    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
+        const int64_t q = cell_of(rows, L.nz, t);
+        int i, j, k;
+        coords(q, L.ny, L.nz, i, j, k);
+        const bool fluid = interior(i, j, k, L.nx, L.ny, L.nz) && (op ? (op[q] & 8) != 0 : L.mask[q] > 0.0f);
+        const double zv = (double)z[q], dold = d[q], wv = w[q], qold = qv[q];
+        double dv = 0.0, qn = 0.0;
+        if (fluid) {
+            dv = first ? zv : zv + beta * dold;
+            qn = first ? wv : wv + beta * qold;
+        }
+        d[q] = dv;
+        qv[q] = qn;
+        acc[0] += dv * qn;
+    }
+    block_sums<1>(acc, partials);
+}
+
+// x += alpha d, r -= alpha q, r32 = float(r); partial r.r.  With e (the
+// packed level-0 operator op given), also the correction array of the next
+// preconditioner cycle after its first red half-sweep from e = 0 on r32:
+// e = fmaf(-dd, r32, +0) / diag on the red interior fluid cells with diag > 0
+// (every neighbour term w * 0 is +0), 0 elsewhere, as zero_rows and
+// rbgs_rows_packed_kernel would leave it.
+__global__ __launch_bounds__(kT) void fcg_update_kernel(Level L, double *__restrict__ x, double *__restrict__ r,
+                                                       float *__restrict__ r32, const double *__restrict__ d,
+                                                       const double *__restrict__ qv, const double *__restrict__ s,
+                                                       const int32_t *__restrict__ rows, int64_t work,
+                                                       const uint8_t *__restrict__ op, float *__restrict__ e, float dd,
+                                                       double *__restrict__ partials) {
+    const double alpha = s[2];
+    double acc[1] = {0.0};
+    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
+        const int64_t q = cell_of(rows, L.nz, t);
+        const double xv = x[q] + alpha * d[q];
+        const double rv = r[q] - alpha * qv[q];
+        const float rf = (float)rv;
+        x[q] = xv;
+        r[q] = rv;
+        r32[q] = rf;
+        acc[0] += rv * rv;
+        if (e) {
+            int i, j, k;
+            coords(q, L.ny, L.nz, i, j, k);
+            const uint32_t o = op[q];
+            const float dg = (float)(o >> 4);
+            float ev = 0.0f;
+            if (((i + j + k) & 1) == 0 && (o & 8) && interior(i, j, k, L.nx, L.ny, L.nz) && dg > 0.0f)
+                ev = fmaf(-dd, rf, 0.0f) / dg;  // multigrid.cl:227 on e = 0
+            e[q] = ev;
+        }
+    }
+    block_sums<1>(acc, partials);
+}
+
+// w = A z on the interior fluid cells (0 elsewhere); partial z.q, z.r.  One
+// contiguous chunk of work items per block, chunks laid out per XCD (the
+// stencil's x-neighbour rows stay in that XCD's L2).
+__global__ __launch_bounds__(kT) void fcg_dots_kernel(Level L, const uint8_t *__restrict__ op,
+                                                     const float *__restrict__ z, const double *__restrict__ qv,
+                                                     const double *__restrict__ r, double *__restrict__ w,
+                                                     double inv_h2, const int32_t *__restrict__ rows, int64_t work,
+                                                     double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
     const int64_t chunk = (work + (int64_t)gridDim.x * kT - 1) / ((int64_t)gridDim.x * kT) * kT;
     const int64_t t0 = (int64_t)xcd_item(blockIdx.x, gridDim.x) * chunk, t1 = min(t0 + chunk, work);
     for (int64_t t = t0 + threadIdx.x; t < t1; t += kT) {
         const int64_t q = cell_of(rows, L.nz, t);
-        double dv = 0.0, av = 0.0;
+        double wv = 0.0;
         int i, j, k;
         coords(q, L.ny, L.nz, i, j, k);
         if (interior(i, j, k, L.nx, L.ny, L.nz)) {
             // neighbours outside the fluid carry zero weight: their values are
             // loaded with the rest but never enter the sum
-            auto dn = [&](int64_t u) { return first ? (double)z[u] : (double)z[u] + beta * d_old[u]; };
+            auto zv = [&](int64_t u) { return (double)z[u]; };
             const bool fluid = op ? (op[q] & 8) != 0 : L.mask[q] > 0.0f;
-            const double dq = dn(q), aq = op ? apply_a_packed(L, op, q, dn, inv_h2) : apply_a(L, q, dn, inv_h2);
-            if (fluid) {
-                dv = dq;
-                av = aq;
-            }
+            const double aq = op ? apply_a_packed(L, op, q, zv, inv_h2) : apply_a(L, q, zv, inv_h2);
+            if (fluid) wv = aq;
         }
-        d_new[q] = dv;
-        qv[q] = av;
-        acc[0] += dv * av;
-    }
-    block_sums<1>(acc, partials);
-}
-
-// x += alpha d, r -= alpha q, r32 = float(r); partial r.r.
-__global__ __launch_bounds__(kT) void fcg_update_kernel(double *__restrict__ x, double *__restrict__ r,
-                                                       float *__restrict__ r32, const double *__restrict__ d,
-                                                       const double *__restrict__ qv, const double *__restrict__ s,
-                                                       const int32_t *__restrict__ rows, int nz, int64_t work,
-                                                       double *__restrict__ partials) {
-    const double alpha = s[2];
-    double acc[1] = {0.0};
-    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
-        const int64_t q = cell_of(rows, nz, t);
-        const double xv = x[q] + alpha * d[q];
-        const double rv = r[q] - alpha * qv[q];
-        x[q] = xv;
-        r[q] = rv;
-        r32[q] = (float)rv;
-        acc[0] += rv * rv;
-    }
-    block_sums<1>(acc, partials);
-}
-
-// partial z.q, z.r
-__global__ __launch_bounds__(kT) void fcg_dots_kernel(const float *__restrict__ z, const double *__restrict__ qv,
-                                                     const double *__restrict__ r, const int32_t *__restrict__ rows,
-                                                     int nz, int64_t work, double *__restrict__ partials) {
-    double acc[2] = {0.0, 0.0};
-    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
-        const int64_t q = cell_of(rows, nz, t);
-        const double zv = (double)z[q];
-        acc[0] += zv * qv[q];
-        acc[1] += zv * r[q];
+        w[q] = wv;
+        const double zq = (double)z[q];
+        acc[0] += zq * qv[q];
+        acc[1] += zq * r[q];
     }
     block_sums<2>(acc, partials);
 }
@@ -230,29 +260,31 @@ void launch_fcg_init(const Level &L, const float *p32, const float *b, double in
 
 static int64_t work_of(const Level &L, const int32_t *rows, int64_t nrows) { return rows ? nrows * L.nz : L.C; }
 
-void launch_fcg_dots(const Level &L, const float *z, const double *qv, const double *r, int first,
-                     const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st) {
+void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const double *qv, const double *r, double *w,
+                     int first, double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s,
+                     hipStream_t st) {
     const int64_t work = work_of(L, rows, nrows);
     const int nb = blocks_for(work);
-    hipLaunchKernelGGL(fcg_dots_kernel, dim3(nb), dim3(kT), 0, st, z, qv, r, rows, L.nz, work, partials);
+    hipLaunchKernelGGL(fcg_dots_kernel, dim3(nb), dim3(kT), 0, st, L, op, z, qv, r, w, inv_h2, rows, work, partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 3, first, s);
 }
 
-void launch_fcg_direction(const Level &L, const uint8_t *op, const float *z, const double *d_old, double *d_new,
-                          double *qv, int first, double inv_h2, const int32_t *rows, int64_t nrows, double *partials,
-                          double *s, hipStream_t st) {
+void launch_fcg_direction(const Level &L, const uint8_t *op, const float *z, double *d, const double *w, double *qv,
+                          int first, const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st) {
     const int64_t work = work_of(L, rows, nrows);
     const int nb = blocks_for(work);
-    hipLaunchKernelGGL(fcg_direction_kernel, dim3(nb), dim3(kT), 0, st, L, op, z, d_old, d_new, qv, s, first, inv_h2,
-                       rows, work, partials);
+    hipLaunchKernelGGL(fcg_direction_kernel, dim3(nb), dim3(kT), 0, st, L, op, z, d, w, qv, s, first, rows, work,
+                       partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 1, 0, s);
 }
 
 void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const double *d, const double *qv,
-                       const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st) {
+                       const int32_t *rows, int64_t nrows, const uint8_t *op, float *e, float delta, double *partials,
+                       double *s, hipStream_t st) {
     const int64_t work = work_of(L, rows, nrows);
     const int nb = blocks_for(work);
-    hipLaunchKernelGGL(fcg_update_kernel, dim3(nb), dim3(kT), 0, st, x, r, r32, d, qv, s, rows, L.nz, work, partials);
+    hipLaunchKernelGGL(fcg_update_kernel, dim3(nb), dim3(kT), 0, st, L, x, r, r32, d, qv, s, rows, work, op, e,
+                       delta * delta, partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 2, 0, s);
 }
 

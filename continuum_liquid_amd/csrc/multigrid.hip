@@ -223,12 +223,27 @@ __global__ __launch_bounds__(256) void zero_inactive_rows_kernel(Level L, const 
     }
 }
 
-// One block per active row and colour half: lane t takes the row's same-colour
-// cells t and t + 256 (rows of up to 1024 cells), all loads issued first.
-__global__ __launch_bounds__(256) void rbgs_rows_kernel(Level L, float dd, int parity, const int32_t *__restrict__ rows) {
-    const uint32_t row = (uint32_t)rows[xcd_item(blockIdx.x, gridDim.x)];
-    const int half = (L.nz + 1) >> 1;
-    for (int kk = threadIdx.x; kk < half; kk += 256) rbgs_cell_eager(L, dd, parity, row, kk);
+// The row kernels take one lane per same-colour cell of the active rows:
+// item t is cell kk = t mod half of active row t / half (half = cells of one
+// colour in a row), so no lane idles on rows shorter than a block.
+__device__ __forceinline__ bool row_item(const int32_t *__restrict__ rows, int64_t nrows, int half, uint32_t &row,
+                                        int &kk) {
+    const uint32_t t = xcd_item(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+    const uint32_t ar = t / (uint32_t)half;
+    if (ar >= (uint32_t)nrows) return false;
+    row = (uint32_t)rows[ar];
+    kk = (int)(t - ar * (uint32_t)half);
+    return true;
+}
+
+static unsigned row_blocks(int64_t nrows, int nz) { return (unsigned)((nrows * ((nz + 1) >> 1) + 255) / 256); }
+
+// all loads issued first
+__global__ __launch_bounds__(256) void rbgs_rows_kernel(Level L, float dd, int parity, const int32_t *__restrict__ rows,
+                                                        int64_t nrows) {
+    uint32_t row;
+    int kk;
+    if (row_item(rows, nrows, (L.nz + 1) >> 1, row, kk)) rbgs_cell_eager(L, dd, parity, row, kk);
 }
 
 __global__ __launch_bounds__(256) void zero_rows_kernel(float *__restrict__ p, const int32_t *__restrict__ rows, int nz) {
@@ -288,48 +303,50 @@ void launch_pack_operator(const Level &L, uint8_t *op, int *bad, hipStream_t s) 
 // rbgs_cell on the packed operator: same loads of p and b, the operator from
 // op (own byte and the -x, -y, -z neighbours' bytes), same arithmetic.
 __global__ __launch_bounds__(256) void rbgs_rows_packed_kernel(Level L, const uint8_t *__restrict__ op, float dd,
-                                                               int parity, const int32_t *__restrict__ rows) {
-    const uint32_t row = (uint32_t)rows[xcd_item(blockIdx.x, gridDim.x)];
-    const int nx = L.nx, ny = L.ny, nz = L.nz, half = (nz + 1) >> 1;
+                                                               int parity, const int32_t *__restrict__ rows,
+                                                               int64_t nrows) {
+    uint32_t row;
+    int kk;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    if (!row_item(rows, nrows, (nz + 1) >> 1, row, kk)) return;
     const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
+    const int k = 2 * kk + ((parity + i + j) & 1);
+    if (k >= nz) return;
     const int64_t sx = (int64_t)ny * nz, sy = nz;
     float *__restrict__ p = L.p;
-    for (int kk = threadIdx.x; kk < half; kk += 256) {
-        const int k = 2 * kk + ((parity + i + j) & 1);
-        if (k >= nz) continue;
-        const int64_t idx = (int64_t)row * nz + k;
-        const bool inner = interior(i, j, k, nx, ny, nz);
-        const uint32_t o = op[idx];
-        float num = 0.0f;
-        if (inner) {
-            const uint32_t ox = op[idx - sx], oy = op[idx - sy], oz = op[idx - 1];
-            float t = bitw(ox, 1) * p[idx - sx];
-            t = fmaf(bitw(o, 1), p[idx + sx], t);
-            t = fmaf(bitw(o, 2), p[idx + sy], t);
-            t = fmaf(bitw(oy, 2), p[idx - sy], t);
-            t = fmaf(bitw(o, 4), p[idx + 1], t);
-            t = fmaf(bitw(oz, 4), p[idx - 1], t);
-            num = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), t);
-        }
-        if (!(o & 8)) {
-            p[idx] = 0.0f;
-            continue;
-        }
-        const float d = (float)(o >> 4);
-        if (inner && d > 0.0f) p[idx] = num / d;  // multigrid.cl:227
+    const int64_t idx = (int64_t)row * nz + k;
+    const bool inner = interior(i, j, k, nx, ny, nz);
+    const uint32_t o = op[idx];
+    float num = 0.0f;
+    if (inner) {
+        const uint32_t ox = op[idx - sx], oy = op[idx - sy], oz = op[idx - 1];
+        float t = bitw(ox, 1) * p[idx - sx];
+        t = fmaf(bitw(o, 1), p[idx + sx], t);
+        t = fmaf(bitw(o, 2), p[idx + sy], t);
+        t = fmaf(bitw(oy, 2), p[idx - sy], t);
+        t = fmaf(bitw(o, 4), p[idx + 1], t);
+        t = fmaf(bitw(oz, 4), p[idx - 1], t);
+        num = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), t);
     }
+    if (!(o & 8)) {
+        p[idx] = 0.0f;
+        return;
+    }
+    const float d = (float)(o >> 4);
+    if (inner && d > 0.0f) p[idx] = num / d;  // multigrid.cl:227
 }
 
 void launch_rbgs_rows_packed(const Level &L, const uint8_t *op, float delta, int parity, const int32_t *rows,
                              int64_t nrows, hipStream_t s) {
     if (nrows == 0) return;
-    hipLaunchKernelGGL(rbgs_rows_packed_kernel, dim3((unsigned)nrows), dim3(256), 0, s, L, op, delta * delta, parity,
-                       rows);
+    hipLaunchKernelGGL(rbgs_rows_packed_kernel, dim3(row_blocks(nrows, L.nz)), dim3(256), 0, s, L, op, delta * delta,
+                       parity, rows, nrows);
 }
 
 void launch_rbgs_rows(const Level &L, float delta, int parity, const int32_t *rows, int64_t nrows, hipStream_t s) {
     if (nrows == 0) return;
-    hipLaunchKernelGGL(rbgs_rows_kernel, dim3((unsigned)nrows), dim3(256), 0, s, L, delta * delta, parity, rows);
+    hipLaunchKernelGGL(rbgs_rows_kernel, dim3(row_blocks(nrows, L.nz)), dim3(256), 0, s, L, delta * delta, parity, rows,
+                       nrows);
 }
 
 __global__ __launch_bounds__(256) void coarse_operator_kernel(
@@ -691,26 +708,29 @@ __global__ __launch_bounds__(256) void rbgs_prolong_kernel(Level L, Level K, flo
 }
 
 __global__ __launch_bounds__(256) void rbgs_rows_prolong_kernel(Level L, Level K, float dd, int parity, int first,
-                                                                const int32_t *__restrict__ rows) {
-    const uint32_t row = (uint32_t)rows[xcd_item(blockIdx.x, gridDim.x)];
-    const int half = (L.nz + 1) >> 1;
-    for (int kk = threadIdx.x; kk < half; kk += 256) rbgs_cell_prolong(L, K, dd, parity, row, kk, first != 0);
+                                                                const int32_t *__restrict__ rows, int64_t nrows) {
+    uint32_t row;
+    int kk;
+    if (row_item(rows, nrows, (L.nz + 1) >> 1, row, kk)) rbgs_cell_prolong(L, K, dd, parity, row, kk, first != 0);
 }
 
 // the packed level-0 form (rbgs_rows_packed_kernel): a neighbour is fluid when
 // bit 3 of its operator byte is set
 __global__ __launch_bounds__(256) void rbgs_rows_packed_prolong_kernel(Level L, const uint8_t *__restrict__ op,
                                                                        Level K, float dd, int parity, int first,
-                                                                       const int32_t *__restrict__ rows) {
-    const uint32_t row = (uint32_t)rows[xcd_item(blockIdx.x, gridDim.x)];
-    const int nx = L.nx, ny = L.ny, nz = L.nz, half = (nz + 1) >> 1;
+                                                                       const int32_t *__restrict__ rows,
+                                                                       int64_t nrows) {
+    uint32_t row;
+    int kk;
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    if (!row_item(rows, nrows, (nz + 1) >> 1, row, kk)) return;
     const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
     const int64_t sx = (int64_t)ny * nz, sy = nz;
     const float *__restrict__ pc = K.p;
     float *__restrict__ p = L.p;
-    for (int kk = threadIdx.x; kk < half; kk += 256) {
+    {
         const int k = 2 * kk + ((parity + i + j) & 1);
-        if (k >= nz) continue;
+        if (k >= nz) return;
         const int64_t idx = (int64_t)row * nz + k;
         const bool inner = interior(i, j, k, nx, ny, nz);
         const uint32_t o = op[idx];
@@ -754,7 +774,7 @@ __global__ __launch_bounds__(256) void rbgs_rows_packed_prolong_kernel(Level L, 
         }
         if (!(o & 8)) {
             p[idx] = 0.0f;
-            continue;
+            return;
         }
         const float d = (float)(o >> 4);
         if (inner) p[idx] = d > 0.0f ? num / d : own;  // multigrid.cl:227 or the prolongation
@@ -767,11 +787,11 @@ void launch_rbgs_prolong(const Level &L, const Level &K, const uint8_t *op, floa
     if (rows) {
         if (nrows == 0) return;
         if (op)
-            hipLaunchKernelGGL(rbgs_rows_packed_prolong_kernel, dim3((unsigned)nrows), dim3(256), 0, s, L, op, K, dd,
-                               parity, first, rows);
+            hipLaunchKernelGGL(rbgs_rows_packed_prolong_kernel, dim3(row_blocks(nrows, L.nz)), dim3(256), 0, s, L, op,
+                               K, dd, parity, first, rows, nrows);
         else
-            hipLaunchKernelGGL(rbgs_rows_prolong_kernel, dim3((unsigned)nrows), dim3(256), 0, s, L, K, dd, parity,
-                               first, rows);
+            hipLaunchKernelGGL(rbgs_rows_prolong_kernel, dim3(row_blocks(nrows, L.nz)), dim3(256), 0, s, L, K, dd,
+                               parity, first, rows, nrows);
         return;
     }
     const int64_t work = (int64_t)L.nx * L.ny * ((L.nz + 1) >> 1);
