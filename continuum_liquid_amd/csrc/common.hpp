@@ -57,20 +57,39 @@ __device__ __forceinline__ float load_b(const float *__restrict__ b, const uint1
     const float v = kind == 1 ? bits_to_f16f(b16[i]) : bits_to_bf16f(b16[i]);
     return v * scale;
 }
+// load_b without branches: both storage forms are read (the 16-bit one
+// through the float array when there is none) and the value selected, so a
+// batch of such loads can all be in flight together.
+__device__ __forceinline__ float load_b_flat(const float *__restrict__ b, const uint16_t *__restrict__ b16, int kind,
+                                             float scale, int64_t i) {
+    const uint16_t *h = b16 ? b16 : reinterpret_cast<const uint16_t *>(b);
+    const float f = b[i];
+    const uint16_t raw = h[i];
+    const float v = (kind == 1 ? bits_to_f16f(raw) : bits_to_bf16f(raw)) * scale;
+    return kind == 0 ? f : v;
+}
+// The 16-bit storage of v (kind 1 fp16, 2 bf16): v / scale rounded to
+// nearest even (NaN kept quiet).
+__device__ __forceinline__ uint16_t encode_b16(int kind, float scale, float v) {
+    const float x = v * (1.0f / scale);
+    if (kind == 1) return __half_as_ushort(__float2half_rn(x));
+    const uint32_t u = __float_as_uint(x);
+    return (u & 0x7fffffffu) > 0x7f800000u ? (uint16_t)((u >> 16) | 0x40u)
+                                           : (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
 __device__ __forceinline__ void store_b(float *__restrict__ b, uint16_t *__restrict__ b16, int kind, float scale,
                                         int64_t i, float v) {
-    if (kind == 0) {
-        b[i] = v;
-        return;
-    }
-    const float x = v * (1.0f / scale);
-    if (kind == 1) {
-        b16[i] = __half_as_ushort(__float2half_rn(x));
-    } else {
-        uint32_t u = __float_as_uint(x);  // round to nearest even, NaN kept quiet
-        b16[i] = (u & 0x7fffffffu) > 0x7f800000u ? (uint16_t)((u >> 16) | 0x40u)
-                                                 : (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
-    }
+    if (kind == 0) b[i] = v;
+    else b16[i] = encode_b16(kind, scale, v);
+}
+
+// What load_b returns after store_b(v): the stored value, without the round
+// trip through memory.
+__device__ __forceinline__ float stored_b(int kind, float scale, float v) {
+    if (kind == 0) return v;
+    const uint16_t h = encode_b16(kind, scale, v);
+    return (kind == 1 ? bits_to_f16f(h) : bits_to_bf16f(h)) * scale;
 }
 
 // ---- device helpers (reference helper.cl:4-23) -----------------------------

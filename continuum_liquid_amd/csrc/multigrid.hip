@@ -56,8 +56,9 @@ __device__ __forceinline__ void rbgs_cell(const Level &L, float dd, int parity, 
 }
 
 // The same update with every load issued before the first branch on loaded
-// data (one memory round trip instead of three); the extra loads at air
-// cells are cheap on the coarse levels, whose sweeps are latency-bound.
+// data, b read without a branch on its storage form (one memory round trip
+// instead of three); the extra loads at air cells are cheap on the coarse
+// levels, whose sweeps are latency-bound.
 __device__ __forceinline__ void rbgs_cell_eager(const Level &L, float dd, int parity, uint32_t row, int kk) {
     const int nx = L.nx, ny = L.ny, nz = L.nz;
     const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
@@ -70,7 +71,7 @@ __device__ __forceinline__ void rbgs_cell_eager(const Level &L, float dd, int pa
     if (inner) {
         const int64_t sx = (int64_t)ny * nz, sy = nz;
         d = L.diag[idx];
-        num = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), wsum(L.p, L.wx, L.wy, L.wz, idx, sx, sy));
+        num = fmaf(-dd, load_b_flat(L.b, L.b16, L.bkind, L.bscale, idx), wsum(L.p, L.wx, L.wy, L.wz, idx, sx, sy));
     }
     if (m <= 0.0f) {
         L.p[idx] = 0.0f;
@@ -403,7 +404,8 @@ __device__ __forceinline__ float residual_at(const float *__restrict__ p, float 
 // Coarse cell ci's b and pressure after the restriction (p = 0) and, with
 // red, after the first (red) half-sweep of the coarse pre-smoothing too: on
 // p = 0 every neighbour term w * 0 is +0, so the red update is
-// fmaf(-dd, b, +0) / d with b as the smoother reads it back from storage.
+// fmaf(-dd, b, +0) / d with b as the smoother would read it back from
+// storage (stored_b).
 __device__ __forceinline__ void restrict_store(const Level &K, int64_t ci, bool kin, float rhs, float dd, bool red) {
     store_b(K.b, K.b16, K.bkind, K.bscale, ci, rhs);
     float pv = 0.0f;
@@ -412,7 +414,7 @@ __device__ __forceinline__ void restrict_store(const Level &K, int64_t ci, bool 
         coords(ci, K.ny, K.nz, a, b, c);
         if (((a + b + c) & 1) == 0) {
             const float m = K.mask[ci], d = K.diag[ci];
-            if (m > 0.0f && d > 0.0f) pv = fmaf(-dd, load_b(K.b, K.b16, K.bkind, K.bscale, ci), 0.0f) / d;
+            if (m > 0.0f && d > 0.0f) pv = fmaf(-dd, stored_b(K.bkind, K.bscale, rhs), 0.0f) / d;
         }
     }
     K.p[ci] = pv;
@@ -461,7 +463,7 @@ __device__ __forceinline__ void restrict_item_eager(const Level &F, const Level 
             if (in[q]) {
                 const int64_t idx = gidx(i, j, k, F.ny, F.nz);
                 m[q] = F.mask[idx];
-                res[q] = residual_at(F.p, load_b(F.b, F.b16, F.bkind, F.bscale, idx), F.diag, F.wx, F.wy, F.wz, idx,
+                res[q] = residual_at(F.p, load_b_flat(F.b, F.b16, F.bkind, F.bscale, idx), F.diag, F.wx, F.wy, F.wz, idx,
                                      sx, sy, inv_delta2);
             }
         }
@@ -515,7 +517,7 @@ __global__ __launch_bounds__(256) void restrict8_kernel(Level F, Level K, float 
         if (kin && interior(i, j, k, F.nx, F.ny, F.nz)) {
             const int64_t idx = gidx(i, j, k, F.ny, F.nz), sx = (int64_t)F.ny * F.nz, sy = F.nz;
             const float m = F.mask[idx];
-            const float r = residual_at(F.p, load_b(F.b, F.b16, F.bkind, F.bscale, idx), F.diag, F.wx, F.wy, F.wz,
+            const float r = residual_at(F.p, load_b_flat(F.b, F.b16, F.bkind, F.bscale, idx), F.diag, F.wx, F.wy, F.wz,
                                         idx, sx, sy, inv_delta2);
             res = m > 0.0f ? r : 0.0f;
         }
@@ -690,7 +692,7 @@ __device__ __forceinline__ void rbgs_cell_prolong(const Level &L, const Level &K
         } else {
             t = wsum(L.p, L.wx, L.wy, L.wz, idx, sx, sy);
         }
-        num = fmaf(-dd, load_b(L.b, L.b16, L.bkind, L.bscale, idx), t);
+        num = fmaf(-dd, load_b_flat(L.b, L.b16, L.bkind, L.bscale, idx), t);
     }
     if (m <= 0.0f) {
         L.p[idx] = 0.0f;
@@ -960,6 +962,29 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
     // 2 update (interior, fluid, d > 0), 3 fluid (interior, mask > 0)
     float w[6][6], bv[6], dv[6];
     int fl[6];
+    // every load unconditional (indices clamped into the arrays; the values
+    // of cells that do not use them are dropped) and issued for all six cells
+    // before any is used: one memory round trip instead of one per branch
+    float lp[6], lm[6], ld[6], lb[6], lw[6][6];
+    int li[6], lj[6], lk[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        const int cell = tid + s * kCoarseThreads;
+        const int cc = cell < BC ? cell : BC - 1;
+        coords(cc, bny, bnz, li[s], lj[s], lk[s]);
+        const bool inner = interior(li[s], lj[s], lk[s], bnx, bny, bnz);
+        const int cxm = inner ? cc - bsx : cc, cym = inner ? cc - bsy : cc, czm = inner ? cc - 1 : cc;
+        lp[s] = B.p[cc];
+        lm[s] = B.mask[cc];
+        ld[s] = B.diag[cc];
+        lb[s] = load_b_flat(B.b, B.b16, B.bkind, B.bscale, cc);
+        lw[s][0] = B.wx[cxm];
+        lw[s][1] = B.wx[cc];
+        lw[s][2] = B.wy[cc];
+        lw[s][3] = B.wy[cym];
+        lw[s][4] = B.wz[cc];
+        lw[s][5] = B.wz[czm];
+    }
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
         const int cell = tid + s * kCoarseThreads;
@@ -969,24 +994,17 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
 #pragma unroll
         for (int q = 0; q < 6; ++q) w[s][q] = 0.0f;
         if (cell < BC) {
-            pB[cell] = B.p[cell];
-            int i, j, k;
-            coords(cell, bny, bnz, i, j, k);
-            const float m = B.mask[cell];
-            fl[s] = (i + j + k) & 1;
-            if (m <= 0.0f) {
+            pB[cell] = lp[s];
+            fl[s] = (li[s] + lj[s] + lk[s]) & 1;
+            if (lm[s] <= 0.0f) {
                 fl[s] |= 2;
-            } else if (interior(i, j, k, bnx, bny, bnz)) {
+            } else if (interior(li[s], lj[s], lk[s], bnx, bny, bnz)) {
                 fl[s] |= 8;
-                dv[s] = B.diag[cell];
-                if (dv[s] > 0.0f) fl[s] |= 4;
-                bv[s] = load_b(B.b, B.b16, B.bkind, B.bscale, cell);
-                w[s][0] = B.wx[cell - bsx];
-                w[s][1] = B.wx[cell];
-                w[s][2] = B.wy[cell];
-                w[s][3] = B.wy[cell - bsy];
-                w[s][4] = B.wz[cell];
-                w[s][5] = B.wz[cell - 1];
+                dv[s] = ld[s];
+                if (ld[s] > 0.0f) fl[s] |= 4;
+                bv[s] = lb[s];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) w[s][q] = lw[s][q];
             }
         }
     }
@@ -994,19 +1012,25 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
     const int knx = K.nx, kny = K.ny, knz = K.nz, KC = (int)K.C;
     const int ksx = kny * knz, ksy = knz;
     int ka, kb, kc;
-    coords(tid, kny, knz, ka, kb, kc);
+    const int kt = tid < KC ? tid : KC - 1;
+    coords(kt, kny, knz, ka, kb, kc);
     const bool kin = tid < KC && interior(ka, kb, kc, knx, kny, knz);
     bool kact = false;
     T kw[6] = {T(0), T(0), T(0), T(0), T(0), T(0)}, kd = T(1);
-    if (kin && K.mask[tid] > 0.0f && K.diag[tid] > 0.0f) {
-        kact = true;
-        kw[0] = (T)K.wx[tid - ksx];
-        kw[1] = (T)K.wx[tid];
-        kw[2] = (T)K.wy[tid];
-        kw[3] = (T)K.wy[tid - ksy];
-        kw[4] = (T)K.wz[tid];
-        kw[5] = (T)K.wz[tid - 1];
-        kd = (T)K.diag[tid];
+    {
+        const int kxm = kin ? kt - ksx : kt, kym = kin ? kt - ksy : kt, kzm = kin ? kt - 1 : kt;
+        const float m = K.mask[kt], d = K.diag[kt];
+        const float w0 = K.wx[kxm], w1 = K.wx[kt], w2 = K.wy[kt], w3 = K.wy[kym], w4 = K.wz[kt], w5 = K.wz[kzm];
+        if (kin && m > 0.0f && d > 0.0f) {
+            kact = true;
+            kw[0] = (T)w0;
+            kw[1] = (T)w1;
+            kw[2] = (T)w2;
+            kw[3] = (T)w3;
+            kw[4] = (T)w4;
+            kw[5] = (T)w5;
+            kd = (T)d;
+        }
     }
     const int kpar = (ka + kb + kc) & 1;
     __syncthreads();
@@ -1065,7 +1089,7 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
             }
         }
         store_b(K.b, K.b16, K.bkind, K.bscale, tid, rhs);
-        kbv = (T)load_b(K.b, K.b16, K.bkind, K.bscale, tid);
+        kbv = (T)stored_b(K.bkind, K.bscale, rhs);
         pK[tid] = T(0);
     }
     __syncthreads();

@@ -33,7 +33,7 @@ namespace clq {
 // particles once and computes the tent weights of each particle for its own
 // cell (oz = 0), the cell above (the bin is that cell's oz = -1 neighbour) and
 // the cell below (oz = +1).  The neighbours' weights and velocities are handed
-// over with wave shuffles, and each cell adds them with the same fmaf into the
+// over with wavefront shifts, and each cell adds them with the same fmaf into the
 // partial sum of that neighbour index.  The three oz of a row feed three
 // different partials, and every partial still receives its bins in increasing
 // neighbour index and each bin's particles in sorted order, so the sums are
@@ -41,6 +41,15 @@ namespace clq {
 // component/bin pairs that can carry weight are evaluated (the oz = +1 bin
 // only for w).
 constexpr int kSeg = 62;
+
+// Lane l receives lane l - 1's value (lane_up) or lane l + 1's (lane_down)
+// through a DPP wavefront shift (wave_shr:1 / wave_shl:1): a VALU move
+// instead of an LDS permute.  Lanes 0 and 63 receive 0; the callers drop
+// those values.
+__device__ __forceinline__ int lane_up(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int lane_down(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_up(float x) { return __int_as_float(lane_up(__float_as_int(x))); }
+__device__ __forceinline__ float lane_down(float x) { return __int_as_float(lane_down(__float_as_int(x))); }
 
 template <int OX, int OY>
 __device__ __forceinline__ void rowscan_row(const float *__restrict__ pos, const float *__restrict__ vel, int start,
@@ -121,29 +130,29 @@ __device__ __forceinline__ void rowscan_row(const float *__restrict__ pos, const
             // for the cell below (ck - 1): this bin is its oz = +1 neighbour (w only)
             float sd = 0.0f;
             if constexpr (W) sd = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_dn + 1.0f)));
-            const int rf = __shfl_up(up_flags, 1);
-            const int rd = __shfl_down(has ? 1 : 0, 1);
+            const int rf = lane_up(up_flags);
+            const int rd = lane_down(has ? 1 : 0);
             if constexpr (U) {
-                const float wg = __shfl_up(su, 1), vv = __shfl_up(v0, 1);
+                const float wg = lane_up(su), vv = lane_up(v0);
                 if (lane > 0 && (rf & 1)) {
                     vs[L_UP][0] = fmaf(wg, vv, vs[L_UP][0]);
                     ws[L_UP][0] = ws[L_UP][0] + wg;
                 }
             }
             if constexpr (V) {
-                const float wg = __shfl_up(sv, 1), vv = __shfl_up(v1, 1);
+                const float wg = lane_up(sv), vv = lane_up(v1);
                 if (lane > 0 && (rf & 1)) {
                     vs[L_UP][1] = fmaf(wg, vv, vs[L_UP][1]);
                     ws[L_UP][1] = ws[L_UP][1] + wg;
                 }
             }
             if constexpr (W) {
-                const float wg = __shfl_up(sw, 1), vv = __shfl_up(v2, 1);
+                const float wg = lane_up(sw), vv = lane_up(v2);
                 if (lane > 0 && (rf & 1)) {
                     vs[L_UP][2] = fmaf(wg, vv, vs[L_UP][2]);
                     ws[L_UP][2] = ws[L_UP][2] + wg;
                 }
-                const float wd = __shfl_down(sd, 1), vd = __shfl_down(v2, 1);
+                const float wd = lane_down(sd), vd = lane_down(v2);
                 if (lane < 63 && rd) {
                     vs[L_DOWN][2] = fmaf(wd, vd, vs[L_DOWN][2]);
                     ws[L_DOWN][2] = ws[L_DOWN][2] + wd;
