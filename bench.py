@@ -56,13 +56,15 @@ def algorithmic_bytes(n_particles: int, cells: int, swept_cells: float) -> dict:
     }
 
 
-# HBM bytes per swept cell of one finest-level red-black sweep (both colour
-# launches), from rocprofv3 FETCH_SIZE/WRITE_SIZE on the 256^3 dam break +
-# sphere with the FCG solve (profiles/step_256cubed_fcg_pmc_traffic.md:
-# (20213.8 KiB x 2.0 + 16583.0 KiB) x 1024 per launch over 4,245,248 swept
-# cells, two launches per sweep).  Reported as `roofline.traffic` for that
-# workload only; the counters cannot be read from inside this process.
-MEASURED_SWEEP_BYTES_PER_CELL = {(3, "fcg"): 27.50}
+# HBM bytes measured with rocprofv3 FETCH_SIZE (x 2.0, calibrated on a plain
+# copy) and WRITE_SIZE on the 256^3 dam break + sphere with the FCG solve
+# (profiles/step_256cubed_fcg_pmc_traffic.md): per P2G launch
+# (1,467,492.1 KiB x 2.0 + 286,720.0 KiB) x 1024, and per swept cell of one
+# finest-level sweep (two colour launches of 58.3 MB over 4,245,504 cells).
+# Reported as `roofline.traffic` for that workload only; the counters cannot
+# be read from inside this process.
+MEASURED_TRAFFIC = {(3, "fcg"): {"p2g": lambda swept: 3_299_025_152,
+                                 "rbgs_fine_sweep": lambda swept: 27.47 * swept}}
 
 
 def solver_settings(name: str, tolerance: float) -> dict:
@@ -225,10 +227,8 @@ def main() -> int:
         }
     dominant = max(kernels, key=lambda k: ktimes[k][0])
     d = kernels[dominant]
-    traffic = None
-    per_cell = MEASURED_SWEEP_BYTES_PER_CELL.get((args.config, args.solver))
-    if dominant == "rbgs_fine_sweep" and per_cell is not None:
-        traffic = int(per_cell * float(np.mean(swept)))
+    measured = MEASURED_TRAFFIC.get((args.config, args.solver), {}).get(dominant)
+    traffic = int(measured(float(np.mean(swept)))) if measured else None
 
     result = {
         "metric": METRIC,
