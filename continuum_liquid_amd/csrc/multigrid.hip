@@ -483,61 +483,105 @@ __global__ __launch_bounds__(256) void restrict_kernel(Level F, Level K, float i
     restrict_item(F, K, inv_delta2, ci, dd, red != 0);
 }
 
-// Eight lanes per coarse cell, lane q computing child q's residual (0 where
-// the child is outside the interior or not fluid); lane 0 of the eight sums
-// them in restrict_item's order.  Adding the +0 of a skipped child leaves
-// the sum unchanged: it starts at +0 and a round-to-nearest sum is -0 only
-// when both terms are.  Eight times the lanes of restrict_kernel, so the
-// coarse levels' restrictions (a few thousand coarse cells) are no longer
-// bound by one lane's 128 loads: 5.1 and 4.1 us instead of 9.8 and 9.3 us
-// for the 66^3 -> 34^3 and 34^3 -> 18^3 restrictions on MI355X.
-__device__ __forceinline__ float sum_children(float res) {
-    const int base = (int)(threadIdx.x & 63u) & ~7;
-    float v[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = __shfl(res, base + q, 64);
+// G lanes per coarse cell, lane g computing the residuals of children
+// g * 8/G .. (g + 1) * 8/G - 1 (0 where the child is outside the interior or
+// not fluid); lane 0 of the G sums the eight in restrict_item's order.
+// Adding the +0 of a skipped child leaves the sum unchanged: it starts at +0
+// and a round-to-nearest sum is -0 only when both terms are.  More lanes per
+// coarse cell shorten each lane's chain of loads (128 for one lane) at the
+// cost of repeated index work: eight lanes measured 5.1 and 4.1 us instead of
+// 9.8 and 9.3 us for the 66^3 -> 34^3 and 34^3 -> 18^3 restrictions on MI355X.
+template <int G>
+__device__ __forceinline__ float sum_children(const float (&res)[8 / G]) {
+    const int base = (int)(threadIdx.x & 63u) & ~(G - 1);
     float rhs = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) rhs = rhs + v[q];
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int u = 0; u < 8 / G; ++u) rhs = rhs + __shfl(res[u], base + g, 64);
     return rhs;
 }
 
-__global__ __launch_bounds__(256) void restrict8_kernel(Level F, Level K, float inv_delta2, float dd, int red) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t ci = t >> 3;
-    const int q = (int)(t & 7);
-    const bool live = ci < K.C;  // the same for the eight lanes of a coarse cell
+// child q of coarse cell (a, b, c): its residual (residual_at) where it is an
+// interior fluid cell, else 0; from the packed operator when op != NULL
+__device__ __forceinline__ float child_residual(const Level &F, const uint8_t *__restrict__ op, float inv_delta2,
+                                                int a, int b, int c, int q) {
+    const int i = 1 + 2 * (a - 1) + (q >> 2), j = 1 + 2 * (b - 1) + ((q >> 1) & 1), k = 1 + 2 * (c - 1) + (q & 1);
+    if (!interior(i, j, k, F.nx, F.ny, F.nz)) return 0.0f;
+    const int64_t idx = gidx(i, j, k, F.ny, F.nz), sx = (int64_t)F.ny * F.nz, sy = F.nz;
+    if (op) {
+        const uint32_t o = op[idx], ox = op[idx - sx], oy = op[idx - sy], oz = op[idx - 1];
+        const float *__restrict__ p = F.p;
+        float t = bitw(ox, 1) * p[idx - sx];
+        t = fmaf(bitw(o, 1), p[idx + sx], t);
+        t = fmaf(bitw(o, 2), p[idx + sy], t);
+        t = fmaf(bitw(oy, 2), p[idx - sy], t);
+        t = fmaf(bitw(o, 4), p[idx + 1], t);
+        t = fmaf(bitw(oz, 4), p[idx - 1], t);
+        const float r = load_b(F.b, F.b16, F.bkind, F.bscale, idx) - fmaf(-(float)(o >> 4), p[idx], t) * inv_delta2;
+        return (o & 8) ? r : 0.0f;  // residual_at
+    }
+    const float m = F.mask[idx];
+    const float r = residual_at(F.p, load_b_flat(F.b, F.b16, F.bkind, F.bscale, idx), F.diag, F.wx, F.wy, F.wz, idx,
+                                sx, sy, inv_delta2);
+    return m > 0.0f ? r : 0.0f;
+}
+
+// coarse cell w of the work list: krows == NULL: all cells, else the active
+// coarse rows' cells
+template <int G>
+__global__ __launch_bounds__(256) void restrict_group_kernel(Level F, const uint8_t *__restrict__ op, Level K,
+                                                             float inv_delta2, const int32_t *__restrict__ krows,
+                                                             int64_t work, float dd, int red) {
+    const int64_t t = (int64_t)xcd_item(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    const int64_t w = t / G;
+    const int g = (int)(t % G);
+    const bool live = w < work;  // the same for the G lanes of a coarse cell
+    int64_t ci = 0;
     bool kin = false;
-    float res = 0.0f;
+    float res[8 / G];
+#pragma unroll
+    for (int u = 0; u < 8 / G; ++u) res[u] = 0.0f;
     if (live) {
+        if (krows) {
+            const int64_t ar = w / K.nz;
+            ci = (int64_t)krows[ar] * K.nz + (w - ar * K.nz);
+        } else {
+            ci = w;
+        }
         int a, b, c;
         coords(ci, K.ny, K.nz, a, b, c);
         kin = interior(a, b, c, K.nx, K.ny, K.nz);
-        const int i = 1 + 2 * (a - 1) + (q >> 2), j = 1 + 2 * (b - 1) + ((q >> 1) & 1), k = 1 + 2 * (c - 1) + (q & 1);
-        if (kin && interior(i, j, k, F.nx, F.ny, F.nz)) {
-            const int64_t idx = gidx(i, j, k, F.ny, F.nz), sx = (int64_t)F.ny * F.nz, sy = F.nz;
-            const float m = F.mask[idx];
-            const float r = residual_at(F.p, load_b_flat(F.b, F.b16, F.bkind, F.bscale, idx), F.diag, F.wx, F.wy, F.wz,
-                                        idx, sx, sy, inv_delta2);
-            res = m > 0.0f ? r : 0.0f;
-        }
+        if (kin)
+#pragma unroll
+            for (int u = 0; u < 8 / G; ++u) res[u] = child_residual(F, op, inv_delta2, a, b, c, g * (8 / G) + u);
     }
-    const float rhs = sum_children(res);  // every lane takes part in the shuffles
-    if (live && q == 0) {
-        restrict_store(K, ci, kin, kin ? rhs : 0.0f, dd, red != 0);
-    }
+    const float rhs = sum_children<G>(res);  // every lane takes part in the shuffles
+    if (live && g == 0) restrict_store(K, ci, kin, kin ? rhs : 0.0f, dd, red != 0);
 }
+
+template <int G>
+static void launch_restrict_group(const Level &F, const uint8_t *op, const Level &K, float inv_delta2,
+                                  const int32_t *krows, int64_t work, float dd, int red, hipStream_t s) {
+    if (work == 0) return;
+    hipLaunchKernelGGL(restrict_group_kernel<G>, dim3((unsigned)((G * work + 255) / 256)), dim3(256), 0, s, F, op, K,
+                       inv_delta2, krows, work, dd, red);
+}
+
+// lanes per coarse cell: levels below 2M fine cells are latency-bound
+constexpr int kRestrictLanesCoarse = 8, kRestrictLanesFine = 1, kRestrictLanesRows = 1;
 
 void launch_restrict_residual(const Level &F, const Level &K, float inv_delta2, float delta, int red, hipStream_t s) {
     const float dd = delta * delta;
     if (F.C >= kEagerCells) {
-        hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2, dd,
-                           red);
+        if (kRestrictLanesFine == 1)
+            hipLaunchKernelGGL(restrict_kernel, dim3((unsigned)((K.C + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2,
+                               dd, red);
+        else
+            launch_restrict_group<kRestrictLanesFine>(F, nullptr, K, inv_delta2, nullptr, K.C, dd, red, s);
         return;
     }
-    const int64_t lanes = 8 * K.C;
-    hipLaunchKernelGGL(restrict8_kernel, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, F, K, inv_delta2, dd,
-                       red);
+    launch_restrict_group<kRestrictLanesCoarse>(F, nullptr, K, inv_delta2, nullptr, K.C, dd, red, s);
 }
 
 // restrict_item_eager with the fine operator read from the packed bytes
@@ -595,8 +639,11 @@ void launch_restrict_rows(const Level &F, const uint8_t *op, const Level &K, flo
                           int64_t nkrows, float delta, int red, hipStream_t s) {
     const int64_t work = nkrows * K.nz;
     if (work == 0) return;
-    hipLaunchKernelGGL(restrict_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, op, K,
-                       inv_delta2, krows, work, delta * delta, red);
+    if (kRestrictLanesRows == 1)
+        hipLaunchKernelGGL(restrict_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, op, K,
+                           inv_delta2, krows, work, delta * delta, red);
+    else
+        launch_restrict_group<kRestrictLanesRows>(F, op, K, inv_delta2, krows, work, delta * delta, red, s);
 }
 
 __device__ __forceinline__ void prolong_item(const Level &F, const Level &K, int64_t fi) {
