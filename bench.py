@@ -8,7 +8,8 @@ residual target (FP64-checked), pressure correction, and the fused G2P/FLIP
 blend/advection/particle boundary pass.  The default workload is the 256^3
 dam break + falling sphere (dx = 1/256, 8 stratified-jitter FP32 particles
 per fluid cell, ~28.6M particles, dt = 1/240, FLIP 0.95, pressure solve by
-FP64 flexible CG preconditioned with one mixed-precision W(1, 1) cycle).
+FP64 flexible CG preconditioned with one mixed-precision W(1, 1) cycle whose
+coarse-grid corrections are weighted by 1.6).
 
 Prints one JSON line: ms/step (`value`), particle-steps/s, pressure-solve ms,
 a `roofline` object for the dominant kernel (algorithmic bytes per launch
@@ -76,7 +77,12 @@ def solver_settings(name: str, tolerance: float) -> dict:
     with the V-cycle below.  The FCG solves precondition with W(1, 1) cycles:
     at 256^3 they take 16.8 iterations per step against 14.2 with W(2, 2), and
     15.5 against 17.2 ms/step (sweep over smoothing 0-3, V/W level and
-    coarsest sweeps on MI355X)."""
+    coarsest sweeps on MI355X).  Their cycles weigh every coarse-grid
+    correction by 1.6 (over-correction of the piecewise-constant
+    prolongation): 9.0 FCG iterations per 256^3 step against 17.2 unweighted,
+    no breakdown over 20-100 steps of the 64^3, 128^3, 256^3 and 512x256x256
+    workloads; 1.7 is slower over 30 steps at 256^3, 1.8 breaks down in about
+    half of the solves, and with the V-cycle from level 1 even 1.6 does."""
     base = {"pre_smooth": 2, "post_smooth": 2, "coarse_extra": 64, "tolerance": tolerance}
     if name == "fp32":
         return {**base, "mode": "fp32", "max_cycles": 60}
@@ -84,10 +90,10 @@ def solver_settings(name: str, tolerance: float) -> dict:
         return {**base, "mode": "mixed", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16, "vcycle_from": 2}
     if name == "fcg":
         return {**base, "mode": "fcg", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16, "vcycle_from": 2,
-                "pre_smooth": 1, "post_smooth": 1}
+                "pre_smooth": 1, "post_smooth": 1, "coarse_weight": 1.6}
     if name == "learned":
         return {**base, "mode": "learned", "max_cycles": 60, "coarse_dtype": 0, "coarse_extra": 16, "vcycle_from": 2,
-                "pre_smooth": 1, "post_smooth": 1}
+                "pre_smooth": 1, "post_smooth": 1, "coarse_weight": 1.6}
     if name == "fixed":
         return {**base, "mode": "fixed", "max_cycles": 1, "post_smooth": 4}
     raise SystemExit(f"unknown solver {name}")
@@ -148,6 +154,8 @@ def main() -> int:
                     help="smoothing sweeps before and after the coarse-grid correction")
     ap.add_argument("--vcycle-from", type=int, default=None,
                     help="refined solves: W-cycle above this level, V-cycle from it down (0: W everywhere)")
+    ap.add_argument("--coarse-weight", type=float, default=None,
+                    help="weight of the cycle's coarse-grid corrections (1: the reference's plain sum)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -178,6 +186,8 @@ def main() -> int:
         settings["coarse_extra"] = args.coarse_extra
     if args.smooth is not None:
         settings["pre_smooth"], settings["post_smooth"] = args.smooth
+    if args.coarse_weight is not None:
+        settings["coarse_weight"] = args.coarse_weight
     params = host_solver.StepParams(dt=1.0 / 240.0, gravity=scene.gravity, flip_ratio=0.95, solver=settings)
     print(f"# {scene.name}: {scene.particle_count} particles, grid {scene.shape}, solver {args.solver}",
           file=sys.stderr, flush=True)
@@ -253,6 +263,7 @@ def main() -> int:
             "vcycle_from": settings.get("vcycle_from", 0),
             "smoothing": [settings["pre_smooth"], settings["post_smooth"]],
             "coarse_extra": settings["coarse_extra"],
+            "coarse_weight": settings.get("coarse_weight", 1.0),
         },
         "particle_steps_per_s": round(n * args.steps / elapsed, 1),
         "pressure_solve_ms": round(float(np.mean(solve_ms)), 3),
@@ -260,6 +271,9 @@ def main() -> int:
         "relative_residual_max": float(np.max(relres)),
         "swept_cells_per_sweep": int(np.mean(swept)),
         **({"w_cycle_fallback_per_step": round(float(np.mean(fallback)), 2)} if args.solver == "learned" else {}),
+        # FCG iterations run with weight 1 after the weighted cycle broke down
+        **({"unweighted_fallback_per_step": round(float(np.mean(fallback)), 2),
+            "solves_with_fallback": int(np.count_nonzero(fallback))} if args.solver == "fcg" else {}),
         "roofline": {
             "kernel": dominant,
             "bound": "hbm",

@@ -11,7 +11,9 @@ def pressure_solve(ctx, dt: float, rho: float, solver_settings: dict, observer=N
     Reference pressure_solve.py:10-85.  `solver_settings` selects the solve:
     mode "fixed" runs `max_cycles` W-cycles (the reference: `iterations`
     cycles of W(2, 4) with 64 extra coarsest sweeps); "fp32" iterates W-cycles
-    to `tolerance` relative residual.  Returns the solve statistics.
+    to `tolerance` relative residual.  `coarse_weight` scales every
+    coarse-grid correction of the cycle (1: the reference's plain sum).
+    Returns the solve statistics.
     """
     ctx.pressure_rhs(rho, dt)
     ctx.build_operator()
@@ -27,6 +29,7 @@ def pressure_solve(ctx, dt: float, rho: float, solver_settings: dict, observer=N
         tolerance=float(solver_settings.get("tolerance", 1e-6)),
         coarse_dtype=int(solver_settings.get("coarse_dtype", 0)),
         vcycle_from=int(solver_settings.get("vcycle_from", 0)),
+        coarse_weight=float(solver_settings.get("coarse_weight", 1.0)),
     )
     if observer is not None:
         observer(ctx, stats)

@@ -20,8 +20,10 @@ Settings added to the reference schema (all optional; defaults reproduce the
 reference): `settings.gravity` (default (0, 0, -9.81)), `settings.flip_ratio`
 (default 1.0, pure FLIP), and `settings.pressure_solver` with `mode`
 ("fixed" | "fp32" | "mixed" | "fcg" | "learned"), `tolerance`, `max_cycles`,
-`pre_smooth`, `post_smooth`, `coarse_extra` (defaults: fixed, `iterations`
-W(2, 4) cycles with 64 extra coarsest sweeps, reference pressure_solve.py:61-83).
+`pre_smooth`, `post_smooth`, `coarse_extra`, `coarse_dtype`, `vcycle_from` and
+`coarse_weight` (defaults: fixed, `iterations` W(2, 4) cycles with 64 extra
+coarsest sweeps, the W-cycle at every level and unweighted coarse-grid
+corrections, reference pressure_solve.py:61-83).
 """
 
 from __future__ import annotations

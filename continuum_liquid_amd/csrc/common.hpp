@@ -46,6 +46,10 @@ struct Level {
     uint16_t *b16 = nullptr;
     int bkind = 0;
     float bscale = 1.0f;
+    // weight of this level's pressure when it is prolongated into the finer
+    // level: p_fine = fmaf(corr, p[parent], p_fine), which for corr = 1 is
+    // the plain sum p_fine + p[parent] bit for bit
+    float corr = 1.0f;
 };
 
 // b of a level through its storage format (uniform branch per launch).

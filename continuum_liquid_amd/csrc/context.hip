@@ -270,7 +270,7 @@ int clq_create(int32_t nx, int32_t ny, int32_t nz, double dx, int64_t capacity, 
     c->norm_partials = (double *)q;
     if ((rc = c->alloc(&q, 16))) return bail(rc);
     c->norm_out = (double *)q;
-    if (hipHostMalloc((void **)&c->norm_host, 16, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&c->norm_host, 10 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(CLQ_ERR_NOMEM, "hipHostMalloc failed"));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(CLQ_ERR_DEVICE, "initial clear failed"));
     *out = c;
@@ -892,10 +892,25 @@ static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rn
     return CLQ_OK;
 }
 
+// the FCG scalars on the host: norm_host[0] = r.r, [1] = b.b, and the whole
+// block s[0..8) from norm_host[2] (alpha of the last update at [4])
 static int read_scalars(clq_ctx *c) {
-    HIP_TRY(hipMemcpyAsync(c->norm_host, c->fcg_s + 6, 16, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->norm_host + 2, c->fcg_s, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    c->norm_host[0] = c->norm_host[2 + 6];
+    c->norm_host[1] = c->norm_host[2 + 7];
     return CLQ_OK;
+}
+
+// A weighted cycle (coarse_weight != 1) that is not definite for the current
+// residual shows as a step length alpha = z.r / d.Ad <= 0: for a definite
+// preconditioner both dots carry the sign of the (negative definite)
+// operator.  The solve then goes on with weight 1, restarting the direction
+// recurrence; returns whether it did.
+static bool drop_weight_on_breakdown(clq_ctx *c) {
+    if (c->nlevels < 2 || c->lv[1].corr == 1.0f || c->norm_host[2 + 2] > 0.0) return false;
+    for (int l = 1; l < c->nlevels; ++l) c->lv[l].corr = 1.0f;
+    return true;
 }
 
 static int ensure_fcg_buffers(clq_ctx *c) {
@@ -941,18 +956,22 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
         s->cycles++;
         launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
                         c->stream);
-        for (int it = 0;; ++it) {
-            launch_fcg_direction(L0, OP0, c->p, c->dir[0], c->dir[1], c->q64, it == 0, ROWS0, c->fcg_partials,
+        bool first = true, plain = false;
+        for (;;) {
+            launch_fcg_direction(L0, OP0, c->p, c->dir[0], c->dir[1], c->q64, first, ROWS0, c->fcg_partials,
                                  c->fcg_s, c->stream);
             launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[0], c->q64, ROWS0, OP0, fold ? c->p : nullptr,
                               (float)c->dx, c->fcg_partials, c->fcg_s, c->stream);
             if (int rc = read_scalars(c)) return rc;
             r = sqrt(c->norm_host[0]);
             if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
+            first = drop_weight_on_breakdown(c);
+            plain = plain || first;
             if (int rc = precondition_wcycle(c, sp, r, fold)) return rc;
             s->cycles++;
-            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 0, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
-                            c->stream);
+            s->fallback_cycles += plain;
+            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], first, inv_h2, ROWS0, c->fcg_partials,
+                            c->fcg_s, c->stream);
         }
     }
     launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
@@ -1006,8 +1025,9 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
         if (int rc = precondition(r)) return rc;
         launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
                         c->stream);
-        for (int it = 0;; ++it) {
-            launch_fcg_direction(L0, OP0, c->p, c->dir[0], c->dir[1], c->q64, it == 0, ROWS0, c->fcg_partials,
+        bool first = true;
+        for (;;) {
+            launch_fcg_direction(L0, OP0, c->p, c->dir[0], c->dir[1], c->q64, first, ROWS0, c->fcg_partials,
                                  c->fcg_s, c->stream);
             launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[0], c->q64, ROWS0, OP0, nullptr, (float)c->dx,
                               c->fcg_partials, c->fcg_s, c->stream);
@@ -1015,10 +1035,14 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
             const double before = r;
             r = sqrt(c->norm_host[0]);
             if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
+            // the weighted W-cycle's breakdown guard (the network's steps keep
+            // the stall test only)
+            const bool restart = !use_net && drop_weight_on_breakdown(c);
             if (use_net && r > c->stall_ratio * before) use_net = false;
             if (int rc = precondition(r)) return rc;
-            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 0, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
-                            c->stream);
+            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], restart, inv_h2, ROWS0, c->fcg_partials,
+                            c->fcg_s, c->stream);
+            first = restart;
         }
     }
     launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
@@ -1043,7 +1067,13 @@ static int pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stat
 
 int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
+    const float w = sp->coarse_weight == 0.0f ? 1.0f : sp->coarse_weight;
+    if (!(w > 0.0f && w < 2.0f)) return fail(CLQ_ERR_INVALID, "coarse_weight %g outside (0, 2)", (double)w);
+    // every prolongation of the solve's cycles weighs the coarse pressure by
+    // w; the stage entry points (clq_prolongate, clq_w_cycle) keep weight 1
+    for (int l = 1; l < c->nlevels; ++l) c->lv[l].corr = w;
     const int rc = pressure_solve(c, sp, st);
+    for (int l = 1; l < c->nlevels; ++l) c->lv[l].corr = 1.0f;
     if (!rc && st) st->active_cells = (c->nlevels < 2 ? c->C : c->nrows[0] * c->nz);
     c->use_rows = false;
     return rc;

@@ -651,7 +651,7 @@ __device__ __forceinline__ void prolong_item(const Level &F, const Level &K, int
     coords(fi, F.ny, F.nz, i, j, k);
     if (!interior(i, j, k, F.nx, F.ny, F.nz) || F.mask[fi] <= 0.0f) return;
     const int a = min((i - 1) / 2 + 1, K.nx - 2), b = min((j - 1) / 2 + 1, K.ny - 2), c = min((k - 1) / 2 + 1, K.nz - 2);
-    F.p[fi] = F.p[fi] + K.p[gidx(a, b, c, K.ny, K.nz)];
+    F.p[fi] = fmaf(K.corr, K.p[gidx(a, b, c, K.ny, K.nz)], F.p[fi]);
 }
 
 // prolong_item with its three loads issued together
@@ -661,7 +661,7 @@ __device__ __forceinline__ void prolong_item_eager(const Level &F, const Level &
     if (!interior(i, j, k, F.nx, F.ny, F.nz)) return;
     const int a = min((i - 1) / 2 + 1, K.nx - 2), b = min((j - 1) / 2 + 1, K.ny - 2), c = min((k - 1) / 2 + 1, K.nz - 2);
     const float m = F.mask[fi], pf = F.p[fi], pc = K.p[gidx(a, b, c, K.ny, K.nz)];
-    if (m > 0.0f) F.p[fi] = pf + pc;
+    if (m > 0.0f) F.p[fi] = fmaf(K.corr, pc, pf);
 }
 
 __global__ __launch_bounds__(256) void prolongate_kernel(Level F, Level K) {
@@ -692,10 +692,10 @@ void launch_prolongate_rows(const Level &F, const Level &K, const int32_t *frows
 
 // ---- transfers folded into the neighbouring half-sweeps --------------------------------
 // A level visit runs restrict -> [coarse visit] -> prolong -> red sweep ->
-// black sweep.  The prolongation (prolong_item: p += p_coarse[parent] on the
+// black sweep.  The prolongation (prolong_item: p += w p_coarse[parent] on the
 // interior fluid cells) is folded into the two post-smoothing half-sweeps:
 // the red one reads its black neighbours as they would be after the
-// prolongation (p + p_coarse[parent] where the neighbour is fluid), and each
+// prolongation (p + w p_coarse[parent] where the neighbour is fluid), and each
 // colour adds the prolongation to its own fluid cells that the sweep does not
 // overwrite (d <= 0).  Every value is the one the separate kernels produce.
 __device__ __forceinline__ int64_t parent_of(const Level &K, int i, int j, int k) {
@@ -706,7 +706,7 @@ __device__ __forceinline__ int64_t parent_of(const Level &K, int i, int j, int k
 // the value of cell q = (i, j, k) after the prolongation
 __device__ __forceinline__ float prolonged(const Level &L, const Level &K, int64_t q, int i, int j, int k) {
     const float m = L.mask[q], pv = L.p[q], pc = K.p[parent_of(K, i, j, k)];
-    return m > 0.0f ? pv + pc : pv;
+    return m > 0.0f ? fmaf(K.corr, pc, pv) : pv;
 }
 
 // rbgs_cell_eager with the prolongation from K folded in (first: the red
@@ -724,7 +724,7 @@ __device__ __forceinline__ void rbgs_cell_prolong(const Level &L, const Level &K
     if (inner) {
         const int64_t sx = (int64_t)ny * nz, sy = nz;
         d = L.diag[idx];
-        own = L.p[idx] + K.p[parent_of(K, i, j, k)];  // prolong_item
+        own = fmaf(K.corr, K.p[parent_of(K, i, j, k)], L.p[idx]);  // prolong_item
         float t;
         if (first) {  // wsum over the prolonged neighbours
             const float vxm = prolonged(L, K, idx - sx, i - 1, j, k), vxp = prolonged(L, K, idx + sx, i + 1, j, k);
@@ -776,6 +776,7 @@ __global__ __launch_bounds__(256) void rbgs_rows_packed_prolong_kernel(Level L, 
     const int i = (int)(row / (uint32_t)ny), j = (int)(row - (uint32_t)i * (uint32_t)ny);
     const int64_t sx = (int64_t)ny * nz, sy = nz;
     const float *__restrict__ pc = K.p;
+    const float w = K.corr;
     float *__restrict__ p = L.p;
     {
         const int k = 2 * kk + ((parity + i + j) & 1);
@@ -786,7 +787,7 @@ __global__ __launch_bounds__(256) void rbgs_rows_packed_prolong_kernel(Level L, 
         float num = 0.0f, own = 0.0f;
         if (inner) {
             const uint32_t ox = op[idx - sx], oy = op[idx - sy], oz = op[idx - 1];
-            own = p[idx] + pc[parent_of(K, i, j, k)];  // prolong_item
+            own = fmaf(w, pc[parent_of(K, i, j, k)], p[idx]);  // prolong_item
             float vxm, vxp, vyp, vym, vzp, vzm;
             if (first) {
                 const uint32_t oxp = op[idx + sx], oyp = op[idx + sy], ozp = op[idx + 1];
@@ -799,12 +800,12 @@ __global__ __launch_bounds__(256) void rbgs_rows_packed_prolong_kernel(Level L, 
                 const float cxm = pc[parent_of(K, i - 1, j, k)], cxp = pc[parent_of(K, i + 1, j, k)];
                 const float cyp = pc[parent_of(K, i, j + 1, k)], cym = pc[parent_of(K, i, j - 1, k)];
                 const float czp = pc[parent_of(K, i, j, k + 1)], czm = pc[parent_of(K, i, j, k - 1)];
-                if (ox & 8) vxm = vxm + cxm;
-                if (oxp & 8) vxp = vxp + cxp;
-                if (oyp & 8) vyp = vyp + cyp;
-                if (oy & 8) vym = vym + cym;
-                if (ozp & 8) vzp = vzp + czp;
-                if (oz & 8) vzm = vzm + czm;
+                if (ox & 8) vxm = fmaf(w, cxm, vxm);
+                if (oxp & 8) vxp = fmaf(w, cxp, vxp);
+                if (oyp & 8) vyp = fmaf(w, cyp, vyp);
+                if (oy & 8) vym = fmaf(w, cym, vym);
+                if (ozp & 8) vzp = fmaf(w, czp, vzp);
+                if (oz & 8) vzm = fmaf(w, czm, vzm);
             } else {
                 vxm = p[idx - sx];
                 vxp = p[idx + sx];
@@ -1167,7 +1168,7 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
         int i, j, k;
         coords(c, bny, bnz, i, j, k);
         const int a = min((i - 1) / 2 + 1, knx - 2), b = min((j - 1) / 2 + 1, kny - 2), cc = min((k - 1) / 2 + 1, knz - 2);
-        pB[c] = pB[c] + (float)pK[gidx(a, b, cc, kny, knz)];
+        pB[c] = fmaf(K.corr, (float)pK[gidx(a, b, cc, kny, knz)], pB[c]);
     }
     __syncthreads();
     smooth_b(post);

@@ -22,7 +22,7 @@ import numpy as np
 
 PACKAGE_DIR = Path(__file__).resolve().parent
 HIP_LIBRARY = PACKAGE_DIR / "libcliquid_hip.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # enum clq_status
 OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NOT_CONVERGED = range(6)
@@ -66,6 +66,7 @@ class SolveParams(ctypes.Structure):
         ("max_cycles", c_int32),
         ("coarse_dtype", c_int32),
         ("vcycle_from", c_int32),
+        ("coarse_weight", ctypes.c_float),
         ("tolerance", c_double),
     ]
 
@@ -342,9 +343,11 @@ class Context:
         tolerance: float = 1e-6,
         coarse_dtype: int = 0,
         vcycle_from: int = 0,
+        coarse_weight: float = 1.0,
     ) -> dict:
-        params = SolveParams(int(mode), pre_smooth, post_smooth, coarse_extra, max_cycles, coarse_dtype, vcycle_from,
-                             tolerance)
+        params = SolveParams(mode=int(mode), pre_smooth=pre_smooth, post_smooth=post_smooth,
+                             coarse_extra=coarse_extra, max_cycles=max_cycles, coarse_dtype=coarse_dtype,
+                             vcycle_from=vcycle_from, coarse_weight=coarse_weight, tolerance=tolerance)
         stats = SolveStats()
         self._check(self.lib.clq_pressure_solve(self.handle, byref(params), byref(stats)), "clq_pressure_solve")
         return stats.as_dict()

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define CLQ_ABI_VERSION 2
+#define CLQ_ABI_VERSION 3
 
 enum clq_status {
     CLQ_OK = 0,
@@ -128,13 +128,22 @@ typedef struct clq_solve_params {
     int32_t vcycle_from;   /* 0: W-cycle at every level (multigrid.py:53-187);
                               k > 0: the coarse-grid correction runs twice
                               below levels < k and once from level k down  */
+    float coarse_weight;   /* weight w of every coarse-grid correction of the
+                              cycle, p += w * p_coarse[parent]; 0 selects 1,
+                              the reference's plain sum (multigrid.cl:182-201);
+                              otherwise 0 < w < 2.  Piecewise-constant
+                              prolongation under-corrects smooth errors, and
+                              w near 1.6 roughly halves the FCG iterations    */
     double tolerance;      /* relative residual target (modes 1-4)              */
 } clq_solve_params;
 
 typedef struct clq_solve_stats {
     int32_t cycles;            /* cycles (modes 0-2) or FCG iterations (3, 4)   */
     int32_t fallback_cycles;   /* mode 3: iterations preconditioned by the
-                                  W-cycle after the network stalled          */
+                                  W-cycle after the network stalled; mode 4:
+                                  iterations run with coarse_weight 1 after
+                                  the weighted cycle broke down (a step
+                                  length alpha <= 0)                         */
     double rhs_norm;           /* ||b||_2 (FP64 accumulation)                    */
     double initial_residual;   /* ||b - A p0||_2 before the solve               */
     double final_residual;     /* ||b - A p||_2 after the solve                 */

@@ -93,6 +93,7 @@ struct clq_ctx {
     uint8_t *obstacle;
     int nlevels;
     level_t lv[MAX_LEVELS];
+    float coarse_weight; /* weight of the cycles' prolongations: 1 outside a solve */
     double stage_ms[CLQ_STAGE_COUNT];
     /* mixed-precision solve state, allocated on first use */
     double *p64a, *p64b;
@@ -202,6 +203,7 @@ int clq_create(int32_t nx, int32_t ny, int32_t nz, double dx, int64_t capacity, 
     clq_ctx *c = calloc(1, sizeof *c);
     if (!c) return fail(CLQ_ERR_NOMEM, "context allocation failed");
     c->nx = nx; c->ny = ny; c->nz = nz; c->dx = dx;
+    c->coarse_weight = 1.0f;
     c->C = (long long)nx * ny * nz;
     if (alloc_particles(c, capacity)) { clq_destroy(c); return CLQ_ERR_NOMEM; }
     c->counts = xcalloc(c->C, 4);
@@ -742,9 +744,10 @@ int clq_restrict(clq_ctx *c, int32_t level) {
     return CLQ_OK;
 }
 
-/* prolongate (multigrid.cl:182-201). */
-int clq_prolongate(clq_ctx *c, int32_t level) {
-    if (!c || level < 0 || level + 1 >= c->nlevels) return fail(CLQ_ERR_INVALID, "bad prolongate level %d", level);
+/* prolongate (multigrid.cl:182-201): p += w * p_coarse[parent], w = 1 in the
+ * reference; fmaf(1, x, y) is the plain sum x + y, so weight 1 is the
+ * reference's arithmetic exactly. */
+static void prolongate_weighted(clq_ctx *c, int level, float w) {
     level_t *F = &c->lv[level];
     const level_t *K = &c->lv[level + 1];
 #pragma omp parallel for schedule(static)
@@ -756,8 +759,13 @@ int clq_prolongate(clq_ctx *c, int32_t level) {
         if (a > K->nx - 2) a = K->nx - 2;
         if (b > K->ny - 2) b = K->ny - 2;
         if (d > K->nz - 2) d = K->nz - 2;
-        F->p[fi] = F->p[fi] + K->p[gidx(a, b, d, K->ny, K->nz)];
+        F->p[fi] = fmaf(w, K->p[gidx(a, b, d, K->ny, K->nz)], F->p[fi]);
     }
+}
+
+int clq_prolongate(clq_ctx *c, int32_t level) {
+    if (!c || level < 0 || level + 1 >= c->nlevels) return fail(CLQ_ERR_INVALID, "bad prolongate level %d", level);
+    prolongate_weighted(c, level, 1.0f);
     return CLQ_OK;
 }
 
@@ -792,7 +800,7 @@ static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfr
     clq_restrict(c, level);
     w_cycle(c, level + 1, pre, post, extra, vfrom);
     if (vfrom <= 0 || level < vfrom) w_cycle(c, level + 1, pre, post, extra, vfrom);
-    clq_prolongate(c, level);
+    prolongate_weighted(c, level, c->coarse_weight);
     smooth_level(c, &c->lv[level], post);
 }
 
@@ -898,7 +906,7 @@ static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, i
     clq_restrict(c, level);
     w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
     if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
-    clq_prolongate(c, level);
+    prolongate_weighted(c, level, c->coarse_weight);
     smooth_level(c, &c->lv[level], post);
 }
 
@@ -1061,12 +1069,13 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
         precondition_wcycle(c, sp, rn);
         s->cycles++;
         double rz = 0.0, alpha = 0.0, beta = 0.0;
+        int first = 1, plain = 0;
 #pragma omp parallel for schedule(static) reduction(+ : rz)
         for (long long t = 0; t < c->C; ++t) rz += (double)c->p[t] * r[t];
-        for (int it = 0;; ++it) {
-            /* d_new = z + beta d_old (beta = 0 first), q = A d_new */
+        for (;;) {
+            /* d_new = z + beta d_old (d_new = z first), q = A d_new */
             for (long long t = 0; t < c->C; ++t)
-                dnew[t] = fluid_at(L, t) ? (it == 0 ? (double)c->p[t] : (double)c->p[t] + beta * dold[t]) : 0.0;
+                dnew[t] = fluid_at(L, t) ? (first ? (double)c->p[t] : (double)c->p[t] + beta * dold[t]) : 0.0;
             double dq = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : dq)
             for (long long t = 0; t < c->C; ++t) {
@@ -1086,15 +1095,23 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
             double *tmp = dold; dold = dnew; dnew = tmp;
             rn = sqrt(rr);
             if (rn <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(rn == rn)) break;
+            /* A weighted cycle that is not definite for this residual gives a
+             * step length alpha = z.r / d.Ad <= 0 (for a definite one both
+             * dots carry the sign of the negative definite operator): go on
+             * with weight 1 and restart the direction recurrence. */
+            first = c->coarse_weight != 1.0f && !(alpha > 0.0);
+            if (first) c->coarse_weight = 1.0f;
+            plain = plain || first;
             precondition_wcycle(c, sp, rn);
             s->cycles++;
+            s->fallback_cycles += plain;
             double zq = 0.0, zr = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : zq, zr)
             for (long long t = 0; t < c->C; ++t) {
                 zq += (double)c->p[t] * q[t];
                 zr += (double)c->p[t] * r[t];
             }
-            beta = rz != 0.0 ? -alpha * zq / rz : 0.0;
+            beta = (first || rz == 0.0) ? 0.0 : -alpha * zq / rz;
             rz = zr;
         }
     }
@@ -1122,8 +1139,7 @@ static long long active_cells(const clq_ctx *c) {
     return rows * L->nz;
 }
 
-int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
-    if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
+static int pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
     if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0 ||
         sp->vcycle_from < 0)
         return fail(CLQ_ERR_INVALID, "negative sweep, cycle count or level");
@@ -1170,6 +1186,18 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     c->stage_ms[CLQ_STAGE_PRESSURE_SOLVE] = s.solve_ms;
     if (st) *st = s;
     return CLQ_OK;
+}
+
+/* The solve with the cycles' coarse-grid corrections weighted by
+ * params->coarse_weight (0 selects 1, the reference's plain sum). */
+int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
+    if (!c || !sp) return fail(CLQ_ERR_INVALID, "NULL argument");
+    const float w = sp->coarse_weight == 0.0f ? 1.0f : sp->coarse_weight;
+    if (!(w > 0.0f && w < 2.0f)) return fail(CLQ_ERR_INVALID, "coarse_weight %g outside (0, 2)", (double)w);
+    c->coarse_weight = w;
+    const int rc = pressure_solve(c, sp, st);
+    c->coarse_weight = 1.0f;
+    return rc;
 }
 
 /* pressure_correction (pressure_solve.cl:26-47, gradient helper.cl:25-37). */

@@ -271,21 +271,23 @@ DEEP_SCENES = {"96^3": lambda: scenes.small_scene(96, seed=3),
                "96x80x72": lambda: scenes.dam_break("dam-break 96x80x72", (96, 80, 72), 1.0 / 96, (0.45, 0.55), [], 5)}
 
 
-@pytest.mark.parametrize("coarse_dtype,vcycle_from,smooth", [(0, 0, (2, 2)), (1, 1, (1, 1)), (0, 2, (1, 2)),
-                                                              (1, 0, (3, 1))],
-                         ids=["fp16-W22", "bf16-V1-11", "fp16-V2-12", "bf16-W31"])
+@pytest.mark.parametrize("coarse_dtype,vcycle_from,smooth,weight", [(0, 0, (2, 2), 1.0), (1, 1, (1, 1), 1.0),
+                                                                     (0, 2, (1, 2), 1.0), (1, 0, (3, 1), 1.0),
+                                                                     (0, 2, (1, 1), 1.6), (1, 0, (2, 1), 1.3)],
+                         ids=["fp16-W22", "bf16-V1-11", "fp16-V2-12", "bf16-W31", "fp16-V2-11-w1.6", "bf16-W21-w1.3"])
 @pytest.mark.parametrize("shape", list(DEEP_SCENES), ids=list(DEEP_SCENES))
-def test_deep_cycles_bit_exact(oracle_lib, hip_lib, shape, coarse_dtype, vcycle_from, smooth):
+def test_deep_cycles_bit_exact(oracle_lib, hip_lib, shape, coarse_dtype, vcycle_from, smooth, weight):
     """Hierarchies of five and four levels: the row kernels on levels 0-1, the
     dense colour kernels on level 2, the single-work-group bottom below, and
     the transfers folded into the neighbouring half-sweeps, against the
-    oracle's plain recursion, bit for bit."""
+    oracle's plain recursion, bit for bit; with weight != 1 every folded
+    prolongation weighs the coarse pressure as the oracle's does."""
     scene = DEEP_SCENES[shape]()
     o, h = pair(oracle_lib, hip_lib, scene)
     for c in (o, h):
         run_to_pressure(c, scene)
     kw = dict(pre_smooth=smooth[0], post_smooth=smooth[1], coarse_extra=16, max_cycles=3, tolerance=1e-12,
-              coarse_dtype=coarse_dtype, vcycle_from=vcycle_from)
+              coarse_dtype=coarse_dtype, vcycle_from=vcycle_from, coarse_weight=weight)
     so = o.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, **kw)
     sh = h.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, **kw)
     print(f"{shape} {kw}: cycles {sh['cycles']}, relres {sh['relative_residual']:.3g}", flush=True)
@@ -293,14 +295,16 @@ def test_deep_cycles_bit_exact(oracle_lib, hip_lib, shape, coarse_dtype, vcycle_
     assert_fields_equal(o, h, ("pressure", "pressure_fp64"))
 
 
-def test_deep_fixed_cycle_bit_exact(oracle_lib, hip_lib):
-    """The reference's fixed FP32 W(2, 4) cycle on a five-level hierarchy."""
+@pytest.mark.parametrize("weight", [1.0, 1.6])
+def test_deep_fixed_cycle_bit_exact(oracle_lib, hip_lib, weight):
+    """The reference's fixed FP32 W(2, 4) cycle on a five-level hierarchy
+    (weight 1), and the same cycle with weighted coarse corrections."""
     scene = DEEP_SCENES["96^3"]()
     o, h = pair(oracle_lib, hip_lib, scene)
     for c in (o, h):
         run_to_pressure(c, scene)
-    so = o.pressure_solve(runtime.SOLVE_FIXED_CYCLES, post_smooth=4, max_cycles=2)
-    sh = h.pressure_solve(runtime.SOLVE_FIXED_CYCLES, post_smooth=4, max_cycles=2)
+    so = o.pressure_solve(runtime.SOLVE_FIXED_CYCLES, post_smooth=4, max_cycles=2, coarse_weight=weight)
+    sh = h.pressure_solve(runtime.SOLVE_FIXED_CYCLES, post_smooth=4, max_cycles=2, coarse_weight=weight)
     assert so["cycles"] == sh["cycles"] == 2
     assert_fields_equal(o, h, ("pressure",))
 
@@ -321,14 +325,38 @@ def test_voxelise_matches_oracle(oracle_lib, hip_lib, scale):
         assert np.array_equal(ho.astype(bool), h.get_field("fluid_fraction") > 0)
 
 
-def test_fcg_matches_oracle(oracle_lib, hip_lib):
+def test_weighted_cycle_breakdown_matches_oracle(oracle_lib, hip_lib):
+    """Weight 1.9 with the V-cycle below level 1 turns the preconditioner
+    indefinite within a few iterations; the device detects the step length
+    alpha <= 0 as the oracle does, goes on with weight 1 and converges."""
+    scene = scenes.small_scene(48, seed=0)
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+    kw = dict(pre_smooth=1, post_smooth=1, coarse_extra=16, max_cycles=60, tolerance=1e-8, vcycle_from=1,
+              coarse_weight=1.9)
+    so = o.pressure_solve(runtime.SOLVE_FCG, **kw)
+    sh = h.pressure_solve(runtime.SOLVE_FCG, **kw)
+    print(f"oracle {so['cycles']}/{so['fallback_cycles']}, device {sh['cycles']}/{sh['fallback_cycles']}", flush=True)
+    assert sh["fallback_cycles"] > 0 and so["fallback_cycles"] > 0
+    assert abs(sh["cycles"] - so["cycles"]) <= 2
+    assert sh["relative_residual"] <= 1e-8
+    # stage entry points after a weighted solve keep the plain sum
+    for c in (o, h):
+        c.set_field("pressure", np.zeros(c.shape, np.float32))
+        c.w_cycle(2, 2, 16)
+    assert_fields_equal(o, h, ("pressure",))
+
+
+@pytest.mark.parametrize("weight", [1.0, 1.6])
+def test_fcg_matches_oracle(oracle_lib, hip_lib, weight):
     """FP64 FCG: the dot products are summed in different trees on the two
     sides, so iterates agree to rounding, not bit for bit."""
     scene = CASES[0]
     o, h = pair(oracle_lib, hip_lib, scene)
     for c in (o, h):
         run_to_pressure(c, scene)
-    kw = dict(post_smooth=2, coarse_extra=16, max_cycles=40, tolerance=1e-8)
+    kw = dict(post_smooth=2, coarse_extra=16, max_cycles=40, tolerance=1e-8, coarse_weight=weight)
     so = o.pressure_solve(runtime.SOLVE_FCG, **kw)
     sh = h.pressure_solve(runtime.SOLVE_FCG, **kw)
     assert abs(so["cycles"] - sh["cycles"]) <= 1

@@ -441,6 +441,96 @@ def test_mixed_refinement_reaches_1e8_against_fp64_direct_solve(oracle_lib, coar
     assert np.linalg.norm(p - exact) <= 1e-6 * np.linalg.norm(exact)
 
 
+def test_coarse_weight_scales_every_coarse_correction(oracle_lib):
+    """A fixed W-cycle with coarse_weight w on a three-level hierarchy equals
+    the reference's stages with each coarse pressure multiplied by w just
+    before its prolongation.  w = 0.5 makes that product exact, so the two
+    must agree bit for bit; a weight applied to the fine pressure, to the
+    sum, or to only one of the two prolongations would not."""
+    pre, post, extra, w = 2, 3, 16, 0.5
+    ref = _pressure_context(oracle_lib, 24, seed=4)
+    last = len(ref.level_shapes) - 1
+    assert last == 2
+
+    def cycle(level):
+        if level == last:
+            ref.coarse_smooth(pre + extra)
+            return
+        ref.smooth(level, pre)
+        ref.restrict(level)
+        cycle(level + 1)
+        cycle(level + 1)
+        ref.set_level_field(level + 1, "p", ref.get_level_field(level + 1, "p") * F32(w))
+        ref.prolongate(level)
+        ref.smooth(level, post)
+
+    cycle(0)
+    ref.neumann_bc()
+    ctx = _pressure_context(oracle_lib, 24, seed=4)
+    ctx.pressure_solve(runtime.SOLVE_FIXED_CYCLES, pre_smooth=pre, post_smooth=post, coarse_extra=extra,
+                       max_cycles=1, coarse_weight=w)
+    assert np.array_equal(ctx.get_field("pressure"), ref.get_field("pressure"))
+    # weight 0 selects 1, the plain sum of the reference
+    plain = [_pressure_context(oracle_lib, 24, seed=4) for _ in range(2)]
+    plain[0].pressure_solve(runtime.SOLVE_FIXED_CYCLES, max_cycles=1)
+    plain[1].pressure_solve(runtime.SOLVE_FIXED_CYCLES, max_cycles=1, coarse_weight=0.0)
+    assert np.array_equal(plain[0].get_field("pressure"), plain[1].get_field("pressure"))
+    assert not np.array_equal(plain[0].get_field("pressure"), ctx.get_field("pressure"))
+    # the stage entry points after a weighted solve keep the plain sum
+    for c in (ctx, plain[0]):
+        c.set_field("pressure", np.zeros(c.shape, F32))
+        c.w_cycle(pre, post, extra)
+    assert np.array_equal(ctx.get_field("pressure"), plain[0].get_field("pressure"))
+
+
+@pytest.mark.parametrize("w", [2.0, -0.5, float("nan")])
+def test_coarse_weight_outside_range_is_rejected(oracle_lib, w):
+    ctx = _pressure_context(oracle_lib, 12)
+    with pytest.raises(runtime.ClqError):
+        ctx.pressure_solve(runtime.SOLVE_FCG, max_cycles=4, coarse_weight=w)
+
+
+def test_coarse_weight_cuts_fcg_iterations(oracle_lib):
+    """Piecewise-constant prolongation under-corrects smooth error; weighting
+    the coarse corrections by 1.6 makes the FCG preconditioner stronger: the
+    same tolerance in fewer iterations, to the scipy FP64 solution."""
+    kw = dict(pre_smooth=1, post_smooth=1, coarse_extra=16, max_cycles=60, tolerance=1e-8, vcycle_from=2)
+    runs = {}
+    for w in (1.0, 1.6):
+        ctx = _pressure_context(oracle_lib, 48, seed=1)
+        runs[w] = (ctx, ctx.pressure_solve(runtime.SOLVE_FCG, coarse_weight=w, **kw))
+    ctx, s = runs[1.6]
+    assert s["relative_residual"] <= 1e-8
+    assert s["cycles"] <= 0.8 * runs[1.0][1]["cycles"]
+    mask = ctx.get_field("fluid_fraction")
+    h2 = np.float64(F32(ctx.dx)) ** 2
+    A, fluid, _ = operator_matrix(*[ctx.get_field(n) for n in ("diagonal", "weight_x", "weight_y", "weight_z")],
+                                  mask, h2)
+    b = ctx.get_field("rhs").astype(np.float64)[fluid]
+    p64 = ctx.get_field("pressure_fp64")[fluid]
+    assert np.linalg.norm(b - A @ p64) <= 1.0000001e-8 * np.linalg.norm(b)
+
+
+def test_weighted_cycle_breakdown_falls_back_to_weight_one(oracle_lib):
+    """With the V-cycle below level 1 a weight of 1.9 over-corrects the
+    inexact coarse solves until the preconditioner is indefinite: a step
+    length alpha <= 0 appears within a few iterations.  The solve then goes on
+    with weight 1 (restarting the directions) and still reaches the
+    tolerance; without that switch this case stalls far above it."""
+    kw = dict(pre_smooth=1, post_smooth=1, coarse_extra=16, max_cycles=60, tolerance=1e-8, vcycle_from=1)
+    ctx = _pressure_context(oracle_lib, 48, seed=0)
+    s = ctx.pressure_solve(runtime.SOLVE_FCG, coarse_weight=1.9, **kw)
+    assert 0 < s["fallback_cycles"] < s["cycles"] < 30
+    assert s["relative_residual"] <= 1e-8
+    mask = ctx.get_field("fluid_fraction")
+    h2 = np.float64(F32(ctx.dx)) ** 2
+    A, fluid, _ = operator_matrix(*[ctx.get_field(n) for n in ("diagonal", "weight_x", "weight_y", "weight_z")],
+                                  mask, h2)
+    b = ctx.get_field("rhs").astype(np.float64)[fluid]
+    p64 = ctx.get_field("pressure_fp64")[fluid]
+    assert np.linalg.norm(b - A @ p64) <= 1.0000001e-8 * np.linalg.norm(b)
+
+
 def test_fcg_reaches_tolerance_in_fewer_cycles(oracle_lib):
     """Mode 4 (FP64 flexible CG around the mixed-precision W-cycle) reaches the
     target with fewer W-cycles than refinement alone and matches the scipy
