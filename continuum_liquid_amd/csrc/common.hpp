@@ -121,11 +121,10 @@ __device__ __forceinline__ int floor_clamp(float t, int lo, int hi) {
     return (int)t;
 }
 // convert_int_rtz + clamp.
+// trunc(t) clamped to [lo, hi] (lo for NaN), without branches: fmaxf
+// returns the other operand for a NaN
 __device__ __forceinline__ int trunc_clamp(float t, int lo, int hi) {
-    t = truncf(t);
-    if (!(t >= (float)lo)) return lo;
-    if (t > (float)hi) return hi;
-    return (int)t;
+    return (int)fminf(fmaxf(truncf(t), (float)lo), (float)hi);
 }
 
 // ---- host launchers ----------------------------------------------------------

@@ -44,17 +44,32 @@ constexpr int kSeg = 62;
 
 // Lane l receives lane l - 1's value (lane_up) or lane l + 1's (lane_down)
 // through a DPP wavefront shift (wave_shr:1 / wave_shl:1): a VALU move
-// instead of an LDS permute.  Lanes 0 and 63 receive 0; the callers drop
-// those values.
-__device__ __forceinline__ int lane_up(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ int lane_down(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+// instead of an LDS permute.  Lanes 0 and 63 receive 0 (bound_ctrl).
+__device__ __forceinline__ int lane_up(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int lane_down(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, true); }
 __device__ __forceinline__ float lane_up(float x) { return __int_as_float(lane_up(__float_as_int(x))); }
 __device__ __forceinline__ float lane_down(float x) { return __int_as_float(lane_down(__float_as_int(x))); }
 
+// max(0, 1 - |p - c|), the reference's tent: 1 - |p - c| <= 1, so clamping
+// it to [0, 1] gives the same bits, and the clamp is the output modifier of
+// the subtraction (one instruction instead of a subtraction and a max)
+__device__ __forceinline__ float tent(float p, float c) {
+    return __builtin_amdgcn_fmed3f(1.0f - fabsf(p - c), 0.0f, 1.0f);
+}
+
+struct RowParticle {
+    float x, y, z, u, v, w;
+};
+
+// Branch-free row visit: a lane whose bin has no particle left (has false)
+// carries zero weights, so every accumulation runs on every lane.  Adding a
+// zero term changes no bit: the term is +-0 (finite velocities), and a sum
+// that starts at +0 and only ever rounds to nearest is never -0, so
+// s + (+-0) == s.
 template <int OX, int OY>
 __device__ __forceinline__ void rowscan_row(const float *__restrict__ pos, const float *__restrict__ vel, int start,
-                                            int cnt, int lane, int ci, int cj, int ck, int nx, int ny, int nz,
-                                            float inv_dx, float (&vs)[8][3], float (&ws)[8][3], int (&occ)[8]) {
+                                            int cnt, int ci, int cj, int ck, int nx, int ny, int nz, float inv_dx,
+                                            float (&vs)[8][3], float (&ws)[8][3], int (&occ)[8]) {
     constexpr int base = (OX + 1) * 9 + (OY + 1) * 3;
     constexpr int L_UP = (base + 0) & 7;    // this lane's bin is oz = -1 for cell ck + 1
     constexpr int L_OWN = (base + 1) & 7;   // oz = 0
@@ -65,101 +80,81 @@ __device__ __forceinline__ void rowscan_row(const float *__restrict__ pos, const
     } else {
         const float gx = (float)ci, gy = (float)cj;
         const float gz_own = (float)ck, gz_up = (float)(ck + 1), gz_dn = (float)(ck - 1);
-        // particle sidx + 1 is loaded while particle sidx is processed
-        float n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, m0 = 0.0f, m1 = 0.0f, m2 = 0.0f;
-        if (cnt > 0) {
-            const int64_t t = start;
-            n0 = pos[3 * t];
-            n1 = pos[3 * t + 1];
-            n2 = pos[3 * t + 2];
-            m0 = vel[3 * t];
-            m1 = vel[3 * t + 1];
-            m2 = vel[3 * t + 2];
-        }
-        for (int sidx = 0; __any(sidx < cnt); ++sidx) {
-            const bool has = sidx < cnt;
-            const float p0 = n0 * inv_dx, p1 = n1 * inv_dx, p2 = n2 * inv_dx;
-            const float v0 = m0, v1 = m1, v2 = m2;
-            if (sidx + 1 < cnt) {
-                const int64_t t = (int64_t)start + sidx + 1;
-                n0 = pos[3 * t];
-                n1 = pos[3 * t + 1];
-                n2 = pos[3 * t + 2];
-                m0 = vel[3 * t];
-                m1 = vel[3 * t + 1];
-                m2 = vel[3 * t + 2];
+        auto load = [&](int sidx, RowParticle &q) {
+            if (sidx < cnt) {
+                const int64_t t = (int64_t)start + sidx;
+                q.x = pos[3 * t];
+                q.y = pos[3 * t + 1];
+                q.z = pos[3 * t + 2];
+                q.u = vel[3 * t];
+                q.v = vel[3 * t + 1];
+                q.w = vel[3 * t + 2];
             }
+        };
+        auto visit = [&](const RowParticle &q, bool has) {
+            const float p0 = q.x * inv_dx, p1 = q.y * inv_dx, p2 = q.z * inv_dx;
             // x and y factors are shared by the three cells of the column
-            const float ax = fmaxf(0.0f, 1.0f - fabsf(p0 - (gx + 1.0f)));  // u face x
-            const float ay = fmaxf(0.0f, 1.0f - fabsf(p1 - (gy + 0.5f)));
-            const float bx = fmaxf(0.0f, 1.0f - fabsf(p0 - (gx + 0.5f)));  // v, w faces x
-            const float by = fmaxf(0.0f, 1.0f - fabsf(p1 - (gy + 1.0f)));  // v face y
-            const float uxy = ax * ay, vxy = bx * by, wxy = bx * ay;
-            int tocc = 0;
+            const float ay = tent(p1, gy + 0.5f);
+            const float uxy = has ? tent(p0, gx + 1.0f) * ay : 0.0f;             // u face
+            const float vxy = has ? tent(p0, gx + 0.5f) * tent(p1, gy + 1.0f) : 0.0f;  // v face
+            const float wxy = has ? tent(p0, gx + 0.5f) * ay : 0.0f;             // w face
+            int tocc = -1000;
             if constexpr (OCC) {
                 const int tx = trunc_clamp(p0, 1, nx - 2);
                 const int ty = trunc_clamp(p1, 1, ny - 2);
                 const int tz = trunc_clamp(p2, 1, nz - 2);
-                tocc = (tx == ci && ty == cj) ? tz : -1000;
+                if (has && tx == ci && ty == cj) tocc = tz;
             }
             // own cell, oz = 0
-            if (has) {
-                if constexpr (U) {
-                    const float wg = uxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_own + 0.5f)));
-                    vs[L_OWN][0] = fmaf(wg, v0, vs[L_OWN][0]);
-                    ws[L_OWN][0] = ws[L_OWN][0] + wg;
-                }
-                if constexpr (V) {
-                    const float wg = vxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_own + 0.5f)));
-                    vs[L_OWN][1] = fmaf(wg, v1, vs[L_OWN][1]);
-                    ws[L_OWN][1] = ws[L_OWN][1] + wg;
-                }
-                if constexpr (W) {
-                    const float wg = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_own + 1.0f)));
-                    vs[L_OWN][2] = fmaf(wg, v2, vs[L_OWN][2]);
-                    ws[L_OWN][2] = ws[L_OWN][2] + wg;
-                }
-                if constexpr (OCC) occ[L_OWN] += (tocc == ck);
-            }
-            // for the cell above (ck + 1): this bin is its oz = -1 neighbour
-            float su = 0.0f, sv = 0.0f, sw = 0.0f;
-            if constexpr (U) su = uxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_up + 0.5f)));
-            if constexpr (V) sv = vxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_up + 0.5f)));
-            if constexpr (W) sw = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_up + 1.0f)));
-            const int up_flags = has ? (1 | ((OCC && tocc == ck + 1) ? 2 : 0)) : 0;
-            // for the cell below (ck - 1): this bin is its oz = +1 neighbour (w only)
-            float sd = 0.0f;
-            if constexpr (W) sd = wxy * fmaxf(0.0f, 1.0f - fabsf(p2 - (gz_dn + 1.0f)));
-            const int rf = lane_up(up_flags);
-            const int rd = lane_down(has ? 1 : 0);
             if constexpr (U) {
-                const float wg = lane_up(su), vv = lane_up(v0);
-                if (lane > 0 && (rf & 1)) {
-                    vs[L_UP][0] = fmaf(wg, vv, vs[L_UP][0]);
-                    ws[L_UP][0] = ws[L_UP][0] + wg;
-                }
+                const float wg = uxy * tent(p2, gz_own + 0.5f);
+                vs[L_OWN][0] = fmaf(wg, q.u, vs[L_OWN][0]);
+                ws[L_OWN][0] = ws[L_OWN][0] + wg;
             }
             if constexpr (V) {
-                const float wg = lane_up(sv), vv = lane_up(v1);
-                if (lane > 0 && (rf & 1)) {
-                    vs[L_UP][1] = fmaf(wg, vv, vs[L_UP][1]);
-                    ws[L_UP][1] = ws[L_UP][1] + wg;
-                }
+                const float wg = vxy * tent(p2, gz_own + 0.5f);
+                vs[L_OWN][1] = fmaf(wg, q.v, vs[L_OWN][1]);
+                ws[L_OWN][1] = ws[L_OWN][1] + wg;
             }
             if constexpr (W) {
-                const float wg = lane_up(sw), vv = lane_up(v2);
-                if (lane > 0 && (rf & 1)) {
-                    vs[L_UP][2] = fmaf(wg, vv, vs[L_UP][2]);
-                    ws[L_UP][2] = ws[L_UP][2] + wg;
-                }
-                const float wd = lane_down(sd), vd = lane_down(v2);
-                if (lane < 63 && rd) {
-                    vs[L_DOWN][2] = fmaf(wd, vd, vs[L_DOWN][2]);
-                    ws[L_DOWN][2] = ws[L_DOWN][2] + wd;
-                }
+                const float wg = wxy * tent(p2, gz_own + 1.0f);
+                vs[L_OWN][2] = fmaf(wg, q.w, vs[L_OWN][2]);
+                ws[L_OWN][2] = ws[L_OWN][2] + wg;
             }
-            if constexpr (OCC)
-                if (lane > 0 && (rf & 2)) occ[L_UP] += 1;
+            if constexpr (OCC) occ[L_OWN] += (tocc == ck);
+            // the cell above (ck + 1) takes this bin as its oz = -1 neighbour
+            // from the lane below (lane 0's sums are dropped)
+            if constexpr (U) {
+                const float wg = lane_up(uxy * tent(p2, gz_up + 0.5f));
+                vs[L_UP][0] = fmaf(wg, lane_up(q.u), vs[L_UP][0]);
+                ws[L_UP][0] = ws[L_UP][0] + wg;
+            }
+            if constexpr (V) {
+                const float wg = lane_up(vxy * tent(p2, gz_up + 0.5f));
+                vs[L_UP][1] = fmaf(wg, lane_up(q.v), vs[L_UP][1]);
+                ws[L_UP][1] = ws[L_UP][1] + wg;
+            }
+            if constexpr (W) {
+                const float wg = lane_up(wxy * tent(p2, gz_up + 1.0f));
+                vs[L_UP][2] = fmaf(wg, lane_up(q.w), vs[L_UP][2]);
+                ws[L_UP][2] = ws[L_UP][2] + wg;
+                // the cell below (ck - 1) takes it as its oz = +1 neighbour (w only)
+                const float wd = lane_down(wxy * tent(p2, gz_dn + 1.0f));
+                vs[L_DOWN][2] = fmaf(wd, lane_down(q.w), vs[L_DOWN][2]);
+                ws[L_DOWN][2] = ws[L_DOWN][2] + wd;
+            }
+            if constexpr (OCC) occ[L_UP] += lane_up(tocc == ck + 1 ? 1 : 0);
+        };
+        // two particle buffers in turn: the next particle is loaded while the
+        // current one is processed, with no register copies between them
+        RowParticle a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, b = a;
+        load(0, a);
+        for (int sidx = 0; __any(sidx < cnt); sidx += 2) {
+            load(sidx + 1, b);
+            visit(a, sidx < cnt);
+            if (!__any(sidx + 1 < cnt)) break;
+            load(sidx + 2, a);
+            visit(b, sidx + 1 < cnt);
         }
     }
 }
@@ -188,24 +183,44 @@ __global__ __launch_bounds__(256) void p2g_rowscan_kernel(const float *__restric
             ws[l][a] = 0.0f;
         }
     }
-    auto row = [&](auto ox_c, auto oy_c) {
+    // the bin range of each neighbour row is fetched one row ahead, so the
+    // waves over empty rows do not wait for eight round trips in turn
+    auto fetch = [&](auto ox_c, auto oy_c, int &start, int &cnt) {
         constexpr int OX = decltype(ox_c)::value, OY = decltype(oy_c)::value;
         const int bx = ci + OX, by = cj + OY;
-        if (bx < 0 || bx >= nx - 1 || by < 0 || by >= ny - 1) return;  // uniform
-        int start = 0, cnt = 0;
-        if (ck >= 0 && ck < nz - 1) {
+        start = 0;
+        cnt = 0;
+        if (bx >= 0 && bx < nx - 1 && by >= 0 && by < ny - 1 && ck >= 0 && ck < nz - 1) {
             const int64_t bin = gidx(bx, by, ck, ny, nz);
             start = offsets[bin];
             cnt = counts[bin];
         }
-        rowscan_row<OX, OY>(pos, vel, start, cnt, lane, ci, cj, ck, nx, ny, nz, inv_dx, vs, ws, occ);
+    };
+    auto scan = [&](auto ox_c, auto oy_c, int start, int cnt) {
+        constexpr int OX = decltype(ox_c)::value, OY = decltype(oy_c)::value;
+        rowscan_row<OX, OY>(pos, vel, start, cnt, ci, cj, ck, nx, ny, nz, inv_dx, vs, ws, occ);
     };
     using m1 = std::integral_constant<int, -1>;
     using z0 = std::integral_constant<int, 0>;
     using p1 = std::integral_constant<int, 1>;
-    row(m1{}, m1{}); row(m1{}, z0{}); row(m1{}, p1{});
-    row(z0{}, m1{}); row(z0{}, z0{}); row(z0{}, p1{});
-    row(p1{}, m1{}); row(p1{}, z0{}); row(p1{}, p1{});
+    // rows in the reference's neighbour order; (+1, +1) carries no weight
+    int s0, c0, s1, c1;
+    fetch(m1{}, m1{}, s0, c0);
+    fetch(m1{}, z0{}, s1, c1);
+    scan(m1{}, m1{}, s0, c0);
+    fetch(m1{}, p1{}, s0, c0);
+    scan(m1{}, z0{}, s1, c1);
+    fetch(z0{}, m1{}, s1, c1);
+    scan(m1{}, p1{}, s0, c0);
+    fetch(z0{}, z0{}, s0, c0);
+    scan(z0{}, m1{}, s1, c1);
+    fetch(z0{}, p1{}, s1, c1);
+    scan(z0{}, z0{}, s0, c0);
+    fetch(p1{}, m1{}, s0, c0);
+    scan(z0{}, p1{}, s1, c1);
+    fetch(p1{}, z0{}, s1, c1);
+    scan(p1{}, m1{}, s0, c0);
+    scan(p1{}, z0{}, s1, c1);
     if (lane == 0 || lane == 63 || ck >= nz) return;
 #pragma unroll
     for (int off = 4; off > 0; off >>= 1) {
