@@ -39,7 +39,8 @@ namespace clq {
 // neighbour index and each bin's particles in sorted order, so the sums are
 // bit-identical to the reference's per-cell, per-partial order.  Only the
 // component/bin pairs that can carry weight are evaluated (the oz = +1 bin
-// only for w).
+// only for w).  Each row's positions are first staged in LDS (below), which
+// takes the scattered per-lane reads off the vector memory path.
 constexpr int kSeg = 62;
 
 // Lane l receives lane l - 1's value (lane_up) or lane l + 1's (lane_down)
@@ -57,9 +58,24 @@ __device__ __forceinline__ float tent(float p, float c) {
     return __builtin_amdgcn_fmed3f(1.0f - fabsf(p - c), 0.0f, 1.0f);
 }
 
+// a particle of the row scan: position in cells (x * inv_dx) and velocity
 struct RowParticle {
     float x, y, z, u, v, w;
 };
+
+// The bins of one neighbour row are contiguous in the sorted particle
+// arrays.  A wave stages the row's positions (in cells) in LDS with
+// coalesced loads, one particle per lane, before its lanes walk their bins:
+// the walk's position reads then come from LDS instead of 64 scattered
+// global reads per step (velocities stay global: staging them too costs more
+// occupancy than it saves).  Slot q + q / 8 spreads consecutive bins over
+// the LDS banks.  A row segment of 64 bins holds 512 particles at 8 per bin
+// and up to ~575 once the fluid compresses (256^3 dam break); kStageCap = 592
+// keeps five 4-wave blocks per CU within 160 KiB of LDS.  Rows with more
+// particles read global memory.
+constexpr int kStageCap = 592;
+constexpr int kStageSlots = kStageCap + kStageCap / 8 + 8;
+__device__ __forceinline__ int stage_slot(int q) { return q + (q >> 3); }
 
 // Branch-free row visit: a lane whose bin has no particle left (has false)
 // carries zero weights, so every accumulation runs on every lane.  Adding a
@@ -68,8 +84,9 @@ struct RowParticle {
 // s + (+-0) == s.
 template <int OX, int OY>
 __device__ __forceinline__ void rowscan_row(const float *__restrict__ pos, const float *__restrict__ vel, int start,
-                                            int cnt, int ci, int cj, int ck, int nx, int ny, int nz, float inv_dx,
-                                            float (&vs)[8][3], float (&ws)[8][3], int (&occ)[8]) {
+                                            int cnt, int lane, int fl, int ll, float (*stage)[kStageSlots], int ci,
+                                            int cj, int ck, int nx, int ny, int nz, float inv_dx, float (&vs)[8][3],
+                                            float (&ws)[8][3], int (&occ)[8]) {
     constexpr int base = (OX + 1) * 9 + (OY + 1) * 3;
     constexpr int L_UP = (base + 0) & 7;    // this lane's bin is oz = -1 for cell ck + 1
     constexpr int L_OWN = (base + 1) & 7;   // oz = 0
@@ -80,19 +97,21 @@ __device__ __forceinline__ void rowscan_row(const float *__restrict__ pos, const
     } else {
         const float gx = (float)ci, gy = (float)cj;
         const float gz_own = (float)ck, gz_up = (float)(ck + 1), gz_dn = (float)(ck - 1);
-        auto load = [&](int sidx, RowParticle &q) {
-            if (sidx < cnt) {
-                const int64_t t = (int64_t)start + sidx;
-                q.x = pos[3 * t];
-                q.y = pos[3 * t + 1];
-                q.z = pos[3 * t + 2];
-                q.u = vel[3 * t];
-                q.v = vel[3 * t + 1];
-                q.w = vel[3 * t + 2];
+        // the row's particles: lanes fl..ll hold its bins, in bin order
+        const int r0 = __builtin_amdgcn_readlane(start, fl), r1 = __builtin_amdgcn_readlane(start + cnt, ll);
+        const int n = r1 - r0;
+        const bool staged = n <= kStageCap;  // uniform
+        if (staged)
+            for (int q = lane; q < n; q += 64) {
+                const int64_t t = (int64_t)r0 + q;
+                const float x = pos[3 * t], y = pos[3 * t + 1], z = pos[3 * t + 2];
+                const int sl = stage_slot(q);
+                stage[0][sl] = x * inv_dx;
+                stage[1][sl] = y * inv_dx;
+                stage[2][sl] = z * inv_dx;
             }
-        };
         auto visit = [&](const RowParticle &q, bool has) {
-            const float p0 = q.x * inv_dx, p1 = q.y * inv_dx, p2 = q.z * inv_dx;
+            const float p0 = q.x, p1 = q.y, p2 = q.z;
             // x and y factors are shared by the three cells of the column
             const float ay = tent(p1, gy + 0.5f);
             const float uxy = has ? tent(p0, gx + 1.0f) * ay : 0.0f;             // u face
@@ -145,16 +164,46 @@ __device__ __forceinline__ void rowscan_row(const float *__restrict__ pos, const
             }
             if constexpr (OCC) occ[L_UP] += lane_up(tocc == ck + 1 ? 1 : 0);
         };
-        // two particle buffers in turn: the next particle is loaded while the
-        // current one is processed, with no register copies between them
-        RowParticle a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, b = a;
-        load(0, a);
-        for (int sidx = 0; __any(sidx < cnt); sidx += 2) {
-            load(sidx + 1, b);
-            visit(a, sidx < cnt);
-            if (!__any(sidx + 1 < cnt)) break;
-            load(sidx + 2, a);
-            visit(b, sidx + 1 < cnt);
+        if (staged) {
+            // positions from LDS when they are used; the velocities, from
+            // global memory, in two buffers in turn (the next one loads while
+            // the current one is used, with no register copies between them)
+            auto load = [&](int sidx, float (&vv)[3]) {
+                if (sidx < cnt) {
+                    const int64_t t = (int64_t)start + sidx;
+                    vv[0] = vel[3 * t];
+                    vv[1] = vel[3 * t + 1];
+                    vv[2] = vel[3 * t + 2];
+                }
+            };
+            auto step = [&](int sidx, const float (&vv)[3]) {
+                const int sl = stage_slot(min(start - r0 + sidx, kStageCap - 1));
+                const RowParticle q = {stage[0][sl], stage[1][sl], stage[2][sl], vv[0], vv[1], vv[2]};
+                visit(q, sidx < cnt);
+            };
+            float va[3] = {0.0f, 0.0f, 0.0f}, vb[3] = {0.0f, 0.0f, 0.0f};
+            load(0, va);
+            for (int sidx = 0; __any(sidx < cnt); sidx += 2) {
+                load(sidx + 1, vb);
+                step(sidx, va);
+                if (!__any(sidx + 1 < cnt)) break;
+                load(sidx + 2, va);
+                step(sidx + 1, vb);
+            }
+        } else {  // a crowded row (rare): global memory, one particle at a time
+            RowParticle a = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            for (int sidx = 0; __any(sidx < cnt); ++sidx) {
+                if (sidx < cnt) {
+                    const int64_t t = (int64_t)start + sidx;
+                    a.x = pos[3 * t] * inv_dx;
+                    a.y = pos[3 * t + 1] * inv_dx;
+                    a.z = pos[3 * t + 2] * inv_dx;
+                    a.u = vel[3 * t];
+                    a.v = vel[3 * t + 1];
+                    a.w = vel[3 * t + 2];
+                }
+                visit(a, sidx < cnt);
+            }
         }
     }
 }
@@ -171,7 +220,10 @@ __global__ __launch_bounds__(256) void p2g_rowscan_kernel(const float *__restric
     // fewer fetched bytes than per-XCD plane ranges or plane-ordered columns.
     const int ci = blockIdx.z, cj = blockIdx.y * 4 + wave;
     if (cj >= ny) return;  // whole wave
-    const int ck = blockIdx.x * kSeg - 1 + lane;  // this lane's bin; also its cell
+    const int ck0 = (int)blockIdx.x * kSeg - 1, ck = ck0 + lane;  // this lane's bin; also its cell
+    // lanes fl..ll hold the bins 0..nz-2 of the segment
+    const int fl = ck0 < 0 ? -ck0 : 0, ll = min(63, nz - 2 - ck0);
+    __shared__ float stage[4][3][kStageSlots];
     float vs[8][3], ws[8][3];
     int occ[8];
 #pragma unroll
@@ -198,7 +250,8 @@ __global__ __launch_bounds__(256) void p2g_rowscan_kernel(const float *__restric
     };
     auto scan = [&](auto ox_c, auto oy_c, int start, int cnt) {
         constexpr int OX = decltype(ox_c)::value, OY = decltype(oy_c)::value;
-        rowscan_row<OX, OY>(pos, vel, start, cnt, ci, cj, ck, nx, ny, nz, inv_dx, vs, ws, occ);
+        rowscan_row<OX, OY>(pos, vel, start, cnt, lane, fl, ll, stage[wave], ci, cj, ck, nx, ny, nz, inv_dx, vs, ws,
+                            occ);
     };
     using m1 = std::integral_constant<int, -1>;
     using z0 = std::integral_constant<int, 0>;

@@ -31,13 +31,15 @@ def _scene_cases():
     # uniform random particles over the whole box, ghosts and edges included
     pos, vel = random_particles(rng, (20, 18, 22), 1.0 / 20, 30000)
     cases.append(scene_from_arrays("random 20x18x22", (20, 18, 22), 1.0 / 20, pos, vel))
-    # adversarial: positions on bin boundaries x * inv_dx - 0.5 = integer, and a crowded bin
+    # adversarial: positions on bin boundaries x * inv_dx - 0.5 = integer, and a
+    # crowded bin whose bin row holds more particles than P2G stages in LDS (592),
+    # so that row takes the kernel's global-memory path
     n = 16
     dx = 1.0 / n
     grid = np.arange(1, n - 1, dtype=np.float64)
     bx = np.stack(np.meshgrid(grid, grid, grid, indexing="ij"), -1).reshape(-1, 3)
     edge = ((bx + 0.5) * dx).astype(np.float32)
-    crowd = np.full((400, 3), 0.5, np.float32) + rng.uniform(-0.2, 0.2, (400, 3)).astype(np.float32) * dx
+    crowd = np.full((700, 3), 0.5, np.float32) + rng.uniform(-0.2, 0.2, (700, 3)).astype(np.float32) * dx
     outside = rng.uniform(-0.5, 1.5, (500, 3)).astype(np.float32)
     pos = np.concatenate([edge, crowd, outside])
     vel = rng.normal(0, 1, pos.shape).astype(np.float32)
