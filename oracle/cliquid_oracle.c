@@ -897,15 +897,18 @@ static void coarse_smooth_f64(clq_ctx *c, int iterations) {
     free(p);
 }
 
-static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, int vfrom) {
+/* f64: the coarsest sweeps in FP64 (mode 2's refinement); otherwise the
+ * reference's FP32 coarse_smooth (the FCG preconditioner). */
+static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, int f64) {
     if (level == c->nlevels - 1) {
-        coarse_smooth_f64(c, pre + extra);
+        if (f64) coarse_smooth_f64(c, pre + extra);
+        else clq_coarse_smooth(c, pre + extra);
         return;
     }
     smooth_level(c, &c->lv[level], pre);
     clq_restrict(c, level);
-    w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
-    if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom);
+    w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64);
+    if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64);
     prolongate_weighted(c, level, c->coarse_weight);
     smooth_level(c, &c->lv[level], post);
 }
@@ -980,7 +983,7 @@ static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
         }
         L0->b = c->r32;
         memset(c->p, 0, (size_t)c->C * sizeof(float));
-        w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
+        w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, 1);
         L0->b = c->rhs;
         ir_update_residual(c, 0, c->p64a, c->p64b, &rr, &bb);
         double *t = c->p64a; c->p64a = c->p64b; c->p64b = t;
@@ -1017,6 +1020,8 @@ static inline int fluid_at(const level_t *L, long long q) {
     return interior(i, j, k, L->nx, L->ny, L->nz) && L->mask[q] > 0.0f;
 }
 
+/* z = M r: the mixed cycle on A e = r32 from e = 0, 16-bit coarse b and the
+ * FP32 coarse smoother on the coarsest level. */
 static void precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rnorm) {
     const int kind = sp->coarse_dtype == 1 ? 2 : 1;
     const int es = (int)lround(log2(rnorm / sqrt((double)c->C) + 1e-300));
@@ -1026,7 +1031,7 @@ static void precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double r
     }
     c->lv[0].b = c->r32;
     memset(c->p, 0, (size_t)c->C * sizeof(float));
-    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from);
+    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, 0);
     c->lv[0].b = c->rhs;
     for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
 }
