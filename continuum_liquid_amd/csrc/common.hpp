@@ -215,17 +215,15 @@ void launch_residual_norm(const Level &L, double inv_h2, double *partials, doubl
 int fcg_partials_count();
 void launch_fcg_init(const Level &L, const float *p32, const float *b, double inv_h2, double *x, double *r, float *r32,
                      double *partials, double *s, hipStream_t st);
-// w = A z and the dots z.q, z.r -> beta
-void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const double *qv, const double *r, double *w,
-                     int first, double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s,
-                     hipStream_t st);
-// d = z + beta d, q = w + beta q and d.q -> alpha
-void launch_fcg_direction(const Level &L, const uint8_t *op, const float *z, double *d, const double *w, double *qv,
-                          int first, const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st);
-// e != NULL (with op): also the next cycle's e after its first red half-sweep
-void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const double *d, const double *qv,
-                       const int32_t *rows, int64_t nrows, const uint8_t *op, float *e, float delta, double *partials,
-                       double *s, hipStream_t st);
+// the dots z.q, z.r, z.w (w = A z) -> beta, d.q of the new direction, alpha
+void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const double *qv, const double *r, int first,
+                     double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st);
+// d = z + beta d, q = A z + beta q, x += alpha d, r -= alpha q and r.r;
+// e != NULL (with op, e != z): also the next cycle's e after its first red
+// half-sweep
+void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double *x, double *r, float *r32, double *d,
+                       double *qv, int first, double inv_h2, const int32_t *rows, int64_t nrows, float *e, float delta,
+                       double *partials, double *s, hipStream_t st);
 
 // context.hip: sets the message returned by clq_last_error(), returns code
 int set_error(int code, const char *fmt, ...);

@@ -81,8 +81,10 @@ struct clq_ctx {
     float *r32 = nullptr;
     bool p64_valid = false;  // p64a holds the pressure of the last solve (modes 2-4)
     // flexible-CG state, allocated on first use
-    // FCG: dir[0] the direction d, dir[1] w = A z
-    double *r64 = nullptr, *dir[2] = {nullptr, nullptr}, *q64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
+    // FCG: the direction d, q = A d, and the second level-0 correction array
+    // (the update writes the next cycle's correction while it reads z)
+    double *r64 = nullptr, *dir = nullptr, *q64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
+    float *p_alt = nullptr;
     // active rows of levels 0 and 1 (rows holding an interior fluid cell),
     // used by the smoother, transfers and FCG vectors inside clq_pressure_solve
     int32_t *row_flags[2] = {nullptr, nullptr}, *rows[2] = {nullptr, nullptr}, *row_pos = nullptr;
@@ -889,8 +891,8 @@ static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rn
     L0.b = c->r32;
     // e = 0 where the cycle reads it with a non-zero weight: the active rows
     if (!red_done) {
-        if (c->use_rows) launch_zero_rows(c->p, c->rows[0], c->nrows[0], c->nz, c->stream);
-        else HIP_TRY(hipMemsetAsync(c->p, 0, (size_t)c->C * 4, c->stream));
+        if (c->use_rows) launch_zero_rows(L0.p, c->rows[0], c->nrows[0], c->nz, c->stream);
+        else HIP_TRY(hipMemsetAsync(L0.p, 0, (size_t)c->C * 4, c->stream));
     }
     w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, false, red_done);
     L0.b = c->rhs;
@@ -925,12 +927,12 @@ static int ensure_fcg_buffers(clq_ctx *c) {
         void *q;
         if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
         c->r64 = (double *)q;
-        for (int k = 0; k < 2; ++k) {
-            if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
-            c->dir[k] = (double *)q;
-        }
+        if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
+        c->dir = (double *)q;
         if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
         c->q64 = (double *)q;
+        if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 4)) return rc;
+        c->p_alt = (float *)q;
         if (int rc = alloc_zeroed(c, &q, 16 * 8)) return rc;
         c->fcg_s = (double *)q;
         if (int rc = alloc_zeroed(c, &q, (size_t)fcg_partials_count() * 8)) return rc;
@@ -958,27 +960,32 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     // the update makes the next cycle's first level-0 red half-sweep
     const bool fold = fold_transfers(c) && c->op0_ok && sp->pre_smooth > 0 && swept_level(c, 0);
     if (r > sp->tolerance * bn && sp->max_cycles > 0) {
+        // z: this iteration's preconditioned residual, the level-0
+        // correction array of the cycle that made it; with the fold, the
+        // update writes the next cycle's correction into the other array
+        float *z = c->p, *zn = c->p_alt;
         if (int rc = precondition_wcycle(c, sp, r)) return rc;
         s->cycles++;
-        launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
-                        c->stream);
+        launch_fcg_dots(L0, OP0, z, c->q64, c->r64, 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         bool first = true, plain = false;
         for (;;) {
-            launch_fcg_direction(L0, OP0, c->p, c->dir[0], c->dir[1], c->q64, first, ROWS0, c->fcg_partials,
-                                 c->fcg_s, c->stream);
-            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[0], c->q64, ROWS0, OP0, fold ? c->p : nullptr,
-                              (float)c->dx, c->fcg_partials, c->fcg_s, c->stream);
+            launch_fcg_update(L0, OP0, z, c->p64a, c->r64, c->r32, c->dir, c->q64, first, inv_h2, ROWS0,
+                              fold ? zn : nullptr, (float)c->dx, c->fcg_partials, c->fcg_s, c->stream);
             if (int rc = read_scalars(c)) return rc;
             r = sqrt(c->norm_host[0]);
             if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
             first = drop_weight_on_breakdown(c);
             plain = plain || first;
+            if (fold) {
+                std::swap(z, zn);
+                c->lv[0].p = z;
+            }
             if (int rc = precondition_wcycle(c, sp, r, fold)) return rc;
             s->cycles++;
             s->fallback_cycles += plain;
-            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], first, inv_h2, ROWS0, c->fcg_partials,
-                            c->fcg_s, c->stream);
+            launch_fcg_dots(L0, OP0, z, c->q64, c->r64, first, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         }
+        c->lv[0].p = c->p;
     }
     launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
     s->final_residual = r;
@@ -1029,14 +1036,11 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
     };
     if (r > sp->tolerance * bn && sp->max_cycles > 0) {
         if (int rc = precondition(r)) return rc;
-        launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
-                        c->stream);
+        launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         bool first = true;
         for (;;) {
-            launch_fcg_direction(L0, OP0, c->p, c->dir[0], c->dir[1], c->q64, first, ROWS0, c->fcg_partials,
-                                 c->fcg_s, c->stream);
-            launch_fcg_update(L0, c->p64a, c->r64, c->r32, c->dir[0], c->q64, ROWS0, OP0, nullptr, (float)c->dx,
-                              c->fcg_partials, c->fcg_s, c->stream);
+            launch_fcg_update(L0, OP0, c->p, c->p64a, c->r64, c->r32, c->dir, c->q64, first, inv_h2, ROWS0, nullptr,
+                              (float)c->dx, c->fcg_partials, c->fcg_s, c->stream);
             if (int rc = read_scalars(c)) return rc;
             const double before = r;
             r = sqrt(c->norm_host[0]);
@@ -1046,8 +1050,8 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
             const bool restart = !use_net && drop_weight_on_breakdown(c);
             if (use_net && r > c->stall_ratio * before) use_net = false;
             if (int rc = precondition(r)) return rc;
-            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, c->dir[1], restart, inv_h2, ROWS0, c->fcg_partials,
-                            c->fcg_s, c->stream);
+            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, restart, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
+                            c->stream);
             first = restart;
         }
     }

@@ -4,15 +4,21 @@
 // runs a fixed number of stationary W-cycles (pressure_solve.py:61-83).
 //
 // Vectors over all C cells, FP64: x (iterate), r (residual), d (direction),
-// q = A d, w = A z.  The preconditioned residual z arrives as FP32 in the
-// level-0 correction array e.  The matrix-vector product is taken of z, right
-// after the preconditioner and before beta is known, so the direction update
-// streams: d = z + beta d, q = A z + beta q (= A d).  Scalars live in device
-// memory so that alpha and beta never round-trip through the host:
-//   s[0] rz = r.z   s[1] d.q   s[2] alpha   s[3] z.q   s[4] z.r   s[5] beta
+// q = A d.  The preconditioned residual z arrives as FP32 in the level-0
+// correction array.  Two kernels per iteration after the preconditioner:
+//   dots:   w = A z (not stored); z.q, z.r, z.w
+//           -> beta = -(z.q) / (d.q)_old, and the new direction's
+//              d.q = z.w + 2 beta z.q + beta^2 (d.q)_old (A symmetric, so
+//              d_old . A z = z . q_old), alpha = z.r / d.q
+//   update: w = A z again, d = z + beta d, q = w + beta q, x += alpha d,
+//           r -= alpha q; r.r
+// (the direction needs no reduction of its own, so it streams with the
+// update).  Scalars live in device memory, so alpha and beta never
+// round-trip through the host:
+//   s[0] rz = z.r   s[1] d.q   s[2] alpha   s[3] z.q   s[4] z.w   s[5] beta
 //   s[6] r.r        s[7] b.b
-// beta = -alpha (z_{k+1} . q_k) / (z_k . r_k), the flexible (Polak-Ribiere)
-// form written with q_k = -(r_{k+1} - r_k) / alpha, so r_k need not be kept.
+// beta = -(z_{k+1} . q_k) / (d_k . q_k), the flexible (Polak-Ribiere) form
+// written with q_k = -(r_{k+1} - r_k) / alpha_k, so r_k need not be kept.
 // The initialisation writes all cells (0 outside the interior fluid cells);
 // the per-iteration kernels visit the active rows only (rows holding an
 // interior fluid cell, or all cells when no row list is given): outside them
@@ -47,13 +53,6 @@ __device__ __forceinline__ double apply_a(const Level &L, int64_t q, F v, double
     if (w4 != 0.0f) s = s + (double)w4 * v4;
     if (w5 != 0.0f) s = s + (double)w5 * v5;
     return (s - d * vc) * inv_h2;
-}
-
-// cell of work item t: row-major over the listed rows, or t itself
-__device__ __forceinline__ int64_t cell_of(const int32_t *__restrict__ rows, int nz, int64_t t) {
-    if (!rows) return t;
-    const int64_t ar = t / nz;
-    return (int64_t)rows[ar] * nz + (t - ar * nz);
 }
 
 // apply_a with the operator decoded from the packed level-0 bytes
@@ -112,110 +111,130 @@ __global__ __launch_bounds__(kT) void fcg_init_kernel(Level L, const float *__re
     block_sums<2>(acc, partials);
 }
 
-// d = z + beta d, q = w + beta q on the interior fluid cells (0 elsewhere);
-// partial d.q.  first: beta = 0.
-__global__ __launch_bounds__(kT) void fcg_direction_kernel(Level L, const uint8_t *__restrict__ op,
-                                                          const float *__restrict__ z, double *__restrict__ d,
-                                                          const double *__restrict__ w, double *__restrict__ qv,
-                                                          const double *__restrict__ s, int first,
-                                                          const int32_t *__restrict__ rows, int64_t work,
-                                                          double *__restrict__ partials) {
-    const double beta = first ? 0.0 : s[5];
-    double acc[1] = {0.0};
-    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
-        const int64_t q = cell_of(rows, L.nz, t);
-        int i, j, k;
-        coords(q, L.ny, L.nz, i, j, k);
-        const bool fluid = interior(i, j, k, L.nx, L.ny, L.nz) && (op ? (op[q] & 8) != 0 : L.mask[q] > 0.0f);
-        const double zv = (double)z[q], dold = d[q], wv = w[q], qold = qv[q];
-        double dv = 0.0, qn = 0.0;
-        if (fluid) {
-            dv = first ? zv : zv + beta * dold;
-            qn = first ? wv : wv + beta * qold;
-        }
-        d[q] = dv;
-        qv[q] = qn;
-        acc[0] += dv * qn;
+// A z at cell q = (i, j, k) (0 off the interior fluid cells), from the
+// packed level-0 operator (PACKED) or the float arrays.  Every load is issued
+// before any test: off the interior the neighbour indices fall back to q
+// itself (in bounds), and the result is dropped.
+template <bool PACKED>
+__device__ __forceinline__ double apply_a_z(const Level &L, const uint8_t *__restrict__ op,
+                                            const float *__restrict__ z, int64_t q, bool inner, double inv_h2,
+                                            bool &fluid, double &dg) {
+    const int64_t sx = inner ? (int64_t)L.ny * L.nz : 0, sy = inner ? L.nz : 0, sz = inner ? 1 : 0;
+    const double z0 = z[q + sx], z1 = z[q - sx], z2 = z[q + sy], z3 = z[q - sy], z4 = z[q + sz], z5 = z[q - sz];
+    const double zc = z[q];
+    double s = 0.0;
+    bool fl;
+    if constexpr (PACKED) {
+        const uint32_t o = op[q], ox = op[q - sx], oy = op[q - sy], oz = op[q - sz];
+        fl = (o & 8) != 0;
+        dg = (double)(o >> 4);
+        if (o & 1) s = s + z0;
+        if (ox & 1) s = s + z1;
+        if (o & 2) s = s + z2;
+        if (oy & 2) s = s + z3;
+        if (o & 4) s = s + z4;
+        if (oz & 4) s = s + z5;
+    } else {
+        const float w0 = L.wx[q], w1 = L.wx[q - sx], w2 = L.wy[q], w3 = L.wy[q - sy], w4 = L.wz[q], w5 = L.wz[q - sz];
+        fl = L.mask[q] > 0.0f;
+        dg = (double)L.diag[q];
+        if (w0 != 0.0f) s = s + (double)w0 * z0;
+        if (w1 != 0.0f) s = s + (double)w1 * z1;
+        if (w2 != 0.0f) s = s + (double)w2 * z2;
+        if (w3 != 0.0f) s = s + (double)w3 * z3;
+        if (w4 != 0.0f) s = s + (double)w4 * z4;
+        if (w5 != 0.0f) s = s + (double)w5 * z5;
     }
-    block_sums<1>(acc, partials);
+    fluid = inner && fl;
+    return fluid ? (s - dg * zc) * inv_h2 : 0.0;
 }
 
-// x += alpha d, r -= alpha q, r32 = float(r); partial r.r.  With e (the
-// packed level-0 operator op given), also the correction array of the next
-// preconditioner cycle after its first red half-sweep from e = 0 on r32:
-// e = fmaf(-dd, r32, +0) / diag on the red interior fluid cells with diag > 0
-// (every neighbour term w * 0 is +0), 0 elsewhere, as zero_rows and
-// rbgs_rows_packed_kernel would leave it.
-__global__ __launch_bounds__(kT) void fcg_update_kernel(Level L, double *__restrict__ x, double *__restrict__ r,
-                                                       float *__restrict__ r32, const double *__restrict__ d,
-                                                       const double *__restrict__ qv, const double *__restrict__ s,
-                                                       const int32_t *__restrict__ rows, int64_t work,
-                                                       const uint8_t *__restrict__ op, float *__restrict__ e, float dd,
-                                                       double *__restrict__ partials) {
-    const double alpha = s[2];
+// The per-iteration kernels walk rows: block b takes a contiguous run of the
+// work's rows (the listed active rows, or all nx * ny rows), runs laid out
+// per XCD so the stencil's neighbour rows stay in that XCD's L2; a row's index
+// and (i, j) are block-uniform, and thread t takes the cells k = t, t + kT, ...
+template <typename F>
+__device__ __forceinline__ void for_rows(const Level &L, const int32_t *__restrict__ rows, int64_t nrows, F cell) {
+    const int64_t per = (nrows + gridDim.x - 1) / gridDim.x;
+    const int64_t a0 = (int64_t)xcd_item(blockIdx.x, gridDim.x) * per, a1 = min(a0 + per, nrows);
+    for (int64_t a = a0; a < a1; ++a) {
+        const int64_t row = rows ? rows[a] : a;
+        const int i = (int)(row / L.ny), j = (int)(row - (int64_t)i * L.ny);
+        const bool inner_row = i >= 1 && i <= L.nx - 2 && j >= 1 && j <= L.ny - 2;
+        for (int k = threadIdx.x; k < L.nz; k += kT) cell(row * L.nz + k, inner_row && k >= 1 && k <= L.nz - 2, i, j, k);
+    }
+}
+
+// w = A z (not stored); partial z.q, z.r, z.w
+template <bool PACKED>
+__global__ __launch_bounds__(kT) void fcg_dots_kernel(Level L, const uint8_t *__restrict__ op,
+                                                     const float *__restrict__ z, const double *__restrict__ qv,
+                                                     const double *__restrict__ r, double inv_h2,
+                                                     const int32_t *__restrict__ rows, int64_t nrows,
+                                                     double *__restrict__ partials) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    for_rows(L, rows, nrows, [&](int64_t q, bool inner, int, int, int) {
+        const double qo = qv[q], ro = r[q], zq = (double)z[q];
+        bool fluid;
+        double dg;
+        const double wv = apply_a_z<PACKED>(L, op, z, q, inner, inv_h2, fluid, dg);
+        acc[0] += zq * qo;
+        acc[1] += zq * ro;
+        acc[2] += zq * wv;
+    });
+    block_sums<3>(acc, partials);
+}
+
+// d = z + beta d, q = A z + beta q on the interior fluid cells (0 elsewhere;
+// beta = 0 on the first iteration), then x += alpha d, r -= alpha q,
+// r32 = float(r); partial r.r.  FOLD (packed operator): also the correction
+// array e of the next preconditioner cycle after its first red half-sweep
+// from e = 0 on r32: e = fmaf(-dd, r32, +0) / diag on the red interior fluid
+// cells with diag > 0 (every neighbour term w * 0 is +0), 0 elsewhere, as
+// zero_rows and rbgs_rows_packed_kernel would leave it.  e must not be z (z
+// is read at the neighbours).
+template <bool PACKED, bool FOLD>
+__global__ __launch_bounds__(kT) void fcg_update_kernel(Level L, const uint8_t *__restrict__ op,
+                                                       const float *__restrict__ z, double *__restrict__ x,
+                                                       double *__restrict__ r, float *__restrict__ r32,
+                                                       double *__restrict__ d, double *__restrict__ qv,
+                                                       const double *__restrict__ s, int first, double inv_h2,
+                                                       const int32_t *__restrict__ rows, int64_t nrows,
+                                                       float *__restrict__ e, float dd, double *__restrict__ partials) {
+    const double alpha = s[2], beta = first ? 0.0 : s[5];
     double acc[1] = {0.0};
-    for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < work; t += (int64_t)gridDim.x * kT) {
-        const int64_t q = cell_of(rows, L.nz, t);
-        const double xv = x[q] + alpha * d[q];
-        const double rv = r[q] - alpha * qv[q];
+    for_rows(L, rows, nrows, [&](int64_t q, bool inner, int i, int j, int k) {
+        const double xo = x[q], ro = r[q], dold = d[q], qold = qv[q], zv = (double)z[q];
+        bool fluid;
+        double dg;
+        const double wv = apply_a_z<PACKED>(L, op, z, q, inner, inv_h2, fluid, dg);
+        const double dv = fluid ? zv + beta * dold : 0.0;
+        const double qn = fluid ? wv + beta * qold : 0.0;
+        d[q] = dv;
+        qv[q] = qn;
+        const double rv = ro - alpha * qn;
         const float rf = (float)rv;
-        x[q] = xv;
+        x[q] = xo + alpha * dv;
         r[q] = rv;
         r32[q] = rf;
         acc[0] += rv * rv;
-        if (e) {
-            int i, j, k;
-            coords(q, L.ny, L.nz, i, j, k);
-            const uint32_t o = op[q];
-            const float dg = (float)(o >> 4);
+        if constexpr (FOLD) {  // dg: the packed diagonal, a small integer
             float ev = 0.0f;
-            if (((i + j + k) & 1) == 0 && (o & 8) && interior(i, j, k, L.nx, L.ny, L.nz) && dg > 0.0f)
-                ev = fmaf(-dd, rf, 0.0f) / dg;  // multigrid.cl:227 on e = 0
+            if (((i + j + k) & 1) == 0 && fluid && dg > 0.0)
+                ev = fmaf(-dd, rf, 0.0f) / (float)dg;  // multigrid.cl:227 on e = 0
             e[q] = ev;
         }
-    }
+    });
     block_sums<1>(acc, partials);
 }
 
-// w = A z on the interior fluid cells (0 elsewhere); partial z.q, z.r.  One
-// contiguous chunk of work items per block, chunks laid out per XCD (the
-// stencil's x-neighbour rows stay in that XCD's L2).
-__global__ __launch_bounds__(kT) void fcg_dots_kernel(Level L, const uint8_t *__restrict__ op,
-                                                     const float *__restrict__ z, const double *__restrict__ qv,
-                                                     const double *__restrict__ r, double *__restrict__ w,
-                                                     double inv_h2, const int32_t *__restrict__ rows, int64_t work,
-                                                     double *__restrict__ partials) {
-    double acc[2] = {0.0, 0.0};
-    const int64_t chunk = (work + (int64_t)gridDim.x * kT - 1) / ((int64_t)gridDim.x * kT) * kT;
-    const int64_t t0 = (int64_t)xcd_item(blockIdx.x, gridDim.x) * chunk, t1 = min(t0 + chunk, work);
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += kT) {
-        const int64_t q = cell_of(rows, L.nz, t);
-        double wv = 0.0;
-        int i, j, k;
-        coords(q, L.ny, L.nz, i, j, k);
-        if (interior(i, j, k, L.nx, L.ny, L.nz)) {
-            // neighbours outside the fluid carry zero weight: their values are
-            // loaded with the rest but never enter the sum
-            auto zv = [&](int64_t u) { return (double)z[u]; };
-            const bool fluid = op ? (op[q] & 8) != 0 : L.mask[q] > 0.0f;
-            const double aq = op ? apply_a_packed(L, op, q, zv, inv_h2) : apply_a(L, q, zv, inv_h2);
-            if (fluid) wv = aq;
-        }
-        w[q] = wv;
-        const double zq = (double)z[q];
-        acc[0] += zq * qv[q];
-        acc[1] += zq * r[q];
-    }
-    block_sums<2>(acc, partials);
-}
-
 // Reduce `n` partial arrays of `blocks` entries and update the scalars.
-// stage 0: init (s6 = r.r, s7 = b.b); 1: d.q -> alpha; 2: r.r;
-// 3: z.q, z.r -> beta (unless first), rz.
+// stage 0: init (s6 = r.r, s7 = b.b); 2: r.r; 3: z.q, z.r, z.w -> beta,
+// d.q, alpha (first: beta = 0).
 __global__ __launch_bounds__(kT) void fcg_scalars_kernel(const double *__restrict__ partials, int blocks, int stage,
                                                         int first, double *__restrict__ s) {
-    __shared__ double sh[2][kT];
-    const int n = (stage == 0 || stage == 3) ? 2 : 1;
+    __shared__ double sh[3][kT];
+    const int n = stage == 0 ? 2 : stage == 3 ? 3 : 1;
     for (int a = 0; a < n; ++a) {
         double v = 0.0;
         for (int q = threadIdx.x; q < blocks; q += kT) v += partials[a * kB + q];
@@ -231,16 +250,18 @@ __global__ __launch_bounds__(kT) void fcg_scalars_kernel(const double *__restric
     if (stage == 0) {
         s[6] = sh[0][0];
         s[7] = sh[1][0];
-    } else if (stage == 1) {
-        s[1] = sh[0][0];
-        s[2] = s[1] != 0.0 ? s[0] / s[1] : 0.0;
     } else if (stage == 2) {
         s[6] = sh[0][0];
     } else {
-        s[3] = sh[0][0];
-        s[4] = sh[1][0];
-        s[5] = (first || s[0] == 0.0) ? 0.0 : -s[2] * s[3] / s[0];
-        s[0] = s[4];
+        const double zq = sh[0][0], zr = sh[1][0], zw = sh[2][0], dq_old = s[1];
+        const double beta = (first || dq_old == 0.0) ? 0.0 : -zq / dq_old;
+        const double dq = first ? zw : zw + 2.0 * beta * zq + beta * beta * dq_old;
+        s[0] = zr;
+        s[1] = dq;
+        s[2] = dq != 0.0 ? zr / dq : 0.0;
+        s[3] = zq;
+        s[4] = zw;
+        s[5] = beta;
     }
 }
 
@@ -249,7 +270,7 @@ static int blocks_for(int64_t C) {
     return (int)(b < kB ? b : kB);
 }
 
-int fcg_partials_count() { return 2 * kB; }
+int fcg_partials_count() { return 3 * kB; }
 
 void launch_fcg_init(const Level &L, const float *p32, const float *b, double inv_h2, double *x, double *r, float *r32,
                      double *partials, double *s, hipStream_t st) {
@@ -258,33 +279,40 @@ void launch_fcg_init(const Level &L, const float *p32, const float *b, double in
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 0, 0, s);
 }
 
-static int64_t work_of(const Level &L, const int32_t *rows, int64_t nrows) { return rows ? nrows * L.nz : L.C; }
+// rows of the per-iteration kernels' work, and their block count
+static int64_t rows_of(const Level &L, const int32_t *rows, int64_t nrows) {
+    return rows ? nrows : (int64_t)L.nx * L.ny;
+}
+static int row_blocks_for(int64_t nr) { return (int)(nr < kB ? (nr > 0 ? nr : 1) : kB); }
 
-void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const double *qv, const double *r, double *w,
-                     int first, double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s,
-                     hipStream_t st) {
-    const int64_t work = work_of(L, rows, nrows);
-    const int nb = blocks_for(work);
-    hipLaunchKernelGGL(fcg_dots_kernel, dim3(nb), dim3(kT), 0, st, L, op, z, qv, r, w, inv_h2, rows, work, partials);
+void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const double *qv, const double *r, int first,
+                     double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st) {
+    const int64_t nr = rows_of(L, rows, nrows);
+    const int nb = row_blocks_for(nr);
+    if (op)
+        hipLaunchKernelGGL(fcg_dots_kernel<true>, dim3(nb), dim3(kT), 0, st, L, op, z, qv, r, inv_h2, rows, nr,
+                           partials);
+    else
+        hipLaunchKernelGGL(fcg_dots_kernel<false>, dim3(nb), dim3(kT), 0, st, L, op, z, qv, r, inv_h2, rows, nr,
+                           partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 3, first, s);
 }
 
-void launch_fcg_direction(const Level &L, const uint8_t *op, const float *z, double *d, const double *w, double *qv,
-                          int first, const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st) {
-    const int64_t work = work_of(L, rows, nrows);
-    const int nb = blocks_for(work);
-    hipLaunchKernelGGL(fcg_direction_kernel, dim3(nb), dim3(kT), 0, st, L, op, z, d, w, qv, s, first, rows, work,
-                       partials);
-    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 1, 0, s);
-}
-
-void launch_fcg_update(const Level &L, double *x, double *r, float *r32, const double *d, const double *qv,
-                       const int32_t *rows, int64_t nrows, const uint8_t *op, float *e, float delta, double *partials,
-                       double *s, hipStream_t st) {
-    const int64_t work = work_of(L, rows, nrows);
-    const int nb = blocks_for(work);
-    hipLaunchKernelGGL(fcg_update_kernel, dim3(nb), dim3(kT), 0, st, L, x, r, r32, d, qv, s, rows, work, op, e,
-                       delta * delta, partials);
+void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double *x, double *r, float *r32, double *d,
+                       double *qv, int first, double inv_h2, const int32_t *rows, int64_t nrows, float *e, float delta,
+                       double *partials, double *s, hipStream_t st) {
+    const int64_t nr = rows_of(L, rows, nrows);
+    const int nb = row_blocks_for(nr);
+    const float dd = delta * delta;
+    if (op && e)
+        hipLaunchKernelGGL((fcg_update_kernel<true, true>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, d, qv, s,
+                           first, inv_h2, rows, nr, e, dd, partials);
+    else if (op)
+        hipLaunchKernelGGL((fcg_update_kernel<true, false>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, d, qv, s,
+                           first, inv_h2, rows, nr, e, dd, partials);
+    else
+        hipLaunchKernelGGL((fcg_update_kernel<false, false>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, d, qv,
+                           s, first, inv_h2, rows, nr, e, dd, partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 2, 0, s);
 }
 

@@ -1014,6 +1014,19 @@ static inline double apply_a64(const level_t *L, long long q, const double *v, d
     return (s - (double)L->diag[q] * v[q]) * inv_h2;
 }
 
+/* apply_a64 of an FP32 vector (the preconditioned residual z) */
+static inline double apply_a64_f32(const level_t *L, long long q, const float *v, double inv_h2) {
+    const long long sx = (long long)L->ny * L->nz, sy = L->nz;
+    double s = 0.0;
+    if (L->wx[q] != 0.0f) s = s + (double)L->wx[q] * v[q + sx];
+    if (L->wx[q - sx] != 0.0f) s = s + (double)L->wx[q - sx] * v[q - sx];
+    if (L->wy[q] != 0.0f) s = s + (double)L->wy[q] * v[q + sy];
+    if (L->wy[q - sy] != 0.0f) s = s + (double)L->wy[q - sy] * v[q - sy];
+    if (L->wz[q] != 0.0f) s = s + (double)L->wz[q] * v[q + 1];
+    if (L->wz[q - 1] != 0.0f) s = s + (double)L->wz[q - 1] * v[q - 1];
+    return (s - (double)L->diag[q] * v[q]) * inv_h2;
+}
+
 static inline int fluid_at(const level_t *L, long long q) {
     int i, j, k;
     coords(q, L->ny, L->nz, &i, &j, &k);
@@ -1071,33 +1084,44 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     s->initial_residual = rn;
     s->rhs_norm = bn;
     if (rn > sp->tolerance * bn && sp->max_cycles > 0) {
+        /* The direction recurrence streams with the update: the dots of z
+         * give beta = -(z.q)/(d.q)_old and the new direction's
+         * d.q = z.w + 2 beta z.q + beta^2 (d.q)_old with w = A z (A
+         * symmetric, so d_old . A z = z . q_old), hence alpha = z.r / d.q
+         * before d and q = w + beta q are formed. */
+        double alpha = 0.0, beta = 0.0, dq = 0.0;
+        int first = 1, plain = 0;
+        double *w = dnew;
         precondition_wcycle(c, sp, rn);
         s->cycles++;
-        double rz = 0.0, alpha = 0.0, beta = 0.0;
-        int first = 1, plain = 0;
-#pragma omp parallel for schedule(static) reduction(+ : rz)
-        for (long long t = 0; t < c->C; ++t) rz += (double)c->p[t] * r[t];
         for (;;) {
-            /* d_new = z + beta d_old (d_new = z first), q = A d_new */
-            for (long long t = 0; t < c->C; ++t)
-                dnew[t] = fluid_at(L, t) ? (first ? (double)c->p[t] : (double)c->p[t] + beta * dold[t]) : 0.0;
-            double dq = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : dq)
+            double zq = 0.0, zr = 0.0, zw = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : zq, zr, zw)
             for (long long t = 0; t < c->C; ++t) {
-                const double av = fluid_at(L, t) ? apply_a64(L, t, dnew, inv_h2) : 0.0;
-                q[t] = av;
-                dq += dnew[t] * av;
+                const double wv = fluid_at(L, t) ? apply_a64_f32(L, t, c->p, inv_h2) : 0.0;
+                w[t] = wv;
+                zq += (double)c->p[t] * q[t];
+                zr += (double)c->p[t] * r[t];
+                zw += (double)c->p[t] * wv;
             }
-            alpha = dq != 0.0 ? rz / dq : 0.0;
+            beta = (first || dq == 0.0) ? 0.0 : -zq / dq;
+            dq = first ? zw : zw + 2.0 * beta * zq + beta * beta * dq;
+            alpha = dq != 0.0 ? zr / dq : 0.0;
             rr = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : rr)
             for (long long t = 0; t < c->C; ++t) {
-                x[t] = x[t] + alpha * dnew[t];
-                r[t] = r[t] - alpha * q[t];
+                double dv = 0.0, qn = 0.0;
+                if (fluid_at(L, t)) {
+                    dv = first ? (double)c->p[t] : (double)c->p[t] + beta * dold[t];
+                    qn = first ? w[t] : w[t] + beta * q[t];
+                }
+                dold[t] = dv;
+                q[t] = qn;
+                x[t] = x[t] + alpha * dv;
+                r[t] = r[t] - alpha * qn;
                 c->r32[t] = (float)r[t];
                 rr += r[t] * r[t];
             }
-            double *tmp = dold; dold = dnew; dnew = tmp;
             rn = sqrt(rr);
             if (rn <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(rn == rn)) break;
             /* A weighted cycle that is not definite for this residual gives a
@@ -1110,14 +1134,6 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
             precondition_wcycle(c, sp, rn);
             s->cycles++;
             s->fallback_cycles += plain;
-            double zq = 0.0, zr = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : zq, zr)
-            for (long long t = 0; t < c->C; ++t) {
-                zq += (double)c->p[t] * q[t];
-                zr += (double)c->p[t] * r[t];
-            }
-            beta = (first || rz == 0.0) ? 0.0 : -alpha * zq / rz;
-            rz = zr;
         }
     }
     for (long long t = 0; t < c->C; ++t) c->p[t] = (float)x[t];
