@@ -31,20 +31,24 @@ __device__ __forceinline__ Stencil stagger(const float *x, float inv_dx, float o
     return s;
 }
 
-__device__ __forceinline__ float sample(const float *__restrict__ f, const Stencil &s, int64_t sx, int64_t sy) {
+// The eight corner values of a field around a stencil, corner c = 4 ox + 2 oy + oz.
+__device__ __forceinline__ void corners(const float *__restrict__ f, const Stencil &s, int64_t sx, int64_t sy,
+                                        float (&c)[8]) {
+#pragma unroll
+    for (int n = 0; n < 8; ++n) c[n] = f[s.base + (n >> 2) * sx + ((n >> 1) & 1) * sy + (n & 1)];
+}
+
+// the trilinear sum over the corners, in the reference's corner order
+__device__ __forceinline__ float blend(const float (&c)[8], const Stencil &s) {
     float value = 0.0f;
 #pragma unroll
-    for (int ox = 0; ox <= 1; ++ox)
-#pragma unroll
-        for (int oy = 0; oy <= 1; ++oy)
-#pragma unroll
-            for (int oz = 0; oz <= 1; ++oz) {
-                const float wx = ox ? s.f0 : 1.0f - s.f0;
-                const float wy = oy ? s.f1 : 1.0f - s.f1;
-                const float wz = oz ? s.f2 : 1.0f - s.f2;
-                const float w = wx * wy * wz;
-                value = fmaf(w, f[s.base + ox * sx + oy * sy + oz], value);  // particles.cl:37 (fmuladd)
-            }
+    for (int n = 0; n < 8; ++n) {
+        const float wx = (n >> 2) ? s.f0 : 1.0f - s.f0;
+        const float wy = ((n >> 1) & 1) ? s.f1 : 1.0f - s.f1;
+        const float wz = (n & 1) ? s.f2 : 1.0f - s.f2;
+        const float w = wx * wy * wz;
+        value = fmaf(w, c[n], value);  // particles.cl:37 (fmuladd)
+    }
     return value;
 }
 
@@ -112,10 +116,20 @@ __global__ __launch_bounds__(256) void particle_update_kernel(float *__restrict_
         const Stencil su = stagger(x, pu.inv_dx, 1.0f, 0.5f, 0.5f, nx, ny, nz);
         const Stencil sv = stagger(x, pu.inv_dx, 0.5f, 1.0f, 0.5f, nx, ny, nz);
         const Stencil sw = stagger(x, pu.inv_dx, 0.5f, 0.5f, 1.0f, nx, ny, nz);
-        const float nu = sample(u, su, sx, sy), nv = sample(v, sv, sx, sy), nw = sample(w, sw, sx, sy);
-        const float f0 = vv[0] + (nu - sample(uo, su, sx, sy));
-        const float f1 = vv[1] + (nv - sample(vo, sv, sx, sy));
-        const float f2 = vv[2] + (nw - sample(wo, sw, sx, sy));
+        // all 48 corner values are loaded before any is used, so they are in
+        // flight together (sampled one field at a time they were waited for
+        // in pairs)
+        float cu[8], cv[8], cw[8], cuo[8], cvo[8], cwo[8];
+        corners(u, su, sx, sy, cu);
+        corners(v, sv, sx, sy, cv);
+        corners(w, sw, sx, sy, cw);
+        corners(uo, su, sx, sy, cuo);
+        corners(vo, sv, sx, sy, cvo);
+        corners(wo, sw, sx, sy, cwo);
+        const float nu = blend(cu, su), nv = blend(cv, sv), nw = blend(cw, sw);
+        const float f0 = vv[0] + (nu - blend(cuo, su));
+        const float f1 = vv[1] + (nv - blend(cvo, sv));
+        const float f2 = vv[2] + (nw - blend(cwo, sw));
         if (pu.pure_flip) {
             vv[0] = f0; vv[1] = f1; vv[2] = f2;
         } else {
