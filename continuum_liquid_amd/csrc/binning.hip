@@ -194,13 +194,25 @@ __global__ __launch_bounds__(256) void bin_gather_kernel(const int32_t *__restri
     const int slot = blockIdx.x * 256 + threadIdx.x;
     if (slot >= n) return;
     const int i = perm[slot];
-    const int b = key[i];
-    const int start = offsets[b], cnt = counts[b];
-    int r = 0;
-    for (int t = 0; t < cnt; ++t) r += perm[start + t] < i;
-    const int64_t dst = 3LL * (start + r), src = 3LL * i;
+    // the particle's data is loaded first: it depends on i alone
+    const int64_t src = 3LL * i;
     const float p0 = pos[src], p1 = pos[src + 1], p2 = pos[src + 2];
     const float v0 = vel[src], v1 = vel[src + 1], v2 = vel[src + 2];
+    const int b = key[i];
+    const int start = offsets[b], cnt = counts[b];
+    // rank among the bin's members by original index, eight members' loads
+    // in flight at a time (the members are slots start .. start + cnt - 1;
+    // cnt >= 1, and indices past the last member read the last one, whose
+    // comparison is dropped)
+    int r = 0;
+    for (int t0 = 0; t0 < cnt; t0 += 8) {
+        int m[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) m[u] = perm[start + min(t0 + u, cnt - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) r += (t0 + u < cnt) & (m[u] < i);
+    }
+    const int64_t dst = 3LL * (start + r);
     pos_out[dst] = p0; pos_out[dst + 1] = p1; pos_out[dst + 2] = p2;
     vel_out[dst] = v0; vel_out[dst + 1] = v1; vel_out[dst + 2] = v2;
 }
