@@ -77,14 +77,48 @@ void launch_grid_obstacle_bc(float *u, float *v, float *w, float *ff, const uint
     hipLaunchKernelGGL(extrapolate_obstacle_kernel, cells_grid(C), dim3(256), 0, s, u, v, w, ob, nx, ny, nz);
 }
 
+// The cells the domain condition touches: i, j or k in {0, n-2, n-1}.
+// Item t enumerates them without repeats: first the three x planes, then the
+// three y planes of the remaining x range, then the three z planes of the
+// remaining (x, y) range.
+__device__ __forceinline__ bool domain_cell(int64_t t, int nx, int ny, int nz, int &i, int &j, int &k) {
+    const int64_t a = 3LL * ny * nz, b = 3LL * (nx - 3) * nz, c = 3LL * (nx - 3) * (ny - 3);
+    auto side = [](int64_t s, int n) { return s == 0 ? 0 : (int)(n - 3 + s); };  // 0, n-2, n-1
+    if (t < a) {
+        const int64_t pl = t / ((int64_t)ny * nz), r = t - pl * ny * nz;
+        i = side(pl, nx);
+        j = (int)(r / nz);
+        k = (int)(r - (int64_t)j * nz);
+        return true;
+    }
+    t -= a;
+    if (t < b) {
+        const int64_t pl = t / ((int64_t)(nx - 3) * nz), r = t - pl * (nx - 3) * nz;
+        j = side(pl, ny);
+        i = 1 + (int)(r / nz);
+        k = (int)(r - (int64_t)(i - 1) * nz);
+        return true;
+    }
+    t -= b;
+    if (t < c) {
+        const int64_t pl = t / ((int64_t)(nx - 3) * (ny - 3)), r = t - pl * (nx - 3) * (ny - 3);
+        k = side(pl, nz);
+        i = 1 + (int)(r / (ny - 3));
+        j = 1 + (int)(r - (int64_t)(i - 1) * (ny - 3));
+        return true;
+    }
+    return false;
+}
+
 // A ghost cell copies only components that are not walls at its clamped
 // interior cell, and interior lanes write only wall components: race free.
+// Launched over the cells that domain_cell enumerates (every other cell is
+// neither a ghost nor a wall face).
 __global__ __launch_bounds__(256) void grid_domain_kernel(float *__restrict__ u, float *__restrict__ v,
                                                           float *__restrict__ w, int nx, int ny, int nz) {
-    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (index >= (int64_t)nx * ny * nz) return;
     int i, j, k;
-    coords(index, ny, nz, i, j, k);
+    if (!domain_cell((int64_t)blockIdx.x * 256 + threadIdx.x, nx, ny, nz, i, j, k)) return;
+    const int64_t index = gidx(i, j, k, ny, nz);
     const bool ghost = !interior(i, j, k, nx, ny, nz);
     const bool wu = i == 0 || i >= nx - 2;
     const bool wv = j == 0 || j >= ny - 2;
@@ -104,8 +138,8 @@ __global__ __launch_bounds__(256) void grid_domain_kernel(float *__restrict__ u,
 }
 
 void launch_grid_domain_bc(float *u, float *v, float *w, int nx, int ny, int nz, hipStream_t s) {
-    const int64_t C = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(grid_domain_kernel, cells_grid(C), dim3(256), 0, s, u, v, w, nx, ny, nz);
+    const int64_t cells = 3LL * ny * nz + 3LL * (nx - 3) * nz + 3LL * (nx - 3) * (ny - 3);
+    hipLaunchKernelGGL(grid_domain_kernel, cells_grid(cells), dim3(256), 0, s, u, v, w, nx, ny, nz);
 }
 
 // laplacian (helper.cl:54-66): fma(-6, centre, neighbour_sum) * inv_dx2.
