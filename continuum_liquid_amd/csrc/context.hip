@@ -943,6 +943,8 @@ static int ensure_fcg_buffers(clq_ctx *c) {
 
 // level-0 active-row list of the FCG vector kernels (all cells without one)
 #define ROWS0 (c->use_rows ? c->rows[0] : nullptr), c->nrows[0]
+// level-0 row flags when the row lists are in use
+#define FLAGS0 (c->use_rows ? c->row_flags[0] : nullptr)
 // packed level-0 operator when the row lists are in use and it is exact
 #define OP0 (c->use_rows && c->op0_ok ? c->op0 : nullptr)
 
@@ -951,7 +953,8 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     const Level &L0 = c->lv[0];
     const double h = (double)(float)c->dx;
     const double inv_h2 = 1.0 / (h * h);
-    launch_fcg_init(L0, c->p, c->rhs, inv_h2, c->p64a, c->r64, c->r32, c->fcg_partials, c->fcg_s, c->stream);
+    launch_fcg_init(L0, c->p, c->rhs, inv_h2, c->p64a, c->r64, c->r32, FLAGS0, c->fcg_partials, c->fcg_s,
+                    c->stream);
     if (int rc = read_scalars(c)) return rc;
     double r = sqrt(c->norm_host[0]);
     const double bn = sqrt(c->norm_host[1]);
@@ -1012,12 +1015,14 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
     double r_warm = 0.0, bn = 0.0;
     if (int rc = residual_norm(c, &r_warm, &bn)) return rc;
     unet_predict(c->net, c->rhs, L0.mask, c->p, c->guess32, c->stream);
-    launch_fcg_init(L0, c->guess32, c->rhs, inv_h2, c->p64a, c->r64, c->r32, c->fcg_partials, c->fcg_s, c->stream);
+    launch_fcg_init(L0, c->guess32, c->rhs, inv_h2, c->p64a, c->r64, c->r32, FLAGS0, c->fcg_partials, c->fcg_s,
+                    c->stream);
     if (int rc = read_scalars(c)) return rc;
     double r = sqrt(c->norm_host[0]);
     bn = sqrt(c->norm_host[1]);
     if (!(r < r_warm)) {
-        launch_fcg_init(L0, c->p, c->rhs, inv_h2, c->p64a, c->r64, c->r32, c->fcg_partials, c->fcg_s, c->stream);
+        launch_fcg_init(L0, c->p, c->rhs, inv_h2, c->p64a, c->r64, c->r32, FLAGS0, c->fcg_partials, c->fcg_s,
+                    c->stream);
         if (int rc = read_scalars(c)) return rc;
         r = sqrt(c->norm_host[0]);
     }

@@ -111,6 +111,41 @@ __global__ __launch_bounds__(kT) void fcg_init_kernel(Level L, const float *__re
     block_sums<2>(acc, partials);
 }
 
+// fcg_init_kernel by rows with the level-0 row flags: a row without an
+// interior fluid cell (flag 0) only gets x = 0 (the pressure correction reads
+// x at every interior cell; r, r32 are read there under zero weights or a
+// fluid select only, and b is 0 there), an active row the full update.
+__global__ __launch_bounds__(kT) void fcg_init_rows_kernel(Level L, const int32_t *__restrict__ flags,
+                                                          const float *__restrict__ p32, const float *__restrict__ b,
+                                                          double inv_h2, double *__restrict__ x,
+                                                          double *__restrict__ r, float *__restrict__ r32,
+                                                          double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    const int64_t nrows = (int64_t)L.nx * L.ny, per = (nrows + gridDim.x - 1) / gridDim.x;
+    const int64_t a0 = (int64_t)blockIdx.x * per, a1 = min(a0 + per, nrows);
+    for (int64_t a = a0; a < a1; ++a) {
+        const bool active = flags[a] != 0;
+        for (int k = threadIdx.x; k < L.nz; k += kT) {
+            const int64_t q = a * L.nz + k;
+            if (!active) {
+                x[q] = 0.0;
+                continue;
+            }
+            double xv = 0.0, rv = 0.0;
+            if (fluid_cell(L, q)) {
+                xv = (double)p32[q];
+                rv = (double)b[q] - apply_a(L, q, [&](int64_t t) { return (double)p32[t]; }, inv_h2);
+            }
+            x[q] = xv;
+            r[q] = rv;
+            r32[q] = (float)rv;
+            acc[0] += rv * rv;
+            acc[1] += (double)b[q] * (double)b[q];
+        }
+    }
+    block_sums<2>(acc, partials);
+}
+
 // A z at cell q = (i, j, k) (0 off the interior fluid cells), from the
 // packed level-0 operator (PACKED) or the float arrays.  Every load is issued
 // before any test: off the interior the neighbour indices fall back to q
@@ -272,18 +307,26 @@ static int blocks_for(int64_t C) {
 
 int fcg_partials_count() { return 3 * kB; }
 
-void launch_fcg_init(const Level &L, const float *p32, const float *b, double inv_h2, double *x, double *r, float *r32,
-                     double *partials, double *s, hipStream_t st) {
-    const int nb = blocks_for(L.C);
-    hipLaunchKernelGGL(fcg_init_kernel, dim3(nb), dim3(kT), 0, st, L, p32, b, inv_h2, x, r, r32, partials);
-    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 0, 0, s);
-}
-
 // rows of the per-iteration kernels' work, and their block count
 static int64_t rows_of(const Level &L, const int32_t *rows, int64_t nrows) {
     return rows ? nrows : (int64_t)L.nx * L.ny;
 }
 static int row_blocks_for(int64_t nr) { return (int)(nr < kB ? (nr > 0 ? nr : 1) : kB); }
+
+void launch_fcg_init(const Level &L, const float *p32, const float *b, double inv_h2, double *x, double *r, float *r32,
+                     const int32_t *flags, double *partials, double *s, hipStream_t st) {
+    int nb;
+    if (flags) {
+        nb = row_blocks_for((int64_t)L.nx * L.ny);
+        hipLaunchKernelGGL(fcg_init_rows_kernel, dim3(nb), dim3(kT), 0, st, L, flags, p32, b, inv_h2, x, r, r32,
+                           partials);
+    } else {
+        nb = blocks_for(L.C);
+        hipLaunchKernelGGL(fcg_init_kernel, dim3(nb), dim3(kT), 0, st, L, p32, b, inv_h2, x, r, r32, partials);
+    }
+    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 0, 0, s);
+}
+
 
 void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const double *qv, const double *r, int first,
                      double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st) {
