@@ -204,6 +204,10 @@ void launch_restrict_rows(const Level &F, const uint8_t *op, const Level &K, flo
 void launch_rbgs_prolong(const Level &L, const Level &K, const uint8_t *op, float delta, int parity, int first,
                          const int32_t *rows, int64_t nrows, hipStream_t s);
 void launch_prolongate_rows(const Level &F, const Level &K, const int32_t *frows, int64_t nfrows, hipStream_t s);
+// both post-smoothing half-sweeps of a dense level with the prolongation from K
+// folded in; writes every cell of L into out (L.p is only read): the caller
+// makes out the level's pressure array
+void launch_rbgs_post_fused(const Level &L, const Level &K, float delta, float *out, hipStream_t s);
 void launch_ir_update_residual(const Level &L, int from32, const float *p32, const double *pin, double *pout,
                                float *r32, const float *b, double inv_h2, double *partials, double *out2,
                                hipStream_t s);

@@ -85,6 +85,9 @@ struct clq_ctx {
     // (the update writes the next cycle's correction while it reads z)
     double *r64 = nullptr, *dir = nullptr, *q64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
     float *p_alt = nullptr;
+    // second pressure array of each dense level (>= 2): the fused
+    // post-smoothing writes it, and it becomes the level's pressure
+    float *p_swap[kMaxLevels] = {};
     // active rows of levels 0 and 1 (rows holding an interior fluid cell),
     // used by the smoother, transfers and FCG vectors inside clq_pressure_solve
     int32_t *row_flags[2] = {nullptr, nullptr}, *rows[2] = {nullptr, nullptr}, *row_pos = nullptr;
@@ -267,6 +270,11 @@ int clq_create(int32_t nx, int32_t ny, int32_t nz, double dx, int64_t capacity, 
             (void)hipMemsetAsync(q, 0, (size_t)L.C * 4, c->stream);
         }
         a = coarse_size(a); b = coarse_size(b); d = coarse_size(d);
+    }
+    for (int l = 2; l < c->nlevels; ++l) {
+        if ((rc = c->alloc(&q, (size_t)c->lv[l].C * 4))) return bail(rc);
+        c->p_swap[l] = (float *)q;
+        (void)hipMemsetAsync(q, 0, (size_t)c->lv[l].C * 4, c->stream);
     }
     if ((rc = c->alloc(&q, (size_t)residual_partials_count(c->C) * 8))) return bail(rc);
     c->norm_partials = (double *)q;
@@ -606,6 +614,11 @@ static bool fold_transfers(const clq_ctx *c) { return c->use_rows; }
 // the first post-smoothing sweep of `level` with the prolongation from level + 1
 static void smooth_prolong(clq_ctx *c, int level) {
     const bool sparse = level < 2;
+    if (!sparse && c->p_swap[level]) {  // both half-sweeps in one launch
+        launch_rbgs_post_fused(c->lv[level], c->lv[level + 1], (float)c->dx, c->p_swap[level], c->stream);
+        std::swap(c->lv[level].p, c->p_swap[level]);
+        return;
+    }
     const uint8_t *op = level == 0 && c->op0_ok ? c->op0 : nullptr;
     for (int parity = 0; parity < 2; ++parity)
         launch_rbgs_prolong(c->lv[level], c->lv[level + 1], op, (float)c->dx, parity, parity == 0,

@@ -831,6 +831,88 @@ __global__ __launch_bounds__(256) void rbgs_rows_packed_prolong_kernel(Level L, 
     }
 }
 
+// Both post-smoothing half-sweeps of a dense level with the prolongation
+// from K folded in, in one launch.  A block owns a tile of kTi x kTj x kTk
+// cells: it computes the red half-sweep's values (rbgs_cell_prolong, first)
+// for the red cells of the tile and a one-cell halo into LDS, then the black
+// half-sweep's values (rbgs_cell_prolong, not first) of its black cells from
+// them, and writes every cell of the tile to `out`.  The red values of a
+// halo cell are the same numbers its own tile computes; L.p is only read
+// (other tiles' blocks still read their halos' old values from it), so the
+// caller swaps `out` in as the level's pressure.  Every value equals the two
+// separate launches'.
+constexpr int kTi = 4, kTj = 8, kTk = 32;
+constexpr int kBi = kTi + 2, kBj = kTj + 2, kBk = kTk + 2;
+
+// the value rbgs_cell_prolong leaves in cell (i, j, k) after its half-sweep:
+// red (first) from the prolonged black neighbours in L.p, black from the red
+// values in r (the LDS box, origin (i0, j0, k0) of the box)
+template <bool RED>
+__device__ __forceinline__ float post_value(const Level &L, const Level &K, float dd, int i, int j, int k,
+                                            const float *r, int i0, int j0, int k0) {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int64_t idx = gidx(i, j, k, ny, nz), sx = (int64_t)ny * nz, sy = nz;
+    const float m = L.mask[idx], pold = L.p[idx];
+    if (m <= 0.0f) return 0.0f;
+    if (!interior(i, j, k, nx, ny, nz)) return pold;
+    const float d = L.diag[idx];
+    const float own = fmaf(K.corr, K.p[parent_of(K, i, j, k)], pold);  // prolong_item
+    float vxm, vxp, vyp, vym, vzp, vzm;
+    if constexpr (RED) {
+        vxm = prolonged(L, K, idx - sx, i - 1, j, k);
+        vxp = prolonged(L, K, idx + sx, i + 1, j, k);
+        vyp = prolonged(L, K, idx + sy, i, j + 1, k);
+        vym = prolonged(L, K, idx - sy, i, j - 1, k);
+        vzp = prolonged(L, K, idx + 1, i, j, k + 1);
+        vzm = prolonged(L, K, idx - 1, i, j, k - 1);
+    } else {
+        auto at = [&](int a, int b, int c) { return r[((a - i0) * kBj + (b - j0)) * kBk + (c - k0)]; };
+        vxm = at(i - 1, j, k);
+        vxp = at(i + 1, j, k);
+        vyp = at(i, j + 1, k);
+        vym = at(i, j - 1, k);
+        vzp = at(i, j, k + 1);
+        vzm = at(i, j, k - 1);
+    }
+    float t = L.wx[idx - sx] * vxm;
+    t = fmaf(L.wx[idx], vxp, t);
+    t = fmaf(L.wy[idx], vyp, t);
+    t = fmaf(L.wy[idx - sy], vym, t);
+    t = fmaf(L.wz[idx], vzp, t);
+    t = fmaf(L.wz[idx - 1], vzm, t);
+    const float num = fmaf(-dd, load_b_flat(L.b, L.b16, L.bkind, L.bscale, idx), t);
+    return d > 0.0f ? num / d : own;  // multigrid.cl:227 or the prolongation
+}
+
+__global__ __launch_bounds__(1024) void rbgs_post_fused_kernel(Level L, Level K, float dd, float *__restrict__ out) {
+    __shared__ float r[kBi * kBj * kBk];
+    const int ti = (int)blockIdx.z * kTi, tj = (int)blockIdx.y * kTj, tk = (int)blockIdx.x * kTk;
+    const int i0 = ti - 1, j0 = tj - 1, k0 = tk - 1;  // box origin
+    // red cells of the box: (bi, bj) pairs x kBk / 2 cells each
+    for (int e = threadIdx.x; e < kBi * kBj * (kBk / 2); e += 1024) {
+        const int pair = e / (kBk / 2), kk = e - pair * (kBk / 2);
+        const int bi = pair / kBj, bj = pair - bi * kBj;
+        const int i = i0 + bi, j = j0 + bj, k = k0 + 2 * kk + ((i + j + k0) & 1);  // i + j + k even
+        if (i < 0 || i >= L.nx || j < 0 || j >= L.ny || k < 0 || k >= L.nz) continue;
+        r[(bi * kBj + bj) * kBk + (k - k0)] = post_value<true>(L, K, dd, i, j, k, r, i0, j0, k0);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTi * kTj * kTk; e += 1024) {
+        const int li = e / (kTj * kTk), rest = e - li * (kTj * kTk), lj = rest / kTk, lk = rest - lj * kTk;
+        const int i = ti + li, j = tj + lj, k = tk + lk;
+        if (i >= L.nx || j >= L.ny || k >= L.nz) continue;
+        const float v = ((i + j + k) & 1) == 0 ? r[((li + 1) * kBj + (lj + 1)) * kBk + (lk + 1)]
+                                                 : post_value<false>(L, K, dd, i, j, k, r, i0, j0, k0);
+        out[gidx(i, j, k, L.ny, L.nz)] = v;
+    }
+}
+
+void launch_rbgs_post_fused(const Level &L, const Level &K, float delta, float *out, hipStream_t s) {
+    const dim3 grid((unsigned)((L.nz + kTk - 1) / kTk), (unsigned)((L.ny + kTj - 1) / kTj),
+                    (unsigned)((L.nx + kTi - 1) / kTi));
+    hipLaunchKernelGGL(rbgs_post_fused_kernel, grid, dim3(1024), 0, s, L, K, delta * delta, out);
+}
+
 void launch_rbgs_prolong(const Level &L, const Level &K, const uint8_t *op, float delta, int parity, int first,
                          const int32_t *rows, int64_t nrows, hipStream_t s) {
     const float dd = delta * delta;
