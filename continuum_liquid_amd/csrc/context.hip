@@ -76,6 +76,7 @@ struct clq_ctx {
     Level lv[kMaxLevels];
     bool coarse_ops_valid = false;  // coarse operators match the current level-0 operator
     double *norm_partials = nullptr, *norm_out = nullptr, *norm_host = nullptr;
+    hipEvent_t ev_scal = nullptr;  // behind each FCG scalar read
     // mixed-precision solve state, allocated on first use
     double *p64a = nullptr, *p64b = nullptr;
     float *r32 = nullptr;
@@ -197,6 +198,7 @@ int clq_destroy(clq_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *q : c->allocations) (void)hipFree(q);
     if (c->norm_host) (void)hipHostFree(c->norm_host);
+    if (c->ev_scal) (void)hipEventDestroy(c->ev_scal);
     for (int s = 0; s < CLQ_STAGE_COUNT; ++s) {
         if (c->ev_start[s]) (void)hipEventDestroy(c->ev_start[s]);
         if (c->ev_stop[s]) (void)hipEventDestroy(c->ev_stop[s]);
@@ -282,6 +284,8 @@ int clq_create(int32_t nx, int32_t ny, int32_t nz, double dx, int64_t capacity, 
     c->norm_out = (double *)q;
     if (hipHostMalloc((void **)&c->norm_host, 10 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return bail(fail(CLQ_ERR_NOMEM, "hipHostMalloc failed"));
+    if (hipEventCreateWithFlags(&c->ev_scal, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(CLQ_ERR_DEVICE, "hipEventCreate failed"));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(CLQ_ERR_DEVICE, "initial clear failed"));
     *out = c;
     return CLQ_OK;
@@ -585,15 +589,17 @@ static int begin_sparse_solve(clq_ctx *c) {
     return CLQ_OK;
 }
 
-// red_done: the restriction onto this level already made the first (red)
-// half-sweep (restrict_level with red).
-static void smooth_level(clq_ctx *c, int level, int iterations, bool red_done = false) {
+// done: half-sweeps of the first sweep already made: 1 when the restriction
+// onto this level made the red one (restrict_level with red) or the FCG
+// update did on level 0, 2 when the FCG loop has also run the black one.
+static void smooth_level(clq_ctx *c, int level, int iterations, int done = 0) {
     const float delta = (float)c->dx;
     const bool sparse = c->use_rows && level < 2;
     for (int it = 0; it < iterations; ++it) {
-        const bool whole = !(it == 0 && red_done);
+        const int skip = it == 0 ? done : 0;
+        const bool whole = skip == 0;
         if (level == 0 && whole) kernel_mark(c, CLQ_KERNEL_RBGS_FINE);
-        for (int parity = whole ? 0 : 1; parity < 2; ++parity) {
+        for (int parity = skip; parity < 2; ++parity) {
             if (sparse && level == 0 && c->op0_ok)
                 launch_rbgs_rows_packed(c->lv[0], c->op0, delta, parity, c->rows[0], c->nrows[0], c->stream);
             else if (sparse)
@@ -697,8 +703,8 @@ static bool swept_level(const clq_ctx *c, int level) {
 // restriction, then (after the caller's coarse visits) the prolongation and
 // post-smoothing.  Returns whether the coarse level's first red half-sweep
 // was made by the restriction.
-static bool visit_down(clq_ctx *c, int level, int pre, bool red_done) {
-    smooth_level(c, level, pre, red_done);
+static bool visit_down(clq_ctx *c, int level, int pre, int done) {
+    smooth_level(c, level, pre, done);
     const bool red = fold_transfers(c) && pre > 0 && swept_level(c, level + 1);
     restrict_level(c, level, red);
     return red;
@@ -714,7 +720,7 @@ static void visit_up(clq_ctx *c, int level, int post) {
     }
 }
 
-static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool red_done = false) {
+static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, int done = 0) {
     if (level > 0 && level == bottom_level(c)) {
         launch_bottom(c, level, pre, post, extra, vfrom, false);
         return;
@@ -723,7 +729,7 @@ static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfr
         launch_coarse_smooth(c->lv[level], (float)c->dx, pre + extra, c->stream);
         return;
     }
-    const bool red = visit_down(c, level, pre, red_done);
+    const bool red = visit_down(c, level, pre, done);
     w_cycle(c, level + 1, pre, post, extra, vfrom, red);
     if (vfrom <= 0 || level < vfrom) w_cycle(c, level + 1, pre, post, extra, vfrom);
     visit_up(c, level, post);
@@ -796,7 +802,7 @@ int clq_residual_norm(clq_ctx *c, double *residual, double *rhs_norm) {
 // f64: the coarsest level's sweeps in FP64 (the refinement of mode 2);
 // otherwise the reference's FP32 coarse smoother
 static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool f64,
-                          bool red_done = false) {
+                          int done = 0) {
     if (level > 0 && level == bottom_level(c)) {
         launch_bottom(c, level, pre, post, extra, vfrom, f64);
         return;
@@ -808,7 +814,7 @@ static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, i
         else launch_coarse_smooth(L, (float)c->dx, pre + extra, c->stream);
         return;
     }
-    const bool red = visit_down(c, level, pre, red_done);
+    const bool red = visit_down(c, level, pre, done);
     w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64, red);
     if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64);
     visit_up(c, level, post);
@@ -891,9 +897,10 @@ static int ensure_mixed_buffers(clq_ctx *c) {
 // level-0 pressure array), 16-bit coarse b scaled from the residual norm and
 // FP32 coarsest sweeps (FP64 ones measured the same FCG iterations and
 // residuals at 256^3, and cost 0.1 ms per step).
-// red_done: the FCG update already left e after the cycle's first red
-// half-sweep on level 0 (fcg_update with e), in place of zeroing it.
-static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rnorm, bool red_done = false) {
+// done: the FCG loop already left e after the cycle's first red half-sweep
+// on level 0 (1: fcg_update with e) or after its whole first sweep (2: and
+// the black half-sweep queued during the scalar read), in place of zeroing it.
+static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rnorm, int done = 0) {
     const int kind = sp->coarse_dtype == 1 ? 2 : 1;
     const int es = (int)lround(log2(rnorm / sqrt((double)c->C) + 1e-300));
     for (int l = 1; l < c->nlevels; ++l) {
@@ -903,11 +910,11 @@ static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rn
     Level &L0 = c->lv[0];
     L0.b = c->r32;
     // e = 0 where the cycle reads it with a non-zero weight: the active rows
-    if (!red_done) {
+    if (!done) {
         if (c->use_rows) launch_zero_rows(L0.p, c->rows[0], c->nrows[0], c->nz, c->stream);
         else HIP_TRY(hipMemsetAsync(L0.p, 0, (size_t)c->C * 4, c->stream));
     }
-    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, false, red_done);
+    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, false, done);
     L0.b = c->rhs;
     for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
     return CLQ_OK;
@@ -915,12 +922,22 @@ static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rn
 
 // the FCG scalars on the host: norm_host[0] = r.r, [1] = b.b, and the whole
 // block s[0..8) from norm_host[2] (alpha of the last update at [4])
-static int read_scalars(clq_ctx *c) {
+// The copy is waited for through an event recorded behind it, so that work
+// queued after it keeps the GPU busy while the host reads the scalars.
+static int queue_scalars(clq_ctx *c) {
     HIP_TRY(hipMemcpyAsync(c->norm_host + 2, c->fcg_s, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipEventRecord(c->ev_scal, c->stream));
+    return CLQ_OK;
+}
+static int wait_scalars(clq_ctx *c) {
+    HIP_TRY(hipEventSynchronize(c->ev_scal));
     c->norm_host[0] = c->norm_host[2 + 6];
     c->norm_host[1] = c->norm_host[2 + 7];
     return CLQ_OK;
+}
+static int read_scalars(clq_ctx *c) {
+    if (int rc = queue_scalars(c)) return rc;
+    return wait_scalars(c);
 }
 
 // A weighted cycle (coarse_weight != 1) that is not definite for the current
@@ -987,7 +1004,18 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
         for (;;) {
             launch_fcg_update(L0, OP0, z, c->p64a, c->r64, c->r32, c->dir, c->q64, first, inv_h2, ROWS0,
                               fold ? zn : nullptr, (float)c->dx, c->fcg_partials, c->fcg_s, c->stream);
-            if (int rc = read_scalars(c)) return rc;
+            if (int rc = queue_scalars(c)) return rc;
+            if (fold) {
+                // the next cycle's level-0 black half-sweep (on e = zn with
+                // b = r32) runs while the host reads the scalars; it needs
+                // neither the residual norm (the coarse b scale) nor the
+                // weight, and when the solve stops here it is simply unused
+                Level Lz = c->lv[0];
+                Lz.p = zn;
+                Lz.b = c->r32;
+                launch_rbgs_rows_packed(Lz, c->op0, (float)c->dx, 1, c->rows[0], c->nrows[0], c->stream);
+            }
+            if (int rc = wait_scalars(c)) return rc;
             r = sqrt(c->norm_host[0]);
             if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
             first = drop_weight_on_breakdown(c);
@@ -996,7 +1024,7 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
                 std::swap(z, zn);
                 c->lv[0].p = z;
             }
-            if (int rc = precondition_wcycle(c, sp, r, fold)) return rc;
+            if (int rc = precondition_wcycle(c, sp, r, fold ? 2 : 0)) return rc;
             s->cycles++;
             s->fallback_cycles += plain;
             launch_fcg_dots(L0, OP0, z, c->q64, c->r64, first, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
