@@ -91,9 +91,10 @@ struct clq_ctx {
     float *p_swap[kMaxLevels] = {};
     // active rows of levels 0 and 1 (rows holding an interior fluid cell),
     // used by the smoother, transfers and FCG vectors inside clq_pressure_solve
-    int32_t *row_flags[2] = {nullptr, nullptr}, *rows[2] = {nullptr, nullptr}, *row_pos = nullptr;
+    // active rows of levels 0-2 (level 2's only for the restriction onto it)
+    int32_t *row_flags[3] = {nullptr, nullptr, nullptr}, *rows[3] = {nullptr, nullptr, nullptr}, *row_pos = nullptr;
     int32_t *row_count = nullptr, *row_count_host = nullptr;
-    int64_t nrows[2] = {0, 0};
+    int64_t nrows[3] = {0, 0, 0};
     bool rows_valid = false, use_rows = false;
     uint8_t *op0 = nullptr;  // packed level-0 operator (valid when op0_ok)
     bool op0_ok = false;
@@ -539,13 +540,17 @@ static void ensure_coarse_operators(clq_ctx *c) {
 
 // Active-row lists of levels 0 and 1 for the current operator (one host
 // read-back of the two counts per operator build).
+// levels with active-row lists: 0 and 1 are swept by rows, level 2's list
+// serves the restriction onto it (its sweeps are dense)
+static int row_levels(const clq_ctx *c) { return c->nlevels < 3 ? c->nlevels : 3; }
+
 static int ensure_active_rows(clq_ctx *c) {
     if (c->rows_valid) return CLQ_OK;
     if (c->nlevels < 2) return CLQ_OK;
     if (!c->row_pos) {
         const int64_t r0 = (int64_t)c->nx * c->ny;
         void *q;
-        for (int l = 0; l < 2; ++l) {
+        for (int l = 0; l < row_levels(c); ++l) {
             const int64_t r = (int64_t)c->lv[l].nx * c->lv[l].ny;
             if (int rc = c->alloc(&q, (size_t)r * 4)) return rc;
             c->row_flags[l] = (int32_t *)q;
@@ -560,20 +565,20 @@ static int ensure_active_rows(clq_ctx *c) {
         if (int rc = c->alloc(&q, (size_t)c->C)) return rc;
         c->op0 = (uint8_t *)q;
     }
-    HIP_TRY(hipMemsetAsync(c->row_count + 2, 0, 4, c->stream));
-    launch_pack_operator(c->lv[0], c->op0, c->row_count + 2, c->stream);
+    HIP_TRY(hipMemsetAsync(c->row_count + 3, 0, 4, c->stream));
+    launch_pack_operator(c->lv[0], c->op0, c->row_count + 3, c->stream);
     launch_row_flags(c->lv[0], c->row_flags[0], c->stream);
-    launch_coarse_row_flags(c->row_flags[0], c->lv[0], c->lv[1], c->row_flags[1], c->stream);
-    for (int l = 0; l < 2; ++l) {
+    for (int l = 1; l < row_levels(c); ++l)
+        launch_coarse_row_flags(c->row_flags[l - 1], c->lv[l - 1], c->lv[l], c->row_flags[l], c->stream);
+    for (int l = 0; l < row_levels(c); ++l) {
         const int64_t r = (int64_t)c->lv[l].nx * c->lv[l].ny;
         launch_exclusive_scan(c->row_flags[l], c->row_pos, r, c->scan_scratch, c->stream);
         launch_compact_rows(c->row_flags[l], c->row_pos, r, c->rows[l], c->row_count + l, c->stream);
     }
-    HIP_TRY(hipMemcpyAsync(c->row_count_host, c->row_count, 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->row_count_host, c->row_count, 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    c->nrows[0] = c->row_count_host[0];
-    c->nrows[1] = c->row_count_host[1];
-    c->op0_ok = c->row_count_host[2] == 0;
+    for (int l = 0; l < row_levels(c); ++l) c->nrows[l] = c->row_count_host[l];
+    c->op0_ok = c->row_count_host[3] == 0;
     c->rows_valid = true;
     return check_launch("active rows");
 }
@@ -583,8 +588,7 @@ static int ensure_active_rows(clq_ctx *c) {
 static int begin_sparse_solve(clq_ctx *c) {
     if (int rc = ensure_active_rows(c)) return rc;
     if (c->nlevels < 2) return CLQ_OK;
-    launch_zero_inactive_rows(c->lv[0], c->row_flags[0], 0, c->stream);
-    launch_zero_inactive_rows(c->lv[1], c->row_flags[1], 1, c->stream);
+    for (int l = 0; l < row_levels(c); ++l) launch_zero_inactive_rows(c->lv[l], c->row_flags[l], l > 0, c->stream);
     c->use_rows = true;
     return CLQ_OK;
 }
@@ -633,9 +637,9 @@ static void smooth_prolong(clq_ctx *c, int level) {
 
 static void restrict_level(clq_ctx *c, int level, bool red = false) {
     const float delta = (float)c->dx;
-    if (c->use_rows && level == 0)
-        launch_restrict_rows(c->lv[0], c->op0_ok ? c->op0 : nullptr, c->lv[1], 1.0f / (delta * delta), c->rows[1],
-                             c->nrows[1], delta, red, c->stream);
+    if (c->use_rows && level + 1 < row_levels(c))  // onto the coarse level's active rows
+        launch_restrict_rows(c->lv[level], level == 0 && c->op0_ok ? c->op0 : nullptr, c->lv[level + 1],
+                             1.0f / (delta * delta), c->rows[level + 1], c->nrows[level + 1], delta, red, c->stream);
     else
         launch_restrict_residual(c->lv[level], c->lv[level + 1], 1.0f / (delta * delta), delta, red, c->stream);
 }

@@ -639,6 +639,8 @@ void launch_restrict_rows(const Level &F, const uint8_t *op, const Level &K, flo
                           int64_t nkrows, float delta, int red, hipStream_t s) {
     const int64_t work = nkrows * K.nz;
     if (work == 0) return;
+    // one lane per coarse cell measured best for the restrictions onto the
+    // active rows of levels 1 (11.5 against 12.2 us with eight lanes) and 2
     if (kRestrictLanesRows == 1)
         hipLaunchKernelGGL(restrict_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, op, K,
                            inv_delta2, krows, work, delta * delta, red);
