@@ -640,7 +640,8 @@ void launch_restrict_rows(const Level &F, const uint8_t *op, const Level &K, flo
     const int64_t work = nkrows * K.nz;
     if (work == 0) return;
     // one lane per coarse cell measured best for the restrictions onto the
-    // active rows of levels 1 (11.5 against 12.2 us with eight lanes) and 2
+    // active rows of levels 1 and 2 (onto level 2 at 256^3: 11.5 against
+    // 12.2 us with eight lanes per coarse cell)
     if (kRestrictLanesRows == 1)
         hipLaunchKernelGGL(restrict_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, F, op, K,
                            inv_delta2, krows, work, delta * delta, red);
