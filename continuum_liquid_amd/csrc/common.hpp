@@ -156,8 +156,10 @@ void launch_build_finest_operator(const float *mask, float *diag, float *wx, flo
 void launch_neumann_bc(float *p, int nx, int ny, int nz, hipStream_t s);
 void launch_pressure_correction(float *u, float *v, float *w, const float *p, float dt_over_rho, float inv_dx,
                                 int nx, int ny, int nz, hipStream_t s);
-void launch_pressure_correction64(float *u, float *v, float *w, const double *p, double dt_over_rho, double inv_dx,
-                                  int nx, int ny, int nz, hipStream_t s);
+// flags (may be NULL): level-0 row flags of a pressure that is 0 off the
+// interior fluid cells; rows that cannot change are skipped
+void launch_pressure_correction64(float *u, float *v, float *w, const double *p, const int32_t *flags,
+                                  double dt_over_rho, double inv_dx, int nx, int ny, int nz, hipStream_t s);
 
 // multigrid.hip
 void launch_rbgs(const Level &L, float delta, int parity, hipStream_t s);

@@ -1201,9 +1201,12 @@ static int pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stat
 int clq_pressure_correction(clq_ctx *c, double rho, double dt) {
     if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
     DeviceGuard g(c->device);
+    // the FP64 pressure of modes 2-4 is 0 off the interior fluid cells: with
+    // the operator's row flags, rows that cannot change are skipped
     if (c->p64_valid)
-        launch_pressure_correction64(c->u, c->v, c->w, c->p64a, (double)(float)(dt / rho),
-                                     (double)(float)(1.0 / c->dx), c->nx, c->ny, c->nz, c->stream);
+        launch_pressure_correction64(c->u, c->v, c->w, c->p64a, c->rows_valid ? c->row_flags[0] : nullptr,
+                                     (double)(float)(dt / rho), (double)(float)(1.0 / c->dx), c->nx, c->ny, c->nz,
+                                     c->stream);
     else
         launch_pressure_correction(c->u, c->v, c->w, c->p, (float)(dt / rho), (float)(1.0 / c->dx), c->nx, c->ny,
                                    c->nz, c->stream);

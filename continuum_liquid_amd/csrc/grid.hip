@@ -309,8 +309,41 @@ __global__ __launch_bounds__(256) void pressure_correction64_kernel(float *__res
     w[index] = (float)((double)w[index] - dt_over_rho * gz);
 }
 
-void launch_pressure_correction64(float *u, float *v, float *w, const double *p, double dt_over_rho, double inv_dx,
-                                  int nx, int ny, int nz, hipStream_t s) {
+// pressure_correction64_kernel by rows, given the level-0 row flags (rows
+// holding an interior fluid cell) of a pressure that is 0 at every other
+// cell: a row whose flag and whose +x and +y neighbour rows' flags (indices
+// clamped as the gradient clamps them) are all 0 sees only zero gradients,
+// u - dt/rho * 0 == u, so its block returns at once.  Same values as the
+// kernel over all cells.
+__global__ __launch_bounds__(256) void pressure_correction64_rows_kernel(float *__restrict__ u, float *__restrict__ v,
+                                                                         float *__restrict__ w,
+                                                                         const double *__restrict__ p,
+                                                                         const int32_t *__restrict__ flags,
+                                                                         double dt_over_rho, double inv_dx, int nx,
+                                                                         int ny, int nz) {
+    const int64_t row = blockIdx.x;
+    const int i = (int)(row / ny), j = (int)(row - (int64_t)i * ny);
+    if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2) return;  // no interior cell in the row
+    if (!flags[row] && !flags[(int64_t)min(i + 1, nx - 2) * ny + j] && !flags[row - j + min(j + 1, ny - 2)]) return;
+    for (int k = 1 + threadIdx.x; k <= nz - 2; k += 256) {
+        const int64_t index = row * nz + k;
+        const double centre = p[index];
+        const double gx = (p[gidx(min(i + 1, nx - 2), j, k, ny, nz)] - centre) * inv_dx;
+        const double gy = (p[gidx(i, min(j + 1, ny - 2), k, ny, nz)] - centre) * inv_dx;
+        const double gz = (p[gidx(i, j, min(k + 1, nz - 2), ny, nz)] - centre) * inv_dx;
+        u[index] = (float)((double)u[index] - dt_over_rho * gx);
+        v[index] = (float)((double)v[index] - dt_over_rho * gy);
+        w[index] = (float)((double)w[index] - dt_over_rho * gz);
+    }
+}
+
+void launch_pressure_correction64(float *u, float *v, float *w, const double *p, const int32_t *flags,
+                                  double dt_over_rho, double inv_dx, int nx, int ny, int nz, hipStream_t s) {
+    if (flags) {
+        hipLaunchKernelGGL(pressure_correction64_rows_kernel, dim3((unsigned)((int64_t)nx * ny)), dim3(256), 0, s, u,
+                           v, w, p, flags, dt_over_rho, inv_dx, nx, ny, nz);
+        return;
+    }
     const int64_t C = (int64_t)nx * ny * nz;
     hipLaunchKernelGGL(pressure_correction64_kernel, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, s, u, v, w, p,
                        dt_over_rho, inv_dx, nx, ny, nz);
