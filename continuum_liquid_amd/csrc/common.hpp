@@ -214,6 +214,7 @@ void launch_ir_update_residual(const Level &L, int from32, const float *p32, con
                                float *r32, const float *b, double inv_h2, double *partials, double *out2,
                                hipStream_t s);
 void launch_widen_narrow(const double *p64, float *p32, int64_t C, hipStream_t s);
+void launch_narrow_rows(const double *p64, float *p32, const int32_t *rows, int64_t nrows, int nz, hipStream_t s);
 int residual_partials_count(int64_t C);
 void launch_residual_norm(const Level &L, double inv_h2, double *partials, double *out2, hipStream_t s);
 

@@ -1035,7 +1035,11 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
         }
         c->lv[0].p = c->p;
     }
-    launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
+    // the pressure: float(x) on the active rows; the other rows' interior
+    // cells are 0 already (cleared at the solve's start, never written by
+    // the row kernels), and their ghost cells get the Neumann copy next
+    if (c->use_rows) launch_narrow_rows(c->p64a, c->p, c->rows[0], c->nrows[0], c->nz, c->stream);
+    else launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
     s->final_residual = r;
     s->relative_residual = bn > 0 ? r / bn : 0.0;
     return check_launch("solve_fcg");

@@ -77,13 +77,16 @@ void launch_grid_obstacle_bc(float *u, float *v, float *w, float *ff, const uint
     hipLaunchKernelGGL(extrapolate_obstacle_kernel, cells_grid(C), dim3(256), 0, s, u, v, w, ob, nx, ny, nz);
 }
 
-// The cells the domain condition touches: i, j or k in {0, n-2, n-1}.
-// Item t enumerates them without repeats: first the three x planes, then the
-// three y planes of the remaining x range, then the three z planes of the
-// remaining (x, y) range.
-__device__ __forceinline__ bool domain_cell(int64_t t, int nx, int ny, int nz, int &i, int &j, int &k) {
-    const int64_t a = 3LL * ny * nz, b = 3LL * (nx - 3) * nz, c = 3LL * (nx - 3) * (ny - 3);
-    auto side = [](int64_t s, int n) { return s == 0 ? 0 : (int)(n - 3 + s); };  // 0, n-2, n-1
+// The cells within S - 1 of a face: i, j or k in {0, n-S+1, ..., n-1} (S = 3:
+// {0, n-2, n-1}, the cells the domain condition touches; S = 2: {0, n-1}, the
+// ghost layer).  Item t enumerates them without repeats: first the S x planes,
+// then the S y planes of the remaining x range [1, n-S], then the S z planes
+// of the remaining (x, y) range.
+template <int S>
+__device__ __forceinline__ bool face_cell(int64_t t, int nx, int ny, int nz, int &i, int &j, int &k) {
+    const int mx = nx - S, my = ny - S;  // remaining ranges [1, n-S]
+    const int64_t a = (int64_t)S * ny * nz, b = (int64_t)S * mx * nz, c = (int64_t)S * mx * my;
+    auto side = [](int64_t s, int n) { return s == 0 ? 0 : (int)(n - S + s); };
     if (t < a) {
         const int64_t pl = t / ((int64_t)ny * nz), r = t - pl * ny * nz;
         i = side(pl, nx);
@@ -93,7 +96,7 @@ __device__ __forceinline__ bool domain_cell(int64_t t, int nx, int ny, int nz, i
     }
     t -= a;
     if (t < b) {
-        const int64_t pl = t / ((int64_t)(nx - 3) * nz), r = t - pl * (nx - 3) * nz;
+        const int64_t pl = t / ((int64_t)mx * nz), r = t - pl * mx * nz;
         j = side(pl, ny);
         i = 1 + (int)(r / nz);
         k = (int)(r - (int64_t)(i - 1) * nz);
@@ -101,23 +104,27 @@ __device__ __forceinline__ bool domain_cell(int64_t t, int nx, int ny, int nz, i
     }
     t -= b;
     if (t < c) {
-        const int64_t pl = t / ((int64_t)(nx - 3) * (ny - 3)), r = t - pl * (nx - 3) * (ny - 3);
+        const int64_t pl = t / ((int64_t)mx * my), r = t - pl * mx * my;
         k = side(pl, nz);
-        i = 1 + (int)(r / (ny - 3));
-        j = 1 + (int)(r - (int64_t)(i - 1) * (ny - 3));
+        i = 1 + (int)(r / my);
+        j = 1 + (int)(r - (int64_t)(i - 1) * my);
         return true;
     }
     return false;
 }
+template <int S>
+static int64_t face_cells(int nx, int ny, int nz) {
+    return (int64_t)S * ny * nz + (int64_t)S * (nx - S) * nz + (int64_t)S * (nx - S) * (ny - S);
+}
 
 // A ghost cell copies only components that are not walls at its clamped
 // interior cell, and interior lanes write only wall components: race free.
-// Launched over the cells that domain_cell enumerates (every other cell is
+// Launched over the cells that face_cell<3> enumerates (every other cell is
 // neither a ghost nor a wall face).
 __global__ __launch_bounds__(256) void grid_domain_kernel(float *__restrict__ u, float *__restrict__ v,
                                                           float *__restrict__ w, int nx, int ny, int nz) {
     int i, j, k;
-    if (!domain_cell((int64_t)blockIdx.x * 256 + threadIdx.x, nx, ny, nz, i, j, k)) return;
+    if (!face_cell<3>((int64_t)blockIdx.x * 256 + threadIdx.x, nx, ny, nz, i, j, k)) return;
     const int64_t index = gidx(i, j, k, ny, nz);
     const bool ghost = !interior(i, j, k, nx, ny, nz);
     const bool wu = i == 0 || i >= nx - 2;
@@ -138,8 +145,8 @@ __global__ __launch_bounds__(256) void grid_domain_kernel(float *__restrict__ u,
 }
 
 void launch_grid_domain_bc(float *u, float *v, float *w, int nx, int ny, int nz, hipStream_t s) {
-    const int64_t cells = 3LL * ny * nz + 3LL * (nx - 3) * nz + 3LL * (nx - 3) * (ny - 3);
-    hipLaunchKernelGGL(grid_domain_kernel, cells_grid(cells), dim3(256), 0, s, u, v, w, nx, ny, nz);
+    hipLaunchKernelGGL(grid_domain_kernel, cells_grid(face_cells<3>(nx, ny, nz)), dim3(256), 0, s, u, v, w, nx, ny,
+                       nz);
 }
 
 // laplacian (helper.cl:54-66): fma(-6, centre, neighbour_sum) * inv_dx2.
@@ -244,19 +251,16 @@ void launch_build_finest_operator(const float *mask, float *diag, float *wx, flo
                        nz);
 }
 
+// launched over the ghost layer (face_cell<2>): interior cells are only read
 __global__ __launch_bounds__(256) void neumann_kernel(float *__restrict__ p, int nx, int ny, int nz) {
-    const int64_t index = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (index >= (int64_t)nx * ny * nz) return;
     int i, j, k;
-    coords(index, ny, nz, i, j, k);
-    if (interior(i, j, k, nx, ny, nz)) return;
+    if (!face_cell<2>((int64_t)blockIdx.x * 256 + threadIdx.x, nx, ny, nz, i, j, k)) return;
     const int ii = min(max(i, 1), nx - 2), jj = min(max(j, 1), ny - 2), kk = min(max(k, 1), nz - 2);
-    p[index] = p[gidx(ii, jj, kk, ny, nz)];
+    p[gidx(i, j, k, ny, nz)] = p[gidx(ii, jj, kk, ny, nz)];
 }
 
 void launch_neumann_bc(float *p, int nx, int ny, int nz, hipStream_t s) {
-    const int64_t C = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(neumann_kernel, cells_grid(C), dim3(256), 0, s, p, nx, ny, nz);
+    hipLaunchKernelGGL(neumann_kernel, cells_grid(face_cells<2>(nx, ny, nz)), dim3(256), 0, s, p, nx, ny, nz);
 }
 
 __global__ __launch_bounds__(256) void pressure_correction_kernel(float *__restrict__ u, float *__restrict__ v,

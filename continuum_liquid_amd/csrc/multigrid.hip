@@ -1348,6 +1348,23 @@ void launch_widen_narrow(const double *p64, float *p32, int64_t C, hipStream_t s
     hipLaunchKernelGGL(narrow_kernel, dim3((unsigned)((C + 255) / 256)), dim3(256), 0, s, p64, p32, C);
 }
 
+// narrow_kernel on the cells of the listed rows (item t: cell t mod nz of
+// row t / nz)
+__global__ __launch_bounds__(256) void narrow_rows_kernel(const double *__restrict__ p64, float *__restrict__ p32,
+                                                          const int32_t *__restrict__ rows, int64_t work, int nz) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= work) return;
+    const int64_t a = t / nz, q = (int64_t)rows[a] * nz + (t - a * nz);
+    p32[q] = (float)p64[q];
+}
+
+void launch_narrow_rows(const double *p64, float *p32, const int32_t *rows, int64_t nrows, int nz, hipStream_t s) {
+    const int64_t work = nrows * nz;
+    if (work == 0) return;
+    hipLaunchKernelGGL(narrow_rows_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, p64, p32, rows,
+                       work, nz);
+}
+
 // ---- FP64 residual norm -------------------------------------------------------------
 constexpr int kNormThreads = 256;
 constexpr int kNormBlocks = 2048;

@@ -228,6 +228,9 @@ def test_fcg_steps_match_oracle(oracle_lib, hip_lib):
         assert sh["relative_residual"] <= 1e-8 and abs(so["cycles"] - sh["cycles"]) <= 1
         po, ph = o.get_field("pressure_fp64"), h.get_field("pressure_fp64")
         assert np.max(np.abs(po - ph)) <= 1e-6 * np.max(np.abs(po))
+        # the FP32 pressure (next step's initial guess), ghost layer included
+        po, ph = o.get_field("pressure"), h.get_field("pressure")
+        assert np.max(np.abs(po - ph)) <= 1e-6 * np.max(np.abs(po))
         for name in ("u", "v", "w"):
             fo, fh = o.get_field(name), h.get_field(name)
             assert np.max(np.abs(fo - fh)) <= 1e-5 * max(1.0, np.max(np.abs(fo))), name
