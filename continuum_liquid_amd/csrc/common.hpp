@@ -220,9 +220,14 @@ void launch_residual_norm(const Level &L, double inv_h2, double *partials, doubl
 
 // fcg.hip
 int fcg_partials_count();
-// flags (level-0 row flags, may be NULL): rows without interior fluid only get x = 0
-void launch_fcg_init(const Level &L, const float *p32, const float *b, double inv_h2, double *x, double *r, float *r32,
-                     const int32_t *flags, double *partials, double *s, hipStream_t st);
+// x = p32 on the interior fluid cells, r = b - A x, r32; r.r and b.b.  With
+// the level-0 row flags (flags, rows, nrows; else NULL): the listed active
+// rows get the full update (op: packed operator, may be NULL), an inactive
+// row only x = 0, and only when x_rows marks it as possibly non-zero or
+// x_rows_known is 0; x_rows then holds the row flags.
+void launch_fcg_init(const Level &L, const uint8_t *op, const float *p32, const float *b, double inv_h2, double *x,
+                     double *r, float *r32, const int32_t *flags, const int32_t *rows, int64_t nrows,
+                     int32_t *x_rows, int x_rows_known, double *partials, double *s, hipStream_t st);
 // the dots z.q, z.r, z.w (w = A z) -> beta, d.q of the new direction, alpha
 void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const double *qv, const double *r, int first,
                      double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st);

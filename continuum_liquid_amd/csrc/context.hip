@@ -86,6 +86,11 @@ struct clq_ctx {
     // (the update writes the next cycle's correction while it reads z)
     double *r64 = nullptr, *dir = nullptr, *q64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
     float *p_alt = nullptr;
+    // level-0 rows on which x (p64a) may be non-zero: the row flags of the
+    // last FCG initialisation by rows; x_rows_known is false after any other
+    // write of p64a (mixed refinement, a dense FCG solve, a field upload)
+    int32_t *x_rows = nullptr;
+    bool x_rows_known = false;
     // second pressure array of each dense level (>= 2): the fused
     // post-smoothing writes it, and it becomes the level's pressure
     float *p_swap[kMaxLevels] = {};
@@ -380,6 +385,7 @@ int clq_set_field(clq_ctx *c, int32_t f, const void *host, int64_t bytes) {
     HIP_TRY(hipMemcpyAsync(p, host, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (f == CLQ_FIELD_PRESSURE) c->p64_valid = false;
+    if (f == CLQ_FIELD_PRESSURE_FP64) c->x_rows_known = false;
     if (f == CLQ_FIELD_OBSTACLE) {
         const uint8_t *o = (const uint8_t *)host;
         c->has_obstacle = std::any_of(o, o + bytes, [](uint8_t x) { return x != 0; });
@@ -842,6 +848,7 @@ static int ensure_mixed_buffers(clq_ctx *c);
 
 static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
     if (int rc = ensure_mixed_buffers(c)) return rc;
+    c->x_rows_known = false;
     double r = 0.0, bn = 0.0;
     if (int rc = ir_residual(c, 1, nullptr, c->p64a, &r, &bn)) return rc;
     s->initial_residual = r;
@@ -971,6 +978,8 @@ static int ensure_fcg_buffers(clq_ctx *c) {
         c->fcg_s = (double *)q;
         if (int rc = alloc_zeroed(c, &q, (size_t)fcg_partials_count() * 8)) return rc;
         c->fcg_partials = (double *)q;
+        if (int rc = alloc_zeroed(c, &q, (size_t)c->nx * c->ny * 4)) return rc;
+        c->x_rows = (int32_t *)q;
     }
     return CLQ_OK;
 }
@@ -982,13 +991,21 @@ static int ensure_fcg_buffers(clq_ctx *c) {
 // packed level-0 operator when the row lists are in use and it is exact
 #define OP0 (c->use_rows && c->op0_ok ? c->op0 : nullptr)
 
+// the FCG initialisation from the level-0 pressure guess p32 (x, r, r32,
+// r.r and b.b)
+static void fcg_init(clq_ctx *c, const float *p32) {
+    const double h = (double)(float)c->dx;
+    launch_fcg_init(c->lv[0], OP0, p32, c->rhs, 1.0 / (h * h), c->p64a, c->r64, c->r32, FLAGS0, ROWS0, c->x_rows,
+                    c->x_rows_known, c->fcg_partials, c->fcg_s, c->stream);
+    c->x_rows_known = c->use_rows;
+}
+
 static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
     if (int rc = ensure_fcg_buffers(c)) return rc;
     const Level &L0 = c->lv[0];
     const double h = (double)(float)c->dx;
     const double inv_h2 = 1.0 / (h * h);
-    launch_fcg_init(L0, c->p, c->rhs, inv_h2, c->p64a, c->r64, c->r32, FLAGS0, c->fcg_partials, c->fcg_s,
-                    c->stream);
+    fcg_init(c, c->p);
     if (int rc = read_scalars(c)) return rc;
     double r = sqrt(c->norm_host[0]);
     const double bn = sqrt(c->norm_host[1]);
@@ -1064,14 +1081,12 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
     double r_warm = 0.0, bn = 0.0;
     if (int rc = residual_norm(c, &r_warm, &bn)) return rc;
     unet_predict(c->net, c->rhs, L0.mask, c->p, c->guess32, c->stream);
-    launch_fcg_init(L0, c->guess32, c->rhs, inv_h2, c->p64a, c->r64, c->r32, FLAGS0, c->fcg_partials, c->fcg_s,
-                    c->stream);
+    fcg_init(c, c->guess32);
     if (int rc = read_scalars(c)) return rc;
     double r = sqrt(c->norm_host[0]);
     bn = sqrt(c->norm_host[1]);
     if (!(r < r_warm)) {
-        launch_fcg_init(L0, c->p, c->rhs, inv_h2, c->p64a, c->r64, c->r32, FLAGS0, c->fcg_partials, c->fcg_s,
-                    c->stream);
+        fcg_init(c, c->p);
         if (int rc = read_scalars(c)) return rc;
         r = sqrt(c->norm_host[0]);
     }

@@ -19,10 +19,11 @@
 //   s[6] r.r        s[7] b.b
 // beta = -(z_{k+1} . q_k) / (d_k . q_k), the flexible (Polak-Ribiere) form
 // written with q_k = -(r_{k+1} - r_k) / alpha_k, so r_k need not be kept.
-// The initialisation writes all cells (0 outside the interior fluid cells);
-// the per-iteration kernels visit the active rows only (rows holding an
-// interior fluid cell, or all cells when no row list is given): outside them
-// x, r, z, d and q are zero or never read.
+// The initialisation writes all cells (0 outside the interior fluid cells),
+// or with the row lists the active rows (rows holding an interior fluid cell)
+// and x = 0 on the inactive rows that may hold another value; the
+// per-iteration kernels visit the active rows only (all cells when no row
+// list is given): outside them x, r, z, d and q are zero or never read.
 #include "common.hpp"
 
 namespace clq {
@@ -111,41 +112,6 @@ __global__ __launch_bounds__(kT) void fcg_init_kernel(Level L, const float *__re
     block_sums<2>(acc, partials);
 }
 
-// fcg_init_kernel by rows with the level-0 row flags: a row without an
-// interior fluid cell (flag 0) only gets x = 0 (the pressure correction reads
-// x at every interior cell; r, r32 are read there under zero weights or a
-// fluid select only, and b is 0 there), an active row the full update.
-__global__ __launch_bounds__(kT) void fcg_init_rows_kernel(Level L, const int32_t *__restrict__ flags,
-                                                          const float *__restrict__ p32, const float *__restrict__ b,
-                                                          double inv_h2, double *__restrict__ x,
-                                                          double *__restrict__ r, float *__restrict__ r32,
-                                                          double *__restrict__ partials) {
-    double acc[2] = {0.0, 0.0};
-    const int64_t nrows = (int64_t)L.nx * L.ny, per = (nrows + gridDim.x - 1) / gridDim.x;
-    const int64_t a0 = (int64_t)blockIdx.x * per, a1 = min(a0 + per, nrows);
-    for (int64_t a = a0; a < a1; ++a) {
-        const bool active = flags[a] != 0;
-        for (int k = threadIdx.x; k < L.nz; k += kT) {
-            const int64_t q = a * L.nz + k;
-            if (!active) {
-                x[q] = 0.0;
-                continue;
-            }
-            double xv = 0.0, rv = 0.0;
-            if (fluid_cell(L, q)) {
-                xv = (double)p32[q];
-                rv = (double)b[q] - apply_a(L, q, [&](int64_t t) { return (double)p32[t]; }, inv_h2);
-            }
-            x[q] = xv;
-            r[q] = rv;
-            r32[q] = (float)rv;
-            acc[0] += rv * rv;
-            acc[1] += (double)b[q] * (double)b[q];
-        }
-    }
-    block_sums<2>(acc, partials);
-}
-
 // A z at cell q = (i, j, k) (0 off the interior fluid cells), from the
 // packed level-0 operator (PACKED) or the float arrays.  Every load is issued
 // before any test: off the interior the neighbour indices fall back to q
@@ -197,6 +163,50 @@ __device__ __forceinline__ void for_rows(const Level &L, const int32_t *__restri
         const int i = (int)(row / L.ny), j = (int)(row - (int64_t)i * L.ny);
         const bool inner_row = i >= 1 && i <= L.nx - 2 && j >= 1 && j <= L.ny - 2;
         for (int k = threadIdx.x; k < L.nz; k += kT) cell(row * L.nz + k, inner_row && k >= 1 && k <= L.nz - 2, i, j, k);
+    }
+}
+
+// fcg_init_kernel over the listed active rows (rows holding an interior
+// fluid cell): the same values, A x from the packed operator (PACKED) or the
+// float arrays, every load issued before the tests.
+template <bool PACKED>
+__global__ __launch_bounds__(kT) void fcg_init_rows_kernel(Level L, const uint8_t *__restrict__ op,
+                                                          const float *__restrict__ p32, const float *__restrict__ b,
+                                                          double inv_h2, const int32_t *__restrict__ rows,
+                                                          int64_t nrows, double *__restrict__ x,
+                                                          double *__restrict__ r, float *__restrict__ r32,
+                                                          double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    for_rows(L, rows, nrows, [&](int64_t q, bool inner, int, int, int) {
+        const double bv = (double)b[q], pv = (double)p32[q];
+        bool fluid;
+        double dg;
+        const double av = apply_a_z<PACKED>(L, op, p32, q, inner, inv_h2, fluid, dg);
+        const double rv = fluid ? bv - av : 0.0;
+        x[q] = fluid ? pv : 0.0;
+        r[q] = rv;
+        r32[q] = (float)rv;
+        acc[0] += rv * rv;
+        acc[1] += bv * bv;
+    });
+    block_sums<2>(acc, partials);
+}
+
+// x = 0 on the inactive level-0 rows that may hold a non-zero value (x_rows
+// set, or every inactive row when !known); x_rows = flags afterwards.  On
+// those rows the pressure correction reads x, while r, r32, b are read there
+// only under zero weights or a fluid select.  Block b takes a contiguous run
+// of rows; the row test is block-uniform.
+__global__ __launch_bounds__(kT) void fcg_clear_rows_kernel(const int32_t *__restrict__ flags,
+                                                           int32_t *__restrict__ x_rows, int known, int64_t nrows,
+                                                           int nz, double *__restrict__ x) {
+    const int64_t per = (nrows + gridDim.x - 1) / gridDim.x;
+    const int64_t a0 = (int64_t)blockIdx.x * per, a1 = min(a0 + per, nrows);
+    for (int64_t a = a0; a < a1; ++a) {
+        const int32_t f = flags[a];
+        if (f == 0 && (!known || x_rows[a] != 0))
+            for (int k = threadIdx.x; k < nz; k += kT) x[a * nz + k] = 0.0;
+        if (threadIdx.x == 0) x_rows[a] = f;
     }
 }
 
@@ -313,13 +323,21 @@ static int64_t rows_of(const Level &L, const int32_t *rows, int64_t nrows) {
 }
 static int row_blocks_for(int64_t nr) { return (int)(nr < kB ? (nr > 0 ? nr : 1) : kB); }
 
-void launch_fcg_init(const Level &L, const float *p32, const float *b, double inv_h2, double *x, double *r, float *r32,
-                     const int32_t *flags, double *partials, double *s, hipStream_t st) {
+void launch_fcg_init(const Level &L, const uint8_t *op, const float *p32, const float *b, double inv_h2, double *x,
+                     double *r, float *r32, const int32_t *flags, const int32_t *rows, int64_t nrows,
+                     int32_t *x_rows, int x_rows_known, double *partials, double *s, hipStream_t st) {
     int nb;
     if (flags) {
-        nb = row_blocks_for((int64_t)L.nx * L.ny);
-        hipLaunchKernelGGL(fcg_init_rows_kernel, dim3(nb), dim3(kT), 0, st, L, flags, p32, b, inv_h2, x, r, r32,
-                           partials);
+        const int64_t all = (int64_t)L.nx * L.ny;
+        hipLaunchKernelGGL(fcg_clear_rows_kernel, dim3(row_blocks_for(all)), dim3(kT), 0, st, flags, x_rows,
+                           x_rows_known, all, L.nz, x);
+        nb = row_blocks_for(nrows);
+        if (op)
+            hipLaunchKernelGGL(fcg_init_rows_kernel<true>, dim3(nb), dim3(kT), 0, st, L, op, p32, b, inv_h2, rows,
+                               nrows, x, r, r32, partials);
+        else
+            hipLaunchKernelGGL(fcg_init_rows_kernel<false>, dim3(nb), dim3(kT), 0, st, L, op, p32, b, inv_h2, rows,
+                               nrows, x, r, r32, partials);
     } else {
         nb = blocks_for(L.C);
         hipLaunchKernelGGL(fcg_init_kernel, dim3(nb), dim3(kT), 0, st, L, p32, b, inv_h2, x, r, r32, partials);

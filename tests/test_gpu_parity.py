@@ -377,6 +377,34 @@ def test_weighted_cycle_breakdown_matches_oracle(oracle_lib, hip_lib):
     assert_fields_equal(o, h, ("pressure",))
 
 
+def test_fcg_clears_stale_fp64_pressure(oracle_lib, hip_lib):
+    """The FCG start clears the FP64 pressure on rows without interior fluid
+    only where it may be non-zero: after an upload of the FP64 pressure or a
+    mixed-refinement solve every such row is cleared again."""
+    scene = CASES[0]
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for c in (o, h):
+        run_to_pressure(c, scene)
+    kw = dict(post_smooth=2, coarse_extra=16, max_cycles=40, tolerance=1e-8)
+    p0 = h.get_field("pressure")
+    so = o.pressure_solve(runtime.SOLVE_FCG, **kw)
+    fluid = o.get_field("pressure_fp64") != 0
+    po = o.get_field("pressure_fp64")
+    for stale in ("first", "upload", "mixed", "none"):
+        if stale == "upload":
+            h.set_field("pressure_fp64", np.full(h.shape, 3.0, np.float64))
+        elif stale == "mixed":
+            h.set_field("pressure_fp64", np.full(h.shape, -2.0, np.float64))
+            h.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, max_cycles=1, tolerance=1e-8)
+        h.set_field("pressure", p0)
+        sh = h.pressure_solve(runtime.SOLVE_FCG, **kw)
+        print(f"{stale}: oracle {so['cycles']} device {sh['cycles']}", flush=True)
+        assert abs(so["cycles"] - sh["cycles"]) <= 1
+        ph = h.get_field("pressure_fp64")
+        assert not np.any(ph[~fluid])
+        assert np.max(np.abs(ph - po)) <= 1e-6 * np.max(np.abs(po))
+
+
 @pytest.mark.parametrize("weight", [1.0, 1.6])
 def test_fcg_matches_oracle(oracle_lib, hip_lib, weight):
     """FP64 FCG: the dot products are summed in different trees on the two
