@@ -151,8 +151,9 @@ void launch_update_velocity(const float *u, const float *v, const float *w, floa
                             int nx, int ny, int nz, hipStream_t s);
 void launch_pressure_rhs(const float *u, const float *v, const float *w, const float *ff, float *rhs,
                          float rho_over_dt, float inv_dx, int nx, int ny, int nz, hipStream_t s);
-void launch_build_finest_operator(const float *mask, float *diag, float *wx, float *wy, float *wz, int nx, int ny,
-                                  int nz, hipStream_t s);
+// op, flags (may be NULL): also the packed level-0 operator and row flags
+void launch_build_finest_operator(const float *mask, float *diag, float *wx, float *wy, float *wz, uint8_t *op,
+                                  int32_t *flags, int nx, int ny, int nz, hipStream_t s);
 void launch_neumann_bc(float *p, int nx, int ny, int nz, hipStream_t s);
 void launch_pressure_correction(float *u, float *v, float *w, const float *p, float dt_over_rho, float inv_dx,
                                 int nx, int ny, int nz, hipStream_t s);

@@ -103,6 +103,9 @@ struct clq_ctx {
     bool rows_valid = false, use_rows = false;
     uint8_t *op0 = nullptr;  // packed level-0 operator (valid when op0_ok)
     bool op0_ok = false;
+    // op0 and the level-0 row flags were written by the operator build from
+    // the current level-0 operator (cleared by every write of its inputs)
+    bool level0_packed = false;
     // learned preconditioner (not owned) and its FP32 initial-guess buffer
     clq_net *net = nullptr;
     double stall_ratio = 0.5;
@@ -392,7 +395,7 @@ int clq_set_field(clq_ctx *c, int32_t f, const void *host, int64_t bytes) {
     }
     if (f == CLQ_FIELD_FLUID_FRACTION || f == CLQ_FIELD_DIAGONAL || f == CLQ_FIELD_WEIGHT_X ||
         f == CLQ_FIELD_WEIGHT_Y || f == CLQ_FIELD_WEIGHT_Z)
-        c->coarse_ops_valid = false;
+        c->coarse_ops_valid = c->level0_packed = false;
     return CLQ_OK;
 }
 
@@ -439,7 +442,7 @@ int clq_set_level_field(clq_ctx *c, int32_t l, int32_t f, const float *host, int
     DeviceGuard g(c->device);
     HIP_TRY(hipMemcpyAsync(a, host, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (f >= CLQ_LEVEL_MASK) c->coarse_ops_valid = false;
+    if (f >= CLQ_LEVEL_MASK) c->coarse_ops_valid = c->level0_packed = false;
     return CLQ_OK;
 }
 
@@ -483,7 +486,7 @@ int clq_particle_to_grid(clq_ctx *c, double particles_per_cell) {
                (float)(1.0 / particles_per_cell), c->nx, c->ny, c->nz, c->stream);
     kernel_mark(c, CLQ_KERNEL_P2G);
     stage_end(c, CLQ_STAGE_P2G);
-    c->coarse_ops_valid = false;
+    c->coarse_ops_valid = c->level0_packed = false;
     return check_launch("particle_to_grid");
 }
 
@@ -493,7 +496,7 @@ int clq_grid_obstacle_bc(clq_ctx *c) {
     if (!c->has_obstacle) return CLQ_OK;  // every write of the two kernels needs a solid voxel
     DeviceGuard g(c->device);
     launch_grid_obstacle_bc(c->u, c->v, c->w, c->ff, c->obstacle, c->nx, c->ny, c->nz, c->stream);
-    c->coarse_ops_valid = false;
+    c->coarse_ops_valid = c->level0_packed = false;
     return check_launch("grid_obstacle_bc");
 }
 
@@ -529,11 +532,21 @@ int clq_pressure_rhs(clq_ctx *c, double rho, double dt) {
     return check_launch("pressure_rhs");
 }
 
+static int ensure_row_buffers(clq_ctx *c);
+
 int clq_build_operator(clq_ctx *c) {
     if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
     DeviceGuard g(c->device);
-    launch_build_finest_operator(c->ff, c->diag, c->wx, c->wy, c->wz, c->nx, c->ny, c->nz, c->stream);
+    // with the row lists in use (two levels or more), the build also packs
+    // the level-0 operator and flags the active rows
+    const bool pack = c->nlevels >= 2;
+    if (pack)
+        if (int rc = ensure_row_buffers(c)) return rc;
+    launch_build_finest_operator(c->ff, c->diag, c->wx, c->wy, c->wz, pack ? c->op0 : nullptr,
+                                 pack ? c->row_flags[0] : nullptr, c->nx, c->ny, c->nz, c->stream);
     c->coarse_ops_valid = false;
+    c->rows_valid = false;  // the row flags now follow the new operator
+    c->level0_packed = pack;
     return check_launch("build_operator");
 }
 
@@ -550,9 +563,7 @@ static void ensure_coarse_operators(clq_ctx *c) {
 // serves the restriction onto it (its sweeps are dense)
 static int row_levels(const clq_ctx *c) { return c->nlevels < 3 ? c->nlevels : 3; }
 
-static int ensure_active_rows(clq_ctx *c) {
-    if (c->rows_valid) return CLQ_OK;
-    if (c->nlevels < 2) return CLQ_OK;
+static int ensure_row_buffers(clq_ctx *c) {
     if (!c->row_pos) {
         const int64_t r0 = (int64_t)c->nx * c->ny;
         void *q;
@@ -571,9 +582,18 @@ static int ensure_active_rows(clq_ctx *c) {
         if (int rc = c->alloc(&q, (size_t)c->C)) return rc;
         c->op0 = (uint8_t *)q;
     }
+    return CLQ_OK;
+}
+
+static int ensure_active_rows(clq_ctx *c) {
+    if (c->rows_valid) return CLQ_OK;
+    if (c->nlevels < 2) return CLQ_OK;
+    if (int rc = ensure_row_buffers(c)) return rc;
     HIP_TRY(hipMemsetAsync(c->row_count + 3, 0, 4, c->stream));
-    launch_pack_operator(c->lv[0], c->op0, c->row_count + 3, c->stream);
-    launch_row_flags(c->lv[0], c->row_flags[0], c->stream);
+    if (!c->level0_packed) {  // the operator build's bytes are exact
+        launch_pack_operator(c->lv[0], c->op0, c->row_count + 3, c->stream);
+        launch_row_flags(c->lv[0], c->row_flags[0], c->stream);
+    }
     for (int l = 1; l < row_levels(c); ++l)
         launch_coarse_row_flags(c->row_flags[l - 1], c->lv[l - 1], c->lv[l], c->row_flags[l], c->stream);
     for (int l = 0; l < row_levels(c); ++l) {

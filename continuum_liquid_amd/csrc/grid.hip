@@ -244,11 +244,51 @@ __global__ __launch_bounds__(256) void build_finest_operator_kernel(const float 
     wz[index] = c;
 }
 
-void launch_build_finest_operator(const float *mask, float *diag, float *wx, float *wy, float *wz, int nx, int ny,
-                                  int nz, hipStream_t s) {
+// build_finest_operator_kernel by rows (block = row (i, j), thread = k), also
+// writing the packed byte of each cell (bits 0-2: wx, wy, wz != 0, bit 3:
+// mask > 0, bits 4-7: the diagonal, which is an integer 0..6 here) and the
+// row flag (1 when the row holds an interior cell with mask > 0).
+__global__ __launch_bounds__(256) void build_finest_operator_rows_kernel(
+    const float *__restrict__ mask, float *__restrict__ diag, float *__restrict__ wx, float *__restrict__ wy,
+    float *__restrict__ wz, uint8_t *__restrict__ op, int32_t *__restrict__ flags, int nx, int ny, int nz) {
+    const int64_t row = blockIdx.x;
+    const int i = (int)(row / ny), j = (int)(row - (int64_t)i * ny);
+    const bool inner_row = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2;
+    const int64_t sx = inner_row && i < nx - 2 ? (int64_t)ny * nz : 0, sy = inner_row && j < ny - 2 ? nz : 0;
+    bool any = false;
+    for (int k = threadIdx.x; k < nz; k += 256) {
+        const int64_t index = row * nz + k;
+        const bool inner = inner_row && k >= 1 && k <= nz - 2;
+        const float m = mask[index], mx = mask[index + sx], my = mask[index + sy];
+        const float mz = mask[index + (inner && k < nz - 2 ? 1 : 0)];
+        const bool fluid = inner && m > 0.0f;
+        int d = 0, a = 0, b = 0, c = 0;
+        if (fluid) {
+            d = 6 - (i == 1) - (i == nx - 2) - (j == 1) - (j == ny - 2) - (k == 1) - (k == nz - 2);
+            a = i < nx - 2 && mx > 0.0f;
+            b = j < ny - 2 && my > 0.0f;
+            c = k < nz - 2 && mz > 0.0f;
+        }
+        any |= fluid;
+        diag[index] = (float)d;
+        wx[index] = (float)a;
+        wy[index] = (float)b;
+        wz[index] = (float)c;
+        op[index] = (uint8_t)(a | (b << 1) | (c << 2) | (m > 0.0f ? 8 : 0) | (d << 4));
+    }
+    const int active = __syncthreads_or(any);
+    if (threadIdx.x == 0) flags[row] = active;
+}
+
+void launch_build_finest_operator(const float *mask, float *diag, float *wx, float *wy, float *wz, uint8_t *op,
+                                  int32_t *flags, int nx, int ny, int nz, hipStream_t s) {
     const int64_t C = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(build_finest_operator_kernel, cells_grid(C), dim3(256), 0, s, mask, diag, wx, wy, wz, nx, ny,
-                       nz);
+    if (op && flags)
+        hipLaunchKernelGGL(build_finest_operator_rows_kernel, dim3((unsigned)((int64_t)nx * ny)), dim3(256), 0, s,
+                           mask, diag, wx, wy, wz, op, flags, nx, ny, nz);
+    else
+        hipLaunchKernelGGL(build_finest_operator_kernel, cells_grid(C), dim3(256), 0, s, mask, diag, wx, wy, wz, nx,
+                           ny, nz);
 }
 
 // launched over the ghost layer (face_cell<2>): interior cells are only read
