@@ -276,20 +276,35 @@ __global__ __launch_bounds__(kT) void fcg_update_kernel(Level L, const uint8_t *
 // Reduce `n` partial arrays of `blocks` entries and update the scalars.
 // stage 0: init (s6 = r.r, s7 = b.b); 2: r.r; 3: z.q, z.r, z.w -> beta,
 // d.q, alpha (first: beta = 0).
-__global__ __launch_bounds__(kT) void fcg_scalars_kernel(const double *__restrict__ partials, int blocks, int stage,
-                                                        int first, double *__restrict__ s) {
-    __shared__ double sh[3][kT];
+constexpr int kScalarThreads = 1024;
+
+__global__ __launch_bounds__(kScalarThreads) void fcg_scalars_kernel(const double *__restrict__ partials, int blocks,
+                                                                    int stage, int first, double *__restrict__ s) {
+    // each lane sums its partials (all loads in flight together), a
+    // butterfly sums each wave, and wave 0 adds the 16 wave sums
+    __shared__ double wsum[3][kScalarThreads / 64];
     const int n = stage == 0 ? 2 : stage == 3 ? 3 : 1;
-    for (int a = 0; a < n; ++a) {
-        double v = 0.0;
-        for (int q = threadIdx.x; q < blocks; q += kT) v += partials[a * kB + q];
-        sh[a][threadIdx.x] = v;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < blocks; q += kScalarThreads)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (a < n) v[a] += partials[a * kB + q];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[a] += __shfl_xor(v[a], off, 64);
+        if (lane == 0) wsum[a][wave] = v[a];
     }
     __syncthreads();
-    for (int off = kT / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off)
-            for (int a = 0; a < n; ++a) sh[a][threadIdx.x] += sh[a][threadIdx.x + off];
-        __syncthreads();
+    if (wave != 0) return;
+    double sh[3][1];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double t = lane < kScalarThreads / 64 ? wsum[a][lane] : 0.0;
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        sh[a][0] = t;
     }
     if (threadIdx.x != 0) return;
     if (stage == 0) {
@@ -342,7 +357,7 @@ void launch_fcg_init(const Level &L, const uint8_t *op, const float *p32, const 
         nb = blocks_for(L.C);
         hipLaunchKernelGGL(fcg_init_kernel, dim3(nb), dim3(kT), 0, st, L, p32, b, inv_h2, x, r, r32, partials);
     }
-    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 0, 0, s);
+    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kScalarThreads), 0, st, partials, nb, 0, 0, s);
 }
 
 
@@ -356,7 +371,7 @@ void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const do
     else
         hipLaunchKernelGGL(fcg_dots_kernel<false>, dim3(nb), dim3(kT), 0, st, L, op, z, qv, r, inv_h2, rows, nr,
                            partials);
-    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 3, first, s);
+    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kScalarThreads), 0, st, partials, nb, 3, first, s);
 }
 
 void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double *x, double *r, float *r32, double *d,
@@ -374,7 +389,7 @@ void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double
     else
         hipLaunchKernelGGL((fcg_update_kernel<false, false>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, d, qv,
                            s, first, inv_h2, rows, nr, e, dd, partials);
-    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kT), 0, st, partials, nb, 2, 0, s);
+    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kScalarThreads), 0, st, partials, nb, 2, 0, s);
 }
 
 }  // namespace clq
