@@ -195,18 +195,27 @@ __global__ __launch_bounds__(kT) void fcg_init_rows_kernel(Level L, const uint8_
 // x = 0 on the inactive level-0 rows that may hold a non-zero value (x_rows
 // set, or every inactive row when !known); x_rows = flags afterwards.  On
 // those rows the pressure correction reads x, while r, r32, b are read there
-// only under zero weights or a fluid select.  Block b takes a contiguous run
-// of rows; the row test is block-uniform.
+// only under zero weights or a fluid select.  Block b tests rows
+// [kT b, kT b + kT), one per thread, lists the rows to clear in LDS, and
+// clears them together (thread = k).
 __global__ __launch_bounds__(kT) void fcg_clear_rows_kernel(const int32_t *__restrict__ flags,
                                                            int32_t *__restrict__ x_rows, int known, int64_t nrows,
                                                            int nz, double *__restrict__ x) {
-    const int64_t per = (nrows + gridDim.x - 1) / gridDim.x;
-    const int64_t a0 = (int64_t)blockIdx.x * per, a1 = min(a0 + per, nrows);
-    for (int64_t a = a0; a < a1; ++a) {
+    __shared__ int32_t list[kT];
+    __shared__ int count;
+    if (threadIdx.x == 0) count = 0;
+    __syncthreads();
+    const int64_t a = (int64_t)blockIdx.x * kT + threadIdx.x;
+    if (a < nrows) {
         const int32_t f = flags[a];
-        if (f == 0 && (!known || x_rows[a] != 0))
-            for (int k = threadIdx.x; k < nz; k += kT) x[a * nz + k] = 0.0;
-        if (threadIdx.x == 0) x_rows[a] = f;
+        if (f == 0 && (!known || x_rows[a] != 0)) list[atomicAdd(&count, 1)] = threadIdx.x;
+        x_rows[a] = f;
+    }
+    __syncthreads();
+    const int n = count;
+    for (int t = 0; t < n; ++t) {
+        const int64_t row = (int64_t)blockIdx.x * kT + list[t];
+        for (int k = threadIdx.x; k < nz; k += kT) x[row * nz + k] = 0.0;
     }
 }
 
@@ -344,8 +353,8 @@ void launch_fcg_init(const Level &L, const uint8_t *op, const float *p32, const 
     int nb;
     if (flags) {
         const int64_t all = (int64_t)L.nx * L.ny;
-        hipLaunchKernelGGL(fcg_clear_rows_kernel, dim3(row_blocks_for(all)), dim3(kT), 0, st, flags, x_rows,
-                           x_rows_known, all, L.nz, x);
+        hipLaunchKernelGGL(fcg_clear_rows_kernel, dim3((unsigned)((all + kT - 1) / kT)), dim3(kT), 0, st, flags,
+                           x_rows, x_rows_known, all, L.nz, x);
         nb = row_blocks_for(nrows);
         if (op)
             hipLaunchKernelGGL(fcg_init_rows_kernel<true>, dim3(nb), dim3(kT), 0, st, L, op, p32, b, inv_h2, rows,
