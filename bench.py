@@ -60,11 +60,11 @@ def algorithmic_bytes(n_particles: int, cells: int, swept_cells: float) -> dict:
 # HBM bytes measured with rocprofv3 FETCH_SIZE (x 2.0, calibrated on a plain
 # copy) and WRITE_SIZE on the 256^3 dam break + sphere with the FCG solve
 # (profiles/step_256cubed_fcg_pmc_traffic.md): per P2G launch
-# (1,046,574.2 KiB x 2.0 + 286,720.0 KiB) x 1024, and per swept cell of one
+# (1,023,232.9 KiB x 2.0 + 286,720.0 KiB) x 1024, and per swept cell of one
 # finest-level sweep (two colour launches of 58.3 MB over 4,245,504 cells).
 # Reported as `roofline.traffic` for that workload only; the counters cannot
 # be read from inside this process.
-MEASURED_TRAFFIC = {(3, "fcg"): {"p2g": lambda swept: 2_436_985_242,
+MEASURED_TRAFFIC = {(3, "fcg"): {"p2g": lambda swept: 2_389_182_259,
                                  "rbgs_fine_sweep": lambda swept: 27.47 * swept}}
 
 
