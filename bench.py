@@ -178,7 +178,7 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
 
-    scene = scenes.baseline_scene(args.config)
+    scene = scenes.bench_scene(args.config)
     settings = solver_settings(args.solver, args.tolerance)
     if args.vcycle_from is not None:
         settings["vcycle_from"] = args.vcycle_from

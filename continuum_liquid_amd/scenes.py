@@ -123,7 +123,7 @@ def dam_break(
     )
 
 
-def baseline_scene(config: int, seed: int = 0) -> Scene:
+def bench_scene(config: int, seed: int = 0) -> Scene:
     """The benchmark configurations 1, 2, 3 and 5 (4 is the learned solver,
     evaluated on the scene of configuration 2)."""
     if config == 1:

@@ -250,7 +250,7 @@ def test_tolerance_solve_matches_oracle(oracle_lib, hip_lib):
 
 def test_binning_full_size_properties(hip_lib):
     """Size-independent properties at the 256^3 workload (28.6M particles)."""
-    scene = scenes.baseline_scene(3)
+    scene = scenes.bench_scene(3)
     h = make_context(hip_lib, scene)
     h.bin_particles()
     counts = h.get_field("bin_counts").ravel().astype(np.int64)
