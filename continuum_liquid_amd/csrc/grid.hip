@@ -81,7 +81,7 @@ void launch_grid_obstacle_bc(float *u, float *v, float *w, float *ff, const uint
 // {0, n-2, n-1}, the cells the domain condition touches; S = 2: {0, n-1}, the
 // ghost layer).  Item t enumerates them without repeats: first the S x planes,
 // then the S y planes of the remaining x range [1, n-S], then the S z planes
-// of the remaining (x, y) range.
+// of the remaining (x, y) range, the S cells of one (x, y) row in a row.
 template <int S>
 __device__ __forceinline__ bool face_cell(int64_t t, int nx, int ny, int nz, int &i, int &j, int &k) {
     const int mx = nx - S, my = ny - S;  // remaining ranges [1, n-S]
@@ -103,9 +103,9 @@ __device__ __forceinline__ bool face_cell(int64_t t, int nx, int ny, int nz, int
         return true;
     }
     t -= b;
-    if (t < c) {
-        const int64_t pl = t / ((int64_t)mx * my), r = t - pl * mx * my;
-        k = side(pl, nz);
+    if (t < c) {  // plane fastest: a row's S cells (and the cells they copy) share cache lines
+        const int64_t r = t / S;
+        k = side(t - r * S, nz);
         i = 1 + (int)(r / my);
         j = 1 + (int)(r - (int64_t)(i - 1) * my);
         return true;
