@@ -1,67 +1,88 @@
-"""Continuum Liquid solver entry point and runtime message channel.
+"""Bake entry point of the add-on's solver package and its progress channel.
 
-Same public names and semantics as the reference's Solver/General/main.py:
-`set_message_callback`, `clear_message_callback` and `emit_message` guard one
-callback with a lock (reference main.py:13-32); `main(config)` dispatches on
-`simulation.settings.solver_backend` and prints the bake runtime on exit
-(reference main.py:35-52).  The "GPU" backend runs the gfx950 HIP step.  The
-reference's "CPU" backend ran its OpenCL kernels on a host CPU device; this
-build has no CPU compute path, so "CPU" raises instead of running elsewhere.
+Interface (reference Solver/General/main.py:13-52, used by the add-on's bake
+operator on a worker thread):
+
+* ``set_message_callback(cb)`` installs one callable that receives the solver's
+  progress dictionaries, ``clear_message_callback()`` removes it and
+  ``emit_message(msg)`` delivers a dictionary to whichever callable is
+  installed at that moment.  Installing, removing and looking up the callable
+  are serialised by one lock; the callable itself runs outside the lock, on
+  the solver's thread, so it may install or remove callables itself.
+* ``main(config)`` runs one bake described by the exported configuration and
+  returns ``None``.  ``simulation.settings.solver_backend`` names the device
+  class; the add-on offers "CPU" (its default, also assumed when the key is
+  absent) and "GPU", and both names start the same solver, which here is the
+  gfx950 HIP step.  Any other name is a ``ValueError``.  Whatever the outcome,
+  the wall time of the call is printed as ``Bake runtime: <seconds> s``
+  followed by a separator line.
 """
 
 from __future__ import annotations
 
 import threading
 from time import perf_counter
-from typing import Any, Callable
+from typing import Any, Callable, Optional
 
 MessageCallback = Callable[[dict[str, Any]], None]
 
-_message_callback: MessageCallback | None = None
-_message_callback_lock = threading.Lock()
+# The add-on writes these two names into the exported configuration.
+BACKENDS = ("CPU", "GPU")
+SEPARATOR = "#" * 64
+
+
+class _ProgressChannel:
+    """Holds the one installed progress callable behind a lock."""
+
+    def __init__(self) -> None:
+        self._guard = threading.Lock()
+        self._target: Optional[MessageCallback] = None
+
+    def install(self, target: Optional[MessageCallback]) -> None:
+        with self._guard:
+            self._target = target
+
+    def current(self) -> Optional[MessageCallback]:
+        with self._guard:
+            return self._target
+
+
+_channel = _ProgressChannel()
 
 
 def set_message_callback(callback: MessageCallback | None) -> None:
-    """Set the callback used for solver runtime messages."""
-    global _message_callback
-    with _message_callback_lock:
-        _message_callback = callback
+    """Install `callback` as the receiver of progress messages (None removes it)."""
+    _channel.install(callback)
 
 
 def clear_message_callback() -> None:
-    """Remove the current solver runtime callback."""
-    global _message_callback
-    with _message_callback_lock:
-        _message_callback = None
+    """Stop delivering progress messages."""
+    _channel.install(None)
 
 
 def emit_message(message: dict[str, Any]) -> None:
-    """Forward a runtime message to the active caller (outside the lock)."""
-    with _message_callback_lock:
-        callback = _message_callback
-    if callback is not None:
-        callback(message)
+    """Hand `message` to the receiver installed now, if there is one."""
+    receiver = _channel.current()
+    if receiver is not None:
+        receiver(message)
+
+
+def _backend_of(config: dict[str, Any]) -> str:
+    settings = (config.get("simulation") or {}).get("settings") or {}
+    return str(settings.get("solver_backend", "CPU")).strip().upper()
 
 
 def main(config: dict[str, Any]) -> None:
-    """Select the configured backend and execute the solver."""
-    total_start_time = perf_counter()
+    """Run the bake described by `config` on the HIP step."""
+    began = perf_counter()
     try:
-        simulation_config = config.get("simulation") or {}
-        settings = simulation_config.get("settings") or {}
-        solver_backend = str(settings.get("solver_backend", "CPU")).strip().upper()
+        backend = _backend_of(config)
+        if backend not in BACKENDS:
+            raise ValueError(f"Unsupported solver backend: {backend}")
+        # Both device classes of the add-on map onto the one HIP step.
+        from ..Kernel.solver import solver as run_bake
 
-        if solver_backend == "GPU":
-            from ..Kernel import solver as solver_kernel_module
-
-            return solver_kernel_module.solver(config)
-        if solver_backend == "CPU":
-            raise RuntimeError(
-                "The CPU backend is not part of the MI355X build: set "
-                "simulation.settings.solver_backend to 'GPU' to run the HIP step"
-            )
-        raise ValueError(f"Unsupported solver backend: {solver_backend}")
+        run_bake(config)
     finally:
-        total_runtime = perf_counter() - total_start_time
-        print(f"Bake runtime: {total_runtime:.3f} s")
-        print("################################################################")
+        print(f"Bake runtime: {perf_counter() - began:.3f} s")
+        print(SEPARATOR)

@@ -35,7 +35,7 @@ import numpy as np
 
 from ... import runtime
 from ..General.main import emit_message
-from . import boundary_conditions, output, particles, pressure_solve, source_particles, timing, velocity_update
+from . import output, pressure_solve, source_particles, timing
 
 FIELD_DTYPE = np.float32
 PARTICLES_PER_CELL = 8
@@ -58,38 +58,41 @@ class StepParams:
 
 def step(ctx, params: StepParams, timings=None) -> dict:
     """One time step on the device, in the reference order.  Returns the
-    pressure-solve statistics."""
+    pressure-solve statistics.  Each stage is one C-ABI call enqueued on the
+    context's stream (the reference's stage wrappers are cited per call)."""
     timings = timings or timing.DisabledTimings()
     with timings.section("solver", "particle_to_grid", ctx):
-        particles.particle_to_grid(ctx, params.particles_per_cell)
+        ctx.bin_particles()  # particles.py:73-140: count, scan, sort, swap
+        ctx.particle_to_grid(params.particles_per_cell)  # particles.py:142-171
     with timings.section("solver", "grid_obstacle_bc", ctx):
-        boundary_conditions.grid_obstacle_bc(ctx)
+        ctx.grid_obstacle_bc()  # boundary_conditions.py:53-79
     with timings.section("solver", "grid_domain_bc", ctx):
-        boundary_conditions.grid_domain_bc(ctx)
+        ctx.grid_domain_bc()  # boundary_conditions.py:36-50
     with timings.section("solver", "update_velocity", ctx):
-        velocity_update.update_velocity(ctx, params.gravity, params.force, params.rho, params.nu, params.dt)
+        # velocity_update.py:8-52 and the buffer swap of solver.py:374-376
+        ctx.update_velocity(params.gravity, params.force, params.rho, params.nu, params.dt)
     with timings.section("solver", "updated_grid_domain_bc", ctx):
-        boundary_conditions.grid_domain_bc(ctx)
-        boundary_conditions.grid_obstacle_bc(ctx)
+        ctx.grid_domain_bc()
+        ctx.grid_obstacle_bc()
     with timings.section("solver", "pressure_solve", ctx):
         stats = pressure_solve.pressure_solve(ctx, params.dt, params.rho, params.solver, params.observer)
     with timings.section("solver", "pressure_correction", ctx):
         pressure_solve.pressure_correction(ctx, params.dt, params.rho)
     with timings.section("solver", "projected_grid_domain_bc", ctx):
-        boundary_conditions.grid_domain_bc(ctx)
-        boundary_conditions.grid_obstacle_bc(ctx)
+        ctx.grid_domain_bc()
+        ctx.grid_obstacle_bc()
     if isinstance(timings, timing.DisabledTimings):
         ctx.grid_to_particle_advect(params.flip_ratio, params.dt)
         return stats
     with timings.section("solver", "grid_to_particle", ctx):
-        particles.grid_to_particle(ctx, params.flip_ratio)
-    ctx.save_previous_positions()
+        ctx.grid_to_particle(params.flip_ratio)  # particles.py:174-206
+    ctx.save_previous_positions()  # solver.py:469-474
     with timings.section("solver", "advect_particles", ctx):
-        particles.advect_particles(ctx, params.dt)
+        ctx.advect_particles(params.dt)  # particles.py:209-226
     with timings.section("solver", "particle_domain_bc", ctx):
-        boundary_conditions.particle_domain_bc(ctx)
+        ctx.particle_domain_bc()  # boundary_conditions.py:82-103
     with timings.section("solver", "particle_obstacle_bc", ctx):
-        boundary_conditions.particle_obstacle_bc(ctx)
+        ctx.particle_obstacle_bc()  # boundary_conditions.py:106-132
     return stats
 
 

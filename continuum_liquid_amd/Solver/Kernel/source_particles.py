@@ -1,16 +1,29 @@
-"""Host seeding of particles from source masks (reference
-Solver/Kernel/source_particles.py).
+"""Particle sources and sinks applied on the host before a step.
 
-Semantics kept: OUTFLOW removes particles whose cell lies in its mask, INFLOW
-(every step) and GEOMETRY (first step only) top every masked, non-solid cell
-up to `particle_density` particles placed uniformly at random in the cell,
-with the source velocity (reference source_particles.py:42-99).
+Semantics (reference Solver/Kernel/source_particles.py:21-92, applied at
+solver.py:253-306 and once at set-up):
 
-The reference builds masks by voxelising USD meshes (voxelise_mesh.py), which
-needs `pxr`; here a source carries its mask directly: `mask` (bool array of
-the grid shape) or `shapes`, a list of {"box": [[x0, y0, z0], [x1, y1, z1]]}
-or {"sphere": [[cx, cy, cz], r]} in domain coordinates, marking cells whose
-centre is inside.
+* A particle at x belongs to the cell ``floor(x / delta)`` per axis, evaluated
+  in the dtype of the position array and clamped into the grid.
+* Every OUTFLOW source first deletes the particles whose cell it marks, in the
+  order the sources are listed; surviving particles keep their order.
+* Then every INFLOW source (on every call) and every GEOMETRY source (on the
+  set-up call only), again in list order, tops up each cell it marks that is
+  not solid to ``particle_density`` particles (at least 1, default 8),
+  counting the particles present at that moment.  The new particles are
+  appended in C order of their cells, a cell missing m particles
+  contributing m consecutive ones.  Their positions are
+  ``(cell + u) * delta`` with u from a single ``rng.random((count, 3))`` draw
+  per source (float64, rounded to float32 once), and their velocity is the
+  source's ``velocity``.  A source that adds nothing draws nothing, so the
+  generator's stream, and with it every later position, matches the
+  reference for the same seed.
+
+A source carries its cells as ``mask`` (bool array of the grid shape),
+``shapes`` (``{"box": [[x0, y0, z0], [x1, y1, z1]]}`` or ``{"sphere": [[cx, cy,
+cz], r]}`` in domain coordinates, marking cells whose centre lies inside) or
+``mesh`` (triangles, voxelised by ray parity on the GPU).  The reference's
+USD ``geometry_inputs`` need the ``pxr`` package, which is not available.
 """
 
 from __future__ import annotations
@@ -18,25 +31,30 @@ from __future__ import annotations
 import numpy as np
 
 FIELD_DTYPE = np.float32
+DEFAULT_DENSITY = 8
 
 
 def shape_mask(shapes, grid_shape, delta) -> np.ndarray:
-    centres = [(np.arange(n) + 0.5) * delta for n in grid_shape]
-    X, Y, Z = np.meshgrid(*centres, indexing="ij")
-    mask = np.zeros(grid_shape, bool)
+    """Cells whose centre lies inside any of the boxes / spheres."""
+    axes = [(np.arange(n) + 0.5) * delta for n in grid_shape]
+    cx, cy, cz = np.meshgrid(*axes, indexing="ij")
+    inside = np.zeros(grid_shape, dtype=bool)
     for shape in shapes or ():
         if "box" in shape:
-            lo, hi = (np.asarray(v, float) for v in shape["box"])
-            mask |= (X >= lo[0]) & (X < hi[0]) & (Y >= lo[1]) & (Y < hi[1]) & (Z >= lo[2]) & (Z < hi[2])
+            lo, hi = (np.asarray(corner, dtype=float) for corner in shape["box"])
+            inside |= (lo[0] <= cx) & (cx < hi[0]) & (lo[1] <= cy) & (cy < hi[1]) & (lo[2] <= cz) & (cz < hi[2])
         elif "sphere" in shape:
-            c, r = shape["sphere"]
-            mask |= (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 < float(r) ** 2
+            centre, radius = shape["sphere"]
+            inside |= (cx - centre[0]) ** 2 + (cy - centre[1]) ** 2 + (cz - centre[2]) ** 2 < float(radius) ** 2
         else:
             raise ValueError(f"unknown source shape {shape}")
-    return mask
+    return inside
 
 
-def resolve_mask(entry: dict, grid_shape, delta) -> np.ndarray:
+def resolve_mask(entry: dict, grid_shape, delta, voxelise_mesh=None) -> np.ndarray:
+    """The bool cell mask a source or obstacle entry describes.  `voxelise_mesh`
+    (vertices, triangles, grid_shape, delta) -> bool mask serves `mesh`
+    entries."""
     if entry.get("mask") is not None:
         mask = np.asarray(entry["mask"], dtype=bool)
         if mask.shape != tuple(grid_shape):
@@ -44,60 +62,68 @@ def resolve_mask(entry: dict, grid_shape, delta) -> np.ndarray:
         return mask
     if entry.get("shapes"):
         return shape_mask(entry["shapes"], grid_shape, delta)
+    if entry.get("mesh") is not None:
+        if voxelise_mesh is None:
+            raise ValueError("a mesh entry needs the GPU voxeliser")
+        mesh = entry["mesh"]
+        return voxelise_mesh(np.asarray(mesh["vertices"]), np.asarray(mesh["triangles"]), grid_shape, delta)
     if entry.get("geometry_inputs"):
         raise NotImplementedError(
-            "USD mesh voxelisation is outside the HIP step; give the source a 'mask' or 'shapes' entry"
+            "USD geometry needs the pxr package; give the entry a 'mesh' (vertices, triangles), 'mask' or 'shapes'"
         )
-    return np.zeros(grid_shape, bool)
+    return np.zeros(grid_shape, dtype=bool)
 
 
-def particle_cells(positions, delta, grid_shape) -> np.ndarray:
+def cell_of(positions, delta, grid_shape) -> np.ndarray:
+    """Integer cell (n, 3) of each particle, clamped into the grid."""
     if len(positions) == 0:
         return np.empty((0, 3), dtype=np.int32)
-    cells = np.floor(np.asarray(positions) / delta).astype(np.int32)
-    return np.clip(cells, 0, np.asarray(grid_shape, dtype=np.int32) - 1)
+    upper = np.asarray(grid_shape, dtype=np.int32) - 1
+    return np.clip(np.floor(np.asarray(positions) / delta).astype(np.int32), 0, upper)
 
 
-def mask_membership(positions, mask, delta) -> np.ndarray:
-    cells = particle_cells(positions, delta, mask.shape)
-    if len(cells) == 0:
-        return np.zeros(0, dtype=bool)
-    return mask[cells[:, 0], cells[:, 1], cells[:, 2]]
+def _flat_cells(positions, delta, grid_shape) -> np.ndarray:
+    return np.ravel_multi_index(cell_of(positions, delta, grid_shape).T, grid_shape)
 
 
-def remove_masked(particles, velocities, mask, delta):
-    keep = ~mask_membership(particles, mask, delta)
-    return particles[keep], velocities[keep]
+def drop_in(positions, velocities, mask, delta):
+    """Particles (and their velocities) outside the cells `mask` marks."""
+    if len(positions) == 0:
+        return positions, velocities
+    keep = ~mask.reshape(-1)[_flat_cells(positions, delta, mask.shape)]
+    return positions[keep], velocities[keep]
 
 
-def fill_source(particles, velocities, source, obstacle_mask, delta, rng):
-    target = max(1, int(source.get("particle_density", 8)))
-    region = np.asarray(source["mask"], dtype=bool) & ~obstacle_mask
-    cells = np.argwhere(region)
-    if len(cells) == 0:
-        return particles, velocities
-    counts = np.zeros(region.size, dtype=np.int64)
-    if len(particles):
-        flat = np.ravel_multi_index(particle_cells(particles, delta, region.shape).T, region.shape)
-        counts = np.bincount(flat, minlength=region.size)
-    missing = np.maximum(0, target - counts.reshape(region.shape)[tuple(cells.T)])
-    spawn = np.repeat(cells, missing, axis=0)
-    if len(spawn) == 0:
-        return particles, velocities
-    new_pos = (spawn + rng.random((len(spawn), 3))) * delta
-    new_vel = np.broadcast_to(np.asarray(source.get("velocity", (0.0, 0.0, 0.0)), FIELD_DTYPE), new_pos.shape)
+def top_up(positions, velocities, source, solid, delta, rng):
+    """Append the particles that bring every open source cell to the density."""
+    density = max(1, int(source.get("particle_density", DEFAULT_DENSITY)))
+    open_cells = np.asarray(source["mask"], dtype=bool) & ~solid
+    flat_open = np.flatnonzero(open_cells)  # C order
+    if flat_open.size == 0:
+        return positions, velocities
+    present = np.zeros(open_cells.size, dtype=np.int64)
+    if len(positions):
+        present = np.bincount(_flat_cells(positions, delta, open_cells.shape), minlength=open_cells.size)
+    need = np.maximum(0, density - present[flat_open])
+    total = int(need.sum())
+    if total == 0:
+        return positions, velocities
+    corner = np.column_stack(np.unravel_index(np.repeat(flat_open, need), open_cells.shape))
+    placed = ((corner + rng.random((total, 3))) * delta).astype(FIELD_DTYPE)
+    speed = np.broadcast_to(np.asarray(source.get("velocity", (0.0, 0.0, 0.0)), dtype=FIELD_DTYPE), placed.shape)
     return (
-        np.ascontiguousarray(np.concatenate((particles, new_pos)), dtype=FIELD_DTYPE),
-        np.ascontiguousarray(np.concatenate((velocities, new_vel)), dtype=FIELD_DTYPE),
+        np.ascontiguousarray(np.concatenate((np.asarray(positions, FIELD_DTYPE), placed))),
+        np.ascontiguousarray(np.concatenate((np.asarray(velocities, FIELD_DTYPE), speed))),
     )
 
 
-def apply_sources(particles, velocities, sources, obstacle_mask, delta, rng, initial=False):
+def apply_sources(positions, velocities, sources, solid, delta, rng, initial=False):
+    """Sinks first, then the top-ups, each in list order."""
     for source in sources:
         if source.get("type") == "OUTFLOW":
-            particles, velocities = remove_masked(particles, velocities, source["mask"], delta)
+            positions, velocities = drop_in(positions, velocities, source["mask"], delta)
+    fill_kinds = ("INFLOW", "GEOMETRY") if initial else ("INFLOW",)
     for source in sources:
-        kind = source.get("type")
-        if kind == "INFLOW" or (initial and kind == "GEOMETRY"):
-            particles, velocities = fill_source(particles, velocities, source, obstacle_mask, delta, rng)
-    return particles, velocities
+        if source.get("type") in fill_kinds:
+            positions, velocities = top_up(positions, velocities, source, solid, delta, rng)
+    return positions, velocities

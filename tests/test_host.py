@@ -96,21 +96,102 @@ def test_debug_timing_reports_reference_sections(tmp_path, oracle_backend, capsy
         assert section in out
 
 
-def test_cpu_backend_is_rejected(tmp_path):
-    cfg = make_config(tmp_path)
-    cfg["simulation"]["settings"]["solver_backend"] = "CPU"
-    with pytest.raises(RuntimeError, match="CPU backend"):
+def test_cpu_backend_runs_the_same_step(tmp_path, oracle_backend):
+    # the add-on's default backend string ("CPU", also assumed when the key is
+    # missing) starts the same solver as "GPU" (reference main.py:41-46)
+    frames = {}
+    for backend in ("GPU", "cpu", None):
+        cfg = make_config(tmp_path / str(backend), frames=2)
+        if backend is None:
+            del cfg["simulation"]["settings"]["solver_backend"]
+        else:
+            cfg["simulation"]["settings"]["solver_backend"] = backend
         solver_main.main(cfg)
+        files = sorted((tmp_path / str(backend) / "bake" / "particles").glob("*.bin"))
+        frames[backend] = [f.read_bytes() for f in files]
+    assert len(frames["GPU"]) == 3
+    assert frames["cpu"] == frames["GPU"] == frames[None]
+    cfg = make_config(tmp_path / "other")
     cfg["simulation"]["settings"]["solver_backend"] = "VULKAN"
     with pytest.raises(ValueError, match="Unsupported solver backend"):
         solver_main.main(cfg)
 
 
-def test_usd_geometry_needs_a_mask(tmp_path, oracle_backend):
+def test_usd_geometry_needs_pxr(tmp_path, oracle_backend):
     cfg = make_config(tmp_path)
     cfg["simulation"]["sources"] = [{"type": "GEOMETRY", "geometry_inputs": [{"mesh_file": "geometry/a.usdc"}]}]
-    with pytest.raises(NotImplementedError, match="mask"):
+    with pytest.raises(NotImplementedError, match="pxr"):
         solver_main.main(cfg)
+
+
+def _seed_by_loops(positions, velocities, sources, solid, delta, rng, initial):
+    """The seeding rules restated with explicit loops over particles and cells
+    (reference source_particles.py:21-92): sinks first, then each filling
+    source tops its open cells up, cells in C order, one uniform draw per
+    source."""
+    shape = solid.shape
+    pos = [tuple(p) for p in np.asarray(positions, np.float32)]
+    vel = [tuple(v) for v in np.asarray(velocities, np.float32)]
+
+    def cell(p):
+        return tuple(min(max(int(np.floor(np.float32(c) / np.float32(delta))), 0), n - 1) for c, n in zip(p, shape))
+
+    for src in sources:
+        if src["type"] == "OUTFLOW":
+            kept = [(p, v) for p, v in zip(pos, vel) if not src["mask"][cell(p)]]
+            pos, vel = [p for p, _ in kept], [v for _, v in kept]
+    for src in sources:
+        if src["type"] == "INFLOW" or (initial and src["type"] == "GEOMETRY"):
+            target = max(1, int(src.get("particle_density", 8)))
+            counts = {}
+            for p in pos:
+                counts[cell(p)] = counts.get(cell(p), 0) + 1
+            spawn = []
+            for i in range(shape[0]):
+                for j in range(shape[1]):
+                    for k in range(shape[2]):
+                        if src["mask"][i, j, k] and not solid[i, j, k]:
+                            spawn += [(i, j, k)] * max(0, target - counts.get((i, j, k), 0))
+            if not spawn:
+                continue
+            draw = rng.random((len(spawn), 3))
+            for c, r in zip(spawn, draw):
+                pos.append(tuple(np.float32((c[a] + r[a]) * delta) for a in range(3)))
+                vel.append(tuple(np.float32(x) for x in src.get("velocity", (0.0, 0.0, 0.0))))
+    return np.array(pos, np.float32).reshape(-1, 3), np.array(vel, np.float32).reshape(-1, 3)
+
+
+def test_source_seeding_matches_the_rules():
+    from continuum_liquid_amd.Solver.Kernel import source_particles as sp
+
+    shape, delta = (6, 5, 7), 1.0 / 6
+    r = np.random.default_rng(3)
+    solid = r.random(shape) < 0.15
+    box = np.zeros(shape, bool)
+    box[1:4, 1:3, 2:6] = True
+    inflow = np.zeros(shape, bool)
+    inflow[4:6, 0:2, 0:3] = True
+    sink = np.zeros(shape, bool)
+    sink[0:3, 3:5, :] = True
+    sources = [{"type": "GEOMETRY", "mask": box, "particle_density": 5, "velocity": (0.5, -1.0, 0.25)},
+               {"type": "OUTFLOW", "mask": sink},
+               {"type": "INFLOW", "mask": inflow, "particle_density": 3, "velocity": (0.0, 0.0, -2.0)},
+               {"type": "INFLOW", "mask": np.zeros(shape, bool)}]
+    # a few particles already inside the inflow, the sink and the box
+    pos = (r.random((40, 3)) * np.array(shape) * delta).astype(np.float32)
+    vel = r.standard_normal((40, 3)).astype(np.float32)
+    for initial in (True, False):
+        got = sp.apply_sources(pos, vel, sources, solid, delta, np.random.default_rng(7), initial=initial)
+        want = _seed_by_loops(pos, vel, sources, solid, delta, np.random.default_rng(7), initial)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        assert got[0].dtype == np.float32 and got[1].dtype == np.float32
+        assert len(got[0]) > len(pos) - 12
+    # a second inflow call adds nothing and draws nothing: the generator stays put
+    once = sp.apply_sources(pos, vel, sources, solid, delta, np.random.default_rng(7))
+    rng = np.random.default_rng(11)
+    again = sp.apply_sources(once[0], once[1], sources[2:], solid, delta, rng)
+    assert np.array_equal(again[0], once[0])
+    assert rng.random() == np.random.default_rng(11).random()
 
 
 def test_stage_errors_propagate(tmp_path, oracle_backend):
