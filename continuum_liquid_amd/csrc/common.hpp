@@ -239,6 +239,22 @@ void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double
                        double *qv, int first, double inv_h2, const int32_t *rows, int64_t nrows, float *e, float delta,
                        double *partials, double *s, hipStream_t st);
 
+// r = b - A x from the FP64 iterate x over the listed rows, r32; r.r and
+// b.b into s[6], s[7]
+void launch_fcg_residual(const Level &L, const uint8_t *op, const double *x, const float *b, double inv_h2,
+                         const int32_t *rows, int64_t nrows, double *r, float *r32, double *partials, double *s,
+                         hipStream_t st);
+// FP64 Jacobi-preconditioned CG check solve (scalars c[0..5], error c[8..11])
+int check_partials_count();
+void launch_check_init(const Level &L, const float *b, double inv_h2, const int32_t *rows, int64_t nrows, double *x,
+                       double *r, double *z, double *p, double *partials, double *c, hipStream_t st);
+void launch_check_iteration(const Level &L, double inv_h2, const int32_t *rows, int64_t nrows, double *x, double *r,
+                            double *z, double *po, double *pn, double *qv, double *partials, double *c,
+                            hipStream_t st);
+// p64 or p32 (one of them NULL) against x: sum (p-x)^2, sum x^2, max|p-x|, max|x|
+void launch_check_error(const Level &L, const double *p64, const float *p32, const double *x, const int32_t *rows,
+                        int64_t nrows, double *partials, double *c, hipStream_t st);
+
 // context.hip: sets the message returned by clq_last_error(), returns code
 int set_error(int code, const char *fmt, ...);
 

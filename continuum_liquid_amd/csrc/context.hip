@@ -7,6 +7,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <vector>
 
 #include "common.hpp"
@@ -48,6 +50,11 @@ int check_launch(const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(CLQ_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
     return CLQ_OK;
+}
+
+int non_finite(double r) {
+    return fail(CLQ_ERR_NOT_CONVERGED, "pressure solve: residual norm is not finite (%g); the velocity field "
+                "or the operator holds NaN or Inf", r);
 }
 
 int coarse_size(int n) {  // multigrid.py:12-15
@@ -106,6 +113,12 @@ struct clq_ctx {
     // op0 and the level-0 row flags were written by the operator build from
     // the current level-0 operator (cleared by every write of its inputs)
     bool level0_packed = false;
+    // FP64 check solve (clq_fp64_check_solve), allocated on first use:
+    // x, r, z, two direction buffers, q; a float residual copy; partials and
+    // scalars
+    double *chk[6] = {};
+    float *chk_r32 = nullptr;
+    double *chk_partials = nullptr, *chk_c = nullptr;
     // learned preconditioner (not owned) and its FP32 initial-guess buffer
     clq_net *net = nullptr;
     double stall_ratio = 0.5;
@@ -871,6 +884,7 @@ static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     c->x_rows_known = false;
     double r = 0.0, bn = 0.0;
     if (int rc = ir_residual(c, 1, nullptr, c->p64a, &r, &bn)) return rc;
+    if (!std::isfinite(r) || !std::isfinite(bn)) return non_finite(r + bn);
     s->initial_residual = r;
     s->rhs_norm = bn;
     const int kind = sp->coarse_dtype == 1 ? 2 : 1;
@@ -890,6 +904,10 @@ static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
         if ((rc = ir_residual(c, 0, c->p64a, c->p64b, &r, &bn))) break;
         std::swap(c->p64a, c->p64b);
         s->cycles++;
+        if (!std::isfinite(r)) {
+            rc = non_finite(r);
+            break;
+        }
         if (r > 0.95 * before) break;
     }
     L0.b = c->rhs;
@@ -1020,6 +1038,17 @@ static void fcg_init(clq_ctx *c, const float *p32) {
     c->x_rows_known = c->use_rows;
 }
 
+// ||b - A x|| from the FP64 iterate (also r, r32 and r.r for a restart);
+// returns through *r
+static int true_residual(clq_ctx *c, double *r) {
+    const double h = (double)(float)c->dx;
+    launch_fcg_residual(c->lv[0], OP0, c->p64a, c->rhs, 1.0 / (h * h), ROWS0, c->r64, c->r32, c->fcg_partials,
+                        c->fcg_s, c->stream);
+    if (int rc = read_scalars(c)) return rc;
+    *r = sqrt(c->norm_host[0]);
+    return CLQ_OK;
+}
+
 static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s) {
     if (int rc = ensure_fcg_buffers(c)) return rc;
     const Level &L0 = c->lv[0];
@@ -1029,19 +1058,27 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     if (int rc = read_scalars(c)) return rc;
     double r = sqrt(c->norm_host[0]);
     const double bn = sqrt(c->norm_host[1]);
+    if (!std::isfinite(r) || !std::isfinite(bn)) return non_finite(r + bn);
     s->initial_residual = r;
     s->rhs_norm = bn;
+    s->recurrence_residual = r;
     // the update makes the next cycle's first level-0 red half-sweep
     const bool fold = fold_transfers(c) && c->op0_ok && sp->pre_smooth > 0 && swept_level(c, 0);
-    if (r > sp->tolerance * bn && sp->max_cycles > 0) {
+    bool plain = false;
+    // each pass of the outer loop starts the direction recurrence from the
+    // residual r = b - A x held in r64 / r32 (the initialisation, or the
+    // recomputation after a pass whose recurrence met the tolerance)
+    while (r > sp->tolerance * bn && s->cycles < sp->max_cycles) {
         // z: this iteration's preconditioned residual, the level-0
         // correction array of the cycle that made it; with the fold, the
         // update writes the next cycle's correction into the other array
         float *z = c->p, *zn = c->p_alt;
+        c->lv[0].p = z;
         if (int rc = precondition_wcycle(c, sp, r)) return rc;
         s->cycles++;
+        s->fallback_cycles += plain;
         launch_fcg_dots(L0, OP0, z, c->q64, c->r64, 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
-        bool first = true, plain = false;
+        bool first = true;
         for (;;) {
             launch_fcg_update(L0, OP0, z, c->p64a, c->r64, c->r32, c->dir, c->q64, first, inv_h2, ROWS0,
                               fold ? zn : nullptr, (float)c->dx, c->fcg_partials, c->fcg_s, c->stream);
@@ -1058,7 +1095,11 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
             }
             if (int rc = wait_scalars(c)) return rc;
             r = sqrt(c->norm_host[0]);
-            if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
+            if (!std::isfinite(r)) {
+                c->lv[0].p = c->p;
+                return non_finite(r);
+            }
+            if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles) break;
             first = drop_weight_on_breakdown(c);
             plain = plain || first;
             if (fold) {
@@ -1071,7 +1112,14 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
             launch_fcg_dots(L0, OP0, z, c->q64, c->r64, first, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         }
         c->lv[0].p = c->p;
+        // the recurrence says converged (or the cycles ran out): recompute
+        // the residual from x; it also seeds the restart if one follows
+        s->recurrence_residual = r;
+        if (int rc = true_residual(c, &r)) return rc;
+        if (!std::isfinite(r)) return non_finite(r);
+        if (r > sp->tolerance * bn && s->cycles < sp->max_cycles) s->restarts++;
     }
+    c->lv[0].p = c->p;
     // the pressure: float(x) on the active rows; the other rows' interior
     // cells are 0 already (cleared at the solve's start, never written by
     // the row kernels), and their ghost cells get the Neumann copy next
@@ -1123,7 +1171,12 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
         s->cycles++;
         return CLQ_OK;
     };
-    if (r > sp->tolerance * bn && sp->max_cycles > 0) {
+    if (!std::isfinite(r) || !std::isfinite(bn)) return non_finite(r + bn);
+    s->recurrence_residual = r;
+    // passes of the direction recurrence as in mode 4: a pass that meets the
+    // tolerance by its recurrence is checked against b - A x, which restarts
+    // the recurrence (with the W-cycle) when it falls short
+    while (r > sp->tolerance * bn && s->cycles < sp->max_cycles) {
         if (int rc = precondition(r)) return rc;
         launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         bool first = true;
@@ -1133,7 +1186,8 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
             if (int rc = read_scalars(c)) return rc;
             const double before = r;
             r = sqrt(c->norm_host[0]);
-            if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(r == r)) break;
+            if (!std::isfinite(r)) return non_finite(r);
+            if (r <= sp->tolerance * bn || s->cycles >= sp->max_cycles) break;
             // the weighted W-cycle's breakdown guard (the network's steps keep
             // the stall test only)
             const bool restart = !use_net && drop_weight_on_breakdown(c);
@@ -1143,11 +1197,84 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
                             c->stream);
             first = restart;
         }
+        s->recurrence_residual = r;
+        if (int rc = true_residual(c, &r)) return rc;
+        if (!std::isfinite(r)) return non_finite(r);
+        if (r > sp->tolerance * bn && s->cycles < sp->max_cycles) {
+            s->restarts++;
+            use_net = false;
+        }
     }
     launch_widen_narrow(c->p64a, c->p, c->C, c->stream);
     s->final_residual = r;
     s->relative_residual = bn > 0 ? r / bn : 0.0;
     return check_launch("solve_learned");
+}
+
+// ---- FP64 check solve -----------------------------------------------------------------------
+int clq_fp64_check_solve(clq_ctx *c, double tolerance, int32_t max_iterations, clq_check_stats *st) {
+    if (!c || !st || !(tolerance > 0.0) || max_iterations < 0) return fail(CLQ_ERR_INVALID, "bad check-solve arguments");
+    DeviceGuard g(c->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!c->chk_c) {
+        void *q;
+        for (double *&a : c->chk) {
+            if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
+            a = (double *)q;
+        }
+        if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 4)) return rc;
+        c->chk_r32 = (float *)q;
+        if (int rc = alloc_zeroed(c, &q, (size_t)check_partials_count() * 8)) return rc;
+        c->chk_partials = (double *)q;
+        if (int rc = alloc_zeroed(c, &q, 16 * 8)) return rc;
+        c->chk_c = (double *)q;
+    }
+    // the active rows of the current operator hold every interior fluid cell
+    if (int rc = ensure_active_rows(c)) return rc;
+    const int32_t *rows = c->nlevels >= 2 ? c->rows[0] : nullptr;
+    const int64_t nrows = c->nlevels >= 2 ? c->nrows[0] : 0;
+    const Level &L0 = c->lv[0];
+    const double h = (double)(float)c->dx;
+    const double inv_h2 = 1.0 / (h * h);
+    double *x = c->chk[0], *r = c->chk[1], *z = c->chk[2], *pa = c->chk[3], *pb = c->chk[4], *qv = c->chk[5];
+    double host[16];
+    auto read = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(host, c->chk_c, sizeof host, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return CLQ_OK;
+    };
+    launch_check_init(L0, c->rhs, inv_h2, rows, nrows, x, r, z, pa, c->chk_partials, c->chk_c, c->stream);
+    if (int rc = read()) return rc;
+    const double bn = sqrt(host[5]);
+    double rn = sqrt(host[4]);
+    int it = 0;
+    // iterations queued in batches; the host reads r.r between batches
+    // the recurrence is driven to half the tolerance, so that the residual
+    // recomputed from x at the end meets it
+    while (rn > 0.5 * tolerance * bn && it < max_iterations) {
+        const int batch = std::min(32, max_iterations - it);
+        for (int b = 0; b < batch; ++b, ++it) {
+            launch_check_iteration(L0, inv_h2, rows, nrows, x, r, z, pa, pb, qv, c->chk_partials, c->chk_c,
+                                   c->stream);
+            std::swap(pa, pb);
+        }
+        if (int rc = read()) return rc;
+        rn = sqrt(host[4]);
+        if (!std::isfinite(rn)) return fail(CLQ_ERR_NOT_CONVERGED, "check solve: residual norm is not finite");
+    }
+    // the residual recomputed from x, and the error of the solver's pressure
+    launch_fcg_residual(L0, nullptr, x, c->rhs, inv_h2, rows, nrows, r, c->chk_r32, c->chk_partials, c->chk_c,
+                        c->stream);
+    launch_check_error(L0, c->p64_valid ? c->p64a : nullptr, c->p64_valid ? nullptr : c->p, x, rows, nrows,
+                       c->chk_partials, c->chk_c, c->stream);
+    if (int rc = read()) return rc;
+    st->iterations = it;
+    st->relative_residual = bn > 0 ? sqrt(host[6]) / bn : 0.0;
+    st->converged = st->relative_residual <= tolerance;
+    st->error_l2 = host[9] > 0 ? sqrt(host[8] / host[9]) : sqrt(host[8]);
+    st->error_max = host[11] > 0 ? host[10] / host[11] : host[10];
+    st->solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return check_launch("fp64_check_solve");
 }
 
 int clq_attach_net(clq_ctx *c, clq_net *net, double stall_ratio) {
@@ -1225,6 +1352,7 @@ static int pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stat
     }
     if (sp->mode == CLQ_SOLVE_FIXED_CYCLES)
         if (int rc = residual_norm(c, &r, &bn)) return rc;
+    if (!std::isfinite(r) || !std::isfinite(bn)) return non_finite(r + bn);
     s.final_residual = r;
     s.relative_residual = bn > 0 ? r / bn : 0.0;
     launch_neumann_bc(c->p, c->nx, c->ny, c->nz, c->stream);  // pressure_solve.py:84

@@ -116,9 +116,9 @@ __global__ __launch_bounds__(kT) void fcg_init_kernel(Level L, const float *__re
 // packed level-0 operator (PACKED) or the float arrays.  Every load is issued
 // before any test: off the interior the neighbour indices fall back to q
 // itself (in bounds), and the result is dropped.
-template <bool PACKED>
+template <bool PACKED, typename T>
 __device__ __forceinline__ double apply_a_z(const Level &L, const uint8_t *__restrict__ op,
-                                            const float *__restrict__ z, int64_t q, bool inner, double inv_h2,
+                                            const T *__restrict__ z, int64_t q, bool inner, double inv_h2,
                                             bool &fluid, double &dg) {
     const int64_t sx = inner ? (int64_t)L.ny * L.nz : 0, sy = inner ? L.nz : 0, sz = inner ? 1 : 0;
     const double z0 = z[q + sx], z1 = z[q - sx], z2 = z[q + sy], z3 = z[q - sy], z4 = z[q + sz], z5 = z[q - sz];
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kT) void fcg_init_rows_kernel(Level L, const uint8_
         const double bv = (double)b[q], pv = (double)p32[q];
         bool fluid;
         double dg;
-        const double av = apply_a_z<PACKED>(L, op, p32, q, inner, inv_h2, fluid, dg);
+        const double av = apply_a_z<PACKED, float>(L, op, p32, q, inner, inv_h2, fluid, dg);
         const double rv = fluid ? bv - av : 0.0;
         x[q] = fluid ? pv : 0.0;
         r[q] = rv;
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kT) void fcg_dots_kernel(Level L, const uint8_t *__
         const double qo = qv[q], ro = r[q], zq = (double)z[q];
         bool fluid;
         double dg;
-        const double wv = apply_a_z<PACKED>(L, op, z, q, inner, inv_h2, fluid, dg);
+        const double wv = apply_a_z<PACKED, float>(L, op, z, q, inner, inv_h2, fluid, dg);
         acc[0] += zq * qo;
         acc[1] += zq * ro;
         acc[2] += zq * wv;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kT) void fcg_update_kernel(Level L, const uint8_t *
         const double xo = x[q], ro = r[q], dold = d[q], qold = qv[q], zv = (double)z[q];
         bool fluid;
         double dg;
-        const double wv = apply_a_z<PACKED>(L, op, z, q, inner, inv_h2, fluid, dg);
+        const double wv = apply_a_z<PACKED, float>(L, op, z, q, inner, inv_h2, fluid, dg);
         const double dv = fluid ? zv + beta * dold : 0.0;
         const double qn = fluid ? wv + beta * qold : 0.0;
         d[q] = dv;
@@ -399,6 +399,241 @@ void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double
         hipLaunchKernelGGL((fcg_update_kernel<false, false>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, d, qv,
                            s, first, inv_h2, rows, nr, e, dd, partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kScalarThreads), 0, st, partials, nb, 2, 0, s);
+}
+
+
+// ---- the true residual of an FP64 iterate -----------------------------------------------
+// r = b - A x on the interior fluid cells of the listed rows (all rows when
+// rows is NULL), 0 elsewhere; r32 = float(r); partial r.r and b.b.  Run after
+// the FCG loop stops, so the reported residual is recomputed from x rather
+// than carried by the r -= alpha q recurrence, and as the restart of the
+// iteration when the two disagree.
+template <bool PACKED>
+__global__ __launch_bounds__(kT) void fcg_residual_rows_kernel(Level L, const uint8_t *__restrict__ op,
+                                                              const double *__restrict__ x,
+                                                              const float *__restrict__ b, double inv_h2,
+                                                              const int32_t *__restrict__ rows, int64_t nrows,
+                                                              double *__restrict__ r, float *__restrict__ r32,
+                                                              double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    for_rows(L, rows, nrows, [&](int64_t q, bool inner, int, int, int) {
+        const double bv = (double)b[q];
+        bool fluid;
+        double dg;
+        const double av = apply_a_z<PACKED, double>(L, op, x, q, inner, inv_h2, fluid, dg);
+        const double rv = fluid ? bv - av : 0.0;
+        r[q] = rv;
+        r32[q] = (float)rv;
+        acc[0] += rv * rv;
+        acc[1] += bv * bv;
+    });
+    block_sums<2>(acc, partials);
+}
+
+void launch_fcg_residual(const Level &L, const uint8_t *op, const double *x, const float *b, double inv_h2,
+                         const int32_t *rows, int64_t nrows, double *r, float *r32, double *partials, double *s,
+                         hipStream_t st) {
+    const int64_t nr = rows_of(L, rows, nrows);
+    const int nb = row_blocks_for(nr);
+    if (op)
+        hipLaunchKernelGGL(fcg_residual_rows_kernel<true>, dim3(nb), dim3(kT), 0, st, L, op, x, b, inv_h2, rows, nr,
+                           r, r32, partials);
+    else
+        hipLaunchKernelGGL(fcg_residual_rows_kernel<false>, dim3(nb), dim3(kT), 0, st, L, op, x, b, inv_h2, rows,
+                           nr, r, r32, partials);
+    hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kScalarThreads), 0, st, partials, nb, 0, 0, s);
+}
+
+// ---- FP64 check solve: Jacobi-preconditioned CG -----------------------------------------
+// An independent solve of the same level-0 system A x = b, sharing nothing
+// with the multigrid path but the operator arrays: CG in FP64 preconditioned
+// by the operator's diagonal a = -diag / h^2 (z = r / a; 0 on a fluid cell
+// without open neighbours).  Iteration k:
+//   apply:  p = z + beta p (beta = 0 first; p double-buffered, the
+//           neighbours' new p formed from z and the old p), q = A p; p.q
+//   update: alpha = r.z / p.q; x += alpha p, r -= alpha q, z = r / a; r.z, r.r
+// Scalars on the device: c[0] r.z, c[1] p.q, c[2] alpha, c[3] beta, c[4] r.r,
+// c[5] b.b.  The float operator arrays are read (not the packed bytes).
+__global__ __launch_bounds__(kT) void check_init_kernel(Level L, const float *__restrict__ b, double inv_h2,
+                                                       const int32_t *__restrict__ rows, int64_t nrows,
+                                                       double *__restrict__ x, double *__restrict__ r,
+                                                       double *__restrict__ z, double *__restrict__ p,
+                                                       double *__restrict__ partials) {
+    double acc[3] = {0.0, 0.0, 0.0};
+    for_rows(L, rows, nrows, [&](int64_t q, bool inner, int, int, int) {
+        const double bv = (double)b[q], dg = (double)L.diag[q];
+        const bool fluid = inner && L.mask[q] > 0.0f;
+        const double rv = fluid ? bv : 0.0;
+        const double zv = fluid && dg > 0.0 ? rv / (-dg * inv_h2) : 0.0;
+        x[q] = 0.0;
+        r[q] = rv;
+        z[q] = zv;
+        p[q] = 0.0;
+        acc[0] += rv * zv;
+        acc[1] += rv * rv;
+        acc[2] += bv * bv;
+    });
+    block_sums<3>(acc, partials);
+}
+
+__global__ __launch_bounds__(kT) void check_apply_kernel(Level L, const double *__restrict__ z,
+                                                        const double *__restrict__ po, double *__restrict__ pn,
+                                                        double *__restrict__ qv, const double *__restrict__ c,
+                                                        double inv_h2, const int32_t *__restrict__ rows,
+                                                        int64_t nrows, double *__restrict__ partials) {
+    const double beta = c[3];
+    double acc[1] = {0.0};
+    for_rows(L, rows, nrows, [&](int64_t q, bool inner, int, int, int) {
+        const int64_t sx = inner ? (int64_t)L.ny * L.nz : 0, sy = inner ? L.nz : 0, sz = inner ? 1 : 0;
+        auto pnew = [&](int64_t t) { return z[t] + beta * po[t]; };
+        const double v0 = pnew(q + sx), v1 = pnew(q - sx), v2 = pnew(q + sy), v3 = pnew(q - sy), v4 = pnew(q + sz),
+                     v5 = pnew(q - sz), vc = pnew(q);
+        const float w0 = L.wx[q], w1 = L.wx[q - sx], w2 = L.wy[q], w3 = L.wy[q - sy], w4 = L.wz[q], w5 = L.wz[q - sz];
+        const bool fluid = inner && L.mask[q] > 0.0f;
+        double sum = 0.0;
+        if (w0 != 0.0f) sum = sum + (double)w0 * v0;
+        if (w1 != 0.0f) sum = sum + (double)w1 * v1;
+        if (w2 != 0.0f) sum = sum + (double)w2 * v2;
+        if (w3 != 0.0f) sum = sum + (double)w3 * v3;
+        if (w4 != 0.0f) sum = sum + (double)w4 * v4;
+        if (w5 != 0.0f) sum = sum + (double)w5 * v5;
+        const double pv = fluid ? vc : 0.0;
+        const double av = fluid ? (sum - (double)L.diag[q] * vc) * inv_h2 : 0.0;
+        pn[q] = pv;
+        qv[q] = av;
+        acc[0] += pv * av;
+    });
+    block_sums<1>(acc, partials);
+}
+
+__global__ __launch_bounds__(kT) void check_update_kernel(Level L, const double *__restrict__ p,
+                                                         const double *__restrict__ qv, double *__restrict__ x,
+                                                         double *__restrict__ r, double *__restrict__ z,
+                                                         const double *__restrict__ c, double inv_h2,
+                                                         const int32_t *__restrict__ rows, int64_t nrows,
+                                                         double *__restrict__ partials) {
+    const double alpha = c[2];
+    double acc[2] = {0.0, 0.0};
+    for_rows(L, rows, nrows, [&](int64_t q, bool inner, int, int, int) {
+        const double dg = (double)L.diag[q];
+        const bool fluid = inner && L.mask[q] > 0.0f;
+        const double xv = x[q] + alpha * p[q];
+        const double rv = r[q] - alpha * qv[q];
+        const double zv = fluid && dg > 0.0 ? rv / (-dg * inv_h2) : 0.0;
+        x[q] = xv;
+        r[q] = rv;
+        z[q] = zv;
+        acc[0] += rv * zv;
+        acc[1] += rv * rv;
+    });
+    block_sums<2>(acc, partials);
+}
+
+// Error of a pressure against the check solution over the interior fluid
+// cells: sum (p - x)^2, sum x^2 (block sums) and max |p - x|, max |x| (block
+// maxima).
+template <typename T>
+__global__ __launch_bounds__(kT) void check_error_kernel(Level L, const T *__restrict__ pr,
+                                                        const double *__restrict__ x, const int32_t *__restrict__ rows,
+                                                        int64_t nrows, double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0}, mx[2] = {0.0, 0.0};
+    for_rows(L, rows, nrows, [&](int64_t q, bool inner, int, int, int) {
+        if (!(inner && L.mask[q] > 0.0f)) return;
+        const double xv = x[q], e = (double)pr[q] - xv;
+        acc[0] += e * e;
+        acc[1] += xv * xv;
+        mx[0] = fmax(mx[0], fabs(e));
+        mx[1] = fmax(mx[1], fabs(xv));
+    });
+    block_sums<2>(acc, partials);
+    __shared__ double sh[2][kT];
+    sh[0][threadIdx.x] = mx[0];
+    sh[1][threadIdx.x] = mx[1];
+    __syncthreads();
+    for (int off = kT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + off]);
+            sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * kB + blockIdx.x] = sh[0][0];
+        partials[3 * kB + blockIdx.x] = sh[1][0];
+    }
+}
+
+// stage 0 (init): r.z, r.r, b.b; 1 (apply): p.q -> alpha; 2 (update): r.z,
+// r.r -> beta; 3 (error): two sums and two maxima into c[8..11].
+__global__ __launch_bounds__(1024) void check_scalars_kernel(const double *__restrict__ partials, int blocks,
+                                                            int stage, double *__restrict__ c) {
+    __shared__ double sh[4][1024];
+    const int n = stage == 0 ? 3 : stage == 1 ? 1 : stage == 2 ? 2 : 4;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < blocks; q += 1024)
+        for (int a = 0; a < n; ++a) {
+            const double t = partials[a * kB + q];
+            v[a] = (stage == 3 && a >= 2) ? fmax(v[a], t) : v[a] + t;
+        }
+    for (int a = 0; a < 4; ++a) sh[a][threadIdx.x] = v[a];
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int a = 0; a < n; ++a)
+                sh[a][threadIdx.x] = (stage == 3 && a >= 2) ? fmax(sh[a][threadIdx.x], sh[a][threadIdx.x + off])
+                                                            : sh[a][threadIdx.x] + sh[a][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (stage == 0) {
+        c[0] = sh[0][0];
+        c[3] = 0.0;
+        c[4] = sh[1][0];
+        c[5] = sh[2][0];
+    } else if (stage == 1) {
+        c[1] = sh[0][0];
+        c[2] = c[1] != 0.0 ? c[0] / c[1] : 0.0;
+    } else if (stage == 2) {
+        c[3] = c[0] != 0.0 ? sh[0][0] / c[0] : 0.0;
+        c[0] = sh[0][0];
+        c[4] = sh[1][0];
+    } else {
+        for (int a = 0; a < 4; ++a) c[8 + a] = sh[a][0];
+    }
+}
+
+int check_partials_count() { return 4 * kB; }
+
+void launch_check_init(const Level &L, const float *b, double inv_h2, const int32_t *rows, int64_t nrows, double *x,
+                       double *r, double *z, double *p, double *partials, double *c, hipStream_t st) {
+    const int64_t nr = rows_of(L, rows, nrows);
+    const int nb = row_blocks_for(nr);
+    hipLaunchKernelGGL(check_init_kernel, dim3(nb), dim3(kT), 0, st, L, b, inv_h2, rows, nr, x, r, z, p, partials);
+    hipLaunchKernelGGL(check_scalars_kernel, dim3(1), dim3(1024), 0, st, partials, nb, 0, c);
+}
+
+void launch_check_iteration(const Level &L, double inv_h2, const int32_t *rows, int64_t nrows, double *x, double *r,
+                            double *z, double *po, double *pn, double *qv, double *partials, double *c,
+                            hipStream_t st) {
+    const int64_t nr = rows_of(L, rows, nrows);
+    const int nb = row_blocks_for(nr);
+    hipLaunchKernelGGL(check_apply_kernel, dim3(nb), dim3(kT), 0, st, L, z, po, pn, qv, c, inv_h2, rows, nr,
+                       partials);
+    hipLaunchKernelGGL(check_scalars_kernel, dim3(1), dim3(1024), 0, st, partials, nb, 1, c);
+    hipLaunchKernelGGL(check_update_kernel, dim3(nb), dim3(kT), 0, st, L, pn, qv, x, r, z, c, inv_h2, rows, nr,
+                       partials);
+    hipLaunchKernelGGL(check_scalars_kernel, dim3(1), dim3(1024), 0, st, partials, nb, 2, c);
+}
+
+void launch_check_error(const Level &L, const double *p64, const float *p32, const double *x, const int32_t *rows,
+                        int64_t nrows, double *partials, double *c, hipStream_t st) {
+    const int64_t nr = rows_of(L, rows, nrows);
+    const int nb = row_blocks_for(nr);
+    if (p64)
+        hipLaunchKernelGGL(check_error_kernel<double>, dim3(nb), dim3(kT), 0, st, L, p64, x, rows, nr, partials);
+    else
+        hipLaunchKernelGGL(check_error_kernel<float>, dim3(nb), dim3(kT), 0, st, L, p32, x, rows, nr, partials);
+    hipLaunchKernelGGL(check_scalars_kernel, dim3(1), dim3(1024), 0, st, partials, nb, 3, c);
 }
 
 }  // namespace clq

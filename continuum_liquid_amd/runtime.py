@@ -22,7 +22,7 @@ import numpy as np
 
 PACKAGE_DIR = Path(__file__).resolve().parent
 HIP_LIBRARY = PACKAGE_DIR / "libcliquid_hip.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # enum clq_status
 OK, ERR_INVALID, ERR_DEVICE, ERR_NOMEM, ERR_UNSUPPORTED, ERR_NOT_CONVERGED = range(6)
@@ -81,6 +81,22 @@ class SolveStats(ctypes.Structure):
         ("relative_residual", c_double),
         ("solve_ms", c_double),
         ("active_cells", c_int64),
+        ("recurrence_residual", c_double),
+        ("restarts", c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class CheckStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", c_int32),
+        ("converged", c_int32),
+        ("relative_residual", c_double),
+        ("error_l2", c_double),
+        ("error_max", c_double),
+        ("solve_ms", c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -113,6 +129,7 @@ _SIGNATURES = {
     "clq_pressure_rhs": ([_P, c_double, c_double], ctypes.c_int),
     "clq_build_operator": ([_P], ctypes.c_int),
     "clq_pressure_solve": ([_P, POINTER(SolveParams), POINTER(SolveStats)], ctypes.c_int),
+    "clq_fp64_check_solve": ([_P, c_double, c_int32, POINTER(CheckStats)], ctypes.c_int),
     "clq_w_cycle": ([_P, c_int32, c_int32, c_int32], ctypes.c_int),
     "clq_smooth": ([_P, c_int32, c_int32], ctypes.c_int),
     "clq_restrict": ([_P, c_int32], ctypes.c_int),
@@ -350,6 +367,14 @@ class Context:
                              vcycle_from=vcycle_from, coarse_weight=coarse_weight, tolerance=tolerance)
         stats = SolveStats()
         self._check(self.lib.clq_pressure_solve(self.handle, byref(params), byref(stats)), "clq_pressure_solve")
+        return stats.as_dict()
+
+    def fp64_check_solve(self, tolerance: float = 1e-12, max_iterations: int = 20000) -> dict:
+        """Independent FP64 solve of the current level-0 system and the error of
+        the last pressure solve against it (clq_fp64_check_solve)."""
+        stats = CheckStats()
+        self._check(self.lib.clq_fp64_check_solve(self.handle, tolerance, max_iterations, byref(stats)),
+                    "clq_fp64_check_solve")
         return stats.as_dict()
 
     def attach_net(self, net: "Network | None", stall_ratio: float = 0.5) -> None:

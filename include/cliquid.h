@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define CLQ_ABI_VERSION 3
+#define CLQ_ABI_VERSION 4
 
 enum clq_status {
     CLQ_OK = 0,
@@ -152,6 +152,13 @@ typedef struct clq_solve_stats {
     int64_t active_cells;      /* cells of the level-0 rows (i, j, *) holding an
                                   interior fluid cell: what the finest-level
                                   smoother and FCG vectors visit             */
+    double recurrence_residual; /* modes 3, 4: ||r|| of the r -= alpha q
+                                  recurrence when the iteration stopped; the
+                                  final_residual above is recomputed as
+                                  ||b - A x|| from the FP64 iterate, and when it
+                                  misses the tolerance while cycles remain the
+                                  iteration restarts from it              */
+    int32_t restarts;          /* modes 3, 4: such restarts                      */
 } clq_solve_stats;
 
 typedef struct clq_ctx clq_ctx;
@@ -210,8 +217,28 @@ int clq_update_velocity(clq_ctx *ctx, const double *gravity, const double *force
 int clq_pressure_rhs(clq_ctx *ctx, double rho, double dt);
 /* build_finest_operator (multigrid.cl:63-88). */
 int clq_build_operator(clq_ctx *ctx);
-/* The solve proper followed by neumann_domain_bc (pressure_solve.py:61-84). */
+/* The solve proper followed by neumann_domain_bc (pressure_solve.py:61-84).
+ * A residual that is not finite (a diverged or NaN-fed solve) returns
+ * CLQ_ERR_NOT_CONVERGED: later stages would otherwise advect NaN velocities. */
 int clq_pressure_solve(clq_ctx *ctx, const clq_solve_params *params, clq_solve_stats *stats);
+
+/* FP64 check of a pressure solve.  No counterpart in the reference.  Solves
+ * the current level-0 system (the rhs and the finest operator arrays) once
+ * more, independently of the multigrid path, by Jacobi-preconditioned CG in
+ * FP64 from x = 0 until its residual recurrence is below tolerance / 2 (or
+ * max_iterations), then recomputes ||b - A x|| / ||b|| from x and reports
+ * the error of the last clq_pressure_solve's pressure (the FP64 iterate
+ * after modes 2-4, else the FP32 pressure) against that solution over the
+ * interior fluid cells. */
+typedef struct clq_check_stats {
+    int32_t iterations;        /* CG iterations of the check solve              */
+    int32_t converged;         /* 1 when the tolerance was met                  */
+    double relative_residual;  /* ||b - A x_check|| / ||b||, recomputed          */
+    double error_l2;           /* ||p - x_check||_2 / ||x_check||_2              */
+    double error_max;          /* max |p - x_check| / max |x_check|              */
+    double solve_ms;           /* wall time of the check solve                  */
+} clq_check_stats;
+int clq_fp64_check_solve(clq_ctx *ctx, double tolerance, int32_t max_iterations, clq_check_stats *stats);
 /* One W-cycle on the current level-0 p/b/mask/operator (multigrid.py:53-187). */
 int clq_w_cycle(clq_ctx *ctx, int32_t pre_smooth, int32_t post_smooth, int32_t coarse_extra);
 /* Red-black Gauss-Seidel sweeps on one level (multigrid.py:190-220). */

@@ -113,6 +113,11 @@ static int fail(int code, const char *fmt, ...) {
     return code;
 }
 
+static int non_finite(double r) {
+    return fail(CLQ_ERR_NOT_CONVERGED, "pressure solve: residual norm is not finite (%g); the velocity field "
+                "or the operator holds NaN or Inf", r);
+}
+
 static double now_ms(void) {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -969,6 +974,7 @@ static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     ir_update_residual(c, 1, NULL, c->p64a, &rr, &bb);
     double r = sqrt(rr);
     const double bn = sqrt(bb);
+    if (!isfinite(r) || !isfinite(bn)) return non_finite(r + bn);
     s->initial_residual = r;
     s->rhs_norm = bn;
     const int kind = sp->coarse_dtype == 1 ? 2 : 1;
@@ -990,6 +996,11 @@ static int solve_mixed(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
         s->cycles++;
         const double before = r;
         r = sqrt(rr);
+        if (!isfinite(r)) {
+            for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
+            L0->b = c->rhs;
+            return non_finite(r);
+        }
         if (r > 0.95 * before) break;
     }
     for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
@@ -1081,19 +1092,26 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     }
     double rn = sqrt(rr);
     const double bn = sqrt(bb);
+    if (!isfinite(rn) || !isfinite(bn)) return non_finite(rn + bn);
     s->initial_residual = rn;
     s->rhs_norm = bn;
-    if (rn > sp->tolerance * bn && sp->max_cycles > 0) {
+    s->recurrence_residual = rn;
+    int plain = 0;
+    /* Each pass starts the direction recurrence from r = b - A x: the
+     * initialisation, or the residual recomputed from x after a pass whose
+     * recurrence met the tolerance (or ran out of cycles). */
+    while (rn > sp->tolerance * bn && s->cycles < sp->max_cycles) {
         /* The direction recurrence streams with the update: the dots of z
          * give beta = -(z.q)/(d.q)_old and the new direction's
          * d.q = z.w + 2 beta z.q + beta^2 (d.q)_old with w = A z (A
          * symmetric, so d_old . A z = z . q_old), hence alpha = z.r / d.q
          * before d and q = w + beta q are formed. */
         double alpha = 0.0, beta = 0.0, dq = 0.0;
-        int first = 1, plain = 0;
+        int first = 1;
         double *w = dnew;
         precondition_wcycle(c, sp, rn);
         s->cycles++;
+        s->fallback_cycles += plain;
         for (;;) {
             double zq = 0.0, zr = 0.0, zw = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : zq, zr, zw)
@@ -1123,7 +1141,8 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
                 rr += r[t] * r[t];
             }
             rn = sqrt(rr);
-            if (rn <= sp->tolerance * bn || s->cycles >= sp->max_cycles || !(rn == rn)) break;
+            if (!isfinite(rn)) return non_finite(rn);
+            if (rn <= sp->tolerance * bn || s->cycles >= sp->max_cycles) break;
             /* A weighted cycle that is not definite for this residual gives a
              * step length alpha = z.r / d.Ad <= 0 (for a definite one both
              * dots carry the sign of the negative definite operator): go on
@@ -1135,6 +1154,21 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
             s->cycles++;
             s->fallback_cycles += plain;
         }
+        /* the residual recomputed from x: the reported value, and the start
+         * of another pass when it misses the tolerance */
+        s->recurrence_residual = rn;
+        rr = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rr)
+        for (long long t = 0; t < c->C; ++t) {
+            double rv = 0.0;
+            if (fluid_at(L, t)) rv = (double)c->rhs[t] - apply_a64(L, t, x, inv_h2);
+            r[t] = rv;
+            c->r32[t] = (float)rv;
+            rr += rv * rv;
+        }
+        rn = sqrt(rr);
+        if (!isfinite(rn)) return non_finite(rn);
+        if (rn > sp->tolerance * bn && s->cycles < sp->max_cycles) s->restarts++;
     }
     for (long long t = 0; t < c->C; ++t) c->p[t] = (float)x[t];
     s->final_residual = rn;
@@ -1199,6 +1233,7 @@ static int pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stat
         }
     }
     if (sp->mode == CLQ_SOLVE_FIXED_CYCLES) clq_residual_norm(c, &r, &bn);
+    if (!isfinite(r) || !isfinite(bn)) return non_finite(r + bn);
     s.final_residual = r;
     s.relative_residual = bn > 0 ? r / bn : 0.0;
     clq_neumann_bc(c); /* pressure_solve.py:84 */
@@ -1219,6 +1254,88 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
     const int rc = pressure_solve(c, sp, st);
     c->coarse_weight = 1.0f;
     return rc;
+}
+
+/* FP64 check solve (no reference counterpart): Jacobi-preconditioned CG on
+ * the level-0 system from x = 0, the preconditioner the operator's diagonal
+ * a = -diag / h^2 (z = 0 on a fluid cell without open neighbours); then the
+ * residual recomputed from x and the error of the last solve's pressure
+ * (the FP64 iterate after modes 2-4, else the FP32 pressure). */
+int clq_fp64_check_solve(clq_ctx *c, double tolerance, int32_t max_iterations, clq_check_stats *st) {
+    if (!c || !st || !(tolerance > 0.0) || max_iterations < 0) return fail(CLQ_ERR_INVALID, "bad check-solve arguments");
+    const double t0 = now_ms();
+    const level_t *L = &c->lv[0];
+    const long long C = c->C;
+    const double h = (double)(float)c->dx;
+    const double inv_h2 = 1.0 / (h * h);
+    double *x = xcalloc(C, 8), *r = xcalloc(C, 8), *z = xcalloc(C, 8), *p = xcalloc(C, 8), *q = xcalloc(C, 8);
+    if (!x || !r || !z || !p || !q) {
+        free(x); free(r); free(z); free(p); free(q);
+        return fail(CLQ_ERR_NOMEM, "check-solve buffers");
+    }
+    double rz = 0.0, rr = 0.0, bb = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rz, rr, bb)
+    for (long long t = 0; t < C; ++t) {
+        const double bv = (double)c->rhs[t], dg = (double)L->diag[t];
+        const int fl = fluid_at(L, t);
+        r[t] = fl ? bv : 0.0;
+        z[t] = fl && dg > 0.0 ? r[t] / (-dg * inv_h2) : 0.0;
+        p[t] = z[t];
+        rz += r[t] * z[t];
+        rr += r[t] * r[t];
+        bb += bv * bv;
+    }
+    const double bn = sqrt(bb);
+    int it = 0;
+    /* the recurrence is driven to half the tolerance, so that the residual
+     * recomputed from x at the end meets it */
+    while (sqrt(rr) > 0.5 * tolerance * bn && it < max_iterations) {
+        double pq = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : pq)
+        for (long long t = 0; t < C; ++t) {
+            q[t] = fluid_at(L, t) ? apply_a64(L, t, p, inv_h2) : 0.0;
+            pq += p[t] * q[t];
+        }
+        const double alpha = pq != 0.0 ? rz / pq : 0.0;
+        double rz_new = 0.0;
+        rr = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : rz_new, rr)
+        for (long long t = 0; t < C; ++t) {
+            x[t] += alpha * p[t];
+            r[t] -= alpha * q[t];
+            const double dg = (double)L->diag[t];
+            z[t] = fluid_at(L, t) && dg > 0.0 ? r[t] / (-dg * inv_h2) : 0.0;
+            rz_new += r[t] * z[t];
+            rr += r[t] * r[t];
+        }
+        const double beta = rz != 0.0 ? rz_new / rz : 0.0;
+        rz = rz_new;
+#pragma omp parallel for schedule(static)
+        for (long long t = 0; t < C; ++t) p[t] = z[t] + beta * p[t];
+        ++it;
+        if (!isfinite(rr)) break;
+    }
+    double rt = 0.0, e2 = 0.0, x2 = 0.0, emax = 0.0, xmax = 0.0;
+    for (long long t = 0; t < C; ++t) {
+        if (!fluid_at(L, t)) continue;
+        const double rv = (double)c->rhs[t] - apply_a64(L, t, x, inv_h2);
+        rt += rv * rv;
+        const double pv = c->p64_valid ? c->p64a[t] : (double)c->p[t];
+        const double e = pv - x[t];
+        e2 += e * e;
+        x2 += x[t] * x[t];
+        if (fabs(e) > emax) emax = fabs(e);
+        if (fabs(x[t]) > xmax) xmax = fabs(x[t]);
+    }
+    free(x); free(r); free(z); free(p); free(q);
+    if (!isfinite(rt)) return fail(CLQ_ERR_NOT_CONVERGED, "check solve: residual norm is not finite");
+    st->iterations = it;
+    st->relative_residual = bn > 0 ? sqrt(rt) / bn : 0.0;
+    st->converged = st->relative_residual <= tolerance;
+    st->error_l2 = x2 > 0 ? sqrt(e2 / x2) : sqrt(e2);
+    st->error_max = xmax > 0 ? emax / xmax : emax;
+    st->solve_ms = now_ms() - t0;
+    return CLQ_OK;
 }
 
 /* pressure_correction (pressure_solve.cl:26-47, gradient helper.cl:25-37). */

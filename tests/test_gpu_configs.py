@@ -1,0 +1,234 @@
+"""Every benchmark configuration exercised on the GPU through the C-ABI.
+
+  1  64^3 dam break, plain FP32 W(2, 2) to 1e-6, FLIP 0.95: whole steps
+     bit-exact against the oracle (W-cycles and every FP32 kernel are
+     bit-exact; the FP64 residual norms only decide when to stop)
+  2  128^3 dam break + sphere, plain FP32 W-cycle to 1e-6 and occupancy at
+     grid resolution: bit-exact steps against the oracle; then the FCG solve
+     of the benchmark within rounding of the oracle's
+  3  256^3, FP64 refinement around the mixed-precision cycle to 1e-8, and a
+     whole step of the benchmark's FCG solve: size-independent properties
+  4  the learned preconditioner (committed weights) in FCG at 128^3
+  5  512x256x256 wide dam break + 3 spheres: one learned-FCG step,
+     occupancy and smoothed density at twice the resolution
+
+Every solve here is checked in FP64 twice: the residual the solver reports
+is recomputed from its FP64 iterate (`relative_residual`, against the
+`recurrence_residual` of the iteration), and an independent FP64 solve of the
+same system (Jacobi-preconditioned CG, `fp64_check_solve`) measures the error
+of the pressure.  The error bound asserted is the residual target times a
+generous condition-number allowance; the measured values are printed.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from continuum_liquid_amd import runtime, scenes
+from continuum_liquid_amd.Solver.Kernel import solver as host_solver
+
+from .helpers import bin_keys, make_context
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 240.0
+
+
+def fp32_wcycle(tolerance=1e-6):
+    return {"mode": "fp32", "max_cycles": 60, "pre_smooth": 2, "post_smooth": 2, "coarse_extra": 64,
+            "tolerance": tolerance}
+
+
+def fcg(tolerance=1e-6, mode="fcg"):
+    return {"mode": mode, "max_cycles": 60, "pre_smooth": 1, "post_smooth": 1, "coarse_extra": 16,
+            "vcycle_from": 2, "coarse_weight": 1.6, "coarse_dtype": 0, "tolerance": tolerance}
+
+
+def params_for(scene, solver):
+    return host_solver.StepParams(dt=DT, gravity=scene.gravity, flip_ratio=0.95, solver=solver)
+
+
+def particles_equal(a, b):
+    pa, va = a.get_particles()
+    pb, vb = b.get_particles()
+    return np.array_equal(pa, pb) and np.array_equal(va, vb)
+
+
+def check_step_properties(ctx, scene_n, stats, tolerance, error_bound, label):
+    """Size-independent properties after one step: particle count kept, the
+    bins of the step cover every particle, occupancy counts every particle,
+    the reported residual is the recomputed one and meets the target, and the
+    FP64 check solve bounds the pressure error."""
+    assert ctx.particle_count == scene_n
+    counts = ctx.get_field("bin_counts").astype(np.int64)
+    assert counts.sum() == scene_n
+    offsets = ctx.get_field("bin_offsets").ravel().astype(np.int64)
+    assert np.array_equal(offsets[1:], np.cumsum(counts.ravel())[:-1])
+    ff = ctx.get_field("fluid_fraction").astype(np.float64)
+    assert ff.sum() * 8 == scene_n
+    assert stats["relative_residual"] <= tolerance, stats
+    check = ctx.fp64_check_solve(1e-12)
+    print(f"{label}: {stats['cycles']} cycles, residual {stats['relative_residual']:.3e} "
+          f"(recurrence {stats['recurrence_residual']:.3e}, {stats['restarts']} restarts), FP64 error "
+          f"l2 {check['error_l2']:.3e} max {check['error_max']:.3e}, check {check['iterations']} its "
+          f"{check['relative_residual']:.1e}", flush=True)
+    assert check["converged"], check
+    assert check["error_l2"] <= error_bound and check["error_max"] <= error_bound, check
+    return check
+
+
+def test_config1_dam_break_64_fp32_bit_exact(oracle_lib, hip_lib):
+    scene = scenes.bench_scene(1)
+    o, h = make_context(oracle_lib, scene), make_context(hip_lib, scene)
+    params = params_for(scene, fp32_wcycle())
+    for step in range(10):
+        so = host_solver.step(o, params)
+        sh = host_solver.step(h, params)
+        assert so["cycles"] == sh["cycles"], (step, so, sh)
+    assert particles_equal(o, h)
+    for name in ("u", "v", "w", "pressure", "fluid_fraction"):
+        assert np.array_equal(o.get_field(name), h.get_field(name)), name
+    # the FP32 iteration stalls above 1e-6 at this size (FP32 storage of p);
+    # its FP64 error is still small
+    check_step_properties(h, scene.particle_count, sh, 1e-4, 1e-3, "config 1 step 10")
+    # the rest of the configured 100 steps on the device
+    for _ in range(90):
+        sh = host_solver.step(h, params)
+    check_step_properties(h, scene.particle_count, sh, 1e-4, 1e-3, "config 1 step 100")
+
+
+def test_config2_dam_break_sphere_128(oracle_lib, hip_lib):
+    scene = scenes.bench_scene(2)
+    o, h = make_context(oracle_lib, scene), make_context(hip_lib, scene)
+    params = params_for(scene, fp32_wcycle())
+    for step in range(3):
+        so = host_solver.step(o, params)
+        sh = host_solver.step(h, params)
+        assert so["cycles"] == sh["cycles"], (step, so, sh)
+    assert particles_equal(o, h)
+    for name in ("u", "v", "w", "pressure", "fluid_fraction"):
+        assert np.array_equal(o.get_field(name), h.get_field(name)), name
+    occ_o, _ = o.voxelise(1, 8.0)
+    occ_h, _ = h.voxelise(1, 8.0)
+    assert np.array_equal(occ_o, occ_h)
+    check_step_properties(h, scene.particle_count, sh, 1e-4, 1e-3, "config 2 FP32 step 3")
+    # the benchmark's FCG solve: rounding-level agreement with the oracle
+    params = params_for(scene, fcg())
+    for step in range(3):
+        so = host_solver.step(o, params)
+        sh = host_solver.step(h, params)
+        assert abs(so["cycles"] - sh["cycles"]) <= 1
+        po, ph = o.get_field("pressure_fp64"), h.get_field("pressure_fp64")
+        assert np.max(np.abs(po - ph)) <= 1e-5 * np.max(np.abs(po))
+        for name in ("u", "v", "w"):
+            fo, fh = o.get_field(name), h.get_field(name)
+            assert np.max(np.abs(fo - fh)) <= 1e-5 * max(1.0, np.max(np.abs(fo))), name
+    check_step_properties(h, scene.particle_count, sh, 1e-6, 1e-3, "config 2 FCG step 6")
+
+
+def test_config3_256_mixed_refinement_and_fcg(hip_lib):
+    scene = scenes.bench_scene(3)
+    h = make_context(hip_lib, scene)
+    mixed = {"mode": "mixed", "max_cycles": 80, "pre_smooth": 2, "post_smooth": 2, "coarse_extra": 16,
+             "vcycle_from": 2, "coarse_dtype": 0, "tolerance": 1e-8}
+    stats = host_solver.step(h, params_for(scene, mixed))
+    check_step_properties(h, scene.particle_count, stats, 1e-8, 1e-4, "config 3 mixed 1e-8")
+    for _ in range(2):
+        stats = host_solver.step(h, params_for(scene, fcg()))
+    check_step_properties(h, scene.particle_count, stats, 1e-6, 1e-3, "config 3 FCG 1e-6")
+    # sorted, stable bins at full size after the steps
+    h.bin_particles()
+    pos, _ = h.get_particles()
+    keys = bin_keys(pos, scene.dx, scene.shape)
+    assert np.all(np.diff(keys) >= 0)
+
+
+def test_config4_learned_fcg_128(hip_lib):
+    from continuum_liquid_amd import learned
+
+    scene = scenes.bench_scene(4)
+    h = make_context(hip_lib, scene)
+    net = learned.load_weights(hip_lib, scene.shape, scene.dx)
+    h.attach_net(net, 0.5)
+    params = params_for(scene, fcg(mode="learned"))
+    for step in range(3):
+        stats = host_solver.step(h, params)
+        check_step_properties(h, scene.particle_count, stats, 1e-6, 1e-3, f"config 4 learned step {step}")
+    h.attach_net(None)
+
+
+def test_config5_wide_512_learned_and_voxelised(hip_lib):
+    from continuum_liquid_amd import learned
+
+    scene = scenes.bench_scene(5)
+    h = make_context(hip_lib, scene)
+    net = learned.load_weights(hip_lib, scene.shape, scene.dx)
+    h.attach_net(net, 0.5)
+    stats = host_solver.step(h, params_for(scene, fcg(mode="learned")))
+    check_step_properties(h, scene.particle_count, stats, 1e-6, 1e-3, "config 5 learned step")
+    h.attach_net(None)
+    occ1, _ = h.voxelise(1, 8.0)
+    occ2, dens2 = h.voxelise(2, 8.0, density=True)
+    assert occ2.shape == tuple(2 * n for n in scene.shape)
+    assert float(dens2.sum(dtype=np.float64)) == pytest.approx(scene.particle_count, rel=1e-5)
+    nx, ny, nz = scene.shape
+    coarse = occ2.reshape(nx, 2, ny, 2, nz, 2).max(axis=(1, 3, 5)).astype(bool)
+    assert not (coarse & ~occ1.astype(bool)).any()
+    assert np.array_equal(occ1.astype(bool), h.get_field("fluid_fraction") > 0)
+
+
+@pytest.mark.parametrize("mode", ["fixed", "fcg"])
+def test_non_finite_solve_is_an_error(hip_lib, mode):
+    """A NaN right-hand side ends the solve with CLQ_ERR_NOT_CONVERGED (every
+    kernel runs its normal bounded loops; the host stops on the residual)."""
+    scene = scenes.small_scene(24, seed=1)
+    h = make_context(hip_lib, scene)
+    from .helpers import run_to_pressure
+
+    run_to_pressure(h, scene)
+    rhs = h.get_field("rhs")
+    rhs[12, 5, 12] = np.nan
+    h.set_field("rhs", rhs)
+    with pytest.raises(runtime.ClqError) as err:
+        h.pressure_solve(runtime.SOLVE_MODES[mode], max_cycles=5, pre_smooth=1, post_smooth=1, coarse_extra=4)
+    assert err.value.status == runtime.ERR_NOT_CONVERGED
+
+
+def test_check_solve_matches_oracle(oracle_lib, hip_lib):
+    """The device's FP64 check solve and the oracle's reach the same solution
+    and report the same error for the same FP32 pressure."""
+    scene = scenes.small_scene(24, seed=2)
+    out = []
+    for lib in (oracle_lib, hip_lib):
+        ctx = make_context(lib, scene)
+        host_solver.step(ctx, params_for(scene, fp32_wcycle(1e-5)))
+        out.append(ctx.fp64_check_solve(1e-12))
+    (co, ch) = out
+    assert co["converged"] and ch["converged"]
+    assert ch["error_l2"] == pytest.approx(co["error_l2"], rel=1e-5)
+    assert ch["error_max"] == pytest.approx(co["error_max"], rel=1e-5)
+
+
+def test_main_bakes_through_the_hip_library(tmp_path, monkeypatch, oracle_lib, hip_lib):
+    """The kept entry point main(config) runs the bake through
+    libcliquid_hip.so; the cache it writes is byte-identical to the one the
+    same call writes with the oracle in its place (reference settings: fixed
+    W(2, 4) cycles, pure FLIP, gravity on z), obstacle and both backend
+    strings included."""
+    from continuum_liquid_amd.Solver.General import main as solver_main
+
+    from .test_host import make_config
+
+    caches = {}
+    for name, lib, backend in (("hip", None, "GPU"), ("hip-cpu", None, "CPU"), ("oracle", oracle_lib, "GPU")):
+        if lib is not None:
+            monkeypatch.setattr(runtime, "load_library", lambda path=None, _l=lib: _l)
+        cfg = make_config(tmp_path / name, frames=4, substeps=1)
+        cfg["simulation"]["settings"]["solver_backend"] = backend
+        solver_main.main(cfg)
+        files = sorted((tmp_path / name / "bake" / "particles").glob("*.bin"))
+        caches[name] = [f.read_bytes() for f in files]
+        monkeypatch.undo()
+    assert len(caches["hip"]) == 5
+    assert caches["hip"] == caches["oracle"] == caches["hip-cpu"]

@@ -552,3 +552,57 @@ def test_fcg_reaches_tolerance_in_fewer_cycles(oracle_lib):
     assert np.linalg.norm(p - exact) <= 1e-6 * np.linalg.norm(exact)
     p64 = ctx.get_field("pressure_fp64")[fluid]
     assert s["final_residual"] == pytest.approx(np.linalg.norm(b - A @ p64), rel=1e-6)
+
+
+def test_fp64_check_solve_matches_direct_solve(oracle_lib):
+    """The check solve (Jacobi-preconditioned CG in FP64) returns the scipy
+    direct solution, and measures the error of an FCG solve against it: at a
+    1e-6 FCG tolerance the error is small but not zero, at 1e-11 it vanishes."""
+    ctx = _pressure_context(oracle_lib, 20, seed=3)
+    h2 = np.float64(F32(ctx.dx)) ** 2
+    A, fluid, _ = operator_matrix(*[ctx.get_field(n) for n in ("diagonal", "weight_x", "weight_y", "weight_z")],
+                                  ctx.get_field("fluid_fraction"), h2)
+    exact = spla.spsolve(A.tocsc(), ctx.get_field("rhs").astype(np.float64)[fluid])
+    loose = ctx.pressure_solve(runtime.SOLVE_FCG, pre_smooth=1, post_smooth=1, coarse_extra=16, max_cycles=60,
+                               tolerance=1e-6)
+    check = ctx.fp64_check_solve(1e-12)
+    assert check["converged"] and check["relative_residual"] <= 1e-12 and check["iterations"] > 10
+    p = ctx.get_field("pressure_fp64")[fluid]
+    want_l2 = np.linalg.norm(p - exact) / np.linalg.norm(exact)
+    assert check["error_l2"] == pytest.approx(want_l2, rel=1e-3, abs=1e-12)
+    assert check["error_max"] == pytest.approx(np.abs(p - exact).max() / np.abs(exact).max(), rel=1e-3, abs=1e-12)
+    assert 1e-12 < check["error_l2"] < 1e-3 and loose["relative_residual"] <= 1e-6
+    tight = _pressure_context(oracle_lib, 20, seed=3)
+    tight.pressure_solve(runtime.SOLVE_FCG, pre_smooth=1, post_smooth=1, coarse_extra=16, max_cycles=100,
+                         tolerance=1e-11)
+    assert tight.fp64_check_solve(1e-12)["error_l2"] < 1e-8
+
+
+def test_fcg_reports_the_recomputed_residual(oracle_lib):
+    """Modes 2-4 report ||b - A x|| recomputed from the FP64 iterate (not the
+    recurrence), checked here against numpy on the assembled operator."""
+    ctx = _pressure_context(oracle_lib, 18, seed=6)
+    h2 = np.float64(F32(ctx.dx)) ** 2
+    A, fluid, _ = operator_matrix(*[ctx.get_field(n) for n in ("diagonal", "weight_x", "weight_y", "weight_z")],
+                                  ctx.get_field("fluid_fraction"), h2)
+    b = ctx.get_field("rhs").astype(np.float64)[fluid]
+    stats = ctx.pressure_solve(runtime.SOLVE_FCG, pre_smooth=1, post_smooth=1, coarse_extra=16, max_cycles=60,
+                               tolerance=1e-7, coarse_weight=1.6)
+    x = ctx.get_field("pressure_fp64")[fluid]
+    true = np.linalg.norm(b - A @ x)
+    assert stats["final_residual"] == pytest.approx(true, rel=1e-6)
+    assert stats["relative_residual"] <= 1e-7
+    assert stats["recurrence_residual"] > 0 and stats["restarts"] >= 0
+
+
+@pytest.mark.parametrize("mode", ["fixed", "fp32", "mixed", "fcg"])
+def test_non_finite_solve_is_an_error(oracle_lib, mode):
+    """A NaN in the right-hand side ends every solve mode with
+    CLQ_ERR_NOT_CONVERGED instead of a silent NaN pressure."""
+    ctx = _pressure_context(oracle_lib, 12, seed=1)
+    rhs = ctx.get_field("rhs")
+    rhs[6, 6, 6] = np.nan
+    ctx.set_field("rhs", rhs)
+    with pytest.raises(runtime.ClqError) as err:
+        ctx.pressure_solve(runtime.SOLVE_MODES[mode], max_cycles=5, pre_smooth=1, post_smooth=1, coarse_extra=4)
+    assert err.value.status == runtime.ERR_NOT_CONVERGED and "not finite" in str(err.value)
