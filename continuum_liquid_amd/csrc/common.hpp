@@ -246,8 +246,8 @@ void launch_fcg_residual(const Level &L, const uint8_t *op, const double *x, con
                          hipStream_t st);
 // FP64 Jacobi-preconditioned CG check solve (scalars c[0..5], error c[8..11])
 int check_partials_count();
-void launch_check_init(const Level &L, const float *b, double inv_h2, const int32_t *rows, int64_t nrows, double *x,
-                       double *r, double *z, double *p, double *partials, double *c, hipStream_t st);
+void launch_check_init(const Level &L, const float *b, double inv_h2, const int32_t *rows, int64_t nrows, int from_x,
+                       double *x, double *r, double *z, double *p, double *partials, double *c, hipStream_t st);
 void launch_check_iteration(const Level &L, double inv_h2, const int32_t *rows, int64_t nrows, double *x, double *r,
                             double *z, double *po, double *pn, double *qv, double *partials, double *c,
                             hipStream_t st);

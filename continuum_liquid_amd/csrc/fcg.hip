@@ -454,18 +454,21 @@ void launch_fcg_residual(const Level &L, const uint8_t *op, const double *x, con
 //   update: alpha = r.z / p.q; x += alpha p, r -= alpha q, z = r / a; r.z, r.r
 // Scalars on the device: c[0] r.z, c[1] p.q, c[2] alpha, c[3] beta, c[4] r.r,
 // c[5] b.b.  The float operator arrays are read (not the packed bytes).
+// from_x 0: x = 0, r = b; 1: restart from the current x, r = b - A x
 __global__ __launch_bounds__(kT) void check_init_kernel(Level L, const float *__restrict__ b, double inv_h2,
-                                                       const int32_t *__restrict__ rows, int64_t nrows,
+                                                       const int32_t *__restrict__ rows, int64_t nrows, int from_x,
                                                        double *__restrict__ x, double *__restrict__ r,
                                                        double *__restrict__ z, double *__restrict__ p,
                                                        double *__restrict__ partials) {
     double acc[3] = {0.0, 0.0, 0.0};
     for_rows(L, rows, nrows, [&](int64_t q, bool inner, int, int, int) {
         const double bv = (double)b[q], dg = (double)L.diag[q];
-        const bool fluid = inner && L.mask[q] > 0.0f;
-        const double rv = fluid ? bv : 0.0;
+        bool fluid;
+        double dd;
+        const double av = apply_a_z<false, double>(L, nullptr, x, q, inner, inv_h2, fluid, dd);
+        const double rv = fluid ? bv - (from_x ? av : 0.0) : 0.0;
         const double zv = fluid && dg > 0.0 ? rv / (-dg * inv_h2) : 0.0;
-        x[q] = 0.0;
+        if (!from_x) x[q] = 0.0;
         r[q] = rv;
         z[q] = zv;
         p[q] = 0.0;
@@ -604,11 +607,12 @@ __global__ __launch_bounds__(1024) void check_scalars_kernel(const double *__res
 
 int check_partials_count() { return 4 * kB; }
 
-void launch_check_init(const Level &L, const float *b, double inv_h2, const int32_t *rows, int64_t nrows, double *x,
-                       double *r, double *z, double *p, double *partials, double *c, hipStream_t st) {
+void launch_check_init(const Level &L, const float *b, double inv_h2, const int32_t *rows, int64_t nrows, int from_x,
+                       double *x, double *r, double *z, double *p, double *partials, double *c, hipStream_t st) {
     const int64_t nr = rows_of(L, rows, nrows);
     const int nb = row_blocks_for(nr);
-    hipLaunchKernelGGL(check_init_kernel, dim3(nb), dim3(kT), 0, st, L, b, inv_h2, rows, nr, x, r, z, p, partials);
+    hipLaunchKernelGGL(check_init_kernel, dim3(nb), dim3(kT), 0, st, L, b, inv_h2, rows, nr, from_x, x, r, z, p,
+                       partials);
     hipLaunchKernelGGL(check_scalars_kernel, dim3(1), dim3(1024), 0, st, partials, nb, 0, c);
 }
 

@@ -70,7 +70,8 @@ def check_step_properties(ctx, scene_n, stats, tolerance, error_bound, label):
     assert stats["relative_residual"] <= tolerance, stats
     check = ctx.fp64_check_solve(1e-12)
     print(f"{label}: {stats['cycles']} cycles, residual {stats['relative_residual']:.3e} "
-          f"(recurrence {stats['recurrence_residual']:.3e}, {stats['restarts']} restarts), FP64 error "
+          f"(recurrence {stats['recurrence_residual'] / max(stats['rhs_norm'], 1e-300):.3e}, "
+          f"{stats['restarts']} restarts), FP64 error "
           f"l2 {check['error_l2']:.3e} max {check['error_max']:.3e}, check {check['iterations']} its "
           f"{check['relative_residual']:.1e}", flush=True)
     assert check["converged"], check
@@ -130,8 +131,10 @@ def test_config2_dam_break_sphere_128(oracle_lib, hip_lib):
 def test_config3_256_mixed_refinement_and_fcg(hip_lib):
     scene = scenes.bench_scene(3)
     h = make_context(hip_lib, scene)
-    mixed = {"mode": "mixed", "max_cycles": 80, "pre_smooth": 2, "post_smooth": 2, "coarse_extra": 16,
-             "vcycle_from": 2, "coarse_dtype": 0, "tolerance": 1e-8}
+    # FP64 refinement around the W-cycle at every level (the hybrid V/W cycle
+    # of the FCG preconditioner contracts too slowly as a stationary method)
+    mixed = {"mode": "mixed", "max_cycles": 200, "pre_smooth": 2, "post_smooth": 2, "coarse_extra": 64,
+             "vcycle_from": 0, "coarse_dtype": 0, "tolerance": 1e-8}
     stats = host_solver.step(h, params_for(scene, mixed))
     check_step_properties(h, scene.particle_count, stats, 1e-8, 1e-4, "config 3 mixed 1e-8")
     for _ in range(2):
@@ -168,6 +171,9 @@ def test_config5_wide_512_learned_and_voxelised(hip_lib):
     stats = host_solver.step(h, params_for(scene, fcg(mode="learned")))
     check_step_properties(h, scene.particle_count, stats, 1e-6, 1e-3, "config 5 learned step")
     h.attach_net(None)
+    # occupancy of the advected particles: gather them again for comparison
+    h.bin_particles()
+    h.particle_to_grid(8.0)
     occ1, _ = h.voxelise(1, 8.0)
     occ2, dens2 = h.voxelise(2, 8.0, density=True)
     assert occ2.shape == tuple(2 * n for n in scene.shape)
@@ -187,8 +193,11 @@ def test_non_finite_solve_is_an_error(hip_lib, mode):
     from .helpers import run_to_pressure
 
     run_to_pressure(h, scene)
+    # a NaN in an interior fluid cell (the system's rows)
+    ff = h.get_field("fluid_fraction")
+    cell = tuple(np.argwhere(ff[1:-1, 1:-1, 1:-1] > 0)[0] + 1)
     rhs = h.get_field("rhs")
-    rhs[12, 5, 12] = np.nan
+    rhs[cell] = np.nan
     h.set_field("rhs", rhs)
     with pytest.raises(runtime.ClqError) as err:
         h.pressure_solve(runtime.SOLVE_MODES[mode], max_cycles=5, pre_smooth=1, post_smooth=1, coarse_extra=4)

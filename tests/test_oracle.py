@@ -600,8 +600,11 @@ def test_non_finite_solve_is_an_error(oracle_lib, mode):
     """A NaN in the right-hand side ends every solve mode with
     CLQ_ERR_NOT_CONVERGED instead of a silent NaN pressure."""
     ctx = _pressure_context(oracle_lib, 12, seed=1)
+    # a NaN in an interior fluid cell (the system's rows)
+    ff = ctx.get_field("fluid_fraction")
+    cell = tuple(np.argwhere(ff[1:-1, 1:-1, 1:-1] > 0)[0] + 1)
     rhs = ctx.get_field("rhs")
-    rhs[6, 6, 6] = np.nan
+    rhs[cell] = np.nan
     ctx.set_field("rhs", rhs)
     with pytest.raises(runtime.ClqError) as err:
         ctx.pressure_solve(runtime.SOLVE_MODES[mode], max_cycles=5, pre_smooth=1, post_smooth=1, coarse_extra=4)
