@@ -16,7 +16,11 @@ a `roofline` object for the dominant kernel (algorithmic bytes per launch
 divided by the launch duration measured with HIP events on the library's
 stream during the timed steps) and a `cpu_baseline` object (the CPU oracle,
 a C/OpenMP restatement of the reference's OpenCL kernels, timed on this
-host's cores on a bounded sample of the same workload).
+host's cores on a bounded sample of the same workload), plus the same oracle
+at the reference's own solver settings per section.  After the timed steps
+(outside the timed region) the last solve is checked against an independent
+FP64 solve (`fp64_error_max`), and the run goes on to the configuration's
+last step, timing the final 20 steps apart (`late_phase`).
 """
 
 from __future__ import annotations
@@ -57,15 +61,30 @@ def algorithmic_bytes(n_particles: int, cells: int, swept_cells: float) -> dict:
     }
 
 
-# HBM bytes measured with rocprofv3 FETCH_SIZE (x 2.0, calibrated on a plain
-# copy) and WRITE_SIZE on the 256^3 dam break + sphere with the FCG solve
-# (profiles/step_256cubed_fcg_pmc_traffic.md): per P2G launch
-# (1,023,232.9 KiB x 2.0 + 286,720.0 KiB) x 1024, and per swept cell of one
-# finest-level sweep (two colour launches of 58.3 MB over 4,245,504 cells).
-# Reported as `roofline.traffic` for that workload only; the counters cannot
-# be read from inside this process.
-MEASURED_TRAFFIC = {(3, "fcg"): {"p2g": lambda swept: 2_389_182_259,
-                                 "rbgs_fine_sweep": lambda swept: 27.47 * swept}}
+def measured_traffic(config: int, solver: str, kernel: str, swept: float):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 (FETCH_SIZE
+    scaled by the copy calibration, plus WRITE_SIZE) for this workload, read
+    from the `bench-traffic:` lines of the committed profile summaries
+    (profiles/*.md); None when no profile covers it.  The counters cannot be
+    read from inside this process, so the figure is as current as the profile
+    named in the returned source."""
+    for path in sorted((ROOT / "profiles").glob("*.md")):
+        for line in path.read_text().splitlines():
+            if not line.startswith("bench-traffic:"):
+                continue
+            fields = dict(item.split("=", 1) for item in line.split()[1:])
+            if fields.get("config") != str(config) or fields.get("solver") != solver or fields.get("kernel") != kernel:
+                continue
+            if "bytes_per_launch" in fields:
+                return int(float(fields["bytes_per_launch"])), f"profiles/{path.name}"
+            if "bytes_per_swept_cell" in fields:
+                return int(float(fields["bytes_per_swept_cell"]) * swept), f"profiles/{path.name}"
+    return None, None
+
+
+# steps of each configuration's run (the timed steps cover the first ones;
+# the late phase times the last ones)
+CONFIG_STEPS = {1: 100, 2: 200, 3: 100, 4: 200, 5: 50}
 
 
 def solver_settings(name: str, tolerance: float) -> dict:
@@ -121,6 +140,31 @@ def cpu_baseline(scene, params, steps: int) -> dict:
     }
 
 
+def cpu_reference_settings(scene, warmup: int = 3) -> dict:
+    """The oracle at the reference's own solver settings (one W(2, 4) cycle
+    with 64 extra coarsest sweeps, pure FLIP), per section with the
+    reference's section names (Solver/Kernel/timing.py), after `warmup`
+    untimed steps; one timed step."""
+    from continuum_liquid_amd.Solver.Kernel import timing
+
+    lib = runtime.bind_library(ctypes.CDLL(str(ROOT / "oracle" / "libcliquid_oracle.so")))
+    params = host_solver.StepParams(dt=1.0 / 240.0, gravity=scene.gravity, flip_ratio=1.0,
+                                    solver=solver_settings("fixed", 1e-6))
+    with runtime.Context(lib, scene.shape, scene.dx, scene.particle_count) as ctx:
+        ctx.set_particles(scene.positions, scene.velocities)
+        for _ in range(warmup):
+            host_solver.step(ctx, params)
+        timings = timing.RunTimings()
+        t0 = time.perf_counter()
+        host_solver.step(ctx, params, timings)
+        ms = (time.perf_counter() - t0) * 1e3
+    sections = {name: round(seconds * 1e3, 3) for (_, name), (_, seconds) in timings.entries.items()}
+    return {"value": round(ms, 3), "unit": "ms/step", "cores": len(os.sched_getaffinity(0)), "kind": "port",
+            "sample": f"1 step after {warmup} untimed steps, reference settings: one W(2,4) cycle with 64 extra "
+                      "coarsest sweeps, pure FLIP; sections timed with device synchronisation",
+            "sections_ms": sections}
+
+
 def calibrate_copy(nbytes: int) -> int:
     """Copy nbytes between two device buffers three times (torch's copy
     kernel: nbytes read and nbytes written per launch)."""
@@ -158,6 +202,9 @@ def main() -> int:
                     help="weight of the cycle's coarse-grid corrections (1: the reference's plain sum)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 check solves after the timed steps")
+    ap.add_argument("--no-late", dest="late", action="store_false",
+                    help="skip running on to the configuration's last step (late-phase timing)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
@@ -204,12 +251,14 @@ def main() -> int:
             host_solver.step(ctx, params)
         ctx.sync()
         ctx.kernel_timing(True)
-        solve_ms, relres, cycles = [], [], []
+        solve_ms, relres, cycles, recres, restarts = [], [], [], [], []
         t0 = time.perf_counter()
         for i in range(args.steps):
             stats = host_solver.step(ctx, params)
             solve_ms.append(stats["solve_ms"])
             relres.append(stats["relative_residual"])
+            recres.append(stats["recurrence_residual"] / stats["rhs_norm"] if stats["rhs_norm"] > 0 else 0.0)
+            restarts.append(stats["restarts"])
             cycles.append(stats["cycles"])
             fallback.append(stats["fallback_cycles"])
             swept.append(stats["active_cells"])
@@ -217,6 +266,35 @@ def main() -> int:
         elapsed = time.perf_counter() - t0
         ktimes = ctx.kernel_times()
         ctx.kernel_timing(False)
+        # untimed from here on: the FP64 check of the last timed solve, then
+        # the rest of the configuration's steps with the last 20 timed apart
+        checks = [] if args.no_check else [ctx.fp64_check_solve(1e-12)]
+        late = None
+        total = CONFIG_STEPS[args.config]
+        done = args.warmup + args.steps
+        if args.late and total > done:
+            tail = min(20, total - done)
+            for _ in range(total - done - tail):
+                host_solver.step(ctx, params)
+            ctx.sync()
+            ctx.kernel_timing(True)
+            late_solve, late_res = [], []
+            t1 = time.perf_counter()
+            for _ in range(tail):
+                st = host_solver.step(ctx, params)
+                late_solve.append(st["solve_ms"])
+                late_res.append(st["relative_residual"])
+            ctx.sync()
+            late_elapsed = time.perf_counter() - t1
+            late_k = ctx.kernel_times()
+            ctx.kernel_timing(False)
+            late = {"steps": [total - tail + 1, total], "ms_per_step": round(late_elapsed * 1e3 / tail, 3),
+                    "pressure_solve_ms": round(float(np.mean(late_solve)), 3),
+                    "relative_residual_max": float(np.max(late_res)),
+                    "kernel_avg_ms": {k: round(t / u, 5) for k, (t, u) in late_k.items() if u}}
+            relres.extend(late_res)
+            if not args.no_check:
+                checks.append(ctx.fp64_check_solve(1e-12))
 
     ms_per_step = elapsed * 1e3 / args.steps
     n, cells = scene.particle_count, int(np.prod(scene.shape))
@@ -237,8 +315,7 @@ def main() -> int:
         }
     dominant = max(kernels, key=lambda k: ktimes[k][0])
     d = kernels[dominant]
-    measured = MEASURED_TRAFFIC.get((args.config, args.solver), {}).get(dominant)
-    traffic = int(measured(float(np.mean(swept)))) if measured else None
+    traffic, traffic_source = measured_traffic(args.config, args.solver, dominant, float(np.mean(swept)))
 
     result = {
         "metric": METRIC,
@@ -268,7 +345,16 @@ def main() -> int:
         "particle_steps_per_s": round(n * args.steps / elapsed, 1),
         "pressure_solve_ms": round(float(np.mean(solve_ms)), 3),
         "w_cycles_per_step": round(float(np.mean(cycles)), 2),
-        "relative_residual_max": float(np.max(relres)),
+        # the solver's reported residual is recomputed as ||b - A x|| from its
+        # FP64 iterate; the recurrence value is the iteration's own estimate
+        "true_relative_residual_max": float(np.max(relres)),
+        **({"recurrence_relative_residual_max": float(np.max(recres)), "restarts": int(np.sum(restarts))}
+           if args.solver in ("fcg", "learned") else {}),
+        **({"fp64_error_max": float(max(c["error_max"] for c in checks)),
+            "fp64_error_l2_max": float(max(c["error_l2"] for c in checks)),
+            "fp64_check": [{k: (round(v, 3) if k == "solve_ms" else v) for k, v in c.items()} for c in checks]}
+           if checks else {}),
+        **({"late_phase": late} if late else {}),
         "swept_cells_per_sweep": int(np.mean(swept)),
         **({"w_cycle_fallback_per_step": round(float(np.mean(fallback)), 2)} if args.solver == "learned" else {}),
         # FCG iterations run with weight 1 after the weighted cycle broke down
@@ -282,6 +368,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": d["frac_hbm_peak"],
             "traffic": traffic,
+            "traffic_source": traffic_source,
         },
         "kernels": kernels,
     }
@@ -292,6 +379,7 @@ def main() -> int:
             cpu_params = host_solver.StepParams(dt=params.dt, gravity=params.gravity, flip_ratio=params.flip_ratio,
                                                 solver=solver_settings("fcg", args.tolerance))
         result["cpu_baseline"] = cpu_baseline(scene, cpu_params, args.cpu_steps)
+        result["cpu_baseline_reference_settings"] = cpu_reference_settings(scene)
     print(json.dumps(result), flush=True)
     return 0
 
