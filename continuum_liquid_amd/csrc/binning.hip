@@ -29,6 +29,29 @@ __device__ __forceinline__ int bin_key(const float *p, float inv_dx, int nx, int
     return (int)gidx(bx, by, bz, ny, nz);
 }
 
+// Runs of equal keys within the wave (lanes in order): whether this lane
+// heads a run, the run's head lane and the lane after its end.
+struct WaveRun {
+    bool head;
+    int head_lane, end_lane;
+};
+__device__ __forceinline__ WaveRun wave_run(int k, bool valid, int lane) {
+    const int prev = __shfl_up(k, 1);
+    const bool head = valid && (lane == 0 || prev != k);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long live = __ballot(valid);
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes <= lane (lane 63: all ones)
+    const unsigned long long below = heads & upto;
+    const unsigned long long above = heads & ~upto;
+    WaveRun r;
+    r.head = head;
+    r.head_lane = below ? 63 - __clzll((long long)below) : lane;
+    r.end_lane = above ? __ffsll((long long)above) - 1 : 64 - __clzll((long long)live);
+    return r;
+}
+
+// One lane per particle: its bin key and a provisional slot in its bin (one
+// atomic per run of equal keys in the wave, handed out in lane order).
 __global__ __launch_bounds__(256) void bin_count_kernel(const float *__restrict__ pos, int n, float inv_dx, int nx,
                                                         int ny, int nz, int32_t *__restrict__ key,
                                                         int32_t *__restrict__ rank, int32_t *__restrict__ counts) {
@@ -37,23 +60,13 @@ __global__ __launch_bounds__(256) void bin_count_kernel(const float *__restrict_
     const bool valid = i < n;
     int k = -1;
     if (valid) k = bin_key(pos + 3LL * i, inv_dx, nx, ny, nz);
-    // Runs of equal keys within the wave: the head lane of each run takes
-    // run-length slots with one atomic and hands them out in lane order.
-    const int prev = __shfl_up(k, 1);
-    const bool head = valid && (lane == 0 || prev != k);
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long live = __ballot(valid);
-    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes <= lane (lane 63: all ones)
-    const unsigned long long below = heads & upto;
-    const unsigned long long above = heads & ~upto;
-    const int h = below ? 63 - __clzll((long long)below) : lane;
-    const int run_end = above ? __ffsll((long long)above) - 1 : 64 - __clzll((long long)live);
+    const WaveRun r = wave_run(k, valid, lane);
     int base = 0;
-    if (head) base = atomicAdd(counts + k, run_end - lane);
-    base = __shfl(base, h);
+    if (r.head) base = atomicAdd(counts + k, r.end_lane - lane);
+    base = __shfl(base, r.head_lane);
     if (valid) {
         key[i] = k;
-        rank[i] = base + (lane - h);
+        rank[i] = base + (lane - r.head_lane);
     }
 }
 
