@@ -231,7 +231,9 @@ __global__ __launch_bounds__(kThreads) void p2g_staged_kernel(const float *__res
         // occupancy (particles.cl:186-188): the cell clamp(trunc(x inv_dx),
         // 1, n - 2) of a particle is its bin or the bin above on each axis
         // (clamps and NaN included), so every particle whose cell the patch
-        // owns is staged by this block once; integer counts, exact
+        // owns is staged by this block once, from one of the rows
+        // (ci0 - 1 .. ci0 + 1, cj0 - 1 .. cj0 + 1); integer counts, exact
+        if ((r >> 2) > 2 || (r & 3) > 2) return;
 #pragma unroll
         for (int q = 0; q < kPerThread; ++q) {
             const int t = tid + q * kThreads;
@@ -246,7 +248,16 @@ __global__ __launch_bounds__(kThreads) void p2g_staged_kernel(const float *__res
     };
     int r = 0, lo = row_lo[0];
     settle(r, lo);
-    if (r < kRowsStaged) {
+    if (r == kRowsStaged) {  // no particle within reach: every sum is +0 (block-uniform)
+        if (ci >= nx || cj >= ny || lane == 0 || lane == 63 || ck >= nz) return;
+        const int64_t index = gidx(ci, cj, ck, ny, nz);
+        u[index] = 0.0f;
+        v[index] = 0.0f;
+        w[index] = 0.0f;
+        ff[index] = 0.0f;
+        return;
+    }
+    {
         load_item(r, lo);
         store_item(r, lo, buf[0]);
     }
