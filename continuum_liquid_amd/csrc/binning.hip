@@ -9,13 +9,19 @@
 //   1. bin_count: one lane per particle computes its bin key and takes a
 //      provisional slot with one atomicAdd per run of equal keys in the
 //      wavefront (particles arrive nearly sorted from the previous step, so a
-//      64-lane wave usually touches ~8 bins).
+//      64-lane wave usually touches ~8 bins); a wave handles four groups of
+//      64 particles so that their atomics are in flight together.
 //   2. exclusive scan of the counts: tile reduce, scan of tile sums, tile scan.
 //   3. bin_permute: perm[offset[key] + provisional] = particle index.
-//   4. bin_gather: every output slot ranks its particle among the members of
-//      its bin by original index (stable order), then moves position and
-//      velocity.  The reference's atomic_inc scatter leaves the order within a
-//      bin to chance; this one is the stable order, deterministic run to run.
+//   4. bin_order: a block takes 128 consecutive bins, stages their slice of
+//      perm in LDS, ranks each member among its bin's members by original
+//      index (stable order) and moves its position and velocity to the
+//      ranked slot.  Bins of more than kBigBin members are listed instead and
+//      sorted by a block-level merge sort (bin_big_sort, O(c log^2 c) for c
+//      members, so no bin size makes the ranking quadratic), then moved
+//      (bin_big_move).
+//   The reference's atomic_inc scatter leaves the order within a bin to
+//   chance; this one is the stable order, deterministic run to run.
 #include "common.hpp"
 
 namespace clq {
@@ -51,30 +57,65 @@ __device__ __forceinline__ WaveRun wave_run(int k, bool valid, int lane) {
 }
 
 // One lane per particle: its bin key and a provisional slot in its bin (one
-// atomic per run of equal keys in the wave, handed out in lane order).
+// atomic per run of equal keys in each group of 64 particles, handed out in
+// lane order).  Wave w of the grid handles particles 256 w .. 256 w + 255 as
+// four groups.  The heads of all runs of the four groups are listed in LDS
+// (at most 256, typically ~32) and their atomics issued together, one lane
+// per head, so that the wave waits for one round trip instead of four.
+constexpr int kCountGroups = 4;
 __global__ __launch_bounds__(256) void bin_count_kernel(const float *__restrict__ pos, int n, float inv_dx, int nx,
                                                         int ny, int nz, int32_t *__restrict__ key,
                                                         int32_t *__restrict__ rank, int32_t *__restrict__ counts) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    const bool valid = i < n;
-    int k = -1;
-    if (valid) k = bin_key(pos + 3LL * i, inv_dx, nx, ny, nz);
-    const WaveRun r = wave_run(k, valid, lane);
-    int base = 0;
-    if (r.head) base = atomicAdd(counts + k, r.end_lane - lane);
-    base = __shfl(base, r.head_lane);
-    if (valid) {
-        key[i] = k;
-        rank[i] = base + (lane - r.head_lane);
+    __shared__ int head_key[4][64 * kCountGroups], head_len[4][64 * kCountGroups], head_base[4][64 * kCountGroups];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t first = ((int64_t)blockIdx.x * 4 + wave) * (64 * kCountGroups) + lane;
+    // every load issued first (lanes past the end read the last particle)
+    float x[kCountGroups][3];
+#pragma unroll
+    for (int g = 0; g < kCountGroups; ++g) {
+        const int64_t i = min(first + 64 * g, (int64_t)n - 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[g][c] = pos[3 * i + c];
+    }
+    int k[kCountGroups], slot[kCountGroups];
+    WaveRun r[kCountGroups];
+    int heads = 0;  // runs listed so far (wave-uniform)
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int g = 0; g < kCountGroups; ++g) {
+        const bool valid = first + 64 * g < n;
+        k[g] = valid ? bin_key(x[g], inv_dx, nx, ny, nz) : -1;
+        r[g] = wave_run(k[g], valid, lane);
+        const unsigned long long hm = __ballot(r[g].head);
+        slot[g] = heads + __popcll(hm & below);  // this lane's list entry when it heads a run
+        if (r[g].head) {
+            head_key[wave][slot[g]] = k[g];
+            head_len[wave][slot[g]] = r[g].end_lane - lane;
+        }
+        heads += __popcll(hm);
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int j = lane; j < heads; j += 64)
+        head_base[wave][j] = atomicAdd(counts + head_key[wave][j], head_len[wave][j]);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int g = 0; g < kCountGroups; ++g) {
+        const int64_t i = first + 64 * g;
+        // the list entry of this lane's run head, from the head lane
+        const int hs = __shfl(slot[g], r[g].head_lane);
+        if (i < n) {
+            key[i] = k[g];
+            rank[i] = head_base[wave][hs] + (lane - r[g].head_lane);
+        }
     }
 }
 
 void launch_bin_count(const float *pos, int n, float inv_dx, int nx, int ny, int nz, int32_t *key, int32_t *rank,
                       int32_t *counts, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(bin_count_kernel, dim3((n + 255) / 256), dim3(256), 0, s, pos, n, inv_dx, nx, ny, nz, key,
-                       rank, counts);
+    constexpr int per_block = 256 * kCountGroups;
+    hipLaunchKernelGGL(bin_count_kernel, dim3((n + per_block - 1) / per_block), dim3(256), 0, s, pos, n, inv_dx, nx,
+                       ny, nz, key, rank, counts);
 }
 
 // ---- exclusive scan ----------------------------------------------------------
@@ -181,60 +222,204 @@ void launch_exclusive_scan(const int32_t *in, int32_t *out, int64_t count, int32
     hipLaunchKernelGGL(scan_tile_apply, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, in, count, scratch, out);
 }
 
-// ---- permutation and stable gather ---------------------------------------------
+// ---- permutation and stable order ---------------------------------------------
+constexpr int kPermPer = 4;
 __global__ __launch_bounds__(256) void bin_permute_kernel(const int32_t *__restrict__ key,
                                                           const int32_t *__restrict__ rank,
                                                           const int32_t *__restrict__ offsets, int n,
                                                           int32_t *__restrict__ perm) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    perm[offsets[key[i]] + rank[i]] = i;
+    const int base = blockIdx.x * (256 * kPermPer) + threadIdx.x;
+    int k[kPermPer], r[kPermPer], o[kPermPer];
+#pragma unroll
+    for (int q = 0; q < kPermPer; ++q) {
+        const int i = base + 256 * q;
+        k[q] = i < n ? key[i] : 0;
+        r[q] = i < n ? rank[i] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kPermPer; ++q) o[q] = offsets[k[q]];
+#pragma unroll
+    for (int q = 0; q < kPermPer; ++q) {
+        const int i = base + 256 * q;
+        if (i < n) perm[o[q] + r[q]] = i;
+    }
 }
 
 void launch_bin_permute(const int32_t *key, const int32_t *rank, const int32_t *offsets, int n, int32_t *perm,
                         hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(bin_permute_kernel, dim3((n + 255) / 256), dim3(256), 0, s, key, rank, offsets, n, perm);
+    constexpr int per_block = 256 * kPermPer;
+    hipLaunchKernelGGL(bin_permute_kernel, dim3((n + per_block - 1) / per_block), dim3(256), 0, s, key, rank, offsets,
+                       n, perm);
 }
 
-__global__ __launch_bounds__(256) void bin_gather_kernel(const int32_t *__restrict__ perm,
-                                                         const int32_t *__restrict__ key,
-                                                         const int32_t *__restrict__ offsets,
-                                                         const int32_t *__restrict__ counts,
-                                                         const float *__restrict__ pos, const float *__restrict__ vel,
-                                                         int n, float *__restrict__ pos_out,
-                                                         float *__restrict__ vel_out) {
-    const int slot = blockIdx.x * 256 + threadIdx.x;
-    if (slot >= n) return;
-    const int i = perm[slot];
-    // the particle's data is loaded first: it depends on i alone
-    const int64_t src = 3LL * i;
-    const float p0 = pos[src], p1 = pos[src + 1], p2 = pos[src + 2];
-    const float v0 = vel[src], v1 = vel[src + 1], v2 = vel[src + 2];
-    const int b = key[i];
-    const int start = offsets[b], cnt = counts[b];
-    // rank among the bin's members by original index, eight members' loads
-    // in flight at a time (the members are slots start .. start + cnt - 1;
-    // cnt >= 1, and indices past the last member read the last one, whose
-    // comparison is dropped)
-    int r = 0;
-    for (int t0 = 0; t0 < cnt; t0 += 8) {
-        int m[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) m[u] = perm[start + min(t0 + u, cnt - 1)];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) r += (t0 + u < cnt) & (m[u] < i);
+constexpr int kOrderBins = 128;   // bins per block
+constexpr int kOrderCap = 4096;   // perm entries staged in LDS
+constexpr int kBigBin = 64;       // larger bins take the merge-sort path
+
+int big_bin_capacity(int64_t n) { return (int)(n / (kBigBin + 1) + 1); }
+
+// Block b: bins [128 b, 128 b + 128) and their slots [S0, S1).  Slot s holds
+// provisionally particle i = perm[s]; its stable slot is start + (number of
+// its bin's members with a smaller index), counted over the staged slice
+// (read from perm when the slice exceeds kOrderCap).  Bins of more than
+// kBigBin members are appended to big_list and skipped.
+__global__ __launch_bounds__(256) void bin_order_kernel(const int32_t *__restrict__ perm,
+                                                        const int32_t *__restrict__ offsets,
+                                                        const int32_t *__restrict__ counts, int64_t C, int n,
+                                                        const float *__restrict__ pos, const float *__restrict__ vel,
+                                                        float *__restrict__ pos_out, float *__restrict__ vel_out,
+                                                        int32_t *__restrict__ big_list,
+                                                        int32_t *__restrict__ big_count, int big_cap) {
+    __shared__ int off_s[kOrderBins], cnt_s[kOrderBins];
+    __shared__ int p_s[kOrderCap];
+    const int t = threadIdx.x;
+    const int64_t b0 = (int64_t)blockIdx.x * kOrderBins;
+    if (t < kOrderBins) {
+        const int64_t b = b0 + t;
+        const int c = b < C ? counts[b] : 0;
+        off_s[t] = b < C ? offsets[b] : n;
+        cnt_s[t] = c;
+        if (c > kBigBin) {
+            const int at = atomicAdd(big_count, 1);
+            if (at < big_cap) big_list[at] = (int)b;
+        }
     }
-    const int64_t dst = 3LL * (start + r);
-    pos_out[dst] = p0; pos_out[dst + 1] = p1; pos_out[dst + 2] = p2;
-    vel_out[dst] = v0; vel_out[dst + 1] = v1; vel_out[dst + 2] = v2;
+    __syncthreads();
+    const int S0 = off_s[0], S1 = off_s[kOrderBins - 1] + cnt_s[kOrderBins - 1];
+    const int span = S1 - S0;
+    if (span == 0) return;  // block-uniform
+    const bool staged = span <= kOrderCap;  // block-uniform
+    if (staged)
+        for (int q = t; q < span; q += 256) p_s[q] = perm[S0 + q];
+    __syncthreads();
+    // kMoves slots per thread at a time: ranks first, then every load, then
+    // every store
+    constexpr int kMoves = 4;
+    for (int s0 = S0 + t; s0 < S1; s0 += 256 * kMoves) {
+        int src[kMoves], dst[kMoves];
+#pragma unroll
+        for (int q = 0; q < kMoves; ++q) {
+            const int s = s0 + 256 * q;
+            src[q] = -1;
+            dst[q] = 0;
+            if (s >= S1) continue;
+            // the bin holding slot s: the last j with off_s[j] <= s
+            int lo = 0, hi = kOrderBins;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (off_s[mid] <= s) lo = mid;
+                else hi = mid;
+            }
+            const int start = off_s[lo], cnt = cnt_s[lo];
+            if (cnt > kBigBin) continue;
+            const int i = staged ? p_s[s - S0] : perm[s];
+            int r = 0;
+            for (int m = 0; m < cnt; ++m) r += (staged ? p_s[start - S0 + m] : perm[start + m]) < i;
+            src[q] = i;
+            dst[q] = start + r;
+        }
+        float pv[kMoves][6];
+#pragma unroll
+        for (int q = 0; q < kMoves; ++q) {
+            const int64_t i = src[q] < 0 ? 0 : src[q];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                pv[q][c] = pos[3 * i + c];
+                pv[q][3 + c] = vel[3 * i + c];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kMoves; ++q) {
+            if (src[q] < 0) continue;
+            const int64_t f = dst[q];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                pos_out[3 * f + c] = pv[q][c];
+                vel_out[3 * f + c] = pv[q][3 + c];
+            }
+        }
+    }
 }
 
-void launch_bin_gather(const int32_t *perm, const int32_t *key, const int32_t *offsets, const int32_t *counts,
-                       const float *pos, const float *vel, int n, float *pos_out, float *vel_out, hipStream_t s) {
+__device__ __forceinline__ void move_particle(const float *__restrict__ pos, const float *__restrict__ vel,
+                                              float *__restrict__ pos_out, float *__restrict__ vel_out, int64_t i,
+                                              int64_t f) {
+    const float p0 = pos[3 * i], p1 = pos[3 * i + 1], p2 = pos[3 * i + 2];
+    const float v0 = vel[3 * i], v1 = vel[3 * i + 1], v2 = vel[3 * i + 2];
+    pos_out[3 * f] = p0; pos_out[3 * f + 1] = p1; pos_out[3 * f + 2] = p2;
+    vel_out[3 * f] = v0; vel_out[3 * f + 1] = v1; vel_out[3 * f + 2] = v2;
+}
+
+// Number of the len sorted values a[0..len) below x.
+__device__ __forceinline__ int count_below(const int32_t *__restrict__ a, int len, int x) {
+    int lo = 0, hi = len;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Each listed bin's slice of perm sorted by value (particle index) with a
+// bottom-up merge sort in global memory by one block, runs merged by
+// ranking every element in the other run (values are distinct), scratch the
+// same slots of `scratch`.
+__global__ __launch_bounds__(256) void bin_big_sort_kernel(const int32_t *__restrict__ big_list,
+                                                           const int32_t *__restrict__ big_count, int big_cap,
+                                                           const int32_t *__restrict__ offsets,
+                                                           const int32_t *__restrict__ counts, int32_t *perm,
+                                                           int32_t *scratch) {
+    const int nb = min(*big_count, big_cap);
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int bin = big_list[b], start = offsets[bin], cnt = counts[bin];
+        int32_t *src = perm + start, *dst = scratch + start;
+        for (int64_t w = 1; w < cnt; w *= 2) {
+            for (int idx = threadIdx.x; idx < cnt; idx += 256) {
+                const int64_t a = idx - idx % (2 * w), mid = min(a + w, (int64_t)cnt), end = min(a + 2 * w, (int64_t)cnt);
+                const int x = src[idx];
+                const int64_t to = idx < mid ? (idx - a) + count_below(src + mid, (int)(end - mid), x)
+                                             : (idx - mid) + count_below(src + a, (int)(mid - a), x);
+                dst[a + to] = x;
+            }
+            __syncthreads();
+            int32_t *tmp = src;
+            src = dst;
+            dst = tmp;
+        }
+        if (src != perm + start)
+            for (int idx = threadIdx.x; idx < cnt; idx += 256) perm[start + idx] = src[idx];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void bin_big_move_kernel(const int32_t *__restrict__ big_list,
+                                                           const int32_t *__restrict__ big_count, int big_cap,
+                                                           const int32_t *__restrict__ offsets,
+                                                           const int32_t *__restrict__ counts,
+                                                           const int32_t *__restrict__ perm,
+                                                           const float *__restrict__ pos,
+                                                           const float *__restrict__ vel, float *__restrict__ pos_out,
+                                                           float *__restrict__ vel_out) {
+    const int nb = min(*big_count, big_cap);
+    for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int bin = big_list[b], start = offsets[bin], cnt = counts[bin];
+        for (int f = start + threadIdx.x; f < start + cnt; f += 256) move_particle(pos, vel, pos_out, vel_out, perm[f], f);
+    }
+}
+
+void launch_bin_order(const int32_t *perm, const int32_t *offsets, const int32_t *counts, int64_t C, int n,
+                      const float *pos, const float *vel, float *pos_out, float *vel_out, int32_t *big_list,
+                      int32_t *big_count, int big_cap, int32_t *scratch, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(bin_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, s, perm, key, offsets, counts, pos,
-                       vel, n, pos_out, vel_out);
+    const int64_t blocks = (C + kOrderBins - 1) / kOrderBins;
+    hipLaunchKernelGGL(bin_order_kernel, dim3((unsigned)blocks), dim3(256), 0, s, perm, offsets, counts, C, n, pos, vel,
+                       pos_out, vel_out, big_list, big_count, big_cap);
+    hipLaunchKernelGGL(bin_big_sort_kernel, dim3(256), dim3(256), 0, s, big_list, big_count, big_cap, offsets, counts,
+                       const_cast<int32_t *>(perm), scratch);
+    hipLaunchKernelGGL(bin_big_move_kernel, dim3(256), dim3(256), 0, s, big_list, big_count, big_cap, offsets, counts,
+                       perm, pos, vel, pos_out, vel_out);
 }
 
 }  // namespace clq

@@ -113,12 +113,10 @@ __device__ __forceinline__ void coords(int64_t index, int ny, int nz, int &i, in
     j = (int)b;
     k = (int)(r - b * (uint32_t)nz);
 }
-// convert_int_rtn + clamp; NaN and out-of-range inputs clamp.
+// convert_int_rtn + clamp; NaN and out-of-range inputs clamp (NaN to lo:
+// fmaxf returns the other operand), without branches.
 __device__ __forceinline__ int floor_clamp(float t, int lo, int hi) {
-    t = floorf(t);
-    if (!(t >= (float)lo)) return lo;
-    if (t > (float)hi) return hi;
-    return (int)t;
+    return (int)fminf(fmaxf(floorf(t), (float)lo), (float)hi);
 }
 // convert_int_rtz + clamp.
 // trunc(t) clamped to [lo, hi] (lo for NaN), without branches: fmaxf
@@ -135,8 +133,14 @@ int64_t scan_scratch_elems(int64_t count);
 void launch_exclusive_scan(const int32_t *in, int32_t *out, int64_t count, int32_t *scratch, hipStream_t s);
 void launch_bin_permute(const int32_t *key, const int32_t *rank, const int32_t *offsets, int n, int32_t *perm,
                         hipStream_t s);
-void launch_bin_gather(const int32_t *perm, const int32_t *key, const int32_t *offsets, const int32_t *counts,
-                       const float *pos, const float *vel, int n, float *pos_out, float *vel_out, hipStream_t s);
+// stable order within bins and the move of positions and velocities; bins of
+// more than 64 members go through big_list (capacity big_bin_capacity(n),
+// *big_count zeroed by the caller) and are merge-sorted using scratch
+// (n entries, free at this point)
+int big_bin_capacity(int64_t n);
+void launch_bin_order(const int32_t *perm, const int32_t *offsets, const int32_t *counts, int64_t C, int n,
+                      const float *pos, const float *vel, float *pos_out, float *vel_out, int32_t *big_list,
+                      int32_t *big_count, int big_cap, int32_t *scratch, hipStream_t s);
 
 // p2g.hip
 void launch_p2g(const float *pos, const float *vel, const int32_t *offsets, const int32_t *counts, float *u,

@@ -73,6 +73,9 @@ struct clq_ctx {
     int64_t cap = 0, n = 0;
     float *pos = nullptr, *vel = nullptr, *pos_work = nullptr, *vel_work = nullptr, *prev = nullptr;
     int32_t *key = nullptr, *rank = nullptr, *perm = nullptr;
+    // bins too crowded for the in-LDS ranking (allocated with the particles)
+    int32_t *big_list = nullptr, *big_count = nullptr;
+    int big_cap = 0;
     int32_t *counts = nullptr, *offsets = nullptr, *scan_scratch = nullptr;
     float *u = nullptr, *v = nullptr, *w = nullptr, *uo = nullptr, *vo = nullptr, *wo = nullptr;
     float *ff = nullptr, *p = nullptr, *rhs = nullptr, *diag = nullptr, *wx = nullptr, *wy = nullptr, *wz = nullptr;
@@ -150,20 +153,23 @@ struct clq_ctx {
 namespace {
 
 int alloc_particles(clq_ctx *c, int64_t cap) {
-    void *arrs[8];
+    void *arrs[9];
     const size_t fb = (size_t)cap * 12, ib = (size_t)cap * 4;
-    const size_t sizes[8] = {fb, fb, fb, fb, fb, ib, ib, ib};
-    for (int q = 0; q < 8; ++q) {
+    const int big_cap = big_bin_capacity(cap);
+    const size_t sizes[9] = {fb, fb, fb, fb, fb, ib, ib, ib, (size_t)big_cap * 4};
+    for (int q = 0; q < 9; ++q) {
         if (int rc = c->alloc(&arrs[q], sizes[q])) {
             for (int r = 0; r < q; ++r) c->release(arrs[r]);
             return rc;
         }
     }
     c->release(c->pos); c->release(c->vel); c->release(c->pos_work); c->release(c->vel_work); c->release(c->prev);
-    c->release(c->key); c->release(c->rank); c->release(c->perm);
+    c->release(c->key); c->release(c->rank); c->release(c->perm); c->release(c->big_list);
     c->pos = (float *)arrs[0]; c->vel = (float *)arrs[1]; c->pos_work = (float *)arrs[2];
     c->vel_work = (float *)arrs[3]; c->prev = (float *)arrs[4];
     c->key = (int32_t *)arrs[5]; c->rank = (int32_t *)arrs[6]; c->perm = (int32_t *)arrs[7];
+    c->big_list = (int32_t *)arrs[8];
+    c->big_cap = big_cap;
     c->cap = cap;
     return CLQ_OK;
 }
@@ -268,6 +274,8 @@ int clq_create(int32_t nx, int32_t ny, int32_t nz, double dx, int64_t capacity, 
     }
     if ((rc = c->alloc(&q, gb))) return bail(rc);
     c->counts = (int32_t *)q;
+    if ((rc = c->alloc(&q, 16))) return bail(rc);
+    c->big_count = (int32_t *)q;
     if ((rc = c->alloc(&q, gb))) return bail(rc);
     c->offsets = (int32_t *)q;
     if ((rc = c->alloc(&q, (size_t)scan_scratch_elems(c->C) * 4))) return bail(rc);
@@ -481,7 +489,10 @@ int clq_bin_particles(clq_ctx *c) {
     launch_bin_count(c->pos, n, inv_dx, c->nx, c->ny, c->nz, c->key, c->rank, c->counts, c->stream);
     launch_exclusive_scan(c->counts, c->offsets, c->C, c->scan_scratch, c->stream);
     launch_bin_permute(c->key, c->rank, c->offsets, n, c->perm, c->stream);
-    launch_bin_gather(c->perm, c->key, c->offsets, c->counts, c->pos, c->vel, n, c->pos_work, c->vel_work, c->stream);
+    HIP_TRY(hipMemsetAsync(c->big_count, 0, 4, c->stream));
+    // key is free after the permutation: the big bins' merge sort uses it
+    launch_bin_order(c->perm, c->offsets, c->counts, c->C, n, c->pos, c->vel, c->pos_work, c->vel_work, c->big_list,
+                     c->big_count, c->big_cap, c->key, c->stream);
     kernel_mark(c, CLQ_KERNEL_BIN);
     stage_end(c, CLQ_STAGE_BIN);
     std::swap(c->pos, c->pos_work);  // particles.py:136-140

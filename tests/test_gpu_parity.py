@@ -430,3 +430,29 @@ def test_fcg_matches_oracle(oracle_lib, hip_lib, weight):
     res = np.linalg.norm(b - A @ p64) / np.linalg.norm(b)
     assert res <= 1.01e-8
     assert sh["final_residual"] == pytest.approx(np.linalg.norm(b - A @ p64), rel=1e-6)
+
+
+def test_binning_crowded_bins_bit_exact(oracle_lib, hip_lib):
+    """Bins far above the in-LDS ranking limit (a 20,000-particle bin, a few
+    of 65-300, NaN positions clamped to bin 0) take the merge-sort path; the
+    order equals the oracle's stable order bit for bit."""
+    rng = np.random.default_rng(21)
+    shape, dx = (24, 20, 28), 1.0 / 24
+    parts = [(rng.random((40000, 3)) * np.array(shape) * dx).astype(np.float32),
+             (np.float32(dx) * (np.float32(9.6) + rng.uniform(-0.05, 0.05, (20000, 3)).astype(np.float32))),
+             np.full((50, 3), np.nan, np.float32)]
+    for b, m in ((3.7, 65), (6.2, 130), (12.3, 300)):
+        parts.append(np.float32(dx) * (np.float32(b) + rng.uniform(-0.1, 0.1, (m, 3)).astype(np.float32)))
+    pos = np.concatenate(parts)
+    pos = np.ascontiguousarray(pos[rng.permutation(len(pos))])
+    vel = rng.standard_normal(pos.shape).astype(np.float32)
+    scene = scene_from_arrays("crowded bins", shape, dx, pos, vel)
+    o, h = pair(oracle_lib, hip_lib, scene)
+    for _ in range(2):  # the second pass starts from the sorted order
+        o.bin_particles()
+        h.bin_particles()
+        po, vo = o.get_particles()
+        ph, vh = h.get_particles()
+        assert np.array_equal(po.view(np.uint32), ph.view(np.uint32))
+        assert np.array_equal(vo, vh)
+        assert_fields_equal(o, h, ("bin_counts", "bin_offsets"))

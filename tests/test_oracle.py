@@ -609,3 +609,27 @@ def test_non_finite_solve_is_an_error(oracle_lib, mode):
     with pytest.raises(runtime.ClqError) as err:
         ctx.pressure_solve(runtime.SOLVE_MODES[mode], max_cycles=5, pre_smooth=1, post_smooth=1, coarse_extra=4)
     assert err.value.status == runtime.ERR_NOT_CONVERGED and "not finite" in str(err.value)
+
+
+def test_binning_crowded_bin_matches_stable_sort(oracle_lib):
+    """One bin holding 50,000 particles (plus NaN positions, which clamp to
+    bin 0) is sorted stably like any other: the order is numpy's stable
+    argsort of the closed-form keys, compared bit for bit."""
+    rng = np.random.default_rng(12)
+    shape, dx = (12, 10, 14), 1.0 / 12
+    spread = (rng.random((3000, 3)) * np.array(shape) * dx).astype(F32)
+    crowd = (np.full((50000, 3), 5.7, np.float32) * F32(dx) + rng.uniform(-0.1, 0.1, (50000, 3)).astype(F32) * F32(dx))
+    nan = np.full((40, 3), np.nan, np.float32)
+    pos = np.concatenate([spread[:1000], crowd, nan, spread[1000:]])
+    order = rng.permutation(len(pos))
+    pos = np.ascontiguousarray(pos[order])
+    vel = rng.standard_normal(pos.shape).astype(F32)
+    ctx = make_context(oracle_lib, scene_from_arrays("crowded", shape, dx, pos, vel))
+    ctx.bin_particles()
+    keys = bin_keys(np.nan_to_num(pos, nan=-1.0), dx, shape)
+    want = np.argsort(keys, kind="stable")
+    got_p, got_v = ctx.get_particles()
+    assert np.array_equal(got_p.view(np.uint32), pos[want].view(np.uint32))
+    assert np.array_equal(got_v, vel[want])
+    counts = ctx.get_field("bin_counts").ravel()
+    assert counts.max() >= 50000 and np.array_equal(counts, np.bincount(keys, minlength=counts.size))
