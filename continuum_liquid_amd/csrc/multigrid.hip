@@ -14,6 +14,8 @@
 //   coarse_smooth   coarse_smooth (multigrid.cl:232-264) in one work-group
 //                   with the whole coarsest level staged in LDS
 //   residual_norm   FP64 ||b - A p||^2 and ||b||^2, fixed reduction tree
+#include <mutex>
+
 #include "common.hpp"
 
 namespace clq {
@@ -1411,6 +1413,9 @@ __global__ __launch_bounds__(kNormThreads) void residual_norm_kernel(
     }
 }
 
+// out[0], out[1] = the sums of the partials of r.r and b.b (SQRT: their
+// square roots, the norms)
+template <bool SQRT = false>
 __global__ __launch_bounds__(kNormThreads) void reduce_partials_kernel(const double *__restrict__ partials, int count,
                                                                        double *__restrict__ out) {
     __shared__ double s_r[kNormThreads], s_b[kNormThreads];
@@ -1430,8 +1435,8 @@ __global__ __launch_bounds__(kNormThreads) void reduce_partials_kernel(const dou
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[0] = s_r[0];
-        out[1] = s_b[0];
+        out[0] = SQRT ? sqrt(s_r[0]) : s_r[0];
+        out[1] = SQRT ? sqrt(s_b[0]) : s_b[0];
     }
 }
 
@@ -1442,7 +1447,7 @@ void launch_ir_update_residual(const Level &L, int from32, const float *p32, con
     if (blocks > kNormBlocksIR) blocks = kNormBlocksIR;
     hipLaunchKernelGGL(ir_update_residual_kernel, dim3(blocks), dim3(kNormThreadsIR), 0, s, from32, p32, pin, pout, r32,
                        b, L.mask, L.diag, L.wx, L.wy, L.wz, inv_h2, L.nx, L.ny, L.nz, partials);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kNormThreads), 0, s, partials, blocks, out2);
+    hipLaunchKernelGGL(reduce_partials_kernel<false>, dim3(1), dim3(kNormThreads), 0, s, partials, blocks, out2);
 }
 
 void launch_residual_norm(const Level &L, double inv_h2, double *partials, double *out2, hipStream_t s) {
@@ -1450,7 +1455,69 @@ void launch_residual_norm(const Level &L, double inv_h2, double *partials, doubl
     if (blocks > kNormBlocks) blocks = kNormBlocks;
     hipLaunchKernelGGL(residual_norm_kernel, dim3(blocks), dim3(kNormThreads), 0, s, L.p, L.b, L.mask, L.diag, L.wx,
                        L.wy, L.wz, inv_h2, L.nx, L.ny, L.nz, partials);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(kNormThreads), 0, s, partials, blocks, out2);
+    hipLaunchKernelGGL(reduce_partials_kernel<false>, dim3(1), dim3(kNormThreads), 0, s, partials, blocks, out2);
 }
 
 }  // namespace clq
+
+// ---- FP64 residual norm without allocation ----------------------------------------
+// clq_residual_norm_noalloc keeps its per-block partial sums (2 x kNormBlocks
+// doubles, 32 KiB, enough for any grid: the norm kernel is grid-strided over
+// at most kNormBlocks blocks) in this file-scope device array.  Each GPU holds
+// its own copy until the process ends; its device address is looked up once
+// per device (the array's name in host code is only a stand-in).  Every block
+// overwrites its two partials before the reduction reads them, so nothing has
+// to be cleared between calls; calls that overlap on one GPU race on it.
+static __device__ __align__(256) double scratch[2 * clq::kNormBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+double *g_scratch[kMaxDevices] = {};
+
+// scratch's device address on the current device (set up on first use)
+int scratch_on_device(double **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return clq::set_error(CLQ_ERR_DEVICE, "residual_norm_noalloc: no usable current device (%d)", dev);
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return clq::set_error(CLQ_ERR_DEVICE, "residual_norm_noalloc: hipGetSymbolAddress: %s",
+                                  hipGetErrorString(e));
+        g_scratch[dev] = (double *)ptr;
+    }
+    *out = g_scratch[dev];
+    return CLQ_OK;
+}
+}  // namespace
+
+extern "C" int clq_residual_norm_noalloc(const float *p, const float *rhs, const float *mask, const float *diag,
+                                         const float *wx, const float *wy, const float *wz, int32_t nx, int32_t ny,
+                                         int32_t nz, double dx, double *norms, void *stream) {
+    using namespace clq;
+    if (!p || !rhs || !mask || !diag || !wx || !wy || !wz || !norms)
+        return set_error(CLQ_ERR_INVALID, "residual_norm_noalloc: NULL array");
+    if (nx < 3 || ny < 3 || nz < 3)
+        return set_error(CLQ_ERR_INVALID, "residual_norm_noalloc: grid %dx%dx%d too small (need >= 3)", nx, ny, nz);
+    if ((int64_t)nx * ny * nz >= (1LL << 31))
+        return set_error(CLQ_ERR_INVALID,
+                         "residual_norm_noalloc: grid of %lld cells above the maximum of 2^31 - 1 cells",
+                         (long long)nx * ny * nz);
+    if (!(dx > 0.0)) return set_error(CLQ_ERR_INVALID, "residual_norm_noalloc: dx must be positive");
+    double *partials = nullptr;
+    if (int rc = scratch_on_device(&partials)) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    const double h = (double)(float)dx;  // the operator's cell size, as in clq_residual_norm
+    const int64_t C = (int64_t)nx * ny * nz;
+    int blocks = (int)((C + kNormThreads - 1) / kNormThreads);
+    if (blocks > kNormBlocks) blocks = kNormBlocks;
+    hipLaunchKernelGGL(residual_norm_kernel, dim3(blocks), dim3(kNormThreads), 0, st, p, rhs, mask, diag, wx, wy, wz,
+                       1.0 / (h * h), nx, ny, nz, partials);
+    hipLaunchKernelGGL(reduce_partials_kernel<true>, dim3(1), dim3(kNormThreads), 0, st, partials, blocks, norms);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(CLQ_ERR_DEVICE, "residual_norm_noalloc: %s", hipGetErrorString(e));
+    return CLQ_OK;
+}

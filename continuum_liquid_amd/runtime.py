@@ -163,8 +163,14 @@ _NET_SIGNATURES = {
     "clq_net_train_step": ([_P, c_int32, _P, _P, _P, _P, c_double, POINTER(c_double)], ctypes.c_int),
     "clq_attach_net": ([_P, _P, c_double], ctypes.c_int),
 }
-EXPORTED_SYMBOLS = ("clq_last_error", *_SIGNATURES, *_NET_SIGNATURES)
+# Entry points on device pointers (GPU library only).
+_DEVICE_SIGNATURES = {
+    "clq_residual_norm_noalloc": ([_P, _P, _P, _P, _P, _P, _P, c_int32, c_int32, c_int32, c_double, _P, _P],
+                                  ctypes.c_int),
+}
+EXPORTED_SYMBOLS = ("clq_last_error", *_SIGNATURES, *_NET_SIGNATURES, *_DEVICE_SIGNATURES)
 NET_SYMBOLS = tuple(_NET_SIGNATURES)
+GPU_ONLY_SYMBOLS = (*_NET_SIGNATURES, *_DEVICE_SIGNATURES)
 
 
 def bind_library(lib: ctypes.CDLL) -> ctypes.CDLL:
@@ -175,7 +181,7 @@ def bind_library(lib: ctypes.CDLL) -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
-    for name, (args, res) in _NET_SIGNATURES.items():
+    for name, (args, res) in {**_NET_SIGNATURES, **_DEVICE_SIGNATURES}.items():
         if hasattr(lib, name):
             fn = getattr(lib, name)
             fn.argtypes = args

@@ -253,6 +253,21 @@ int clq_coarse_smooth(clq_ctx *ctx, int32_t iterations);
 int clq_neumann_bc(clq_ctx *ctx);
 /* FP64-accumulated ||b - A p||_2 and ||b||_2 on level 0. */
 int clq_residual_norm(clq_ctx *ctx, double *residual, double *rhs_norm);
+/* clq_residual_norm without a context and without allocating.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops).  The arrays are
+ * device pointers of the level-0 system of an (nx, ny, nz) grid of cell size
+ * dx, laid out as the fields of the same names (p = pressure, rhs, mask =
+ * fluid fraction, the finest operator); norms is a device double[2] that
+ * receives ||b - A p||_2 and ||b||_2 (the same values as clq_residual_norm).
+ * Enqueued on `stream` (a hipStream_t, NULL for the default stream) without
+ * waiting.  The per-block partial sums live in a 32 KiB array of the
+ * library's device memory, one per GPU; calls that overlap on one GPU race
+ * on it.  Grids must have at least 3 and fewer than 2^31 cells (otherwise
+ * CLQ_ERR_INVALID). */
+int clq_residual_norm_noalloc(const float *p, const float *rhs, const float *mask, const float *diag,
+                              const float *wx, const float *wy, const float *wz, int32_t nx, int32_t ny, int32_t nz,
+                              double dx, double *norms, void *stream);
 /* pressure_correction (pressure_solve.cl:26-47).  After a solve in modes 2-4
  * the FP64 pressure is used (ghost neighbours read the clamped interior value,
  * which is what neumann_domain_bc stores): the gradient and the update run in

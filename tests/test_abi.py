@@ -29,8 +29,8 @@ def test_library_exports_every_symbol(path):
     assert path.exists(), f"{path} not built"
     lib = ctypes.CDLL(str(path))
     expected = declared_symbols()
-    if path == ORACLE_LIBRARY:  # the oracle restates the step; the network's CPU check is a torch model
-        expected -= set(runtime.NET_SYMBOLS)
+    if path == ORACLE_LIBRARY:  # the oracle restates the step on host arrays; the network's CPU check is a
+        expected -= set(runtime.GPU_ONLY_SYMBOLS)  # torch model
     missing = [name for name in expected if not hasattr(lib, name)]
     assert not missing, missing
     bound = runtime.bind_library(lib)
