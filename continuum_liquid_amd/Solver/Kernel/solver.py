@@ -167,6 +167,9 @@ def solver(config: dict, timings=None):
     try:
         ctx.set_field("obstacle", obstacle_mask.astype(np.uint8))
         ctx.set_particles(positions, velocities)
+        dynamic_sources = any(s.get("type") in {"INFLOW", "OUTFLOW"} for s in sources)
+        if dynamic_sources:
+            ctx.set_sources(sources)
 
         output_dir.mkdir(parents=True, exist_ok=True)
         cache = output.create_cache(output_dir, fps, start_frame, end_frame)
@@ -174,7 +177,6 @@ def solver(config: dict, timings=None):
         try:
             writer.submit(positions, start_frame)
             emit_message({"type": "stats", "frame": 0, "total_frames": total_frames, "particle_count": count})
-            dynamic_sources = any(s.get("type") in {"INFLOW", "OUTFLOW"} for s in sources)
             for solver_step in range(total_solver_steps):
                 if cancel_event is not None and cancel_event.is_set():
                     print("Bake cancellation requested. Stopping the simulation cleanly...")
@@ -182,14 +184,10 @@ def solver(config: dict, timings=None):
                 output_frame = (solver_step + 1) // steps_per_frame
                 last_substep = solver_step % steps_per_frame == steps_per_frame - 1
                 if dynamic_sources:
-                    # host round trip of the reference (solver.py:253-306)
-                    if count:
-                        positions, velocities = ctx.get_particles()
-                    positions, velocities = source_particles.apply_sources(
-                        positions, velocities, sources, obstacle_mask, delta, rng
-                    )
-                    count = len(positions)
-                    ctx.set_particles(positions, velocities)
+                    # the sources of solver.py:253-306 on the device: the same
+                    # particles as the host seeding with the same generator,
+                    # without moving every particle to the host and back
+                    count = ctx.apply_sources(rng)
                 if count == 0:
                     if last_substep:
                         writer.submit(np.empty((0, 3), FIELD_DTYPE), start_frame + output_frame)

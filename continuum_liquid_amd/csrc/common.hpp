@@ -259,6 +259,20 @@ void launch_check_iteration(const Level &L, double inv_h2, const int32_t *rows, 
 void launch_check_error(const Level &L, const double *p64, const float *p32, const double *x, const int32_t *rows,
                         int64_t nrows, double *partials, double *c, hipStream_t st);
 
+// sources.hip
+void launch_outflow_flags(const float *pos, int n, const uint8_t *sink, float delta, int nx, int ny, int nz,
+                          int32_t *keep, hipStream_t s);
+void launch_compact(const float *pos, const float *vel, const int32_t *keep, const int32_t *slot, int n,
+                    float *pos_out, float *vel_out, hipStream_t s);
+void launch_count_open(const float *pos, int n, const uint8_t *open, float delta, int nx, int ny, int nz,
+                       int32_t *present, hipStream_t s);
+void launch_missing(int32_t *present, const uint8_t *open, int density, int64_t C, hipStream_t s);
+// rng: numpy PCG64 state {state hi, lo, inc hi, lo}; the emission starts
+// draws_before draws after it
+void launch_emit(const int32_t *missing, const int32_t *first, int64_t C, int ny, int nz, double delta,
+                 const uint64_t *rng, uint64_t draws_before, int64_t n0, const float *velocity, float *pos, float *vel,
+                 hipStream_t s);
+
 // context.hip: sets the message returned by clq_last_error(), returns code
 int set_error(int code, const char *fmt, ...);
 

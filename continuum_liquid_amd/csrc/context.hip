@@ -76,6 +76,18 @@ struct clq_ctx {
     // bins too crowded for the in-LDS ranking (allocated with the particles)
     int32_t *big_list = nullptr, *big_count = nullptr;
     int big_cap = 0;
+    int32_t *pscan = nullptr;  // scan scratch for particle-length scans
+    // particle sources on the device (clq_set_sources): per source its type,
+    // density, velocity and open cells (mask and not solid), and the union of
+    // the OUTFLOW masks
+    struct Source {
+        int32_t type, density;
+        float velocity[3];
+        uint8_t *open;
+    };
+    std::vector<Source> sources;
+    uint8_t *sink = nullptr;
+    int32_t *src_scalar = nullptr, *src_scalar_host = nullptr;
     int32_t *counts = nullptr, *offsets = nullptr, *scan_scratch = nullptr;
     float *u = nullptr, *v = nullptr, *w = nullptr, *uo = nullptr, *vo = nullptr, *wo = nullptr;
     float *ff = nullptr, *p = nullptr, *rhs = nullptr, *diag = nullptr, *wx = nullptr, *wy = nullptr, *wz = nullptr;
@@ -152,23 +164,35 @@ struct clq_ctx {
 
 namespace {
 
-int alloc_particles(clq_ctx *c, int64_t cap) {
-    void *arrs[9];
+// (Re)allocates the particle arrays for `cap` particles; the first `keep`
+// positions and velocities are carried over.
+int alloc_particles(clq_ctx *c, int64_t cap, int64_t keep = 0) {
+    void *arrs[10];
     const size_t fb = (size_t)cap * 12, ib = (size_t)cap * 4;
     const int big_cap = big_bin_capacity(cap);
-    const size_t sizes[9] = {fb, fb, fb, fb, fb, ib, ib, ib, (size_t)big_cap * 4};
-    for (int q = 0; q < 9; ++q) {
+    const size_t sizes[10] = {fb, fb, fb, fb, fb, ib, ib, ib, (size_t)big_cap * 4,
+                              (size_t)scan_scratch_elems(cap) * 4};
+    for (int q = 0; q < 10; ++q) {
         if (int rc = c->alloc(&arrs[q], sizes[q])) {
             for (int r = 0; r < q; ++r) c->release(arrs[r]);
             return rc;
         }
     }
+    if (keep > 0) {
+        if (hipMemcpyAsync(arrs[0], c->pos, (size_t)keep * 12, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(arrs[1], c->vel, (size_t)keep * 12, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            for (void *q : arrs) c->release(q);
+            return fail(CLQ_ERR_DEVICE, "particle copy on capacity growth failed");
+        }
+    }
     c->release(c->pos); c->release(c->vel); c->release(c->pos_work); c->release(c->vel_work); c->release(c->prev);
-    c->release(c->key); c->release(c->rank); c->release(c->perm); c->release(c->big_list);
+    c->release(c->key); c->release(c->rank); c->release(c->perm); c->release(c->big_list); c->release(c->pscan);
     c->pos = (float *)arrs[0]; c->vel = (float *)arrs[1]; c->pos_work = (float *)arrs[2];
     c->vel_work = (float *)arrs[3]; c->prev = (float *)arrs[4];
     c->key = (int32_t *)arrs[5]; c->rank = (int32_t *)arrs[6]; c->perm = (int32_t *)arrs[7];
     c->big_list = (int32_t *)arrs[8];
+    c->pscan = (int32_t *)arrs[9];
     c->big_cap = big_cap;
     c->cap = cap;
     return CLQ_OK;
@@ -226,6 +250,7 @@ int clq_destroy(clq_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *q : c->allocations) (void)hipFree(q);
     if (c->norm_host) (void)hipHostFree(c->norm_host);
+    if (c->src_scalar_host) (void)hipHostFree(c->src_scalar_host);
     if (c->ev_scal) (void)hipEventDestroy(c->ev_scal);
     for (int s = 0; s < CLQ_STAGE_COUNT; ++s) {
         if (c->ev_start[s]) (void)hipEventDestroy(c->ev_start[s]);
@@ -1463,6 +1488,96 @@ int clq_grid_to_particle_advect(clq_ctx *c, double flip_ratio, double dt) {
     kernel_mark(c, CLQ_KERNEL_PARTICLE_UPDATE);
     stage_end(c, CLQ_STAGE_G2P_ADVECT);
     return rc;
+}
+
+// ---- particle sources --------------------------------------------------------------------
+int clq_set_sources(clq_ctx *c, int32_t count, const int32_t *types, const uint8_t *masks, const int32_t *densities,
+                    const float *velocities) {
+    if (!c || count < 0 || (count > 0 && (!types || !masks || !densities || !velocities)))
+        return fail(CLQ_ERR_INVALID, "bad source arguments");
+    for (int32_t q = 0; q < count; ++q)
+        if (types[q] < CLQ_SOURCE_GEOMETRY || types[q] > CLQ_SOURCE_OUTFLOW)
+            return fail(CLQ_ERR_INVALID, "unknown source type %d", types[q]);
+    DeviceGuard g(c->device);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &src : c->sources) c->release(src.open);
+    c->sources.clear();
+    void *q;
+    if (!c->sink) {
+        if (int rc = c->alloc(&q, (size_t)c->C)) return rc;
+        c->sink = (uint8_t *)q;
+        if (int rc = c->alloc(&q, 16)) return rc;
+        c->src_scalar = (int32_t *)q;
+        HIP_TRY(hipHostMalloc((void **)&c->src_scalar_host, 16, hipHostMallocDefault));
+    }
+    // open cells per source and the union of the sinks, on the host
+    std::vector<uint8_t> solid((size_t)c->C), cells((size_t)c->C), sink((size_t)c->C, 0);
+    HIP_TRY(hipMemcpy(solid.data(), c->obstacle, (size_t)c->C, hipMemcpyDeviceToHost));
+    for (int32_t s = 0; s < count; ++s) {
+        const uint8_t *m = masks + (size_t)s * c->C;
+        clq_ctx::Source src;
+        src.type = types[s];
+        src.density = std::max(1, densities[s]);
+        for (int a = 0; a < 3; ++a) src.velocity[a] = velocities[3 * s + a];
+        for (int64_t i = 0; i < c->C; ++i) {
+            cells[i] = m[i] && !solid[i];
+            if (src.type == CLQ_SOURCE_OUTFLOW && m[i]) sink[i] = 1;
+        }
+        if (int rc = c->alloc(&q, (size_t)c->C)) return rc;
+        src.open = (uint8_t *)q;
+        HIP_TRY(hipMemcpy(src.open, cells.data(), (size_t)c->C, hipMemcpyHostToDevice));
+        c->sources.push_back(src);
+    }
+    HIP_TRY(hipMemcpy(c->sink, sink.data(), (size_t)c->C, hipMemcpyHostToDevice));
+    return CLQ_OK;
+}
+
+int clq_apply_sources(clq_ctx *c, int32_t initial, const uint64_t *rng, int64_t *draws) {
+    if (!c || !rng || !draws) return fail(CLQ_ERR_INVALID, "NULL argument");
+    DeviceGuard g(c->device);
+    const float delta = (float)c->dx;
+    *draws = 0;
+    bool any_sink = false;
+    for (auto &src : c->sources) any_sink = any_sink || src.type == CLQ_SOURCE_OUTFLOW;
+    // sinks: stable compaction of the particles outside every OUTFLOW cell
+    if (any_sink && c->n > 0) {
+        const int n = (int)c->n;
+        launch_outflow_flags(c->pos, n, c->sink, delta, c->nx, c->ny, c->nz, c->key, c->stream);
+        launch_exclusive_scan(c->key, c->rank, n, c->pscan, c->stream);
+        launch_compact(c->pos, c->vel, c->key, c->rank, n, c->pos_work, c->vel_work, c->stream);
+        HIP_TRY(hipMemcpyAsync(c->src_scalar_host, c->rank + (n - 1), 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->src_scalar_host + 1, c->key + (n - 1), 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        std::swap(c->pos, c->pos_work);
+        std::swap(c->vel, c->vel_work);
+        c->n = (int64_t)c->src_scalar_host[0] + c->src_scalar_host[1];
+    }
+    // top-ups, each counting the particles present at that moment
+    uint64_t used = 0;
+    for (auto &src : c->sources) {
+        if (!(src.type == CLQ_SOURCE_INFLOW || (initial && src.type == CLQ_SOURCE_GEOMETRY))) continue;
+        HIP_TRY(hipMemsetAsync(c->counts, 0, (size_t)c->C * 4, c->stream));
+        launch_count_open(c->pos, (int)c->n, src.open, delta, c->nx, c->ny, c->nz, c->counts, c->stream);
+        launch_missing(c->counts, src.open, src.density, c->C, c->stream);
+        launch_exclusive_scan(c->counts, c->offsets, c->C, c->scan_scratch, c->stream);
+        HIP_TRY(hipMemcpyAsync(c->src_scalar_host, c->offsets + (c->C - 1), 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->src_scalar_host + 1, c->counts + (c->C - 1), 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const int64_t total = (int64_t)c->src_scalar_host[0] + c->src_scalar_host[1];
+        if (total == 0) continue;  // no draw, as on the host
+        if (c->n + total >= (1LL << 31)) return fail(CLQ_ERR_INVALID, "particle count would reach 2^31");
+        if (c->n + total > c->cap) {  // solver.py:290-291
+            const int64_t cap = std::max<int64_t>(c->n + total, c->cap * 3 / 2);
+            if (int rc = alloc_particles(c, cap, c->n)) return rc;
+        }
+        launch_emit(c->counts, c->offsets, c->C, c->ny, c->nz, c->dx, rng, used, c->n, src.velocity, c->pos,
+                    c->vel, c->stream);
+        used += 3ull * (uint64_t)total;
+        c->n += total;
+    }
+    *draws = (int64_t)used;
+    c->bins_current = false;
+    return check_launch("apply_sources");
 }
 
 int clq_voxelise(clq_ctx *c, int32_t scale, double particles_per_cell, uint8_t *occupancy, float *density) {

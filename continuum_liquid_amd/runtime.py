@@ -46,6 +46,8 @@ SOLVE_MODES = {
     "fcg": SOLVE_FCG,
 }
 STAGES = ("bin", "p2g", "pressure_solve", "g2p_advect")
+# enum clq_source_type
+SOURCE_TYPES = {"GEOMETRY": 0, "INFLOW": 1, "OUTFLOW": 2}
 KERNEL_CLASSES = ("binning", "p2g", "rbgs_fine_sweep", "particle_update")
 
 
@@ -145,6 +147,8 @@ _SIGNATURES = {
     "clq_particle_obstacle_bc": ([_P], ctypes.c_int),
     "clq_grid_to_particle_advect": ([_P, c_double, c_double], ctypes.c_int),
     "clq_voxelise": ([_P, c_int32, c_double, _P, _P], ctypes.c_int),
+    "clq_set_sources": ([_P, c_int32, _P, _P, _P, _P], ctypes.c_int),
+    "clq_apply_sources": ([_P, c_int32, _P, POINTER(c_int64)], ctypes.c_int),
     "clq_stage_times": ([_P, POINTER(c_double), c_int32], ctypes.c_int),
     "clq_kernel_timing": ([_P, c_int32], ctypes.c_int),
     "clq_kernel_times": ([_P, POINTER(c_double), POINTER(c_int64), c_int32], ctypes.c_int),
@@ -449,6 +453,38 @@ class Context:
             "clq_voxelise",
         )
         return occ, dens
+
+    def set_sources(self, sources) -> None:
+        """Device copies of the sources' cells (clq_set_sources): dicts with
+        `type` (GEOMETRY, INFLOW, OUTFLOW), a bool `mask` of the grid shape,
+        `particle_density` and `velocity`.  Set the obstacle field first."""
+        kinds = [s for s in sources if s.get("type") in SOURCE_TYPES]
+        types = np.array([SOURCE_TYPES[s["type"]] for s in kinds], np.int32)
+        masks = np.zeros((max(1, len(kinds)),) + self.shape, np.uint8)
+        for q, s in enumerate(kinds):
+            masks[q] = np.asarray(s["mask"], bool)
+        dens = np.array([int(s.get("particle_density", 8)) for s in kinds], np.int32)
+        vel = np.array([s.get("velocity", (0.0, 0.0, 0.0)) for s in kinds], np.float32).reshape(-1, 3)
+        self._check(self.lib.clq_set_sources(self.handle, len(kinds), _ptr(types), _ptr(masks), _ptr(dens),
+                                             _ptr(np.ascontiguousarray(vel))), "clq_set_sources")
+
+    def apply_sources(self, rng: np.random.Generator, initial: bool = False) -> int:
+        """The sources' removal and top-up on the device (clq_apply_sources),
+        drawing from `rng` (a PCG64 Generator), which is advanced by the draws
+        used: the same particles, bit for bit, as source_particles.apply_sources
+        with the same generator.  Returns the particle count."""
+        state = rng.bit_generator.state
+        if state.get("bit_generator") != "PCG64" or state.get("has_uint32"):
+            raise ValueError("apply_sources needs a PCG64 generator without a buffered 32-bit draw")
+        s, inc = int(state["state"]["state"]), int(state["state"]["inc"])
+        m64 = (1 << 64) - 1
+        words = np.array([s >> 64, s & m64, inc >> 64, inc & m64], np.uint64)
+        draws = c_int64()
+        self._check(self.lib.clq_apply_sources(self.handle, int(bool(initial)), _ptr(words), byref(draws)),
+                    "clq_apply_sources")
+        if draws.value:
+            rng.bit_generator.advance(draws.value)
+        return self.particle_count
 
     def stage_times(self) -> dict:
         ms = (c_double * len(STAGES))()

@@ -291,6 +291,28 @@ int clq_particle_obstacle_bc(clq_ctx *ctx);
 /* The five calls above in order, as one pass over the particles. */
 int clq_grid_to_particle_advect(clq_ctx *ctx, double flip_ratio, double dt);
 
+/* ---- particle sources (Solver/Kernel/solver.py:253-306) ------------------------
+ * The per-step OUTFLOW removal and INFLOW top-up of
+ * Solver/Kernel/source_particles.py:42-92 on the device, replacing the host
+ * round trip of every particle.  Semantics: a particle's cell is
+ * floor(x / dx) (float32) clamped into the grid; OUTFLOW sources first remove
+ * the particles in their cells (the rest keep their order); then each INFLOW
+ * source (and each GEOMETRY source when `initial`), in order, tops every
+ * open cell (mask and not solid) up to its density counting the particles
+ * present, appending the new ones in C order of their cells at
+ * float32(((double)cell + u) * dx) with the source's velocity.  The uniforms
+ * u continue numpy's Generator.random() stream of the PCG64 state `rng`
+ * ({state >> 64, state & (2^64-1), inc >> 64, inc & (2^64-1)}), three per
+ * new particle as rng.random((count, 3)) draws them; *draws receives the
+ * number of 64-bit draws consumed, by which the caller advances its own
+ * generator.  The result equals the host seeding bit for bit.  Masks are
+ * uint8[count][C] in the grid layout; the obstacle field must be set before
+ * clq_set_sources. */
+enum clq_source_type { CLQ_SOURCE_GEOMETRY = 0, CLQ_SOURCE_INFLOW = 1, CLQ_SOURCE_OUTFLOW = 2 };
+int clq_set_sources(clq_ctx *ctx, int32_t count, const int32_t *types, const uint8_t *masks, const int32_t *densities,
+                    const float *velocities);
+int clq_apply_sources(clq_ctx *ctx, int32_t initial, const uint64_t *rng, int64_t *draws);
+
 /* ---- particle voxelisation ------------------------------------------------ */
 /* Occupancy at `scale` x grid resolution (uint8, 1 where a particle's cell is
  * occupied) and, when density != NULL, smoothed density (float32, trilinear

@@ -101,6 +101,12 @@ struct clq_ctx {
     double *r64, *dir0, *dir1, *q64; /* flexible CG */
     int p64_valid;                    /* p64a holds the pressure of the last solve (modes 2-4) */
     int bins_current; /* counts/offsets describe the current particle order */
+    /* particle sources (clq_set_sources) */
+    int nsources;
+    int32_t *src_type, *src_density;
+    float *src_velocity;
+    uint8_t *src_open; /* [nsources][C]: mask and not solid */
+    uint8_t *sink;     /* union of the OUTFLOW masks */
 };
 
 static __thread char g_error[512];
@@ -194,6 +200,7 @@ int clq_destroy(clq_ctx *c) {
     }
     free(c->p64a); free(c->p64b); free(c->r32);
     free(c->r64); free(c->dir0); free(c->dir1); free(c->q64);
+    free(c->src_type); free(c->src_density); free(c->src_velocity); free(c->src_open); free(c->sink);
     free(c);
     return CLQ_OK;
 }
@@ -1584,6 +1591,124 @@ int clq_grid_to_particle_advect(clq_ctx *c, double flip_ratio, double dt) {
  * refined cell centres, scaled by s^3 / particles_per_cell so a uniformly
  * seeded region reads 1.  Summation runs particle by particle in the current
  * (binned) order, in FP64, and is rounded once. */
+/* ---- particle sources (source_particles.py:42-92, applied at solver.py:253-306) ---- */
+int clq_set_sources(clq_ctx *c, int32_t count, const int32_t *types, const uint8_t *masks, const int32_t *densities,
+                    const float *velocities) {
+    if (!c || count < 0 || (count > 0 && (!types || !masks || !densities || !velocities)))
+        return fail(CLQ_ERR_INVALID, "bad source arguments");
+    for (int32_t q = 0; q < count; ++q)
+        if (types[q] < CLQ_SOURCE_GEOMETRY || types[q] > CLQ_SOURCE_OUTFLOW)
+            return fail(CLQ_ERR_INVALID, "unknown source type %d", types[q]);
+    free(c->src_type); free(c->src_density); free(c->src_velocity); free(c->src_open); free(c->sink);
+    c->nsources = count;
+    c->src_type = xcalloc(count, 4);
+    c->src_density = xcalloc(count, 4);
+    c->src_velocity = xcalloc(3 * (size_t)count, 4);
+    c->src_open = xcalloc((size_t)count * c->C, 1);
+    c->sink = xcalloc(c->C, 1);
+    if (!c->src_type || !c->src_density || !c->src_velocity || !c->src_open || !c->sink)
+        return fail(CLQ_ERR_NOMEM, "source allocation failed");
+    for (int32_t q = 0; q < count; ++q) {
+        c->src_type[q] = types[q];
+        c->src_density[q] = densities[q] > 1 ? densities[q] : 1;
+        for (int a = 0; a < 3; ++a) c->src_velocity[3 * q + a] = velocities[3 * q + a];
+        const uint8_t *m = masks + (size_t)q * c->C;
+        for (long long i = 0; i < c->C; ++i) {
+            c->src_open[(size_t)q * c->C + i] = m[i] && !c->obstacle[i];
+            if (types[q] == CLQ_SOURCE_OUTFLOW && m[i]) c->sink[i] = 1;
+        }
+    }
+    return CLQ_OK;
+}
+
+typedef unsigned __int128 u128;
+static const u128 PCG_MULT = ((u128)0x2360ED051FC65DA4ull << 64) | (u128)0x4385DF649FCCF645ull;
+
+/* numpy's Generator.random() on PCG64 (XSL-RR 128/64): the state steps, then
+ * its output, >> 11, times 2^-53 */
+static double pcg_next_double(u128 *state, u128 inc) {
+    *state = *state * PCG_MULT + inc;
+    const uint64_t x = (uint64_t)(*state >> 64) ^ (uint64_t)*state;
+    const unsigned rot = (unsigned)(*state >> 122);
+    const uint64_t r = (x >> rot) | (x << ((64u - rot) & 63u));
+    return (double)(r >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static long long source_cell(const clq_ctx *c, const float *x) {
+    const float delta = (float)c->dx;
+    return gidx(floor_clamp(x[0] / delta, 0, c->nx - 1), floor_clamp(x[1] / delta, 0, c->ny - 1),
+                floor_clamp(x[2] / delta, 0, c->nz - 1), c->ny, c->nz);
+}
+
+int clq_apply_sources(clq_ctx *c, int32_t initial, const uint64_t *rng, int64_t *draws) {
+    if (!c || !rng || !draws) return fail(CLQ_ERR_INVALID, "NULL argument");
+    u128 state = ((u128)rng[0] << 64) | rng[1];
+    const u128 inc = ((u128)rng[2] << 64) | rng[3];
+    long long used = 0;
+    /* sinks: the particles in any OUTFLOW cell go, the rest keep their order */
+    int any_sink = 0;
+    for (int q = 0; q < c->nsources; ++q) any_sink |= c->src_type[q] == CLQ_SOURCE_OUTFLOW;
+    if (any_sink) {
+        long long m = 0;
+        for (long long i = 0; i < c->n; ++i) {
+            if (c->sink[source_cell(c, c->pos + 3 * i)]) continue;
+            memmove(c->pos + 3 * m, c->pos + 3 * i, 12);
+            memmove(c->vel + 3 * m, c->vel + 3 * i, 12);
+            ++m;
+        }
+        c->n = m;
+    }
+    int32_t *present = xcalloc(c->C, 4);
+    if (!present) return fail(CLQ_ERR_NOMEM, "source counts");
+    for (int q = 0; q < c->nsources; ++q) {
+        const int t = c->src_type[q];
+        if (!(t == CLQ_SOURCE_INFLOW || (initial && t == CLQ_SOURCE_GEOMETRY))) continue;
+        const uint8_t *open = c->src_open + (size_t)q * c->C;
+        memset(present, 0, (size_t)c->C * 4);
+        for (long long i = 0; i < c->n; ++i) {
+            const long long cell = source_cell(c, c->pos + 3 * i);
+            if (open[cell]) present[cell]++;
+        }
+        long long total = 0;
+        for (long long cell = 0; cell < c->C; ++cell)
+            if (open[cell] && present[cell] < c->src_density[q]) total += c->src_density[q] - present[cell];
+        if (total == 0) continue; /* no draw, as on the host */
+        if (c->n + total > c->cap) { /* solver.py:290-291, keeping the particles */
+            long long cap = c->cap * 3 / 2;
+            if (cap < c->n + total) cap = c->n + total;
+            float *np_ = xcalloc(cap * 3, 4), *nv = xcalloc(cap * 3, 4);
+            if (!np_ || !nv) { free(np_); free(nv); free(present); return fail(CLQ_ERR_NOMEM, "source growth"); }
+            memcpy(np_, c->pos, (size_t)c->n * 12);
+            memcpy(nv, c->vel, (size_t)c->n * 12);
+            free(c->pos_work); free(c->vel_work); free(c->prev);
+            c->pos_work = xcalloc(cap * 3, 4);
+            c->vel_work = xcalloc(cap * 3, 4);
+            c->prev = xcalloc(cap * 3, 4);
+            free(c->pos); free(c->vel);
+            c->pos = np_; c->vel = nv; c->cap = cap;
+            if (!c->pos_work || !c->vel_work || !c->prev) { free(present); return fail(CLQ_ERR_NOMEM, "source growth"); }
+        }
+        long long slot = c->n;
+        for (long long cell = 0; cell < c->C; ++cell) {
+            if (!open[cell] || present[cell] >= c->src_density[q]) continue;
+            int i, j, k;
+            coords(cell, c->ny, c->nz, &i, &j, &k);
+            const double corner[3] = {(double)i, (double)j, (double)k};
+            for (int m = present[cell]; m < c->src_density[q]; ++m, ++slot) {
+                for (int a = 0; a < 3; ++a)
+                    c->pos[3 * slot + a] = (float)((corner[a] + pcg_next_double(&state, inc)) * c->dx);
+                for (int a = 0; a < 3; ++a) c->vel[3 * slot + a] = c->src_velocity[3 * q + a];
+            }
+        }
+        used += 3 * total;
+        c->n += total;
+    }
+    free(present);
+    *draws = used;
+    c->bins_current = 0;
+    return CLQ_OK;
+}
+
 int clq_voxelise(clq_ctx *c, int32_t s, double ppc, uint8_t *occ, float *density) {
     if (!c || s < 1 || !(ppc > 0)) return fail(CLQ_ERR_INVALID, "bad voxelise arguments");
     if (s > 2) return fail(CLQ_ERR_UNSUPPORTED, "voxelise scale %d > 2", s);

@@ -456,3 +456,44 @@ def test_binning_crowded_bins_bit_exact(oracle_lib, hip_lib):
         assert np.array_equal(po.view(np.uint32), ph.view(np.uint32))
         assert np.array_equal(vo, vh)
         assert_fields_equal(o, h, ("bin_counts", "bin_offsets"))
+
+
+@pytest.mark.parametrize("initial", [False, True])
+def test_particle_sources_bit_exact(oracle_lib, hip_lib, initial):
+    """clq_apply_sources on the device removes and emits the same particles as
+    the oracle, bit for bit, over several calls (the particle arrays grow past
+    their first capacity); both leave the numpy generator in the same state."""
+    rng = np.random.default_rng(31)
+    shape, delta = (40, 36, 44), 1.0 / 40
+    solid = np.zeros(shape, bool)
+    solid[25:32, 0:9, 10:20] = True
+    inflow = np.zeros(shape, bool)
+    inflow[2:10, 24:31, 5:15] = True
+    inflow[28, 4, 12] = True  # solid: stays closed
+    sink = np.zeros(shape, bool)
+    sink[14:22, 0:6, :] = True
+    sources = [{"type": "INFLOW", "mask": inflow, "particle_density": 7, "velocity": (0.5, -2.0, 0.25)},
+               {"type": "OUTFLOW", "mask": sink},
+               {"type": "INFLOW", "mask": sink[::-1].copy(), "particle_density": 3, "velocity": (0.0, 1.0, 0.0)},
+               {"type": "GEOMETRY", "mask": np.roll(inflow, 13, axis=0), "particle_density": 8}]
+    pos = (rng.random((60000, 3)) * np.array(shape) * delta).astype(np.float32)
+    vel = rng.standard_normal(pos.shape).astype(np.float32)
+    scene = scene_from_arrays("sources 40x36x44", shape, delta, pos, vel)
+    o, h = pair(oracle_lib, hip_lib, scene)
+    rng_o, rng_h = np.random.default_rng(5), np.random.default_rng(5)
+    for ctx in (o, h):
+        ctx.set_field("obstacle", solid.astype(np.uint8))
+        ctx.set_sources(sources)
+    for step in range(4):
+        first = initial and step == 0
+        no, nh = o.apply_sources(rng_o, initial=first), h.apply_sources(rng_h, initial=first)
+        assert no == nh
+        po, vo = o.get_particles()
+        ph, vh = h.get_particles()
+        assert np.array_equal(po.view(np.uint32), ph.view(np.uint32))
+        assert np.array_equal(vo.view(np.uint32), vh.view(np.uint32))
+        assert rng_o.random() == rng_h.random()
+        # move the particles between calls, the same way on both
+        moved = po + np.float32(0.7 * delta) * np.sign(vo)
+        o.set_particles(moved, vo)
+        h.set_particles(moved, vo)

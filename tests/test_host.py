@@ -234,3 +234,82 @@ def test_fused_and_sectioned_steps_agree(oracle_lib):
             host_solver.step(ctx, params, timings)
             out.append(ctx.get_particles())
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def _source_scene(shape=(14, 12, 16), delta=1.0 / 14, seed=4):
+    r = np.random.default_rng(seed)
+    solid = np.zeros(shape, bool)
+    solid[9:12, 0:4, 3:9] = True
+    inflow = np.zeros(shape, bool)
+    inflow[1:4, 8:11, 2:6] = True
+    inflow[10, 1, 4] = True  # partly solid
+    sink = np.zeros(shape, bool)
+    sink[5:9, 0:3, :] = True
+    sources = [{"type": "INFLOW", "mask": inflow, "particle_density": 6, "velocity": (0.5, -2.0, 0.25)},
+               {"type": "OUTFLOW", "mask": sink},
+               {"type": "INFLOW", "mask": sink[::-1].copy(), "particle_density": 3, "velocity": (0.0, 1.0, 0.0)},
+               {"type": "GEOMETRY", "mask": np.roll(inflow, 5, axis=0), "particle_density": 8}]
+    pos = (r.random((3000, 3)) * np.array(shape) * delta).astype(np.float32)
+    vel = r.standard_normal((3000, 3)).astype(np.float32)
+    return shape, delta, solid, sources, pos, vel
+
+
+@pytest.mark.parametrize("initial", [False, True])
+def test_device_sources_match_host_seeding(oracle_lib, initial):
+    """clq_apply_sources (here the oracle's restatement) removes and emits the
+    same particles, bit for bit, as source_particles.apply_sources with the
+    same numpy generator, and leaves the generator in the same state."""
+    from continuum_liquid_amd.Solver.Kernel import source_particles as sp
+
+    shape, delta, solid, sources, pos, vel = _source_scene()
+    host_rng, dev_rng = np.random.default_rng(9), np.random.default_rng(9)
+    hp, hv = pos, vel
+    with runtime.Context(oracle_lib, shape, delta, capacity=100) as ctx:  # small: grows while emitting
+        ctx.set_field("obstacle", solid.astype(np.uint8))
+        ctx.set_sources(sources)
+        ctx.set_particles(pos, vel)
+        for step in range(3):
+            hp, hv = sp.apply_sources(hp, hv, sources, solid, delta, host_rng, initial=initial and step == 0)
+            n = ctx.apply_sources(dev_rng, initial=initial and step == 0)
+            dp, dv = ctx.get_particles()
+            assert n == len(hp)
+            assert np.array_equal(dp, hp) and np.array_equal(dv, hv)
+            assert host_rng.random() == dev_rng.random()
+            hp = hp + np.float32(0.3 * delta)  # move the particles between calls
+            ctx.set_particles(hp, hv)
+
+
+def test_bake_with_inflow_and_outflow_equals_host_seeding(tmp_path, oracle_backend, monkeypatch):
+    """A bake with dynamic sources through the device entry points writes the
+    same frames as the reference's host round trip (particles downloaded,
+    seeded by source_particles.apply_sources, uploaded), reproduced here by
+    swapping Context.apply_sources for it."""
+    from continuum_liquid_amd.Solver.Kernel import source_particles as sp
+
+    def bake(name):
+        cfg = make_config(tmp_path / name, frames=3)
+        cfg["simulation"]["sources"] += [
+            {"type": "INFLOW", "particle_density": 4, "velocity": [1.0, 0.0, -0.5],
+             "shapes": [{"box": [[0.1, 0.6, 0.55], [0.3, 0.8, 0.8]]}]},
+            {"type": "OUTFLOW", "shapes": [{"box": [[0.0, 0.0, 0.0], [0.3, 0.3, 0.15]]}]}]
+        solver_main.main(cfg)
+        return [f.read_bytes() for f in sorted((tmp_path / name / "bake" / "particles").glob("*.bin"))]
+
+    device = bake("device")
+    state = {}
+
+    def host_round_trip(ctx, rng, initial=False):
+        pos, vel = ctx.get_particles() if ctx.particle_count else (np.empty((0, 3), np.float32),) * 2
+        solid = ctx.get_field("obstacle").astype(bool)
+        pos, vel = sp.apply_sources(pos, vel, state["sources"], solid, ctx.dx, rng, initial)
+        ctx.set_particles(pos, vel)
+        return len(pos)
+
+    def keep_sources(ctx, sources):
+        state["sources"] = sources
+
+    monkeypatch.setattr(runtime.Context, "apply_sources", host_round_trip)
+    monkeypatch.setattr(runtime.Context, "set_sources", keep_sources)
+    host = bake("host")
+    assert len(device) == 4 and device == host
+    assert len(device[-1]) != len(device[0])
