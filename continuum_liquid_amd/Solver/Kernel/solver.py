@@ -173,7 +173,7 @@ def solver(config: dict, timings=None):
 
         output_dir.mkdir(parents=True, exist_ok=True)
         cache = output.create_cache(output_dir, fps, start_frame, end_frame)
-        writer = output.ParticleOutput(cache)
+        writer = output.ParticleOutput(cache, ctx)
         try:
             writer.submit(positions, start_frame)
             emit_message({"type": "stats", "frame": 0, "total_frames": total_frames, "particle_count": count})
@@ -197,8 +197,8 @@ def solver(config: dict, timings=None):
                 step(ctx, params, timings)
                 if last_substep:
                     with timings.section("transfer", "particles_to_host", ctx):
-                        frame_positions = ctx.get_positions()
-                        writer.submit(frame_positions, start_frame + output_frame)
+                        # pinned slot, copy in flight while the next steps run
+                        writer.submit_device(start_frame + output_frame)
                     emit_message({"type": "stats", "frame": output_frame, "total_frames": total_frames,
                                   "particle_count": count})
         finally:

@@ -145,6 +145,15 @@ struct clq_ctx {
     std::vector<hipEvent_t> kevents[CLQ_KERNEL_CLASS_COUNT];  // start, stop, start, stop, ...
     std::vector<hipEvent_t> event_pool;
     std::vector<void *> allocations;
+    // frame downloads (clq_positions_to_host_async): a device snapshot of the
+    // positions, the copy stream, the snapshot-ready event and one completion
+    // event per ticket modulo kDownloadEvents
+    static constexpr int kDownloadEvents = 16;
+    float *snap = nullptr;
+    int64_t snap_cap = 0;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_ready = nullptr, copied[kDownloadEvents] = {};
+    int64_t tickets = 0;
 
     int alloc(void **ptr, size_t bytes) {
         void *q = nullptr;
@@ -248,7 +257,12 @@ int clq_destroy(clq_ctx *c) {
     if (!c) return CLQ_OK;
     DeviceGuard g(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     for (void *q : c->allocations) (void)hipFree(q);
+    if (c->snap_ready) (void)hipEventDestroy(c->snap_ready);
+    for (hipEvent_t e : c->copied)
+        if (e) (void)hipEventDestroy(e);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->norm_host) (void)hipHostFree(c->norm_host);
     if (c->src_scalar_host) (void)hipHostFree(c->src_scalar_host);
     if (c->ev_scal) (void)hipEventDestroy(c->ev_scal);
@@ -395,6 +409,75 @@ int clq_get_previous_positions(clq_ctx *c, float *pos, int64_t n) {
 int clq_get_particle_count(clq_ctx *c, int64_t *n) {
     if (!c || !n) return fail(CLQ_ERR_INVALID, "NULL argument");
     *n = c->n;
+    return CLQ_OK;
+}
+
+// ---- frame output ------------------------------------------------------------------------
+int clq_host_alloc(clq_ctx *c, int64_t bytes, void **host) {
+    if (!c || !host || bytes < 0) return fail(CLQ_ERR_INVALID, "bad host allocation");
+    DeviceGuard g(c->device);
+    *host = nullptr;
+    if (hipHostMalloc(host, (size_t)std::max<int64_t>(bytes, 16), hipHostMallocDefault) != hipSuccess)
+        return fail(CLQ_ERR_NOMEM, "hipHostMalloc(%lld) failed", (long long)bytes);
+    return CLQ_OK;
+}
+
+int clq_host_free(clq_ctx *c, void *host) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    if (!host) return CLQ_OK;
+    DeviceGuard g(c->device);
+    // a copy into this memory may still be in flight
+    HIP_TRY(c->copy_stream ? hipStreamSynchronize(c->copy_stream) : hipSuccess);
+    HIP_TRY(hipHostFree(host));
+    return CLQ_OK;
+}
+
+int clq_positions_to_host_async(clq_ctx *c, float *host, int64_t capacity, int64_t *n, int64_t *ticket) {
+    if (!c || !n || !ticket) return fail(CLQ_ERR_INVALID, "NULL argument");
+    *n = c->n;
+    *ticket = -1;
+    if (c->n == 0) return CLQ_OK;
+    if (!host || capacity < c->n)
+        return fail(CLQ_ERR_INVALID, "host buffer holds %lld particles, %lld needed", (long long)capacity,
+                    (long long)c->n);
+    DeviceGuard g(c->device);
+    if (!c->copy_stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+        for (hipEvent_t &e : c->copied) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    if (c->n > c->snap_cap) {
+        HIP_TRY(hipStreamSynchronize(c->copy_stream));  // the old snapshot may still be read
+        c->release(c->snap);
+        c->snap = nullptr;
+        c->snap_cap = 0;
+        void *q = nullptr;
+        if (int rc = c->alloc(&q, (size_t)c->cap * 12)) return rc;
+        c->snap = (float *)q;
+        c->snap_cap = c->cap;
+    }
+    // the snapshot is overwritten only after the previous copy has read it
+    if (c->tickets > 0)
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->copied[(c->tickets - 1) % clq_ctx::kDownloadEvents], 0));
+    const size_t bytes = (size_t)c->n * 12;
+    HIP_TRY(hipMemcpyAsync(c->snap, c->pos, bytes, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipEventRecord(c->snap_ready, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->copy_stream, c->snap_ready, 0));
+    HIP_TRY(hipMemcpyAsync(host, c->snap, bytes, hipMemcpyDeviceToHost, c->copy_stream));
+    HIP_TRY(hipEventRecord(c->copied[c->tickets % clq_ctx::kDownloadEvents], c->copy_stream));
+    *ticket = c->tickets++;
+    return CLQ_OK;
+}
+
+int clq_host_copy_wait(clq_ctx *c, int64_t ticket) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    if (ticket < 0) return CLQ_OK;
+    if (ticket >= c->tickets || ticket < c->tickets - clq_ctx::kDownloadEvents)
+        return fail(CLQ_ERR_INVALID, "unknown download ticket %lld", (long long)ticket);
+    DeviceGuard g(c->device);
+    // copies complete in order on the copy stream: the event of a later ticket
+    // recorded into the same slot completes after this one
+    HIP_TRY(hipEventSynchronize(c->copied[ticket % clq_ctx::kDownloadEvents]));
     return CLQ_OK;
 }
 

@@ -117,6 +117,10 @@ _SIGNATURES = {
     "clq_get_particles": ([_P, _P, _P, c_int64], ctypes.c_int),
     "clq_get_previous_positions": ([_P, _P, c_int64], ctypes.c_int),
     "clq_get_particle_count": ([_P, POINTER(c_int64)], ctypes.c_int),
+    "clq_host_alloc": ([_P, c_int64, POINTER(_P)], ctypes.c_int),
+    "clq_host_free": ([_P, _P], ctypes.c_int),
+    "clq_positions_to_host_async": ([_P, _P, c_int64, POINTER(c_int64), POINTER(c_int64)], ctypes.c_int),
+    "clq_host_copy_wait": ([_P, c_int64], ctypes.c_int),
     "clq_set_field": ([_P, c_int32, _P, c_int64], ctypes.c_int),
     "clq_get_field": ([_P, c_int32, _P, c_int64], ctypes.c_int),
     "clq_level_count": ([_P, POINTER(c_int32)], ctypes.c_int),
@@ -300,6 +304,32 @@ class Context:
         pos = out if out is not None else np.empty((n, 3), np.float32)
         self._check(self.lib.clq_get_particles(self.handle, _ptr(pos), None, n), "clq_get_particles")
         return pos
+
+    def host_alloc(self, rows: int) -> np.ndarray:
+        """A float32 (rows, 3) array in pinned host memory of the context's
+        device (output.py:56-75); release it with host_free."""
+        ptr = ctypes.c_void_p()
+        rows = max(1, int(rows))
+        self._check(self.lib.clq_host_alloc(self.handle, rows * 12, byref(ptr)), "clq_host_alloc")
+        flat = np.ctypeslib.as_array((ctypes.c_float * (rows * 3)).from_address(ptr.value))
+        return flat.reshape(rows, 3)
+
+    def host_free(self, array: np.ndarray) -> None:
+        self._check(self.lib.clq_host_free(self.handle, array.ctypes.data), "clq_host_free")
+
+    def positions_to_host_async(self, out: np.ndarray) -> tuple[int, int]:
+        """Starts the copy of the current positions into the pinned array
+        `out` (output.py:178-195) and returns (count, ticket) at once; the
+        rows [0, count) are valid after wait_host_copy(ticket)."""
+        if out.dtype != np.float32 or out.ndim != 2 or out.shape[1] != 3 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float32 (rows, 3) array")
+        n, ticket = c_int64(), c_int64()
+        self._check(self.lib.clq_positions_to_host_async(self.handle, out.ctypes.data, out.shape[0], byref(n),
+                                                         byref(ticket)), "clq_positions_to_host_async")
+        return n.value, ticket.value
+
+    def wait_host_copy(self, ticket: int) -> None:
+        self._check(self.lib.clq_host_copy_wait(self.handle, int(ticket)), "clq_host_copy_wait")
 
     def get_previous_positions(self) -> np.ndarray:
         n = self.particle_count

@@ -183,6 +183,24 @@ int clq_set_particles(clq_ctx *ctx, const float *pos, const float *vel, int64_t 
 int clq_get_particles(clq_ctx *ctx, float *pos, float *vel, int64_t n);
 int clq_get_previous_positions(clq_ctx *ctx, float *pos, int64_t n);
 int clq_get_particle_count(clq_ctx *ctx, int64_t *n);
+
+/* ---- frame output (Solver/Kernel/output.py:56-75, 178-195) -------------------
+ * The reference maps ALLOC_HOST_PTR buffers as pinned frame slots and copies
+ * the positions into one per output frame.  Here:
+ *   clq_host_alloc / clq_host_free: pinned (page-locked) host memory of the
+ *     context's device, for the frame slots;
+ *   clq_positions_to_host_async: snapshots the current positions on the
+ *     device (stream order: after every step enqueued so far) and copies the
+ *     snapshot into `host` (float32[n][3], n = particle count, at most
+ *     `capacity` rows) on a separate copy stream; returns at once with the
+ *     count and a ticket.  Later steps run while the copy is in flight;
+ *   clq_host_copy_wait: blocks until the copy of `ticket` has landed (a
+ *     ticket of -1, returned for an empty set, needs no wait).
+ * Tickets stay valid while fewer than 16 newer copies have been started. */
+int clq_host_alloc(clq_ctx *ctx, int64_t bytes, void **host);
+int clq_host_free(clq_ctx *ctx, void *host);
+int clq_positions_to_host_async(clq_ctx *ctx, float *host, int64_t capacity, int64_t *n, int64_t *ticket);
+int clq_host_copy_wait(clq_ctx *ctx, int64_t ticket);
 int clq_set_field(clq_ctx *ctx, int32_t field, const void *host, int64_t bytes);
 int clq_get_field(clq_ctx *ctx, int32_t field, void *host, int64_t bytes);
 int clq_level_count(clq_ctx *ctx, int32_t *levels);

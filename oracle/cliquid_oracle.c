@@ -87,6 +87,7 @@ struct clq_ctx {
     long long C;
     double dx;
     long long cap, n;
+    long long tickets; /* frame downloads started (clq_positions_to_host_async) */
     float *pos, *vel, *pos_work, *vel_work, *prev;
     int32_t *counts, *offsets;
     float *u, *v, *w, *uo, *vo, *wo, *ff, *p, *rhs, *diag, *wx, *wy, *wz;
@@ -283,6 +284,42 @@ int clq_get_previous_positions(clq_ctx *c, float *pos, int64_t n) {
 int clq_get_particle_count(clq_ctx *c, int64_t *n) {
     if (!c || !n) return fail(CLQ_ERR_INVALID, "NULL argument");
     *n = c->n;
+    return CLQ_OK;
+}
+
+/* Frame output (Solver/Kernel/output.py:56-75, 178-195): on the CPU the
+ * "pinned" slot is ordinary memory and the copy completes before the call
+ * returns; the ticket rules are the library's (-1 for an empty set, the last
+ * 16 tickets valid). */
+int clq_host_alloc(clq_ctx *c, int64_t bytes, void **host) {
+    if (!c || !host || bytes < 0) return fail(CLQ_ERR_INVALID, "bad host allocation");
+    *host = malloc((size_t)(bytes > 16 ? bytes : 16));
+    return *host ? CLQ_OK : fail(CLQ_ERR_NOMEM, "malloc(%lld) failed", (long long)bytes);
+}
+
+int clq_host_free(clq_ctx *c, void *host) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    free(host);
+    return CLQ_OK;
+}
+
+int clq_positions_to_host_async(clq_ctx *c, float *host, int64_t capacity, int64_t *n, int64_t *ticket) {
+    if (!c || !n || !ticket) return fail(CLQ_ERR_INVALID, "NULL argument");
+    *n = c->n;
+    *ticket = -1;
+    if (c->n == 0) return CLQ_OK;
+    if (!host || capacity < c->n)
+        return fail(CLQ_ERR_INVALID, "host buffer holds %lld particles, %lld needed", (long long)capacity, c->n);
+    memcpy(host, c->pos, (size_t)c->n * 12);
+    *ticket = c->tickets++;
+    return CLQ_OK;
+}
+
+int clq_host_copy_wait(clq_ctx *c, int64_t ticket) {
+    if (!c) return fail(CLQ_ERR_INVALID, "ctx is NULL");
+    if (ticket < 0) return CLQ_OK;
+    if (ticket >= c->tickets || ticket < c->tickets - 16)
+        return fail(CLQ_ERR_INVALID, "unknown download ticket %lld", (long long)ticket);
     return CLQ_OK;
 }
 

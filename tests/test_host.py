@@ -313,3 +313,32 @@ def test_bake_with_inflow_and_outflow_equals_host_seeding(tmp_path, oracle_backe
     host = bake("host")
     assert len(device) == 4 and device == host
     assert len(device[-1]) != len(device[0])
+
+
+def test_device_frames_through_pinned_slots(tmp_path, oracle_lib):
+    """submit_device writes the positions held when it was called, also when
+    the particles move and grow before the writer runs; the download rejects
+    a slot that is too small and an unknown ticket."""
+    rng = np.random.default_rng(3)
+    shape, delta = (10, 9, 8), 0.1
+    pos = (rng.random((500, 3)) * np.array(shape) * delta).astype(np.float32)
+    cache = output.create_cache(tmp_path, 24, 1, 4)
+    with runtime.Context(oracle_lib, shape, delta, capacity=len(pos)) as ctx:
+        ctx.set_particles(pos)
+        writer = output.ParticleOutput(cache, ctx)
+        expected = []
+        for frame in range(1, 5):
+            expected.append(ctx.get_positions())
+            assert writer.submit_device(frame) == len(expected[-1])
+            grown = np.concatenate([expected[-1] + np.float32(0.01), expected[-1][: 50 * frame]])
+            ctx.set_particles(grown)
+        writer.close()
+        for frame, want in enumerate(expected, start=1):
+            got = np.fromfile(output.frame_path(cache, frame), np.float32).reshape(-1, 3)
+            assert np.array_equal(got, want)
+        small = ctx.host_alloc(3)
+        with pytest.raises(runtime.ClqError, match="host buffer holds"):
+            ctx.positions_to_host_async(small)
+        ctx.host_free(small)
+        with pytest.raises(runtime.ClqError, match="unknown download ticket"):
+            ctx.wait_host_copy(10**6)
