@@ -35,7 +35,7 @@ import numpy as np
 
 from ... import runtime
 from ..General.main import emit_message
-from . import output, pressure_solve, source_particles, timing
+from . import output, pressure_solve, source_particles, timing, voxelise_mesh
 
 FIELD_DTYPE = np.float32
 PARTICLES_PER_CELL = 8
@@ -104,15 +104,28 @@ def _solver_settings(settings: dict) -> dict:
     return chosen
 
 
-def _static_masks(simulation: dict, grid_shape, delta):
+def _static_masks(ctx, simulation: dict, grid_shape, delta):
+    """Source and obstacle masks (reference voxelise_mesh.py:138-163): one
+    mask per source, one obstacle mask for every obstacle entry together;
+    meshes are voxelised on the device."""
+    origin = voxelise_mesh.grid_origin(simulation)
+
+    def voxelise(meshes):
+        mask = np.zeros(grid_shape, dtype=np.uint8)
+        for triangles in meshes:
+            if len(triangles):
+                ctx.voxelise_mesh(triangles, origin, mask)
+        return mask.astype(bool)
+
+    bake_path = simulation.get("output_path")
     sources = []
     for entry in simulation.get("sources") or []:
         resolved = dict(entry)
-        resolved["mask"] = source_particles.resolve_mask(entry, grid_shape, delta)
+        resolved["mask"] = source_particles.resolve_mask(entry, grid_shape, delta, voxelise, bake_path)
         sources.append(resolved)
     obstacle = np.zeros(grid_shape, bool)
     for entry in simulation.get("obstacles") or []:
-        obstacle |= source_particles.resolve_mask(entry, grid_shape, delta)
+        obstacle |= source_particles.resolve_mask(entry, grid_shape, delta, voxelise, bake_path)
     return sources, obstacle
 
 
@@ -154,17 +167,16 @@ def solver(config: dict, timings=None):
         solver=_solver_settings(settings),
     )
 
-    sources, obstacle_mask = _static_masks(simulation, grid_shape, delta)
-    rng = np.random.default_rng(0)
-    positions = np.empty((0, 3), dtype=FIELD_DTYPE)
-    velocities = np.empty((0, 3), dtype=FIELD_DTYPE)
-    positions, velocities = source_particles.apply_sources(
-        positions, velocities, sources, obstacle_mask, delta, rng, initial=True
-    )
-    count = len(positions)
-
-    ctx = runtime.Context(lib, grid_shape, delta, capacity=max(1, count))
+    ctx = runtime.Context(lib, grid_shape, delta, capacity=1)
     try:
+        sources, obstacle_mask = _static_masks(ctx, simulation, grid_shape, delta)
+        rng = np.random.default_rng(0)
+        positions = np.empty((0, 3), dtype=FIELD_DTYPE)
+        velocities = np.empty((0, 3), dtype=FIELD_DTYPE)
+        positions, velocities = source_particles.apply_sources(
+            positions, velocities, sources, obstacle_mask, delta, rng, initial=True
+        )
+        count = len(positions)
         ctx.set_field("obstacle", obstacle_mask.astype(np.uint8))
         ctx.set_particles(positions, velocities)
         dynamic_sources = any(s.get("type") in {"INFLOW", "OUTFLOW"} for s in sources)

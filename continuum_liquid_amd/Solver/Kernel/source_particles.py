@@ -22,8 +22,11 @@ solver.py:253-306 and once at set-up):
 A source carries its cells as ``mask`` (bool array of the grid shape),
 ``shapes`` (``{"box": [[x0, y0, z0], [x1, y1, z1]]}`` or ``{"sphere": [[cx, cy,
 cz], r]}`` in domain coordinates, marking cells whose centre lies inside) or
-``mesh`` (triangles, voxelised by ray parity on the GPU).  The reference's
-USD ``geometry_inputs`` need the ``pxr`` package, which is not available.
+``mesh`` (``{"vertices": (V, 3), "triangles": (T, 3)}`` in world coordinates,
+optionally with a ``matrix_world``) or the reference's ``geometry_inputs``
+(USD files with a ``matrix_world``, reference voxelise_mesh.py:111-163;
+reading them needs the ``pxr`` package); meshes become masks by ray parity
+on the GPU.
 """
 
 from __future__ import annotations
@@ -51,10 +54,13 @@ def shape_mask(shapes, grid_shape, delta) -> np.ndarray:
     return inside
 
 
-def resolve_mask(entry: dict, grid_shape, delta, voxelise_mesh=None) -> np.ndarray:
-    """The bool cell mask a source or obstacle entry describes.  `voxelise_mesh`
-    (vertices, triangles, grid_shape, delta) -> bool mask serves `mesh`
-    entries."""
+def resolve_mask(entry: dict, grid_shape, delta, voxelise=None, bake_path=None) -> np.ndarray:
+    """The bool cell mask a source or obstacle entry describes.  `voxelise`
+    (a list of (T, 3, 3) world-triangle arrays -> bool mask, the union of
+    their ray-parity masks) serves mesh entries; `bake_path` is the folder
+    the USD files of `geometry_inputs` are relative to."""
+    from . import voxelise_mesh as vm
+
     if entry.get("mask") is not None:
         mask = np.asarray(entry["mask"], dtype=bool)
         if mask.shape != tuple(grid_shape):
@@ -62,15 +68,19 @@ def resolve_mask(entry: dict, grid_shape, delta, voxelise_mesh=None) -> np.ndarr
         return mask
     if entry.get("shapes"):
         return shape_mask(entry["shapes"], grid_shape, delta)
+    meshes = []
     if entry.get("mesh") is not None:
-        if voxelise_mesh is None:
-            raise ValueError("a mesh entry needs the GPU voxeliser")
         mesh = entry["mesh"]
-        return voxelise_mesh(np.asarray(mesh["vertices"]), np.asarray(mesh["triangles"]), grid_shape, delta)
-    if entry.get("geometry_inputs"):
-        raise NotImplementedError(
-            "USD geometry needs the pxr package; give the entry a 'mesh' (vertices, triangles), 'mask' or 'shapes'"
-        )
+        triangles = np.asarray(mesh["vertices"], dtype=np.float32)[np.asarray(mesh["triangles"], dtype=np.int64)]
+        if mesh.get("matrix_world") is not None:
+            triangles = vm.world_triangles(triangles, mesh["matrix_world"])
+        meshes.append(triangles)
+    for item in entry.get("geometry_inputs") or ():
+        meshes.append(vm.entry_triangles(item, bake_path))
+    if meshes:
+        if voxelise is None:
+            raise ValueError("a mesh entry needs the GPU voxeliser")
+        return voxelise(meshes)
     return np.zeros(grid_shape, dtype=bool)
 
 

@@ -19,7 +19,7 @@ PACKAGE_DIR = Path(__file__).resolve().parent
 CSRC = PACKAGE_DIR / "csrc"
 LIBRARY = PACKAGE_DIR / "libcliquid_hip.so"
 SOURCES = (
-    "context.hip", "binning.hip", "p2g.hip", "grid.hip", "multigrid.hip", "g2p.hip", "voxelise.hip", "fcg.hip", "unet.hip", "sources.hip",
+    "context.hip", "binning.hip", "p2g.hip", "grid.hip", "multigrid.hip", "g2p.hip", "voxelise.hip", "fcg.hip", "unet.hip", "sources.hip", "mesh.hip",
 )
 
 HIPCC_FLAGS = [

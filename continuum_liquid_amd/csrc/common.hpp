@@ -297,4 +297,9 @@ void launch_particle_update(float *pos, float *vel, float *prev, const float *u,
                             const float *uo, const float *vo, const float *wo, const uint8_t *ob, int n, int nx,
                             int ny, int nz, const ParticleUpdate &pu, hipStream_t s);
 
+// mesh.hip: ray-parity mesh voxelisation (scratch: mesh_scratch_bytes(count))
+size_t mesh_scratch_bytes(int64_t count);
+void launch_voxelise_mesh(const float *tris, int64_t count, void *scratch, int nx, int ny, int nz, float delta,
+                          const float origin[3], uint8_t *mask, hipStream_t s);
+
 }  // namespace clq

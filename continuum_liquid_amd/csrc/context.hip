@@ -1663,6 +1663,37 @@ int clq_apply_sources(clq_ctx *c, int32_t initial, const uint64_t *rng, int64_t 
     return check_launch("apply_sources");
 }
 
+int clq_voxelise_mesh(clq_ctx *c, const float *triangles, int64_t count, double ox, double oy, double oz,
+                      uint8_t *mask) {
+    if (!c || count < 0 || (count > 0 && !triangles) || !mask) return fail(CLQ_ERR_INVALID, "bad mesh arguments");
+    if (count == 0) return CLQ_OK;  // no triangle: no cell changes
+    if (count > (1LL << 31)) return fail(CLQ_ERR_INVALID, "more than 2^31 triangles");
+    DeviceGuard g(c->device);
+    // one-time set-up: the triangles, their prepared terms and the mask live
+    // only for this call
+    void *tri = nullptr, *scratch = nullptr, *dmask = nullptr;
+    if (int rc = c->alloc(&tri, (size_t)count * 36)) return rc;
+    if (int rc = c->alloc(&scratch, mesh_scratch_bytes(count))) { c->release(tri); return rc; }
+    if (int rc = c->alloc(&dmask, (size_t)c->C)) { c->release(tri); c->release(scratch); return rc; }
+    const float origin[3] = {(float)ox, (float)oy, (float)oz};
+    int rc = CLQ_OK;
+    if (hipMemcpyAsync(tri, triangles, (size_t)count * 36, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(dmask, mask, (size_t)c->C, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = fail(CLQ_ERR_DEVICE, "mesh upload failed");
+    if (!rc) {
+        launch_voxelise_mesh((const float *)tri, count, scratch, c->nx, c->ny, c->nz, (float)c->dx, origin,
+                             (uint8_t *)dmask, c->stream);
+        rc = check_launch("voxelise_mesh");
+    }
+    if (!rc && hipMemcpyAsync(mask, dmask, (size_t)c->C, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = fail(CLQ_ERR_DEVICE, "mask download failed");
+    if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = fail(CLQ_ERR_DEVICE, "voxelise_mesh sync failed");
+    c->release(tri);
+    c->release(scratch);
+    c->release(dmask);
+    return rc;
+}
+
 int clq_voxelise(clq_ctx *c, int32_t scale, double particles_per_cell, uint8_t *occupancy, float *density) {
     if (!c || scale < 1 || !(particles_per_cell > 0)) return fail(CLQ_ERR_INVALID, "bad voxelise arguments");
     if (scale > voxelise_max_scale()) return fail(CLQ_ERR_UNSUPPORTED, "voxelise scale %d > %d", scale,

@@ -117,6 +117,7 @@ _SIGNATURES = {
     "clq_get_particles": ([_P, _P, _P, c_int64], ctypes.c_int),
     "clq_get_previous_positions": ([_P, _P, c_int64], ctypes.c_int),
     "clq_get_particle_count": ([_P, POINTER(c_int64)], ctypes.c_int),
+    "clq_voxelise_mesh": ([_P, _P, c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P], ctypes.c_int),
     "clq_host_alloc": ([_P, c_int64, POINTER(_P)], ctypes.c_int),
     "clq_host_free": ([_P, _P], ctypes.c_int),
     "clq_positions_to_host_async": ([_P, _P, c_int64, POINTER(c_int64), POINTER(c_int64)], ctypes.c_int),
@@ -304,6 +305,24 @@ class Context:
         pos = out if out is not None else np.empty((n, 3), np.float32)
         self._check(self.lib.clq_get_particles(self.handle, _ptr(pos), None, n), "clq_get_particles")
         return pos
+
+    def voxelise_mesh(self, triangles: np.ndarray, origin, mask: np.ndarray | None = None) -> np.ndarray:
+        """Ray-parity voxelisation of one geometry entry (voxelise_mesh.cl:24-48):
+        `triangles` (T, 3, 3) float32 world coordinates, `origin` the world
+        position of the grid's corner.  The cells an odd number of triangles
+        surround are set in `mask` (a bool or uint8 grid array, updated in
+        place and returned; a new zero mask when None)."""
+        tris = np.ascontiguousarray(triangles, dtype=np.float32)
+        if tris.size and (tris.ndim != 3 or tris.shape[1:] != (3, 3)):
+            raise ValueError(f"triangles must have shape (T, 3, 3), got {tris.shape}")
+        if mask is None:
+            mask = np.zeros(self.shape, np.uint8)
+        if mask.shape != self.shape or mask.dtype not in (np.uint8, np.bool_) or not mask.flags.c_contiguous:
+            raise ValueError("mask must be a C-contiguous uint8 or bool array of the grid's shape")
+        ox, oy, oz = (float(v) for v in origin)
+        self._check(self.lib.clq_voxelise_mesh(self.handle, _ptr(tris), len(tris), ox, oy, oz, mask.ctypes.data),
+                    "clq_voxelise_mesh")
+        return mask
 
     def host_alloc(self, rows: int) -> np.ndarray:
         """A float32 (rows, 3) array in pinned host memory of the context's

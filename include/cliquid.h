@@ -331,6 +331,18 @@ int clq_set_sources(clq_ctx *ctx, int32_t count, const int32_t *types, const uin
                     const float *velocities);
 int clq_apply_sources(clq_ctx *ctx, int32_t initial, const uint64_t *rng, int64_t *draws);
 
+/* ---- mesh voxelisation (Solver/Kernel/voxelise_mesh.py:111-135) --------------
+ * Kernel voxelise_mesh (Solver/Kernel/OpenCL/voxelise_mesh.cl:24-48) for one
+ * geometry entry: `triangles` is float32[count][3][3] in world coordinates;
+ * every interior cell (no index 0 or n-1) whose centre
+ * (o + (index + 0.5) * dx, float32, o = (ox, oy, oz) rounded to float32)
+ * sends a ray along normalize(1, 0.3713907, 0.2178311) that hits an odd
+ * number of triangles (ray_hit, voxelise_mesh.cl:5-22) gets mask[cell] = 1;
+ * other cells keep their value, so calls for several entries give the union.
+ * `mask` is a host uint8[C] array in the grid layout, updated in place. */
+int clq_voxelise_mesh(clq_ctx *ctx, const float *triangles, int64_t count, double ox, double oy, double oz,
+                      uint8_t *mask);
+
 /* ---- particle voxelisation ------------------------------------------------ */
 /* Occupancy at `scale` x grid resolution (uint8, 1 where a particle's cell is
  * occupied) and, when density != NULL, smoothed density (float32, trilinear

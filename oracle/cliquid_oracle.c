@@ -1746,6 +1746,70 @@ int clq_apply_sources(clq_ctx *c, int32_t initial, const uint64_t *rng, int64_t 
     return CLQ_OK;
 }
 
+/* ---- mesh voxelisation (Solver/Kernel/OpenCL/voxelise_mesh.cl) ---------------
+ * The OpenCL builtins are restated by their definitions with separately
+ * rounded operations: cross(a, b) = (a.y b.z - a.z b.y, a.z b.x - a.x b.z,
+ * a.x b.y - a.y b.x), dot as a left-to-right sum of products, normalize(v) =
+ * v / sqrt(dot(v, v)).  The cell centre o + (i + 0.5) * delta is the one
+ * expression of the kernel that fp-contract=on fuses (fmuladd). */
+static void mesh_cross(const float a[3], const float b[3], float out[3]) {
+    out[0] = a[1] * b[2] - a[2] * b[1];
+    out[1] = a[2] * b[0] - a[0] * b[2];
+    out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static float mesh_dot(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* normalize((float3)(1.0f, 0.3713907f, 0.2178311f)), voxelise_mesh.cl:42 */
+static void mesh_direction(float d[3]) {
+    const float v[3] = {1.0f, 0.3713907f, 0.2178311f};
+    const float len = sqrtf(mesh_dot(v, v));
+    for (int a = 0; a < 3; ++a) d[a] = v[a] / len;
+}
+
+/* ray_hit, voxelise_mesh.cl:5-22 (Moller-Trumbore with the kernel's
+ * thresholds); tri = a, b, c */
+static int mesh_ray_hit(const float o[3], const float d[3], const float *tri) {
+    float e1[3], e2[3], p[3], t[3], q[3];
+    for (int a = 0; a < 3; ++a) {
+        e1[a] = tri[3 + a] - tri[a];
+        e2[a] = tri[6 + a] - tri[a];
+    }
+    mesh_cross(d, e2, p);
+    const float det = mesh_dot(e1, p);
+    if (fabsf(det) < 1.0e-7f) return 0;
+    const float inv = 1.0f / det;
+    for (int a = 0; a < 3; ++a) t[a] = o[a] - tri[a];
+    const float u = mesh_dot(t, p) * inv;
+    if (u < 0.0f || u > 1.0f) return 0;
+    mesh_cross(t, e1, q);
+    const float v = mesh_dot(d, q) * inv;
+    if (v < 0.0f || u + v > 1.0f) return 0;
+    return mesh_dot(e2, q) * inv > 1.0e-6f;
+}
+
+/* voxelise_mesh, voxelise_mesh.cl:24-48: every interior cell tests its
+ * centre's ray against every triangle and is marked when the count is odd */
+int clq_voxelise_mesh(clq_ctx *c, const float *triangles, int64_t count, double ox, double oy, double oz,
+                      uint8_t *mask) {
+    if (!c || count < 0 || (count > 0 && !triangles) || !mask) return fail(CLQ_ERR_INVALID, "bad mesh arguments");
+    const float delta = (float)c->dx, o[3] = {(float)ox, (float)oy, (float)oz};
+    float d[3];
+    mesh_direction(d);
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+#pragma omp parallel for collapse(2) schedule(dynamic, 1)
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int k = 1; k < nz - 1; ++k) {
+                const float point[3] = {fmaf((float)i + 0.5f, delta, o[0]), fmaf((float)j + 0.5f, delta, o[1]),
+                                        fmaf((float)k + 0.5f, delta, o[2])};
+                int hits = 0;
+                for (int64_t q = 0; q < count; ++q) hits += mesh_ray_hit(point, d, triangles + 9 * q);
+                if (hits & 1) mask[((long long)i * ny + j) * nz + k] = 1;
+            }
+    return CLQ_OK;
+}
+
 int clq_voxelise(clq_ctx *c, int32_t s, double ppc, uint8_t *occ, float *density) {
     if (!c || s < 1 || !(ppc > 0)) return fail(CLQ_ERR_INVALID, "bad voxelise arguments");
     if (s > 2) return fail(CLQ_ERR_UNSUPPORTED, "voxelise scale %d > 2", s);

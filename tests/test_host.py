@@ -342,3 +342,24 @@ def test_device_frames_through_pinned_slots(tmp_path, oracle_lib):
         ctx.host_free(small)
         with pytest.raises(runtime.ClqError, match="unknown download ticket"):
             ctx.wait_host_copy(10**6)
+
+
+def test_bake_with_a_mesh_obstacle_equals_the_same_box_as_a_shape(tmp_path, oracle_backend):
+    """An obstacle given as a triangle mesh in world coordinates (grid origin
+    (-x/2, -y/2, 0)) is voxelised by ray parity and bakes exactly like the
+    same box given as a shape in domain coordinates."""
+    from .test_oracle_mesh import box_mesh
+
+    lo, hi = [0.55, 0.2, 0.1], [0.8, 0.7, 0.35]
+    tris = box_mesh(np.array(lo) + [-0.5, -0.5, 0.0], np.array(hi) + [-0.5, -0.5, 0.0])
+    vertices, index = np.unique(tris.reshape(-1, 3), axis=0, return_inverse=True)
+
+    def bake(name, obstacle):
+        cfg = make_config(tmp_path / name, frames=2)
+        cfg["simulation"]["obstacles"] = [obstacle]
+        solver_main.main(cfg)
+        return [f.read_bytes() for f in sorted((tmp_path / name / "bake" / "particles").glob("*.bin"))]
+
+    as_mesh = bake("mesh", {"mesh": {"vertices": vertices, "triangles": index.reshape(-1, 3)}})
+    as_shape = bake("shape", {"shapes": [{"box": [lo, hi]}]})
+    assert len(as_mesh) == 3 and as_mesh == as_shape
