@@ -293,9 +293,12 @@ struct ParticleUpdate {
     int pure_flip;
 };
 enum { kG2P = 1, kSavePrev = 2, kAdvect = 4, kDomainBC = 8, kObstacleBC = 16 };
+// offsets/counts: the bins of the current particle order (the G2P samples
+// are then staged per bin-row segment), or NULL
 void launch_particle_update(float *pos, float *vel, float *prev, const float *u, const float *v, const float *w,
-                            const float *uo, const float *vo, const float *wo, const uint8_t *ob, int n, int nx,
-                            int ny, int nz, const ParticleUpdate &pu, hipStream_t s);
+                            const float *uo, const float *vo, const float *wo, const int32_t *offsets,
+                            const int32_t *counts, const uint8_t *ob, int n, int nx, int ny, int nz,
+                            const ParticleUpdate &pu, hipStream_t s);
 
 // mesh.hip: ray-parity mesh voxelisation (scratch: mesh_scratch_bytes(count))
 size_t mesh_scratch_bytes(int64_t count);

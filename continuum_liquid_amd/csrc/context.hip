@@ -1526,10 +1526,13 @@ static ParticleUpdate make_update(clq_ctx *c, int flags, double flip_ratio, doub
 
 static int particle_update(clq_ctx *c, int flags, double flip_ratio, double dt, const char *what) {
     DeviceGuard g(c->device);
+    // with the particles in bin order, the G2P samples are staged per bin row
+    const bool binned = c->bins_current;
     if (flags & (kAdvect | kDomainBC | kObstacleBC)) c->bins_current = false;
     const ParticleUpdate pu = make_update(c, flags, flip_ratio, dt);
-    launch_particle_update(c->pos, c->vel, c->prev, c->u, c->v, c->w, c->uo, c->vo, c->wo, c->obstacle, (int)c->n,
-                           c->nx, c->ny, c->nz, pu, c->stream);
+    launch_particle_update(c->pos, c->vel, c->prev, c->u, c->v, c->w, c->uo, c->vo, c->wo,
+                           binned ? c->offsets : nullptr, c->counts, c->obstacle, (int)c->n, c->nx, c->ny, c->nz, pu,
+                           c->stream);
     return check_launch(what);
 }
 
