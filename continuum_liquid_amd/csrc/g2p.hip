@@ -229,33 +229,39 @@ __global__ __launch_bounds__(256) void particle_update_staged_kernel(float *__re
     const int lo = offsets[first], hi = offsets[last] + counts[last];
     if (lo >= hi) return;  // no particle in the segment (block-uniform)
     const int zlo = k0 - 1;
-    // every staging load is issued before the first store, so the block
-    // waits for one memory round trip rather than one per element; the array
-    // loop is unrolled so each array's pointer is a scalar
-    constexpr int kRow = 9 * kStageZ, kPer = (kRow + 255) / 256;
+    // Only the rows an array is sampled in are staged (the bin argument
+    // above): u rows x = bi - 1 .. bi + 1, y = bj, bj + 1; v rows x = bi,
+    // bi + 1, y = bj - 1 .. bj + 1; w rows x = bi, bi + 1, y = bj, bj + 1.
+    // Every load is issued before the first store, so the block waits for
+    // one memory round trip; the array loop is unrolled so each array's
+    // pointer is a scalar.
+    constexpr int kPer = (6 * kStageZ + 255) / 256;
     float val[6][kPer];
+    int slot[6][kPer];
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
         const float *__restrict__ f = fetch.f[a];
+        const int c = a % 3, rows = c == 2 ? 4 : 6;
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
-            const int e = threadIdx.x + q * 256;
-            const int r = e / kStageZ, z = zlo + e % kStageZ;
-            const int x = bi - 1 + r / 3, y = bj - 1 + r % 3;
+            const int e = threadIdx.x + q * 256, ri = e / kStageZ, z = zlo + e % kStageZ;
+            const int rx = c == 0 ? ri / 2 : 1 + (c == 1 ? ri / 3 : ri / 2);
+            const int ry = c == 1 ? ri % 3 : 1 + ri % 2;
+            const int x = bi - 1 + rx, y = bj - 1 + ry;
             // unconditional load of a clamped cell; the value of a cell
             // outside the grid (never sampled) is dropped
-            const bool inside = e < kRow && x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
-            const float loaded = f[gidx(min(max(x, 0), nx - 1), min(max(y, 0), ny - 1), min(max(z, 0), nz - 1), ny, nz)];
+            const bool inside = e < rows * kStageZ && x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
+            const float loaded =
+                f[gidx(min(max(x, 0), nx - 1), min(max(y, 0), ny - 1), min(max(z, 0), nz - 1), ny, nz)];
             val[a][q] = inside ? loaded : 0.0f;
+            slot[a][q] = e < rows * kStageZ ? ((a * 3 + rx) * 3 + ry) * kStageZ + e % kStageZ : -1;
         }
     }
 #pragma unroll
     for (int a = 0; a < 6; ++a)
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const int e = threadIdx.x + q * 256;
-            if (e < kRow) st[a * kRow + e] = val[a][q];
-        }
+        for (int q = 0; q < kPer; ++q)
+            if (slot[a][q] >= 0) st[slot[a][q]] = val[a][q];
     __syncthreads();
     const StagedFetch sf{st, bi, bj, zlo};
     for (int i = lo + (int)threadIdx.x; i < hi; i += 256)
