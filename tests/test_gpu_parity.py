@@ -506,3 +506,25 @@ def test_particle_sources_bit_exact(oracle_lib, hip_lib, initial):
         moved = po + np.float32(0.7 * delta) * np.sign(vo)
         o.set_particles(moved, vo)
         h.set_particles(moved, vo)
+
+
+def test_domain_boundary_kernels_on_small_grids(oracle_lib, hip_lib):
+    """grid_domain_bc and the pressure's Neumann copy enumerate the boundary
+    faces per launch item; on every grid with each axis in 3..8 (where some
+    face has no interior cells, or one) they write exactly the oracle's
+    values on random fields (boundary_conditions.cl:18-67)."""
+    rng = np.random.default_rng(17)
+    for shape in [(a, b, c) for a in range(3, 9) for b in range(3, 9) for c in range(3, 9)]:
+        fields = {name: rng.standard_normal(shape).astype(np.float32)
+                  for name in ("u", "v", "w", "pressure")}
+        fields["fluid_fraction"] = (rng.random(shape) < 0.5).astype(np.float32)
+        out = []
+        for lib in (oracle_lib, hip_lib):
+            with runtime.Context(lib, shape, 0.1) as ctx:
+                for name, value in fields.items():
+                    ctx.set_field(name, value)
+                ctx.grid_domain_bc()
+                ctx.neumann_bc()
+                out.append({name: ctx.get_field(name) for name in fields})
+        for name in fields:
+            assert np.array_equal(out[0][name], out[1][name]), (shape, name)
