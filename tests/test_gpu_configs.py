@@ -29,6 +29,9 @@ from continuum_liquid_amd import runtime, scenes
 from continuum_liquid_amd.Solver.Kernel import solver as host_solver
 
 from .helpers import bin_keys, make_context
+from .test_golden_inputs import GOLDEN
+
+import hashlib
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +56,14 @@ def particles_equal(a, b):
     pa, va = a.get_particles()
     pb, vb = b.get_particles()
     return np.array_equal(pa, pb) and np.array_equal(va, vb)
+
+
+def assert_recorded_inputs(scene, config):
+    """The scene's seeded arrays are the ones recorded in tests/golden."""
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(scene.positions, dtype=np.float32).tobytes())
+    h.update(np.ascontiguousarray(scene.velocities, dtype=np.float32).tobytes())
+    assert h.hexdigest() == GOLDEN["configs"][str(config)]["sha256"]
 
 
 def check_step_properties(ctx, scene_n, stats, tolerance, error_bound, label):
@@ -130,6 +141,7 @@ def test_config2_dam_break_sphere_128(oracle_lib, hip_lib):
 
 def test_config3_256_mixed_refinement_and_fcg(hip_lib):
     scene = scenes.bench_scene(3)
+    assert_recorded_inputs(scene, 3)
     h = make_context(hip_lib, scene)
     # FP64 refinement around the W-cycle at every level (the hybrid V/W cycle
     # of the FCG preconditioner contracts too slowly as a stationary method)
@@ -165,6 +177,7 @@ def test_config5_wide_512_learned_and_voxelised(hip_lib):
     from continuum_liquid_amd import learned
 
     scene = scenes.bench_scene(5)
+    assert_recorded_inputs(scene, 5)
     h = make_context(hip_lib, scene)
     net = learned.load_weights(hip_lib, scene.shape, scene.dx)
     h.attach_net(net, 0.5)
