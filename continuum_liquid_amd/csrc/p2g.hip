@@ -213,38 +213,36 @@ __global__ __launch_bounds__(kThreads) void p2g_staged_kernel(const float *__res
     };
     auto store_item = [&](int r, int lo, StageBuf &sb) {
         const int n = min(kCap, row_hi[r] - lo);
+        // occupancy (particles.cl:186-188): the cell clamp(trunc(x inv_dx),
+        // 1, n - 2) of a particle is its bin or the bin above on each axis
+        // (clamps and NaN included), so every particle whose cell the patch
+        // owns is staged by this block once, from one of the rows
+        // (ci0 - 1 .. ci0 + 1, cj0 - 1 .. cj0 + 1); integer counts, exact
+        const bool occ_row = (r >> 2) <= 2 && (r & 3) <= 2;  // block-uniform
 #pragma unroll
         for (int q = 0; q < kPerThread; ++q) {
             const int t = tid + q * kThreads;
             if (t < n) {
                 const int sl = stage_slot(t);
+                float pc[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    sb.a[c][sl] = rp[q][c] * inv_dx;
+                    pc[c] = rp[q][c] * inv_dx;
+                    sb.a[c][sl] = pc[c];
                     sb.a[3 + c][sl] = rv[q][c];
+                }
+                if (occ_row) {
+                    const int tx = trunc_clamp(pc[0], 1, nx - 2) - ci0;
+                    const int ty = trunc_clamp(pc[1], 1, ny - 2) - cj0;
+                    const int tz = trunc_clamp(pc[2], 1, nz - 2) - ck0;
+                    if ((unsigned)tx < 2u && (unsigned)ty < 2u && tz >= 1 && tz <= 62)
+                        atomicAdd(&occ_s[2 * tx + ty][tz], 1);
                 }
             }
         }
         if (tid < 64) sb.start[tid] = rb;
         else if (tid < 128) sb.count[tid - 64] = rb;
         if (tid < 6) sb.a[tid][kSentinel] = tid < 3 ? -1.0e9f : 0.0f;
-        // occupancy (particles.cl:186-188): the cell clamp(trunc(x inv_dx),
-        // 1, n - 2) of a particle is its bin or the bin above on each axis
-        // (clamps and NaN included), so every particle whose cell the patch
-        // owns is staged by this block once, from one of the rows
-        // (ci0 - 1 .. ci0 + 1, cj0 - 1 .. cj0 + 1); integer counts, exact
-        if ((r >> 2) > 2 || (r & 3) > 2) return;
-#pragma unroll
-        for (int q = 0; q < kPerThread; ++q) {
-            const int t = tid + q * kThreads;
-            if (t < n) {
-                const int tx = trunc_clamp(rp[q][0] * inv_dx, 1, nx - 2) - ci0;
-                const int ty = trunc_clamp(rp[q][1] * inv_dx, 1, ny - 2) - cj0;
-                const int tz = trunc_clamp(rp[q][2] * inv_dx, 1, nz - 2) - ck0;
-                if ((unsigned)tx < 2u && (unsigned)ty < 2u && tz >= 1 && tz <= 62)
-                    atomicAdd(&occ_s[2 * tx + ty][tz], 1);
-            }
-        }
     };
     int r = 0, lo = row_lo[0];
     settle(r, lo);
