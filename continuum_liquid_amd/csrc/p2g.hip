@@ -46,6 +46,15 @@ namespace clq {
 // neighbour index (n = 9(ox+1) + 3(oy+1) + oz+1, partial n % 8), as the
 // reference's 8-lane work-group does.  A lane past its part of the chunk
 // reads a sentinel particle far outside the grid, whose weights are 0.
+//
+// The kernel is bound by the chain of staged items each block runs through
+// (one memory round trip and one barrier per item), so nothing on that chain
+// waits for LDS it need not: every wave lists the items in its own registers
+// (row ranges in lanes 0..15, chunk counts summed with DPP, the current item
+// found with a ballot and readlanes), every lane keeps the start and count
+// of its own bin of the next item in registers, loaded with the item's
+// particles, and a walk reads each step's particle while it computes the
+// previous one.
 constexpr int kSeg = 62;
 constexpr int kThreads = 256;
 constexpr int kCap = 640;                            // particles per staged chunk
@@ -74,64 +83,101 @@ __device__ __forceinline__ int stage_slot(int q) { return q + (q >> 3); }
 
 struct StageBuf {
     float a[6][kSlots];  // x, y, z in cells; u, v, w
-    int start[64], count[64];
 };
+
+// 16-lane inclusive prefix sum (lanes 0..15) with DPP row shifts
+__device__ __forceinline__ int row_scan16(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);
+    return x;
+}
 
 // One walk of a staged chunk by one wave: the cell row (ci, cj) reads bin row
 // (ci + OX, cj + OY).  Adding a zero term changes no bit: the term is +-0
 // (finite velocities), and a sum that starts at +0 and only ever rounds to
 // nearest is never -0, so s + (+-0) == s.
+struct WalkParticle {
+    float p0, p1, p2, qu, qv, qw;
+};
+
+// the particle in LDS slot sl (only the velocity components the walk uses)
+template <bool U, bool V, bool W>
+__device__ __forceinline__ WalkParticle walk_load(const StageBuf &sb, int sl) {
+    WalkParticle q;
+    q.p0 = sb.a[0][sl];
+    q.p1 = sb.a[1][sl];
+    q.p2 = sb.a[2][sl];
+    q.qu = U ? sb.a[3][sl] : 0.0f;
+    q.qv = V ? sb.a[4][sl] : 0.0f;
+    q.qw = W ? sb.a[5][sl] : 0.0f;
+    return q;
+}
+
 template <int OX, int OY>
-__device__ __forceinline__ void walk(const StageBuf &sb, int lo, int hi, int lane, int ci, int cj, int ck,
-                                     float (&vs)[8][3], float (&ws)[8][3]) {
+__device__ __forceinline__ void walk_step(const WalkParticle &q, float gx, float gy, float gz_own, float gz_up,
+                                          float gz_dn, float (&vs)[8][3], float (&ws)[8][3]) {
     constexpr int base = (OX + 1) * 9 + (OY + 1) * 3;
     constexpr int L_UP = (base + 0) & 7;    // this lane's bin is oz = -1 for cell ck + 1
     constexpr int L_OWN = (base + 1) & 7;   // oz = 0
     constexpr int L_DOWN = (base + 2) & 7;  // oz = +1 for cell ck - 1
     constexpr bool U = OY <= 0, V = OX <= 0, W = OX <= 0 && OY <= 0;
-    const int b0 = sb.start[lane], b1 = b0 + sb.count[lane];
+    const float ay = tent(q.p1, gy + 0.5f);
+    if constexpr (U) {
+        const float uxy = tent(q.p0, gx + 1.0f) * ay;
+        const float wo = uxy * tent(q.p2, gz_own + 0.5f);
+        vs[L_OWN][0] = fmaf(wo, q.qu, vs[L_OWN][0]);
+        ws[L_OWN][0] = ws[L_OWN][0] + wo;
+        const float wu = lane_up(uxy * tent(q.p2, gz_up + 0.5f));
+        vs[L_UP][0] = fmaf(wu, lane_up(q.qu), vs[L_UP][0]);
+        ws[L_UP][0] = ws[L_UP][0] + wu;
+    }
+    if constexpr (V) {
+        const float vxy = tent(q.p0, gx + 0.5f) * tent(q.p1, gy + 1.0f);
+        const float wo = vxy * tent(q.p2, gz_own + 0.5f);
+        vs[L_OWN][1] = fmaf(wo, q.qv, vs[L_OWN][1]);
+        ws[L_OWN][1] = ws[L_OWN][1] + wo;
+        const float wu = lane_up(vxy * tent(q.p2, gz_up + 0.5f));
+        vs[L_UP][1] = fmaf(wu, lane_up(q.qv), vs[L_UP][1]);
+        ws[L_UP][1] = ws[L_UP][1] + wu;
+    }
+    if constexpr (W) {
+        const float wxy = tent(q.p0, gx + 0.5f) * ay;
+        const float wo = wxy * tent(q.p2, gz_own + 1.0f);
+        vs[L_OWN][2] = fmaf(wo, q.qw, vs[L_OWN][2]);
+        ws[L_OWN][2] = ws[L_OWN][2] + wo;
+        const float wu = lane_up(wxy * tent(q.p2, gz_up + 1.0f));
+        vs[L_UP][2] = fmaf(wu, lane_up(q.qw), vs[L_UP][2]);
+        ws[L_UP][2] = ws[L_UP][2] + wu;
+        // the cell below (ck - 1) takes it as its oz = +1 neighbour
+        const float wd = lane_down(wxy * tent(q.p2, gz_dn + 1.0f));
+        vs[L_DOWN][2] = fmaf(wd, lane_down(q.qw), vs[L_DOWN][2]);
+        ws[L_DOWN][2] = ws[L_DOWN][2] + wd;
+    }
+}
+
+template <int OX, int OY>
+__device__ __forceinline__ void walk(const StageBuf &sb, int lo, int hi, int b0, int bc, int ci, int cj, int ck,
+                                     float (&vs)[8][3], float (&ws)[8][3]) {
+    constexpr bool U = OY <= 0, V = OX <= 0, W = OX <= 0 && OY <= 0;
+    const int b1 = b0 + bc;
     const int a = max(b0, lo), e = min(b1, hi);
     const int n = max(e - a, 0), first = a - lo;
     const float gx = (float)ci, gy = (float)cj;
     const float gz_own = (float)ck, gz_up = (float)(ck + 1), gz_dn = (float)(ck - 1);
-    for (int s = 0; __any(s < n); ++s) {
-        const int sl = s < n ? stage_slot(first + s) : kSentinel;
-        const float p0 = sb.a[0][sl], p1 = sb.a[1][sl], p2 = sb.a[2][sl];
-        const float ay = tent(p1, gy + 0.5f);
-        if constexpr (U) {
-            const float qu = sb.a[3][sl];
-            const float uxy = tent(p0, gx + 1.0f) * ay;
-            const float wo = uxy * tent(p2, gz_own + 0.5f);
-            vs[L_OWN][0] = fmaf(wo, qu, vs[L_OWN][0]);
-            ws[L_OWN][0] = ws[L_OWN][0] + wo;
-            const float wu = lane_up(uxy * tent(p2, gz_up + 0.5f));
-            vs[L_UP][0] = fmaf(wu, lane_up(qu), vs[L_UP][0]);
-            ws[L_UP][0] = ws[L_UP][0] + wu;
-        }
-        if constexpr (V) {
-            const float qv = sb.a[4][sl];
-            const float vxy = tent(p0, gx + 0.5f) * tent(p1, gy + 1.0f);
-            const float wo = vxy * tent(p2, gz_own + 0.5f);
-            vs[L_OWN][1] = fmaf(wo, qv, vs[L_OWN][1]);
-            ws[L_OWN][1] = ws[L_OWN][1] + wo;
-            const float wu = lane_up(vxy * tent(p2, gz_up + 0.5f));
-            vs[L_UP][1] = fmaf(wu, lane_up(qv), vs[L_UP][1]);
-            ws[L_UP][1] = ws[L_UP][1] + wu;
-        }
-        if constexpr (W) {
-            const float qw = sb.a[5][sl];
-            const float wxy = tent(p0, gx + 0.5f) * ay;
-            const float wo = wxy * tent(p2, gz_own + 1.0f);
-            vs[L_OWN][2] = fmaf(wo, qw, vs[L_OWN][2]);
-            ws[L_OWN][2] = ws[L_OWN][2] + wo;
-            const float wu = lane_up(wxy * tent(p2, gz_up + 1.0f));
-            vs[L_UP][2] = fmaf(wu, lane_up(qw), vs[L_UP][2]);
-            ws[L_UP][2] = ws[L_UP][2] + wu;
-            // the cell below (ck - 1) takes it as its oz = +1 neighbour
-            const float wd = lane_down(wxy * tent(p2, gz_dn + 1.0f));
-            vs[L_DOWN][2] = fmaf(wd, lane_down(qw), vs[L_DOWN][2]);
-            ws[L_DOWN][2] = ws[L_DOWN][2] + wd;
-        }
+    auto slot_of = [&](int s) { return s < n ? stage_slot(first + s) : kSentinel; };
+    // two steps per trip, each step's particle read while the previous step
+    // is computed
+    WalkParticle A = walk_load<U, V, W>(sb, slot_of(0));
+    for (int s = 0; __any(s < n); s += 2) {
+        const WalkParticle B = walk_load<U, V, W>(sb, slot_of(s + 1));
+        __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the step
+        walk_step<OX, OY>(A, gx, gy, gz_own, gz_up, gz_dn, vs, ws);
+        if (!__any(s + 1 < n)) break;
+        A = walk_load<U, V, W>(sb, slot_of(s + 2));
+        __builtin_amdgcn_sched_barrier(0);
+        walk_step<OX, OY>(B, gx, gy, gz_own, gz_up, gz_dn, vs, ws);
     }
 }
 
@@ -143,7 +189,6 @@ __global__ __launch_bounds__(kThreads) void p2g_staged_kernel(const float *__res
                                                              float *__restrict__ ff, float inv_dx, float inv_ppc,
                                                              int nx, int ny, int nz) {
     __shared__ StageBuf buf[2];
-    __shared__ int row_lo[kRowsStaged], row_hi[kRowsStaged];
     // occupancy counts of the patch's cells, [wave][lane]
     __shared__ int occ_s[4][64];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -154,20 +199,36 @@ __global__ __launch_bounds__(kThreads) void p2g_staged_kernel(const float *__res
     const int ck0 = (int)blockIdx.x * kSeg - 1, ck = ck0 + lane;  // this lane's bin; also its cell
     // lanes fl..ll hold the bins 0..nz-2 of the segment
     const int fl = ck0 < 0 ? -ck0 : 0, ll = min(63, nz - 2 - ck0);
-    // the particle range of each bin row r = 4 (bi - ci0 + 1) + (bj - cj0 + 1)
-    if (tid < kRowsStaged) {
-        const int bi = ci0 - 1 + (tid >> 2), bj = cj0 - 1 + (tid & 3);
-        int lo = 0, hi = 0;
+    // Every wave lists the items itself, in registers: lane r < 16 holds the
+    // particle range [rl, rh) of bin row r = 4 (bi - ci0 + 1) + (bj - cj0 + 1)
+    // and pend, the number of kCap chunks of rows 0..r; item k is chunk
+    // k - (pend[r] - chunks[r]) of the row r with pend[r - 1] <= k < pend[r]
+    // (a ballot and readlanes, all scalar).
+    int rl = 0, rh = 0;
+    if (lane < kRowsStaged) {
+        const int bi = ci0 - 1 + (lane >> 2), bj = cj0 - 1 + (lane & 3);
         if (bi >= 0 && bi < nx - 1 && bj >= 0 && bj < ny - 1 && fl <= ll) {
             const int64_t f = gidx(bi, bj, ck0 + fl, ny, nz), l = gidx(bi, bj, ck0 + ll, ny, nz);
-            lo = offsets[f];
-            hi = offsets[l] + counts[l];
+            rl = offsets[f];
+            rh = offsets[l] + counts[l];
         }
         // (ci0 + 2, cj0 + 2) is the (+1, +1) neighbour of one cell row only
-        if (tid == kRowsStaged - 1) hi = lo;
-        row_lo[tid] = lo;
-        row_hi[tid] = hi;
+        if (lane == kRowsStaged - 1) rh = rl;
     }
+    const int chunks = rh > rl ? (rh - rl + kCap - 1) / kCap : 0;
+    const int pend = row_scan16(chunks), pbeg = pend - chunks;
+    const int items = __builtin_amdgcn_readlane(pend, kRowsStaged - 1);
+    struct Item {
+        int r, lo, hi;
+    };
+    auto item_at = [&](int k) {
+        Item it;
+        it.r = (int)__popcll(__ballot(lane < kRowsStaged && pend <= k));
+        const int j = k - __builtin_amdgcn_readlane(pbeg, it.r);
+        it.lo = __builtin_amdgcn_readlane(rl, it.r) + j * kCap;
+        it.hi = min(it.lo + kCap, __builtin_amdgcn_readlane(rh, it.r));
+        return it;
+    };
     float vs[8][3], ws[8][3];
 #pragma unroll
     for (int l = 0; l < 8; ++l)
@@ -176,49 +237,48 @@ __global__ __launch_bounds__(kThreads) void p2g_staged_kernel(const float *__res
             vs[l][a] = 0.0f;
             ws[l][a] = 0.0f;
         }
-    __syncthreads();
-    // The items are the chunks [lo, lo + kCap) of the non-empty rows in row
-    // order; (r, lo) is the current one, the same in every thread.
-    auto settle = [&](int &r, int &lo) {
-        while (r < kRowsStaged && lo >= row_hi[r]) {
-            ++r;
-            if (r < kRowsStaged) lo = row_lo[r];
-        }
-    };
-    // staging registers: thread t holds the particles t, t + kThreads, ... of an item
+    if (items == 0) {  // no particle within reach: every sum is +0 (block-uniform)
+        if (ci >= nx || cj >= ny || lane == 0 || lane == 63 || ck >= nz) return;
+        const int64_t index = gidx(ci, cj, ck, ny, nz);
+        u[index] = 0.0f;
+        v[index] = 0.0f;
+        w[index] = 0.0f;
+        ff[index] = 0.0f;
+        return;
+    }
+    // staging registers: thread t holds the particles t, t + kThreads, ... of
+    // an item; every lane the start and count of its bin in the item's row
     float rp[kPerThread][3], rv[kPerThread][3];
-    int rb = 0;  // bin start (tid < 64) or count (64 <= tid < 128) of the item's row
-    auto load_item = [&](int r, int lo) {
-        const int n = min(kCap, row_hi[r] - lo);
+    int nb0 = 0, nbc = 0;
+    auto load_item = [&](const Item &it) {
+        const int n = it.hi - it.lo;
 #pragma unroll
         for (int q = 0; q < kPerThread; ++q) {
             const int t = tid + q * kThreads;
-            const int64_t src = 3LL * (lo + min(t, n - 1));  // in bounds (n >= 1); unused when t >= n
+            const int64_t src = 3LL * (it.lo + min(t, n - 1));  // in bounds (n >= 1); unused when t >= n
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 rp[q][c] = pos[src + c];
                 rv[q][c] = vel[src + c];
             }
         }
-        if (tid < 128) {
-            const int bi = ci0 - 1 + (r >> 2), bj = cj0 - 1 + (r & 3);
-            const int kk = ck0 + (tid & 63);
-            int val = 0;
-            if (kk >= 0 && kk < nz - 1) {
-                const int64_t bin = gidx(bi, bj, kk, ny, nz);
-                val = tid < 64 ? offsets[bin] : counts[bin];
-            }
-            rb = val;
+        const int bi = ci0 - 1 + (it.r >> 2), bj = cj0 - 1 + (it.r & 3);
+        nb0 = 0;
+        nbc = 0;
+        if (ck >= 0 && ck < nz - 1) {
+            const int64_t bin = gidx(bi, bj, ck, ny, nz);
+            nb0 = offsets[bin];
+            nbc = counts[bin];
         }
     };
-    auto store_item = [&](int r, int lo, StageBuf &sb) {
-        const int n = min(kCap, row_hi[r] - lo);
+    auto store_item = [&](const Item &it, StageBuf &sb) {
+        const int n = it.hi - it.lo;
         // occupancy (particles.cl:186-188): the cell clamp(trunc(x inv_dx),
         // 1, n - 2) of a particle is its bin or the bin above on each axis
         // (clamps and NaN included), so every particle whose cell the patch
         // owns is staged by this block once, from one of the rows
         // (ci0 - 1 .. ci0 + 1, cj0 - 1 .. cj0 + 1); integer counts, exact
-        const bool occ_row = (r >> 2) <= 2 && (r & 3) <= 2;  // block-uniform
+        const bool occ_row = (it.r >> 2) <= 2 && (it.r & 3) <= 2;  // block-uniform
 #pragma unroll
         for (int q = 0; q < kPerThread; ++q) {
             const int t = tid + q * kThreads;
@@ -240,53 +300,47 @@ __global__ __launch_bounds__(kThreads) void p2g_staged_kernel(const float *__res
                 }
             }
         }
-        if (tid < 64) sb.start[tid] = rb;
-        else if (tid < 128) sb.count[tid - 64] = rb;
         if (tid < 6) sb.a[tid][kSentinel] = tid < 3 ? -1.0e9f : 0.0f;
     };
-    int r = 0, lo = row_lo[0];
-    settle(r, lo);
-    if (r == kRowsStaged) {  // no particle within reach: every sum is +0 (block-uniform)
-        if (ci >= nx || cj >= ny || lane == 0 || lane == 63 || ck >= nz) return;
-        const int64_t index = gidx(ci, cj, ck, ny, nz);
-        u[index] = 0.0f;
-        v[index] = 0.0f;
-        w[index] = 0.0f;
-        ff[index] = 0.0f;
-        return;
-    }
-    {
-        load_item(r, lo);
-        store_item(r, lo, buf[0]);
-    }
+    Item cur = item_at(0);
+    load_item(cur);
+    __syncthreads();  // occ_s cleared
+    store_item(cur, buf[0]);
+    int cb0 = nb0, cbc = nbc;
     __syncthreads();
-    for (int k = 0; r < kRowsStaged; ++k) {
-        int nr = r, nlo = lo + kCap;
-        settle(nr, nlo);
-        if (nr < kRowsStaged) load_item(nr, nlo);  // in flight during the walk
+    for (int k = 0; k < items; ++k) {
+        const bool more = k + 1 < items;  // uniform
+        Item nxt = cur;
+        if (more) {
+            nxt = item_at(k + 1);
+            load_item(nxt);  // in flight during the walk
+        }
         const StageBuf &sb = buf[k & 1];
-        const int ox = ci0 - 1 + (r >> 2) - ci, oy = cj0 - 1 + (r & 3) - cj;
-        const int hi = min(lo + kCap, row_hi[r]);
+        const int ox = ci0 - 1 + (cur.r >> 2) - ci, oy = cj0 - 1 + (cur.r & 3) - cj;
         if (ci < nx && cj < ny && ox >= -1 && ox <= 1 && oy >= -1 && oy <= 1) {
+            const int lo = cur.lo, hi = cur.hi;
             // uniform per wave: the (ox, oy) of this walk
             switch ((ox + 1) * 3 + (oy + 1)) {
-            case 0: walk<-1, -1>(sb, lo, hi, lane, ci, cj, ck, vs, ws); break;
-            case 1: walk<-1, 0>(sb, lo, hi, lane, ci, cj, ck, vs, ws); break;
-            case 2: walk<-1, 1>(sb, lo, hi, lane, ci, cj, ck, vs, ws); break;
-            case 3: walk<0, -1>(sb, lo, hi, lane, ci, cj, ck, vs, ws); break;
-            case 4: walk<0, 0>(sb, lo, hi, lane, ci, cj, ck, vs, ws); break;
-            case 5: walk<0, 1>(sb, lo, hi, lane, ci, cj, ck, vs, ws); break;
-            case 6: walk<1, -1>(sb, lo, hi, lane, ci, cj, ck, vs, ws); break;
-            case 7: walk<1, 0>(sb, lo, hi, lane, ci, cj, ck, vs, ws); break;
+            case 0: walk<-1, -1>(sb, lo, hi, cb0, cbc, ci, cj, ck, vs, ws); break;
+            case 1: walk<-1, 0>(sb, lo, hi, cb0, cbc, ci, cj, ck, vs, ws); break;
+            case 2: walk<-1, 1>(sb, lo, hi, cb0, cbc, ci, cj, ck, vs, ws); break;
+            case 3: walk<0, -1>(sb, lo, hi, cb0, cbc, ci, cj, ck, vs, ws); break;
+            case 4: walk<0, 0>(sb, lo, hi, cb0, cbc, ci, cj, ck, vs, ws); break;
+            case 5: walk<0, 1>(sb, lo, hi, cb0, cbc, ci, cj, ck, vs, ws); break;
+            case 6: walk<1, -1>(sb, lo, hi, cb0, cbc, ci, cj, ck, vs, ws); break;
+            case 7: walk<1, 0>(sb, lo, hi, cb0, cbc, ci, cj, ck, vs, ws); break;
             default: break;  // (+1, +1): no weight
             }
         }
         // the other buffer was last read by the walk of the previous item,
         // which every wave finished before the previous barrier
-        if (nr < kRowsStaged) store_item(nr, nlo, buf[(k + 1) & 1]);
+        if (more) {
+            store_item(nxt, buf[(k + 1) & 1]);
+            cb0 = nb0;
+            cbc = nbc;
+        }
         __syncthreads();
-        r = nr;
-        lo = nlo;
+        cur = nxt;
     }
     if (ci >= nx || cj >= ny || lane == 0 || lane == 63 || ck >= nz) return;
 #pragma unroll
