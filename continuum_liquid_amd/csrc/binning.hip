@@ -12,11 +12,12 @@
 //      64-lane wave usually touches ~8 bins); a wave handles four groups of
 //      64 particles so that their atomics are in flight together.
 //   2. exclusive scan of the counts: tile reduce, scan of tile sums, tile scan.
-//   3. bin_permute: perm[offset[key] + provisional] = particle index.
-//   4. bin_order: a block takes 128 consecutive bins, stages their slice of
-//      perm in LDS, ranks each member among its bin's members by original
-//      index (stable order) and moves its position and velocity to the
-//      ranked slot.  Bins of more than kBigBin members are listed instead and
+//   3. bin_permute: perm[offset[key] + provisional] = particle index (and
+//      the big-bin counter of step 4 set to 0).
+//   4. bin_order: a block takes 256 consecutive bins, stages their slice of
+//      perm in LDS with a slot-to-bin table, ranks each member among its
+//      bin's members by original index (stable order) and moves its
+//      position and velocity to the ranked slot.  Bins of more than kBigBin members are listed instead and
 //      sorted by a block-level merge sort (bin_big_sort, O(c log^2 c) for c
 //      members, so no bin size makes the ranking quadratic), then moved
 //      (bin_big_move).
@@ -227,8 +228,9 @@ constexpr int kPermPer = 4;
 __global__ __launch_bounds__(256) void bin_permute_kernel(const int32_t *__restrict__ key,
                                                           const int32_t *__restrict__ rank,
                                                           const int32_t *__restrict__ offsets, int n,
-                                                          int32_t *__restrict__ perm) {
+                                                          int32_t *__restrict__ perm, int32_t *__restrict__ big_count) {
     const int base = blockIdx.x * (256 * kPermPer) + threadIdx.x;
+    if (base == 0) *big_count = 0;  // read by bin_order, the next launch
     int k[kPermPer], r[kPermPer], o[kPermPer];
 #pragma unroll
     for (int q = 0; q < kPermPer; ++q) {
@@ -246,23 +248,25 @@ __global__ __launch_bounds__(256) void bin_permute_kernel(const int32_t *__restr
 }
 
 void launch_bin_permute(const int32_t *key, const int32_t *rank, const int32_t *offsets, int n, int32_t *perm,
-                        hipStream_t s) {
+                        int32_t *big_count, hipStream_t s) {
     if (n <= 0) return;
     constexpr int per_block = 256 * kPermPer;
     hipLaunchKernelGGL(bin_permute_kernel, dim3((n + per_block - 1) / per_block), dim3(256), 0, s, key, rank, offsets,
-                       n, perm);
+                       n, perm, big_count);
 }
 
-constexpr int kOrderBins = 128;   // bins per block
+constexpr int kOrderBins = 256;   // bins per block
 constexpr int kOrderCap = 4096;   // perm entries staged in LDS
 constexpr int kBigBin = 64;       // larger bins take the merge-sort path
 
 int big_bin_capacity(int64_t n) { return (int)(n / (kBigBin + 1) + 1); }
 
-// Block b: bins [128 b, 128 b + 128) and their slots [S0, S1).  Slot s holds
+// Block b: bins [256 b, 256 b + 256) and their slots [S0, S1).  Slot s holds
 // provisionally particle i = perm[s]; its stable slot is start + (number of
 // its bin's members with a smaller index), counted over the staged slice
-// (read from perm when the slice exceeds kOrderCap).  Bins of more than
+// (read from perm when the slice exceeds kOrderCap).  A staged slot finds
+// its bin in bin_s, which every bin's thread fills for its own slots (one
+// LDS read instead of a binary search of eight dependent reads).  Bins of more than
 // kBigBin members are appended to big_list and skipped.
 __global__ __launch_bounds__(256) void bin_order_kernel(const int32_t *__restrict__ perm,
                                                         const int32_t *__restrict__ offsets,
@@ -273,13 +277,14 @@ __global__ __launch_bounds__(256) void bin_order_kernel(const int32_t *__restric
                                                         int32_t *__restrict__ big_count, int big_cap) {
     __shared__ int off_s[kOrderBins], cnt_s[kOrderBins];
     __shared__ int p_s[kOrderCap];
+    __shared__ uint8_t bin_s[kOrderCap];  // block-local bin of each staged slot
     const int t = threadIdx.x;
     const int64_t b0 = (int64_t)blockIdx.x * kOrderBins;
-    if (t < kOrderBins) {
-        const int64_t b = b0 + t;
+    for (int q = t; q < kOrderBins; q += 256) {
+        const int64_t b = b0 + q;
         const int c = b < C ? counts[b] : 0;
-        off_s[t] = b < C ? offsets[b] : n;
-        cnt_s[t] = c;
+        off_s[q] = b < C ? offsets[b] : n;
+        cnt_s[q] = c;
         if (c > kBigBin) {
             const int at = atomicAdd(big_count, 1);
             if (at < big_cap) big_list[at] = (int)b;
@@ -290,8 +295,11 @@ __global__ __launch_bounds__(256) void bin_order_kernel(const int32_t *__restric
     const int span = S1 - S0;
     if (span == 0) return;  // block-uniform
     const bool staged = span <= kOrderCap;  // block-uniform
-    if (staged)
+    if (staged) {
         for (int q = t; q < span; q += 256) p_s[q] = perm[S0 + q];
+        for (int q = t; q < kOrderBins; q += 256)
+            for (int m = off_s[q] - S0, e = m + cnt_s[q]; m < e; ++m) bin_s[m] = (uint8_t)q;
+    }
     __syncthreads();
     // kMoves slots per thread at a time: ranks first, then every load, then
     // every store
@@ -305,17 +313,28 @@ __global__ __launch_bounds__(256) void bin_order_kernel(const int32_t *__restric
             dst[q] = 0;
             if (s >= S1) continue;
             // the bin holding slot s: the last j with off_s[j] <= s
-            int lo = 0, hi = kOrderBins;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (off_s[mid] <= s) lo = mid;
-                else hi = mid;
+            int lo = 0;
+            if (staged) {
+                lo = bin_s[s - S0];
+            } else {
+                int hi = kOrderBins;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (off_s[mid] <= s) lo = mid;
+                    else hi = mid;
+                }
             }
             const int start = off_s[lo], cnt = cnt_s[lo];
             if (cnt > kBigBin) continue;
             const int i = staged ? p_s[s - S0] : perm[s];
             int r = 0;
-            for (int m = 0; m < cnt; ++m) r += (staged ? p_s[start - S0 + m] : perm[start + m]) < i;
+            if (staged) {
+                const int *ms = p_s + (start - S0);
+#pragma unroll 8
+                for (int m = 0; m < cnt; ++m) r += ms[m] < i;
+            } else {
+                for (int m = 0; m < cnt; ++m) r += perm[start + m] < i;
+            }
             src[q] = i;
             dst[q] = start + r;
         }

@@ -131,11 +131,12 @@ void launch_bin_count(const float *pos, int n, float inv_dx, int nx, int ny, int
                       int32_t *rank, int32_t *counts, hipStream_t s);
 int64_t scan_scratch_elems(int64_t count);
 void launch_exclusive_scan(const int32_t *in, int32_t *out, int64_t count, int32_t *scratch, hipStream_t s);
+// also sets *big_count to 0 for launch_bin_order
 void launch_bin_permute(const int32_t *key, const int32_t *rank, const int32_t *offsets, int n, int32_t *perm,
-                        hipStream_t s);
+                        int32_t *big_count, hipStream_t s);
 // stable order within bins and the move of positions and velocities; bins of
 // more than 64 members go through big_list (capacity big_bin_capacity(n),
-// *big_count zeroed by the caller) and are merge-sorted using scratch
+// *big_count zeroed by launch_bin_permute) and are merge-sorted using scratch
 // (n entries, free at this point)
 int big_bin_capacity(int64_t n);
 void launch_bin_order(const int32_t *perm, const int32_t *offsets, const int32_t *counts, int64_t C, int n,

@@ -596,8 +596,7 @@ int clq_bin_particles(clq_ctx *c) {
     HIP_TRY(hipMemsetAsync(c->counts, 0, (size_t)c->C * 4, c->stream));
     launch_bin_count(c->pos, n, inv_dx, c->nx, c->ny, c->nz, c->key, c->rank, c->counts, c->stream);
     launch_exclusive_scan(c->counts, c->offsets, c->C, c->scan_scratch, c->stream);
-    launch_bin_permute(c->key, c->rank, c->offsets, n, c->perm, c->stream);
-    HIP_TRY(hipMemsetAsync(c->big_count, 0, 4, c->stream));
+    launch_bin_permute(c->key, c->rank, c->offsets, n, c->perm, c->big_count, c->stream);
     // key is free after the permutation: the big bins' merge sort uses it
     launch_bin_order(c->perm, c->offsets, c->counts, c->C, n, c->pos, c->vel, c->pos_work, c->vel_work, c->big_list,
                      c->big_count, c->big_cap, c->key, c->stream);
@@ -721,8 +720,9 @@ static int ensure_active_rows(clq_ctx *c) {
     if (c->rows_valid) return CLQ_OK;
     if (c->nlevels < 2) return CLQ_OK;
     if (int rc = ensure_row_buffers(c)) return rc;
-    HIP_TRY(hipMemsetAsync(c->row_count + 3, 0, 4, c->stream));
-    if (!c->level0_packed) {  // the operator build's bytes are exact
+    const bool pack = !c->level0_packed;  // else the operator build's bytes are exact
+    if (pack) {
+        HIP_TRY(hipMemsetAsync(c->row_count + 3, 0, 4, c->stream));
         launch_pack_operator(c->lv[0], c->op0, c->row_count + 3, c->stream);
         launch_row_flags(c->lv[0], c->row_flags[0], c->stream);
     }
@@ -736,7 +736,7 @@ static int ensure_active_rows(clq_ctx *c) {
     HIP_TRY(hipMemcpyAsync(c->row_count_host, c->row_count, 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     for (int l = 0; l < row_levels(c); ++l) c->nrows[l] = c->row_count_host[l];
-    c->op0_ok = c->row_count_host[3] == 0;
+    c->op0_ok = !pack || c->row_count_host[3] == 0;
     c->rows_valid = true;
     return check_launch("active rows");
 }
