@@ -1062,9 +1062,10 @@ static int ensure_mixed_buffers(clq_ctx *c) {
 }
 
 // z = M r: the mixed-precision W-cycle on A e = r32 from e = 0 (e is the
-// level-0 pressure array), 16-bit coarse b scaled from the residual norm and
-// FP32 coarsest sweeps (FP64 ones measured the same FCG iterations and
-// residuals at 256^3, and cost 0.1 ms per step).
+// level-0 pressure array), FP32 smoothing, 16-bit coarse b scaled from the
+// residual norm and FP64 coarsest sweeps (the coarsest-grid solve of the
+// mixed-precision pattern; FP32 ones give the same iteration counts at
+// 256^3 and save about 1 % of the step).
 // done: the FCG loop already left e after the cycle's first red half-sweep
 // on level 0 (1: fcg_update with e) or after its whole first sweep (2: and
 // the black half-sweep queued during the scalar read), in place of zeroing it.
@@ -1082,7 +1083,7 @@ static int precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rn
         if (c->use_rows) launch_zero_rows(L0.p, c->rows[0], c->nrows[0], c->nz, c->stream);
         else HIP_TRY(hipMemsetAsync(L0.p, 0, (size_t)c->C * 4, c->stream));
     }
-    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, false, done);
+    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, true, done);
     L0.b = c->rhs;
     for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
     return CLQ_OK;

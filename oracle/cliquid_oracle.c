@@ -1089,7 +1089,7 @@ static inline int fluid_at(const level_t *L, long long q) {
 }
 
 /* z = M r: the mixed cycle on A e = r32 from e = 0, 16-bit coarse b and the
- * FP32 coarse smoother on the coarsest level. */
+ * FP64 coarse smoother on the coarsest level. */
 static void precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double rnorm) {
     const int kind = sp->coarse_dtype == 1 ? 2 : 1;
     const int es = (int)lround(log2(rnorm / sqrt((double)c->C) + 1e-300));
@@ -1099,7 +1099,7 @@ static void precondition_wcycle(clq_ctx *c, const clq_solve_params *sp, double r
     }
     c->lv[0].b = c->r32;
     memset(c->p, 0, (size_t)c->C * sizeof(float));
-    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, 0);
+    w_cycle_mixed(c, 0, sp->pre_smooth, sp->post_smooth, sp->coarse_extra, sp->vcycle_from, 1);
     c->lv[0].b = c->rhs;
     for (int l = 1; l < c->nlevels; ++l) c->lv[l].bkind = 0;
 }

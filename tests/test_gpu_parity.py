@@ -220,7 +220,7 @@ def test_full_step_bit_exact(oracle_lib, hip_lib, steps):
 
 def test_fcg_steps_match_oracle(oracle_lib, hip_lib):
     """Eight whole steps with the benchmark's FCG solve (weighted W(1, 1)
-    cycles, FP32 coarsest sweeps, the folded update with its two correction
+    cycles, FP64 coarsest sweeps, the folded update with its two correction
     arrays, the row-restricted initialisation).  The fluid moves, so the
     active rows change between solves; values left in rows that went inactive
     must not reach the pressure or the velocities.  FCG sums its dots in
