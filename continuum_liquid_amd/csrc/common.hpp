@@ -234,15 +234,16 @@ int fcg_partials_count();
 void launch_fcg_init(const Level &L, const uint8_t *op, const float *p32, const float *b, double inv_h2, double *x,
                      double *r, float *r32, const int32_t *flags, const int32_t *rows, int64_t nrows,
                      int32_t *x_rows, int x_rows_known, double *partials, double *s, hipStream_t st);
-// the dots z.q, z.r, z.w (w = A z) -> beta, d.q of the new direction, alpha
-void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const double *qv, const double *r, int first,
+// the dots z.(A d_old), z.r, z.w (w = A z) -> beta, d.A d of the new
+// direction, alpha
+void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const float *dold, const double *r, int first,
                      double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st);
-// d = z + beta d, q = A z + beta q, x += alpha d, r -= alpha q and r.r;
-// e != NULL (with op, e != z): also the next cycle's e after its first red
-// half-sweep
-void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double *x, double *r, float *r32, double *d,
-                       double *qv, int first, double inv_h2, const int32_t *rows, int64_t nrows, float *e, float delta,
-                       double *partials, double *s, hipStream_t st);
+// dnew = float(z + beta dold), q = A dnew, x += alpha dnew, r -= alpha q and
+// r.r (dnew != dold); e != NULL (with op, e != z): also the next cycle's e
+// after its first red half-sweep
+void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double *x, double *r, float *r32,
+                       const float *dold, float *dnew, int first, double inv_h2, const int32_t *rows, int64_t nrows,
+                       float *e, float delta, double *partials, double *s, hipStream_t st);
 
 // r = b - A x from the FP64 iterate x over the listed rows, r32; r.r and
 // b.b into s[6], s[7]

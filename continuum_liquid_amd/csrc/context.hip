@@ -104,9 +104,12 @@ struct clq_ctx {
     float *r32 = nullptr;
     bool p64_valid = false;  // p64a holds the pressure of the last solve (modes 2-4)
     // flexible-CG state, allocated on first use
-    // FCG: the direction d, q = A d, and the second level-0 correction array
-    // (the update writes the next cycle's correction while it reads z)
-    double *r64 = nullptr, *dir = nullptr, *q64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
+    // FCG: the residual, the FP32 direction (two arrays: the update reads
+    // the old direction at the neighbours while it writes the new one), and
+    // the second level-0 correction array (the update writes the next
+    // cycle's correction while it reads z)
+    double *r64 = nullptr, *fcg_s = nullptr, *fcg_partials = nullptr;
+    float *dir[2] = {nullptr, nullptr};
     float *p_alt = nullptr;
     // level-0 rows on which x (p64a) may be non-zero: the row flags of the
     // last FCG initialisation by rows; x_rows_known is false after any other
@@ -1126,10 +1129,10 @@ static int ensure_fcg_buffers(clq_ctx *c) {
         void *q;
         if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
         c->r64 = (double *)q;
-        if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
-        c->dir = (double *)q;
-        if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 8)) return rc;
-        c->q64 = (double *)q;
+        for (float *&d : c->dir) {
+            if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 4)) return rc;
+            d = (float *)q;
+        }
         if (int rc = alloc_zeroed(c, &q, (size_t)c->C * 4)) return rc;
         c->p_alt = (float *)q;
         if (int rc = alloc_zeroed(c, &q, 16 * 8)) return rc;
@@ -1185,6 +1188,7 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     // the update makes the next cycle's first level-0 red half-sweep
     const bool fold = fold_transfers(c) && c->op0_ok && sp->pre_smooth > 0 && swept_level(c, 0);
     bool plain = false;
+    float *dold = c->dir[0], *dnew = c->dir[1];  // the direction, read and written by each update
     // each pass of the outer loop starts the direction recurrence from the
     // residual r = b - A x held in r64 / r32 (the initialisation, or the
     // recomputation after a pass whose recurrence met the tolerance)
@@ -1197,11 +1201,12 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
         if (int rc = precondition_wcycle(c, sp, r)) return rc;
         s->cycles++;
         s->fallback_cycles += plain;
-        launch_fcg_dots(L0, OP0, z, c->q64, c->r64, 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
+        launch_fcg_dots(L0, OP0, z, dold, c->r64, 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         bool first = true;
         for (;;) {
-            launch_fcg_update(L0, OP0, z, c->p64a, c->r64, c->r32, c->dir, c->q64, first, inv_h2, ROWS0,
+            launch_fcg_update(L0, OP0, z, c->p64a, c->r64, c->r32, dold, dnew, first, inv_h2, ROWS0,
                               fold ? zn : nullptr, (float)c->dx, c->fcg_partials, c->fcg_s, c->stream);
+            std::swap(dold, dnew);
             if (int rc = queue_scalars(c)) return rc;
             if (fold) {
                 // the next cycle's level-0 black half-sweep (on e = zn with
@@ -1229,7 +1234,7 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
             if (int rc = precondition_wcycle(c, sp, r, fold ? 2 : 0)) return rc;
             s->cycles++;
             s->fallback_cycles += plain;
-            launch_fcg_dots(L0, OP0, z, c->q64, c->r64, first, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
+            launch_fcg_dots(L0, OP0, z, dold, c->r64, first, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         }
         c->lv[0].p = c->p;
         // the recurrence says converged (or the cycles ran out): recompute
@@ -1293,16 +1298,18 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
     };
     if (!std::isfinite(r) || !std::isfinite(bn)) return non_finite(r + bn);
     s->recurrence_residual = r;
+    float *dold = c->dir[0], *dnew = c->dir[1];
     // passes of the direction recurrence as in mode 4: a pass that meets the
     // tolerance by its recurrence is checked against b - A x, which restarts
     // the recurrence (with the W-cycle) when it falls short
     while (r > sp->tolerance * bn && s->cycles < sp->max_cycles) {
         if (int rc = precondition(r)) return rc;
-        launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
+        launch_fcg_dots(L0, OP0, c->p, dold, c->r64, 1, inv_h2, ROWS0, c->fcg_partials, c->fcg_s, c->stream);
         bool first = true;
         for (;;) {
-            launch_fcg_update(L0, OP0, c->p, c->p64a, c->r64, c->r32, c->dir, c->q64, first, inv_h2, ROWS0, nullptr,
+            launch_fcg_update(L0, OP0, c->p, c->p64a, c->r64, c->r32, dold, dnew, first, inv_h2, ROWS0, nullptr,
                               (float)c->dx, c->fcg_partials, c->fcg_s, c->stream);
+            std::swap(dold, dnew);
             if (int rc = read_scalars(c)) return rc;
             const double before = r;
             r = sqrt(c->norm_host[0]);
@@ -1313,7 +1320,7 @@ static int solve_learned(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats
             const bool restart = !use_net && drop_weight_on_breakdown(c);
             if (use_net && r > c->stall_ratio * before) use_net = false;
             if (int rc = precondition(r)) return rc;
-            launch_fcg_dots(L0, OP0, c->p, c->q64, c->r64, restart, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
+            launch_fcg_dots(L0, OP0, c->p, dold, c->r64, restart, inv_h2, ROWS0, c->fcg_partials, c->fcg_s,
                             c->stream);
             first = restart;
         }

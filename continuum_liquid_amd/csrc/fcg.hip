@@ -3,27 +3,32 @@
 // learned network followed by it).  No reference counterpart: the reference
 // runs a fixed number of stationary W-cycles (pressure_solve.py:61-83).
 //
-// Vectors over all C cells, FP64: x (iterate), r (residual), d (direction),
-// q = A d.  The preconditioned residual z arrives as FP32 in the level-0
-// correction array.  Two kernels per iteration after the preconditioner:
-//   dots:   w = A z (not stored); z.q, z.r, z.w
-//           -> beta = -(z.q) / (d.q)_old, and the new direction's
-//              d.q = z.w + 2 beta z.q + beta^2 (d.q)_old (A symmetric, so
-//              d_old . A z = z . q_old), alpha = z.r / d.q
-//   update: w = A z again, d = z + beta d, q = w + beta q, x += alpha d,
+// Vectors over all C cells: x (iterate) and r (residual) in FP64, the
+// direction d in FP32 (two arrays, read one, write the other).  The
+// preconditioned residual z arrives as FP32 in the level-0 correction array.
+// Two kernels per iteration after the preconditioner:
+//   dots:   w = A z (not stored); (A z).d_old = z.(A d_old), z.r, z.w
+//           -> beta = -(z.A d_old) / (d.A d)_old, and the new direction's
+//              d.A d = z.w + 2 beta z.A d_old + beta^2 (d.A d)_old (A
+//              symmetric), alpha = z.r / d.A d
+//   update: d = fmaf(float(beta), d_old, z) at the cell and its six
+//           neighbours, q = A d in FP64 from those values, x += alpha d,
 //           r -= alpha q; r.r
-// (the direction needs no reduction of its own, so it streams with the
-// update).  Scalars live in device memory, so alpha and beta never
-// round-trip through the host:
-//   s[0] rz = z.r   s[1] d.q   s[2] alpha   s[3] z.q   s[4] z.w   s[5] beta
-//   s[6] r.r        s[7] b.b
-// beta = -(z_{k+1} . q_k) / (d_k . q_k), the flexible (Polak-Ribiere) form
-// written with q_k = -(r_{k+1} - r_k) / alpha_k, so r_k need not be kept.
-// The initialisation writes all cells (0 outside the interior fluid cells),
-// or with the row lists the active rows (rows holding an interior fluid cell)
-// and x = 0 on the inactive rows that may hold another value; the
-// per-iteration kernels visit the active rows only (all cells when no row
-// list is given): outside them x, r, z, d and q are zero or never read.
+// q is formed from the stored (rounded) direction itself, never carried by a
+// recurrence, so x and r move by alpha d and alpha A d of the same d and the
+// recurrence residual tracks b - A x to FP64 rounding; rounding d to FP32
+// only perturbs the direction, not that consistency.  (The direction needs
+// no reduction of its own, so it streams with the update.)  Scalars live in
+// device memory, so alpha and beta never round-trip through the host:
+//   s[0] rz = z.r   s[1] d.A d   s[2] alpha   s[3] z.A d_old   s[4] z.w
+//   s[5] beta       s[6] r.r     s[7] b.b
+// beta = -(z_{k+1} . A d_k) / (d_k . A d_k), the flexible (Polak-Ribiere)
+// form written with A d_k = -(r_{k+1} - r_k) / alpha_k, so r_k need not be
+// kept.  The initialisation writes all cells (0 outside the interior fluid
+// cells), or with the row lists the active rows (rows holding an interior
+// fluid cell) and x = 0 on the inactive rows that may hold another value;
+// the per-iteration kernels visit the active rows only (all cells when no
+// row list is given): outside them x, r, z and d are zero or never read.
 #include "common.hpp"
 
 namespace clq {
@@ -112,17 +117,17 @@ __global__ __launch_bounds__(kT) void fcg_init_kernel(Level L, const float *__re
     block_sums<2>(acc, partials);
 }
 
-// A z at cell q = (i, j, k) (0 off the interior fluid cells), from the
-// packed level-0 operator (PACKED) or the float arrays.  Every load is issued
-// before any test: off the interior the neighbour indices fall back to q
-// itself (in bounds), and the result is dropped.
-template <bool PACKED, typename T>
-__device__ __forceinline__ double apply_a_z(const Level &L, const uint8_t *__restrict__ op,
-                                            const T *__restrict__ z, int64_t q, bool inner, double inv_h2,
-                                            bool &fluid, double &dg) {
+// A v at cell q = (i, j, k) (0 off the interior fluid cells) for the vector
+// whose value at cell t is v(t), from the packed level-0 operator (PACKED) or
+// the float arrays.  Every load is issued before any test: off the interior
+// the neighbour indices fall back to q itself (in bounds), and the result is
+// dropped.
+// The neighbour values arrive as z0..z5 (+x, -x, +y, -y, +z, -z) and zc.
+template <bool PACKED>
+__device__ __forceinline__ double apply_a_vals(const Level &L, const uint8_t *__restrict__ op, int64_t q, bool inner,
+                                               double inv_h2, bool &fluid, double &dg, double z0, double z1,
+                                               double z2, double z3, double z4, double z5, double zc) {
     const int64_t sx = inner ? (int64_t)L.ny * L.nz : 0, sy = inner ? L.nz : 0, sz = inner ? 1 : 0;
-    const double z0 = z[q + sx], z1 = z[q - sx], z2 = z[q + sy], z3 = z[q - sy], z4 = z[q + sz], z5 = z[q - sz];
-    const double zc = z[q];
     double s = 0.0;
     bool fl;
     if constexpr (PACKED) {
@@ -148,6 +153,23 @@ __device__ __forceinline__ double apply_a_z(const Level &L, const uint8_t *__res
     }
     fluid = inner && fl;
     return fluid ? (s - dg * zc) * inv_h2 : 0.0;
+}
+
+template <bool PACKED, typename F>
+__device__ __forceinline__ double apply_a_fn(const Level &L, const uint8_t *__restrict__ op, int64_t q, bool inner,
+                                             double inv_h2, bool &fluid, double &dg, F v) {
+    const int64_t sx = inner ? (int64_t)L.ny * L.nz : 0, sy = inner ? L.nz : 0, sz = inner ? 1 : 0;
+    const double z0 = v(q + sx), z1 = v(q - sx), z2 = v(q + sy), z3 = v(q - sy), z4 = v(q + sz), z5 = v(q - sz);
+    const double zc = v(q);
+    return apply_a_vals<PACKED>(L, op, q, inner, inv_h2, fluid, dg, z0, z1, z2, z3, z4, z5, zc);
+}
+
+// apply_a_fn of an array
+template <bool PACKED, typename T>
+__device__ __forceinline__ double apply_a_z(const Level &L, const uint8_t *__restrict__ op,
+                                            const T *__restrict__ z, int64_t q, bool inner, double inv_h2,
+                                            bool &fluid, double &dg) {
+    return apply_a_fn<PACKED>(L, op, q, inner, inv_h2, fluid, dg, [&](int64_t t) { return (double)z[t]; });
 }
 
 // The per-iteration kernels walk rows: block b takes a contiguous run of the
@@ -219,56 +241,71 @@ __global__ __launch_bounds__(kT) void fcg_clear_rows_kernel(const int32_t *__res
     }
 }
 
-// w = A z (not stored); partial z.q, z.r, z.w
+// w = A z (not stored); partial (A z).d_old (= z.(A d_old), A symmetric),
+// z.r, z.w
 template <bool PACKED>
 __global__ __launch_bounds__(kT) void fcg_dots_kernel(Level L, const uint8_t *__restrict__ op,
-                                                     const float *__restrict__ z, const double *__restrict__ qv,
+                                                     const float *__restrict__ z, const float *__restrict__ dold,
                                                      const double *__restrict__ r, double inv_h2,
                                                      const int32_t *__restrict__ rows, int64_t nrows,
                                                      double *__restrict__ partials) {
     double acc[3] = {0.0, 0.0, 0.0};
     for_rows(L, rows, nrows, [&](int64_t q, bool inner, int, int, int) {
-        const double qo = qv[q], ro = r[q], zq = (double)z[q];
+        const double dq = (double)dold[q], ro = r[q], zq = (double)z[q];
         bool fluid;
         double dg;
         const double wv = apply_a_z<PACKED, float>(L, op, z, q, inner, inv_h2, fluid, dg);
-        acc[0] += zq * qo;
+        acc[0] += wv * dq;
         acc[1] += zq * ro;
         acc[2] += zq * wv;
     });
     block_sums<3>(acc, partials);
 }
 
-// d = z + beta d, q = A z + beta q on the interior fluid cells (0 elsewhere;
-// beta = 0 on the first iteration), then x += alpha d, r -= alpha q,
-// r32 = float(r); partial r.r.  FOLD (packed operator): also the correction
-// array e of the next preconditioner cycle after its first red half-sweep
-// from e = 0 on r32: e = fmaf(-dd, r32, +0) / diag on the red interior fluid
-// cells with diag > 0 (every neighbour term w * 0 is +0), 0 elsewhere, as
-// zero_rows and rbgs_rows_packed_kernel would leave it.  e must not be z (z
-// is read at the neighbours).
+// d = fmaf(float(beta), d_old, z) on the interior fluid cells (0 elsewhere;
+// d = z on the first iteration), q = A d in FP64 from the same values at the
+// neighbours (computed here from z and d_old, the z-neighbours' taken from
+// the adjacent lanes, which hold cells k -/+ 1 of the same row; at a
+// neighbour that is not fluid the operator weight is 0 and the value
+// unused), then x += alpha d,
+// r -= alpha q, r32 = float(r); partial r.r.  FOLD (packed operator): also
+// the correction array e of the next preconditioner cycle after its first
+// red half-sweep from e = 0 on r32: e = fmaf(-dd, r32, +0) / diag on the red
+// interior fluid cells with diag > 0 (every neighbour term w * 0 is +0), 0
+// elsewhere, as zero_rows and rbgs_rows_packed_kernel would leave it.  e must
+// not be z (z is read at the neighbours); dnew must not be dold.
 template <bool PACKED, bool FOLD>
 __global__ __launch_bounds__(kT) void fcg_update_kernel(Level L, const uint8_t *__restrict__ op,
                                                        const float *__restrict__ z, double *__restrict__ x,
                                                        double *__restrict__ r, float *__restrict__ r32,
-                                                       double *__restrict__ d, double *__restrict__ qv,
+                                                       const float *__restrict__ dold, float *__restrict__ dnew,
                                                        const double *__restrict__ s, int first, double inv_h2,
                                                        const int32_t *__restrict__ rows, int64_t nrows,
                                                        float *__restrict__ e, float dd, double *__restrict__ partials) {
-    const double alpha = s[2], beta = first ? 0.0 : s[5];
+    const double alpha = s[2];
+    const float beta = first ? 0.0f : (float)s[5];
     double acc[1] = {0.0};
+    // the new direction at cell t, as the cell's own thread stores it
+    auto dir = [&](int64_t t) -> float { return first ? z[t] : fmaf(beta, dold[t], z[t]); };
+    const int lane = threadIdx.x & 63;
     for_rows(L, rows, nrows, [&](int64_t q, bool inner, int i, int j, int k) {
-        const double xo = x[q], ro = r[q], dold = d[q], qold = qv[q], zv = (double)z[q];
+        const double xo = x[q], ro = r[q];
+        const int64_t sx = inner ? (int64_t)L.ny * L.nz : 0, sy = inner ? L.nz : 0, sz = inner ? 1 : 0;
+        const float dc = dir(q), dxp = dir(q + sx), dxm = dir(q - sx), dyp = dir(q + sy), dym = dir(q - sy);
+        // cells k + 1 and k - 1: the next and previous lanes (a wave's first
+        // and last lanes read theirs); lanes off the row only feed ghost
+        // cells, whose result is dropped
+        float dzp = __shfl_down(dc, 1), dzm = __shfl_up(dc, 1);
+        if (lane == 63) dzp = dir(q + sz);
+        if (lane == 0) dzm = dir(q - sz);
         bool fluid;
         double dg;
-        const double wv = apply_a_z<PACKED, float>(L, op, z, q, inner, inv_h2, fluid, dg);
-        const double dv = fluid ? zv + beta * dold : 0.0;
-        const double qn = fluid ? wv + beta * qold : 0.0;
-        d[q] = dv;
-        qv[q] = qn;
+        const double qn = apply_a_vals<PACKED>(L, op, q, inner, inv_h2, fluid, dg, dxp, dxm, dyp, dym, dzp, dzm, dc);
+        const float dv = fluid ? dc : 0.0f;
+        dnew[q] = dv;
         const double rv = ro - alpha * qn;
         const float rf = (float)rv;
-        x[q] = xo + alpha * dv;
+        x[q] = xo + alpha * (double)dv;
         r[q] = rv;
         r32[q] = rf;
         acc[0] += rv * rv;
@@ -283,8 +320,8 @@ __global__ __launch_bounds__(kT) void fcg_update_kernel(Level L, const uint8_t *
 }
 
 // Reduce `n` partial arrays of `blocks` entries and update the scalars.
-// stage 0: init (s6 = r.r, s7 = b.b); 2: r.r; 3: z.q, z.r, z.w -> beta,
-// d.q, alpha (first: beta = 0).
+// stage 0: init (s6 = r.r, s7 = b.b); 2: r.r; 3: z.(A d_old), z.r, z.w ->
+// beta, d.A d, alpha (first: beta = 0).
 constexpr int kScalarThreads = 1024;
 
 __global__ __launch_bounds__(kScalarThreads) void fcg_scalars_kernel(const double *__restrict__ partials, int blocks,
@@ -370,34 +407,34 @@ void launch_fcg_init(const Level &L, const uint8_t *op, const float *p32, const 
 }
 
 
-void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const double *qv, const double *r, int first,
+void launch_fcg_dots(const Level &L, const uint8_t *op, const float *z, const float *dold, const double *r, int first,
                      double inv_h2, const int32_t *rows, int64_t nrows, double *partials, double *s, hipStream_t st) {
     const int64_t nr = rows_of(L, rows, nrows);
     const int nb = row_blocks_for(nr);
     if (op)
-        hipLaunchKernelGGL(fcg_dots_kernel<true>, dim3(nb), dim3(kT), 0, st, L, op, z, qv, r, inv_h2, rows, nr,
+        hipLaunchKernelGGL(fcg_dots_kernel<true>, dim3(nb), dim3(kT), 0, st, L, op, z, dold, r, inv_h2, rows, nr,
                            partials);
     else
-        hipLaunchKernelGGL(fcg_dots_kernel<false>, dim3(nb), dim3(kT), 0, st, L, op, z, qv, r, inv_h2, rows, nr,
+        hipLaunchKernelGGL(fcg_dots_kernel<false>, dim3(nb), dim3(kT), 0, st, L, op, z, dold, r, inv_h2, rows, nr,
                            partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kScalarThreads), 0, st, partials, nb, 3, first, s);
 }
 
-void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double *x, double *r, float *r32, double *d,
-                       double *qv, int first, double inv_h2, const int32_t *rows, int64_t nrows, float *e, float delta,
-                       double *partials, double *s, hipStream_t st) {
+void launch_fcg_update(const Level &L, const uint8_t *op, const float *z, double *x, double *r, float *r32,
+                       const float *dold, float *dnew, int first, double inv_h2, const int32_t *rows, int64_t nrows,
+                       float *e, float delta, double *partials, double *s, hipStream_t st) {
     const int64_t nr = rows_of(L, rows, nrows);
     const int nb = row_blocks_for(nr);
     const float dd = delta * delta;
     if (op && e)
-        hipLaunchKernelGGL((fcg_update_kernel<true, true>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, d, qv, s,
-                           first, inv_h2, rows, nr, e, dd, partials);
-    else if (op)
-        hipLaunchKernelGGL((fcg_update_kernel<true, false>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, d, qv, s,
-                           first, inv_h2, rows, nr, e, dd, partials);
-    else
-        hipLaunchKernelGGL((fcg_update_kernel<false, false>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, d, qv,
+        hipLaunchKernelGGL((fcg_update_kernel<true, true>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, dold, dnew,
                            s, first, inv_h2, rows, nr, e, dd, partials);
+    else if (op)
+        hipLaunchKernelGGL((fcg_update_kernel<true, false>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, dold,
+                           dnew, s, first, inv_h2, rows, nr, e, dd, partials);
+    else
+        hipLaunchKernelGGL((fcg_update_kernel<false, false>), dim3(nb), dim3(kT), 0, st, L, op, z, x, r, r32, dold,
+                           dnew, s, first, inv_h2, rows, nr, e, dd, partials);
     hipLaunchKernelGGL(fcg_scalars_kernel, dim3(1), dim3(kScalarThreads), 0, st, partials, nb, 2, 0, s);
 }
 

@@ -99,7 +99,8 @@ struct clq_ctx {
     /* mixed-precision solve state, allocated on first use */
     double *p64a, *p64b;
     float *r32;
-    double *r64, *dir0, *dir1, *q64; /* flexible CG */
+    double *r64;                     /* flexible CG: residual */
+    float *dir0, *dir1;              /* FP32 directions, old and new */
     int p64_valid;                    /* p64a holds the pressure of the last solve (modes 2-4) */
     int bins_current; /* counts/offsets describe the current particle order */
     /* particle sources (clq_set_sources) */
@@ -200,7 +201,7 @@ int clq_destroy(clq_ctx *c) {
         free(L->b16);
     }
     free(c->p64a); free(c->p64b); free(c->r32);
-    free(c->r64); free(c->dir0); free(c->dir1); free(c->q64);
+    free(c->r64); free(c->dir0); free(c->dir1);
     free(c->src_type); free(c->src_density); free(c->src_velocity); free(c->src_open); free(c->sink);
     free(c);
     return CLQ_OK;
@@ -1113,15 +1114,15 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
     }
     if (!c->r64) {
         c->r64 = xcalloc(c->C, sizeof(double));
-        c->dir0 = xcalloc(c->C, sizeof(double));
-        c->dir1 = xcalloc(c->C, sizeof(double));
-        c->q64 = xcalloc(c->C, sizeof(double));
+        c->dir0 = xcalloc(c->C, sizeof(float));
+        c->dir1 = xcalloc(c->C, sizeof(float));
     }
-    if (!c->p64a || !c->r64 || !c->dir0 || !c->dir1 || !c->q64) return fail(CLQ_ERR_NOMEM, "FCG buffers");
+    if (!c->p64a || !c->r64 || !c->dir0 || !c->dir1) return fail(CLQ_ERR_NOMEM, "FCG buffers");
     const level_t *L = &c->lv[0];
     const double h = (double)(float)c->dx;
     const double inv_h2 = 1.0 / (h * h);
-    double *x = c->p64a, *r = c->r64, *q = c->q64, *dold = c->dir0, *dnew = c->dir1;
+    double *x = c->p64a, *r = c->r64;
+    float *dold = c->dir0, *dnew = c->dir1;
     /* x = p (fluid), r = b - A x */
     double rr = 0.0, bb = 0.0;
     for (long long t = 0; t < c->C; ++t) x[t] = fluid_at(L, t) ? (double)c->p[t] : 0.0;
@@ -1145,14 +1146,15 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
      * initialisation, or the residual recomputed from x after a pass whose
      * recurrence met the tolerance (or ran out of cycles). */
     while (rn > sp->tolerance * bn && s->cycles < sp->max_cycles) {
-        /* The direction recurrence streams with the update: the dots of z
-         * give beta = -(z.q)/(d.q)_old and the new direction's
-         * d.q = z.w + 2 beta z.q + beta^2 (d.q)_old with w = A z (A
-         * symmetric, so d_old . A z = z . q_old), hence alpha = z.r / d.q
-         * before d and q = w + beta q are formed. */
+        /* The direction streams with the update: the dots of z give
+         * beta = -(z.A d_old)/(d.A d)_old and the new direction's
+         * d.A d = z.w + 2 beta z.A d_old + beta^2 (d.A d)_old with w = A z
+         * (A symmetric, so z.A d_old = w.d_old), hence alpha = z.r / d.A d
+         * before d = fmaf(float(beta), d_old, z) is formed; q = A d is then
+         * applied to that stored FP32 direction, so x and r move by alpha d
+         * and alpha A d of the same d. */
         double alpha = 0.0, beta = 0.0, dq = 0.0;
         int first = 1;
-        double *w = dnew;
         precondition_wcycle(c, sp, rn);
         s->cycles++;
         s->fallback_cycles += plain;
@@ -1161,29 +1163,29 @@ static int solve_fcg(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *s)
 #pragma omp parallel for schedule(static) reduction(+ : zq, zr, zw)
             for (long long t = 0; t < c->C; ++t) {
                 const double wv = fluid_at(L, t) ? apply_a64_f32(L, t, c->p, inv_h2) : 0.0;
-                w[t] = wv;
-                zq += (double)c->p[t] * q[t];
+                zq += wv * (double)dold[t];
                 zr += (double)c->p[t] * r[t];
                 zw += (double)c->p[t] * wv;
             }
             beta = (first || dq == 0.0) ? 0.0 : -zq / dq;
             dq = first ? zw : zw + 2.0 * beta * zq + beta * beta * dq;
             alpha = dq != 0.0 ? zr / dq : 0.0;
+            const float beta32 = (float)beta;
+#pragma omp parallel for schedule(static)
+            for (long long t = 0; t < c->C; ++t)
+                dnew[t] = !fluid_at(L, t) ? 0.0f : first ? c->p[t] : fmaf(beta32, dold[t], c->p[t]);
             rr = 0.0;
 #pragma omp parallel for schedule(static) reduction(+ : rr)
             for (long long t = 0; t < c->C; ++t) {
-                double dv = 0.0, qn = 0.0;
-                if (fluid_at(L, t)) {
-                    dv = first ? (double)c->p[t] : (double)c->p[t] + beta * dold[t];
-                    qn = first ? w[t] : w[t] + beta * q[t];
-                }
-                dold[t] = dv;
-                q[t] = qn;
-                x[t] = x[t] + alpha * dv;
+                const double qn = fluid_at(L, t) ? apply_a64_f32(L, t, dnew, inv_h2) : 0.0;
+                x[t] = x[t] + alpha * (double)dnew[t];
                 r[t] = r[t] - alpha * qn;
                 c->r32[t] = (float)r[t];
                 rr += r[t] * r[t];
             }
+            float *tmp = dold;
+            dold = dnew;
+            dnew = tmp;
             rn = sqrt(rr);
             if (!isfinite(rn)) return non_finite(rn);
             if (rn <= sp->tolerance * bn || s->cycles >= sp->max_cycles) break;
