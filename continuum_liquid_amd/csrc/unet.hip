@@ -178,62 +178,167 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(const void *__restrict__
 }
 
 // ---- weight gradient ------------------------------------------------------------------------
-// Block b covers voxels [b*kWChunk, (b+1)*kWChunk) and writes its partial
-// dW[tap][ci][co] (FP32).  GEMM: A = X^T (ci x voxels), B = Z (voxels x co).
-constexpr int kWChunk = 1024;
+// dW[tap][ci][co] = sum_v X[v + d(tap)][ci] Z[v][co] for the 16 input channels
+// c0 .. c0 + 15 of one launch: per tap the GEMM D (ci x co) += A (ci x 16
+// voxels) B (16 voxels x co) on v_mfma_f32_16x16x16_bf16, K = the 16 voxels
+// of one z-row of a 4 x 4 x 16 brick.  A persistent block walks the bricks
+// blockIdx.x, blockIdx.x + gridDim.x, ...; for each it stages the 6 x 6 x 18
+// input brick with its halo in LDS as [ci][voxel] (zeros off the grid) and
+// Z of the brick's 256 voxels as [co][voxel] (dL/dY masked by Y > 0, BF16),
+// and wave w accumulates the taps w, w + 4, ... for all CO output channels in
+// registers over all the block's bricks.  The block writes its partial
+// dW[tap][ci - c0][co] at the end; the partials are summed over the blocks
+// in a fixed order (reduce_wgrad_kernel).
+constexpr int kWBlocks = 512;                  // persistent blocks (at most)
+constexpr int kXRow = kHalo + 2;               // LDS row of one input channel (odd dword count)
+constexpr int kZRow = kBX * kBY * kBZ + 4;     // LDS row of one output channel (8-byte aligned)
+constexpr int kTapsPerWave = 7;                // ceil(27 / 4)
+
+inline int wgrad_bricks(const Dims &d) {
+    return ((d.X + kBX - 1) / kBX) * ((d.Y + kBY - 1) / kBY) * ((d.Z + kBZ - 1) / kBZ);
+}
+inline int wgrad_blocks(const Dims &d) { return std::min(kWBlocks, wgrad_bricks(d)); }
 
 template <int CI, int CO>
 __global__ __launch_bounds__(256) void conv3_wgrad_kernel(const uint16_t *__restrict__ x, const float *__restrict__ gy,
                                                           const uint16_t *__restrict__ y, float *__restrict__ partials,
-                                                          Dims d) {
-    constexpr int TI = CI / 16, TO = CO / 16, TILES = 27 * TI * TO;
+                                                          Dims d, int c0) {
+    constexpr int TO = CO / 16;
+    __shared__ __align__(16) uint16_t xs[16 * kXRow];
+    __shared__ __align__(16) uint16_t zs[CO * kZRow];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = lane & 15, kg = lane >> 4;
-    const int64_t vbeg = (int64_t)blockIdx.x * kWChunk;
-    const int64_t vend = min(vbeg + (int64_t)kWChunk, d.V);
-    float *dst = partials + (int64_t)blockIdx.x * 27 * CI * CO;
-    for (int tile = wave; tile < TILES; tile += 4) {
-        const int tap = tile / (TI * TO), rem = tile % (TI * TO), ti = rem / TO, to = rem % TO;
-        const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-        float4_t acc = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
-        for (int64_t k0 = vbeg; k0 < vend; k0 += 16) {
-            short4_t a, b;
+    const int nbx = (d.X + kBX - 1) / kBX, nby = (d.Y + kBY - 1) / kBY, nbz = (d.Z + kBZ - 1) / kBZ;
+    const int nb = nbx * nby * nbz;
+    float4_t acc[kTapsPerWave][TO];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int64_t v = k0 + kg * 4 + j;
-                uint16_t av = 0, bv = 0;
-                if (v < vend) {
-                    int px, py, pz;
-                    vox(d, v, px, py, pz);
-                    const int nx = px + dx, ny = py + dy, nz = pz + dz;
-                    if (nx >= 0 && nx < d.X && ny >= 0 && ny < d.Y && nz >= 0 && nz < d.Z)
-                        av = x[(((int64_t)nx * d.Y + ny) * d.Z + nz) * CI + ti * 16 + m];
-                    const int co = to * 16 + m;
-                    bv = bf2f(y[v * CO + co]) > 0.0f ? f2bf(gy[v * CO + co]) : (uint16_t)0;
-                }
-                a[j] = (short)av;
-                b[j] = (short)bv;
+    for (int tt = 0; tt < kTapsPerWave; ++tt)
+#pragma unroll
+        for (int t = 0; t < TO; ++t) acc[tt][t] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int brick = blockIdx.x; brick < nb; brick += gridDim.x) {
+        const int bz = brick % nbz, bxy = brick / nbz, by = bxy % nby, bx = bxy / nby;
+        const int x0 = bx * kBX, y0 = by * kBY, z0 = bz * kBZ;
+        __syncthreads();  // the previous brick's fragments have been read
+        // input brick with halo, 8 channels per item, transposed to [ci][voxel]
+        for (int item = threadIdx.x; item < kHalo * 2; item += 256) {
+            const int hv = item >> 1, half = item & 1;
+            const int hz = hv % kHZ, hxy = hv / kHZ, hy = hxy % kHY, hx = hxy / kHY;
+            const int gx = x0 - 1 + hx, gyy = y0 - 1 + hy, gz = z0 - 1 + hz;
+            uint4 val = make_uint4(0u, 0u, 0u, 0u);
+            if (gx >= 0 && gx < d.X && gyy >= 0 && gyy < d.Y && gz >= 0 && gz < d.Z)
+                val = *reinterpret_cast<const uint4 *>(x + (((int64_t)gx * d.Y + gyy) * d.Z + gz) * CI + c0 + half * 8);
+            const uint32_t w4[4] = {val.x, val.y, val.z, val.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                xs[(half * 8 + 2 * q) * kXRow + hv] = (uint16_t)(w4[q] & 0xffffu);
+                xs[(half * 8 + 2 * q + 1) * kXRow + hv] = (uint16_t)(w4[q] >> 16);
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
         }
-        // D[row = ci][col = co]: the lane holds column m, rows kg*4 + j
+        // Z of the brick's voxels, 8 channels per item, transposed to [co][voxel]
+        for (int item = threadIdx.x; item < kBX * kBY * kBZ * (CO / 8); item += 256) {
+            const int bv = item / (CO / 8), cg = item - bv * (CO / 8);
+            const int lz = bv % kBZ, lxy = bv / kBZ, ly = lxy % kBY, lx = lxy / kBY;
+            const int gx = x0 + lx, gyy = y0 + ly, gz = z0 + lz;
+            uint16_t zv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (gx < d.X && gyy < d.Y && gz < d.Z) {
+                const int64_t off = (((int64_t)gx * d.Y + gyy) * d.Z + gz) * CO + cg * 8;
+                const float4 g0 = *reinterpret_cast<const float4 *>(gy + off);
+                const float4 g1 = *reinterpret_cast<const float4 *>(gy + off + 4);
+                const uint4 yy = *reinterpret_cast<const uint4 *>(y + off);
+                const float gv[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+                const uint32_t y4[4] = {yy.x, yy.y, yy.z, yy.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int ci = ti * 16 + kg * 4 + j, co = to * 16 + m;
-            dst[((int64_t)tap * CI + ci) * CO + co] = acc[j];
+                for (int q = 0; q < 8; ++q) {
+                    const uint16_t yq = (uint16_t)(q & 1 ? y4[q >> 1] >> 16 : y4[q >> 1] & 0xffffu);
+                    zv[q] = bf2f(yq) > 0.0f ? f2bf(gv[q]) : (uint16_t)0;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) zs[(cg * 8 + q) * kZRow + bv] = zv[q];
         }
+        __syncthreads();
+        // z-row r = (lx, ly) of the brick: K = its 16 voxels
+        for (int r = 0; r < kBX * kBY; ++r) {
+            const int lx = r / kBY, ly = r - lx * kBY;
+            short4_t b[TO];
+#pragma unroll
+            for (int t = 0; t < TO; ++t)
+                b[t] = *reinterpret_cast<const short4_t *>(zs + (t * 16 + m) * kZRow + r * kBZ + kg * 4);
+#pragma unroll
+            for (int tt = 0; tt < kTapsPerWave; ++tt) {
+                const int tap = wave + 4 * tt;
+                if (tap < 27) {  // wave-uniform
+                    const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+                    const int hv = ((lx + 1 + dx) * kHY + (ly + 1 + dy)) * kHZ + 1 + dz + kg * 4;
+                    const uint16_t *row = xs + m * kXRow + hv;
+                    const short4_t a = short4_t{(short)row[0], (short)row[1], (short)row[2], (short)row[3]};
+#pragma unroll
+                    for (int t = 0; t < TO; ++t)
+                        acc[tt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[tt][t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D[row = ci][col = co]: the lane holds column m, rows kg*4 + j
+    float *dst = partials + (int64_t)blockIdx.x * 27 * 16 * CO;
+#pragma unroll
+    for (int tt = 0; tt < kTapsPerWave; ++tt) {
+        const int tap = wave + 4 * tt;
+        if (tap >= 27) continue;
+#pragma unroll
+        for (int t = 0; t < TO; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) dst[((int64_t)tap * 16 + kg * 4 + j) * CO + t * 16 + m] = acc[tt][t][j];
     }
 }
 
-// dst[i] (+)= sum_b partials[b * n + i], blocks summed in order.
-__global__ void reduce_blocks_kernel(const float *__restrict__ partials, int blocks, int64_t n, float *__restrict__ dst,
-                                     int accumulate) {
+// dW[tap][c0 + ci][co] (+)= sum_b partials[b][tap][ci][co], blocks in order
+__global__ void reduce_wgrad_kernel(const float *__restrict__ partials, int blocks, int CI, int CO, int c0,
+                                    float *__restrict__ dw, int accumulate) {
+    const int64_t n = 27LL * 16 * CO;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     float s = 0.0f;
     for (int b = 0; b < blocks; ++b) s += partials[(int64_t)b * n + i];
-    dst[i] = accumulate ? dst[i] + s : s;
+    const int tap = (int)(i / (16 * CO)), rem = (int)(i % (16 * CO)), ci = rem / CO, co = rem % CO;
+    float &o = dw[((int64_t)tap * CI + c0 + ci) * CO + co];
+    o = accumulate ? o + s : s;
 }
+
+// Sums over voxels of a [V][C] tensor in a fixed order: block b adds the
+// elements of its range of whole voxels, thread t the channel t % C (C
+// divides 256), and writes partials[b][c]; reduce_channel_kernel adds the
+// blocks.
+constexpr int kSumBlocks = 256;
+
+// dbias[co] partials: sum_v Z[v][co], Z = dL/dY where Y > 0 (FP32)
+__global__ __launch_bounds__(256) void bias_grad_partials_kernel(const float *__restrict__ gy,
+                                                                 const uint16_t *__restrict__ y, int64_t V, int CO,
+                                                                 float *__restrict__ partials) {
+    __shared__ float sh[256];
+    const int64_t per = (V + gridDim.x - 1) / gridDim.x;
+    const int64_t e0 = (int64_t)blockIdx.x * per * CO, e1 = min((int64_t)(blockIdx.x + 1) * per, V) * CO;
+    float s = 0.0f;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) s += bf2f(y[e]) > 0.0f ? gy[e] : 0.0f;
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off >= CO; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < CO) partials[(int64_t)blockIdx.x * CO + threadIdx.x] = sh[threadIdx.x];
+}
+
+// dst[c] (+)= sum_b partials[b][c] for c < C, blocks in order
+__global__ void reduce_channel_kernel(const float *__restrict__ partials, int blocks, int C, float *__restrict__ dst,
+                                      int accumulate) {
+    const int c = threadIdx.x;
+    if (c >= C) return;
+    float s = 0.0f;
+    for (int b = 0; b < blocks; ++b) s += partials[(int64_t)b * C + c];
+    dst[c] = accumulate ? dst[c] + s : s;
+}
+
 
 __device__ __forceinline__ float block_sum(float s) {
     __shared__ float sh[256];
@@ -244,16 +349,6 @@ __device__ __forceinline__ float block_sum(float s) {
         __syncthreads();
     }
     return sh[0];
-}
-
-// dbias[co] (+)= sum_v Z[v][co]; one block per output channel.
-__global__ __launch_bounds__(256) void bias_grad_kernel(const float *__restrict__ gy, const uint16_t *__restrict__ y,
-                                                        int64_t V, int CO, float *__restrict__ db, int accumulate) {
-    const int co = blockIdx.x;
-    float s = 0.0f;
-    for (int64_t v = threadIdx.x; v < V; v += 256) s += bf2f(y[v * CO + co]) > 0.0f ? gy[v * CO + co] : 0.0f;
-    s = block_sum(s);
-    if (threadIdx.x == 0) db[co] = accumulate ? db[co] + s : s;
 }
 
 // ---- pooling, upsampling with concatenation ------------------------------------------------
@@ -482,15 +577,32 @@ __global__ void head_bwd_data_kernel(const float *__restrict__ g, const float *_
     gy[i] = g[i / 16] * w[i % 16];
 }
 
-// dw[c] (+)= sum_v g[v] y[v][c] (c < 16), dw[16] (+)= sum_v g[v]; one block per entry
-__global__ __launch_bounds__(256) void head_bwd_weight_kernel(const float *__restrict__ g,
-                                                              const uint16_t *__restrict__ y, int64_t V,
-                                                              float *__restrict__ dw, int accumulate) {
-    const int c = blockIdx.x;
-    float s = 0.0f;
-    for (int64_t v = threadIdx.x; v < V; v += 256) s += g[v] * (c < 16 ? bf2f(y[v * 16 + c]) : 1.0f);
-    s = block_sum(s);
-    if (threadIdx.x == 0) dw[c] = accumulate ? dw[c] + s : s;
+// head weight gradient partials: sum_v g[v] y[v][c] (c < 16) and sum_v g[v]
+// (c = 16), summed like bias_grad_partials_kernel (thread t: channel t % 16)
+__global__ __launch_bounds__(256) void head_bwd_weight_partials_kernel(const float *__restrict__ g,
+                                                                       const uint16_t *__restrict__ y, int64_t V,
+                                                                       float *__restrict__ partials) {
+    __shared__ float sh[256], shb[256];
+    const int64_t per = (V + gridDim.x - 1) / gridDim.x;
+    const int64_t e0 = (int64_t)blockIdx.x * per * 16, e1 = min((int64_t)(blockIdx.x + 1) * per, V) * 16;
+    float s = 0.0f, sb = 0.0f;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+        const float gv = g[e >> 4];
+        s += gv * bf2f(y[e]);
+        if ((threadIdx.x & 15) == 0) sb += gv;
+    }
+    sh[threadIdx.x] = s;
+    shb[threadIdx.x] = sb;
+    __syncthreads();
+    for (int off = 128; off >= 16; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh[threadIdx.x] += sh[threadIdx.x + off];
+            shb[threadIdx.x] += shb[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 16) partials[(int64_t)blockIdx.x * 17 + threadIdx.x] = sh[threadIdx.x];
+    if (threadIdx.x == 0) partials[(int64_t)blockIdx.x * 17 + 16] = shb[0];
 }
 
 // Adam with bias correction, FP32 master weights.
@@ -536,7 +648,6 @@ int64_t param_count() {
 }
 
 inline dim3 g1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
-inline int wgrad_blocks(const Dims &d) { return (int)((d.V + kWChunk - 1) / kWChunk); }
 
 }  // namespace
 }  // namespace clq
@@ -623,10 +734,13 @@ template <int CI, int CO>
 void conv_wgrad(const uint16_t *x, const float *gy, const uint16_t *y, float *partials, float *dw, float *db, Dims d,
                 int accumulate, hipStream_t s) {
     const int blocks = wgrad_blocks(d);
-    hipLaunchKernelGGL((conv3_wgrad_kernel<CI, CO>), dim3(blocks), dim3(256), 0, s, x, gy, y, partials, d);
-    const int64_t n = 27LL * CI * CO;
-    hipLaunchKernelGGL(reduce_blocks_kernel, g1(n), dim3(256), 0, s, partials, blocks, n, dw, accumulate);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(CO), dim3(256), 0, s, gy, y, d.V, CO, db, accumulate);
+    for (int c0 = 0; c0 < CI; c0 += 16) {
+        hipLaunchKernelGGL((conv3_wgrad_kernel<CI, CO>), dim3(blocks), dim3(256), 0, s, x, gy, y, partials, d, c0);
+        hipLaunchKernelGGL(reduce_wgrad_kernel, g1(27LL * 16 * CO), dim3(256), 0, s, partials, blocks, CI, CO, c0, dw,
+                           accumulate);
+    }
+    hipLaunchKernelGGL(bias_grad_partials_kernel, dim3(kSumBlocks), dim3(256), 0, s, gy, y, d.V, CO, partials);
+    hipLaunchKernelGGL(reduce_channel_kernel, dim3(1), dim3(64), 0, s, partials, kSumBlocks, CO, db, accumulate);
 }
 
 void normalise_and_pack(clq_net *n, const float *rhs, const float *mask, const float *prev, hipStream_t s) {
@@ -667,8 +781,8 @@ void backward(clq_net *n, int acc, hipStream_t s) {
     // the weight gradients of c7 and c9 read their concatenated inputs
     hipLaunchKernelGGL(upcat_fwd_kernel, g1(d1.V * 96), dim3(256), 0, s, n->a6, n->s1, n->u1, d1, d2, 64, 32);
     hipLaunchKernelGGL(upcat_fwd_kernel, g1(d0.V * 48), dim3(256), 0, s, n->a8, n->s0, n->u0, d0, d1, 32, 16);
-    hipLaunchKernelGGL(head_bwd_weight_kernel, dim3(17), dim3(256), 0, s, n->gpred, n->a10, d0.V, gw + n->head_off,
-                       acc);
+    hipLaunchKernelGGL(head_bwd_weight_partials_kernel, dim3(kSumBlocks), dim3(256), 0, s, n->gpred, n->a10, d0.V, P);
+    hipLaunchKernelGGL(reduce_channel_kernel, dim3(1), dim3(64), 0, s, P, kSumBlocks, 17, gw + n->head_off, acc);
     hipLaunchKernelGGL(head_bwd_data_kernel, g1(d0.V * 16), dim3(256), 0, s, n->gpred, n->w + n->head_off, d0.V, A);
     // c10: a9 -> a10
     conv_wgrad<16, 16>(n->a9, A, n->a10, P, gw + n->woff[9], gw + n->boff[9], d0, acc, s);
@@ -717,9 +831,9 @@ int ensure_training(clq_net *n) {
     const int64_t V0 = n->d0.V, V1 = n->d1.V, V2 = n->d2.V;
     const int64_t gmax = std::max(std::max(V0 * 48, V1 * 96), V2 * 64);
     const Dims *lvl[3] = {&n->d0, &n->d1, &n->d2};
-    int64_t pmax = 0;
+    int64_t pmax = (int64_t)kSumBlocks * 64;  // bias and head partials
     for (int i = 0; i < kLayers; ++i)
-        pmax = std::max(pmax, (int64_t)wgrad_blocks(*lvl[kLayerLevel[i]]) * 27 * kLayerIO[i][0] * kLayerIO[i][1]);
+        pmax = std::max(pmax, (int64_t)wgrad_blocks(*lvl[kLayerLevel[i]]) * 27 * 16 * kLayerIO[i][1]);
     if (int rc = n->alloc_n(&n->g, n->nparams)) return rc;
     if (int rc = n->alloc_n(&n->m1, n->nparams)) return rc;
     if (int rc = n->alloc_n(&n->m2, n->nparams)) return rc;

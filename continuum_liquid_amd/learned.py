@@ -64,35 +64,106 @@ def collect_pairs(lib, shape, n_scenes: int, steps: int, seed: int = 0, dt: floa
         if scene.fluid_cells == 0:
             continue
         ctx = runtime.Context(lib, scene.shape, scene.dx, capacity=scene.particle_count)
-        ctx.set_particles(scene.positions, scene.velocities)
-        pending = {}
-        step_index = [0]
-
-        def observer(c, stats):
-            if step_index[0] < skip:
-                return
-            if stats is None:
-                pending["rhs"] = c.get_field("rhs")
-                pending["mask"] = c.get_field("fluid_fraction")
-                pending["prev"] = c.get_field("pressure")
-            else:
-                if stats["relative_residual"] > 1e-6:
-                    return  # keep converged solves only
-                out["rhs"].append(pending["rhs"])
-                out["mask"].append(pending["mask"])
-                out["prev"].append(pending["prev"])
-                out["target"].append(c.get_field("pressure_fp64").astype(np.float32))
-
-        params = host_solver.StepParams(dt=dt, gravity=scene.gravity, flip_ratio=flip_ratio,
-                                        solver=dict(TARGET_SOLVER), observer=observer)
-        for k in range(skip + steps):
-            step_index[0] = k
-            host_solver.step(ctx, params)
+        for k, v in scene_pairs(ctx, scene, steps, skip, dt, flip_ratio).items():
+            out[k].extend(v)
         ctx.close()
         if log:
             log(f"scene {s}: {scene.fluid_cells} fluid cells, {len(out['rhs'])} pairs so far")
     return PairSet(*(np.stack(out[k]) if out[k] else np.zeros((0,) + tuple(shape), np.float32)
                      for k in ("rhs", "mask", "prev", "target")))
+
+
+def scene_pairs(ctx, scene, steps: int, skip: int, dt: float = 1.0 / 240.0, flip_ratio: float = 0.95) -> dict:
+    """Run `skip + steps` steps of `scene` on the context (its particles
+    replace the context's) and return the pairs of the last `steps` solves
+    that converged, as lists of float32 arrays."""
+    out = {k: [] for k in ("rhs", "mask", "prev", "target")}
+    pending = {}
+    step_index = [0]
+
+    def observer(c, stats):
+        if step_index[0] < skip:
+            return
+        if stats is None:
+            pending["rhs"] = c.get_field("rhs")
+            pending["mask"] = c.get_field("fluid_fraction")
+            pending["prev"] = c.get_field("pressure")
+        elif stats["relative_residual"] <= 1e-6:  # keep converged solves only
+            for k in ("rhs", "mask", "prev"):
+                out[k].append(pending[k])
+            out["target"].append(c.get_field("pressure_fp64").astype(np.float32))
+
+    ctx.set_particles(scene.positions, scene.velocities)
+    params = host_solver.StepParams(dt=dt, gravity=scene.gravity, flip_ratio=flip_ratio,
+                                    solver=dict(TARGET_SOLVER), observer=observer)
+    for k in range(skip + steps):
+        step_index[0] = k
+        host_solver.step(ctx, params)
+    return out
+
+
+def train_streaming(net: "runtime.Network", lib, shape, total: int, window: int = 512, passes: int = 2,
+                    batch: int = 4, lr: float = 1e-3, steps: int = 36, skip: int = 4, seed: int = 0,
+                    prev_dropout: float = 0.5, log=None, checkpoint=None) -> list[float]:
+    """Training on `total` pairs generated on the fly, too many to hold at
+    once: consecutive seeded random scenes (seeds seed, seed + 1, ...) fill a
+    window of `window` pairs, the network takes `passes` shuffled passes of
+    Adam steps over it (prev_dropout as in `train`), and the window is
+    refilled.  The next scene is built on a host thread while the device runs
+    the current one.  `checkpoint(net)`, when given, is called after every
+    window.  Returns the mean loss of each window."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    shape = tuple(int(n) for n in shape)
+    keys = ("rhs", "mask", "prev", "target")
+    buf = {k: np.empty((window,) + shape, np.float32) for k in keys}
+    rng = np.random.default_rng(seed)
+    losses: list[float] = []
+    fill = seen = 0
+    t0 = time.perf_counter()
+
+    def train_window(n: int) -> None:
+        total_loss, steps_done = 0.0, 0
+        for _ in range(passes):
+            order = rng.permutation(n)
+            for i in range(0, n - batch + 1, batch):
+                idx = np.sort(order[i:i + batch])
+                prev = buf["prev"][idx].copy()
+                prev[rng.random(len(idx)) < prev_dropout] = 0.0
+                total_loss += net.train_step(buf["rhs"][idx], buf["mask"][idx], prev, buf["target"][idx], lr)
+                steps_done += 1
+        losses.append(total_loss / max(steps_done, 1))
+        if checkpoint is not None:
+            checkpoint(net)
+        if log:
+            log(f"{seen} pairs: window of {n}, loss {losses[-1]:.4e} ({time.perf_counter() - t0:.1f} s)")
+
+    ctx = None
+    scene_seed = seed
+    with ThreadPoolExecutor(1) as pool:
+        upcoming = pool.submit(scenes.random_scene, shape, scene_seed)
+        while seen < total:
+            scene = upcoming.result()
+            scene_seed += 1
+            upcoming = pool.submit(scenes.random_scene, shape, scene_seed)
+            if scene.fluid_cells == 0:
+                continue
+            if ctx is None:
+                ctx = runtime.Context(lib, scene.shape, scene.dx, capacity=scene.particle_count)
+            pairs = scene_pairs(ctx, scene, steps, skip)
+            for j in range(len(pairs["rhs"])):
+                for k in keys:
+                    buf[k][fill] = pairs[k][j]
+                fill += 1
+                seen += 1
+                if fill == window or seen == total:
+                    train_window(fill)
+                    fill = 0
+                if seen == total:
+                    break
+    if ctx is not None:
+        ctx.close()
+    return losses
 
 
 def train(net: "runtime.Network", data: PairSet, epochs: int, batch: int = 4, lr: float = 1e-3, seed: int = 0,
@@ -168,6 +239,10 @@ def main(argv=None) -> int:
     ap.add_argument("--holdout", type=int, default=2, help="scenes kept out of training for the report")
     ap.add_argument("--out", type=Path, default=DEFAULT_WEIGHTS)
     ap.add_argument("--pairs", type=Path, default=None, help="also save the pairs here (.npz)")
+    ap.add_argument("--stream", type=int, default=0, metavar="N",
+                    help="train on N pairs generated on the fly (window by window) instead of a stored set")
+    ap.add_argument("--window", type=int, default=512, help="--stream: pairs held at once")
+    ap.add_argument("--passes", type=int, default=2, help="--stream: passes over each window")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -176,6 +251,20 @@ def main(argv=None) -> int:
     lib = runtime.load_library()
     shape = tuple(args.shape)
     t0 = time.perf_counter()
+    if args.stream:
+        net = runtime.Network(lib, shape, 1.0 / max(shape), seed=args.seed)
+        # held out: scenes of seeds the stream never reaches
+        test_set = collect_pairs(lib, shape, args.holdout, 3, seed=10**6 + args.seed, skip=args.skip)
+        before = relative_error(net, test_set) if len(test_set) else float("nan")
+        train_streaming(net, lib, shape, args.stream, window=args.window, passes=args.passes, batch=args.batch,
+                        lr=args.lr, steps=args.steps, skip=args.skip, seed=args.seed, log=log,
+                        checkpoint=lambda trained: save_weights(trained, args.out))
+        after = relative_error(net, test_set) if len(test_set) else float("nan")
+        after_np = relative_error(net, test_set, with_prev=False) if len(test_set) else float("nan")
+        log(f"held-out relative error: untrained {before:.3e}, trained {after:.3e} "
+            f"(no previous pressure: {after_np:.3e}); {time.perf_counter() - t0:.1f} s")
+        log(f"weights written to {save_weights(net, args.out)}")
+        return 0
     data = collect_pairs(lib, shape, args.scenes, args.steps, seed=args.seed, skip=args.skip, log=log)
     log(f"{len(data)} pairs in {time.perf_counter() - t0:.1f} s")
     if args.pairs:
