@@ -80,6 +80,7 @@ __device__ __forceinline__ void vox(const Dims &d, int64_t v, int &x, int &y, in
 constexpr int kBX = 4, kBY = 4, kBZ = 16;
 constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
+constexpr int kStageIters = (kHalo * 2 + 255) / 256;  // staging items per thread
 
 // CA > 0 (forward only): the input is the concatenation [up(coarse), in] with
 // CA channels from the coarse tensor (dims dc, read at the parent voxel) and
@@ -100,37 +101,54 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(const void *__restrict__
     for (int mt = 0; mt < kBY; ++mt)
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[mt][t] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    // the bias of this lane's output channels, loaded once
+    float bco[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) bco[t] = MODE == 0 ? bias[t * 16 + m] : 0.0f;
+    // the 16 input channels c0.. of halo voxel hv, half `half` (8 channels)
+    auto stage_value = [&](int c0, int hv, int half) -> uint4 {
+        const int hz = hv % kHZ, hxy = hv / kHZ, hy = hxy % kHY, hx = hxy / kHY;
+        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+        uint4 val = make_uint4(0u, 0u, 0u, 0u);
+        if (gx >= 0 && gx < d.X && gy >= 0 && gy < d.Y && gz >= 0 && gz < d.Z) {
+            const int64_t vox_index = ((int64_t)gx * d.Y + gy) * d.Z + gz;
+            const int64_t off = vox_index * CI + c0 + half * 8;
+            if (CA > 0 && c0 < CA) {
+                const int64_t pv = ((int64_t)(gx >> 1) * dc.Y + (gy >> 1)) * dc.Z + (gz >> 1);
+                val = *reinterpret_cast<const uint4 *>(coarse + pv * CA + c0 + half * 8);
+            } else if (CA > 0) {
+                val = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(in) + vox_index * (CI - CA) +
+                                                       (c0 - CA) + half * 8);
+            } else if (MODE == 0) {
+                val = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(in) + off);
+            } else {
+                const float4 g0 = *reinterpret_cast<const float4 *>(static_cast<const float *>(in) + off);
+                const float4 g1 = *reinterpret_cast<const float4 *>(static_cast<const float *>(in) + off + 4);
+                const uint4 y = *reinterpret_cast<const uint4 *>(ymask + off);
+                auto z = [](uint32_t ypair, float ga, float gb) {
+                    const uint32_t lo = bf2f((uint16_t)(ypair & 0xffffu)) > 0.0f ? f2bf(ga) : 0u;
+                    const uint32_t hi = bf2f((uint16_t)(ypair >> 16)) > 0.0f ? f2bf(gb) : 0u;
+                    return lo | (hi << 16);
+                };
+                val = make_uint4(z(y.x, g0.x, g0.y), z(y.y, g0.z, g0.w), z(y.z, g1.x, g1.y), z(y.w, g1.z, g1.w));
+            }
+        }
+        return val;
+    };
     for (int c0 = 0; c0 < CI; c0 += 16) {
         __syncthreads();  // the previous chunk's fragments have been read
-        for (int item = threadIdx.x; item < kHalo * 2; item += 256) {
-            const int hv = item >> 1, half = item & 1;
-            const int hz = hv % kHZ, hxy = hv / kHZ, hy = hxy % kHY, hx = hxy / kHY;
-            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
-            uint4 val = make_uint4(0u, 0u, 0u, 0u);
-            if (gx >= 0 && gx < d.X && gy >= 0 && gy < d.Y && gz >= 0 && gz < d.Z) {
-                const int64_t vox_index = ((int64_t)gx * d.Y + gy) * d.Z + gz;
-                const int64_t off = vox_index * CI + c0 + half * 8;
-                if (CA > 0 && c0 < CA) {
-                    const int64_t pv = ((int64_t)(gx >> 1) * dc.Y + (gy >> 1)) * dc.Z + (gz >> 1);
-                    val = *reinterpret_cast<const uint4 *>(coarse + pv * CA + c0 + half * 8);
-                } else if (CA > 0) {
-                    val = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(in) + vox_index * (CI - CA) +
-                                                           (c0 - CA) + half * 8);
-                } else if (MODE == 0) {
-                    val = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(in) + off);
-                } else {
-                    const float4 g0 = *reinterpret_cast<const float4 *>(static_cast<const float *>(in) + off);
-                    const float4 g1 = *reinterpret_cast<const float4 *>(static_cast<const float *>(in) + off + 4);
-                    const uint4 y = *reinterpret_cast<const uint4 *>(ymask + off);
-                    auto z = [](uint32_t ypair, float ga, float gb) {
-                        const uint32_t lo = bf2f((uint16_t)(ypair & 0xffffu)) > 0.0f ? f2bf(ga) : 0u;
-                        const uint32_t hi = bf2f((uint16_t)(ypair >> 16)) > 0.0f ? f2bf(gb) : 0u;
-                        return lo | (hi << 16);
-                    };
-                    val = make_uint4(z(y.x, g0.x, g0.y), z(y.y, g0.z, g0.w), z(y.z, g1.x, g1.y), z(y.w, g1.z, g1.w));
-                }
-            }
-            *reinterpret_cast<uint4 *>(tile + hv * 16 + half * 8) = val;
+        // every item's loads first, then the LDS stores (one memory round
+        // trip for the chunk's staging)
+        uint4 staged[kStageIters];
+#pragma unroll
+        for (int it = 0; it < kStageIters; ++it) {
+            const int item = threadIdx.x + it * 256;
+            staged[it] = item < kHalo * 2 ? stage_value(c0, item >> 1, item & 1) : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int it = 0; it < kStageIters; ++it) {
+            const int item = threadIdx.x + it * 256;
+            if (item < kHalo * 2) *reinterpret_cast<uint4 *>(tile + (item >> 1) * 16 + (item & 1) * 8) = staged[it];
         }
         __syncthreads();
         for (int tap = 0; tap < 27; ++tap) {
@@ -170,7 +188,7 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(const void *__restrict__
                 const int64_t v = ((int64_t)ox * d.Y + oy) * d.Z + oz;
                 const int co = t * 16 + m;
                 if (MODE == 0)
-                    static_cast<uint16_t *>(out)[v * CO + co] = f2bf(fmaxf(acc[mt][t][j] + bias[co], 0.0f));
+                    static_cast<uint16_t *>(out)[v * CO + co] = f2bf(fmaxf(acc[mt][t][j] + bco[t], 0.0f));
                 else
                     static_cast<float *>(out)[v * CO + co] = acc[mt][t][j];
             }
@@ -189,13 +207,16 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(const void *__restrict__
 // registers over all the block's bricks.  The block writes its partial
 // dW[tap][ci - c0][co] at the end; the partials are summed over the blocks
 // in a fixed order (reduce_wgrad_kernel).
+constexpr int kWX = 4, kWY = 4, kWZ = 16;        // brick of one step
+constexpr int kWHX = kWX + 2, kWHY = kWY + 2, kWHZ = kWZ + 2;
+constexpr int kWHalo = kWHX * kWHY * kWHZ;
 constexpr int kWBlocks = 512;                  // persistent blocks (at most)
-constexpr int kXRow = kHalo + 2;               // LDS row of one input channel (odd dword count)
-constexpr int kZRow = kBX * kBY * kBZ + 4;     // LDS row of one output channel (8-byte aligned)
+constexpr int kXRow = kWHalo + 2;               // LDS row of one input channel (odd dword count)
+constexpr int kZRow = kWX * kWY * kWZ + 4;     // LDS row of one output channel (8-byte aligned)
 constexpr int kTapsPerWave = 7;                // ceil(27 / 4)
 
 inline int wgrad_bricks(const Dims &d) {
-    return ((d.X + kBX - 1) / kBX) * ((d.Y + kBY - 1) / kBY) * ((d.Z + kBZ - 1) / kBZ);
+    return ((d.X + kWX - 1) / kWX) * ((d.Y + kWY - 1) / kWY) * ((d.Z + kWZ - 1) / kWZ);
 }
 inline int wgrad_blocks(const Dims &d) { return std::min(kWBlocks, wgrad_bricks(d)); }
 
@@ -208,7 +229,7 @@ __global__ __launch_bounds__(256) void conv3_wgrad_kernel(const uint16_t *__rest
     __shared__ __align__(16) uint16_t zs[CO * kZRow];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m = lane & 15, kg = lane >> 4;
-    const int nbx = (d.X + kBX - 1) / kBX, nby = (d.Y + kBY - 1) / kBY, nbz = (d.Z + kBZ - 1) / kBZ;
+    const int nbx = (d.X + kWX - 1) / kWX, nby = (d.Y + kWY - 1) / kWY, nbz = (d.Z + kWZ - 1) / kWZ;
     const int nb = nbx * nby * nbz;
     float4_t acc[kTapsPerWave][TO];
 #pragma unroll
@@ -217,12 +238,12 @@ __global__ __launch_bounds__(256) void conv3_wgrad_kernel(const uint16_t *__rest
         for (int t = 0; t < TO; ++t) acc[tt][t] = float4_t{0.0f, 0.0f, 0.0f, 0.0f};
     for (int brick = blockIdx.x; brick < nb; brick += gridDim.x) {
         const int bz = brick % nbz, bxy = brick / nbz, by = bxy % nby, bx = bxy / nby;
-        const int x0 = bx * kBX, y0 = by * kBY, z0 = bz * kBZ;
+        const int x0 = bx * kWX, y0 = by * kWY, z0 = bz * kWZ;
         __syncthreads();  // the previous brick's fragments have been read
         // input brick with halo, 8 channels per item, transposed to [ci][voxel]
-        for (int item = threadIdx.x; item < kHalo * 2; item += 256) {
+        for (int item = threadIdx.x; item < kWHalo * 2; item += 256) {
             const int hv = item >> 1, half = item & 1;
-            const int hz = hv % kHZ, hxy = hv / kHZ, hy = hxy % kHY, hx = hxy / kHY;
+            const int hz = hv % kWHZ, hxy = hv / kWHZ, hy = hxy % kWHY, hx = hxy / kWHY;
             const int gx = x0 - 1 + hx, gyy = y0 - 1 + hy, gz = z0 - 1 + hz;
             uint4 val = make_uint4(0u, 0u, 0u, 0u);
             if (gx >= 0 && gx < d.X && gyy >= 0 && gyy < d.Y && gz >= 0 && gz < d.Z)
@@ -235,9 +256,9 @@ __global__ __launch_bounds__(256) void conv3_wgrad_kernel(const uint16_t *__rest
             }
         }
         // Z of the brick's voxels, 8 channels per item, transposed to [co][voxel]
-        for (int item = threadIdx.x; item < kBX * kBY * kBZ * (CO / 8); item += 256) {
+        for (int item = threadIdx.x; item < kWX * kWY * kWZ * (CO / 8); item += 256) {
             const int bv = item / (CO / 8), cg = item - bv * (CO / 8);
-            const int lz = bv % kBZ, lxy = bv / kBZ, ly = lxy % kBY, lx = lxy / kBY;
+            const int lz = bv % kWZ, lxy = bv / kWZ, ly = lxy % kWY, lx = lxy / kWY;
             const int gx = x0 + lx, gyy = y0 + ly, gz = z0 + lz;
             uint16_t zv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             if (gx < d.X && gyy < d.Y && gz < d.Z) {
@@ -258,18 +279,18 @@ __global__ __launch_bounds__(256) void conv3_wgrad_kernel(const uint16_t *__rest
         }
         __syncthreads();
         // z-row r = (lx, ly) of the brick: K = its 16 voxels
-        for (int r = 0; r < kBX * kBY; ++r) {
-            const int lx = r / kBY, ly = r - lx * kBY;
+        for (int r = 0; r < kWX * kWY; ++r) {
+            const int lx = r / kWY, ly = r - lx * kWY;
             short4_t b[TO];
 #pragma unroll
             for (int t = 0; t < TO; ++t)
-                b[t] = *reinterpret_cast<const short4_t *>(zs + (t * 16 + m) * kZRow + r * kBZ + kg * 4);
+                b[t] = *reinterpret_cast<const short4_t *>(zs + (t * 16 + m) * kZRow + r * kWZ + kg * 4);
 #pragma unroll
             for (int tt = 0; tt < kTapsPerWave; ++tt) {
                 const int tap = wave + 4 * tt;
                 if (tap < 27) {  // wave-uniform
                     const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-                    const int hv = ((lx + 1 + dx) * kHY + (ly + 1 + dy)) * kHZ + 1 + dz + kg * 4;
+                    const int hv = ((lx + 1 + dx) * kWHY + (ly + 1 + dy)) * kWHZ + 1 + dz + kg * 4;
                     const uint16_t *row = xs + m * kXRow + hv;
                     const short4_t a = short4_t{(short)row[0], (short)row[1], (short)row[2], (short)row[3]};
 #pragma unroll
