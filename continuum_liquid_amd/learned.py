@@ -143,29 +143,31 @@ def train_streaming(net: "runtime.Network", lib, shape, total: int, window: int 
 
     ctx = None
     scene_seed = seed
-    with ThreadPoolExecutor(1) as pool:
-        upcoming = pool.submit(scenes.random_scene, shape, scene_seed)
-        while seen < total:
-            scene = upcoming.result()
-            scene_seed += 1
+    try:
+        with ThreadPoolExecutor(1) as pool:
             upcoming = pool.submit(scenes.random_scene, shape, scene_seed)
-            if scene.fluid_cells == 0:
-                continue
-            if ctx is None:
-                ctx = runtime.Context(lib, scene.shape, scene.dx, capacity=scene.particle_count)
-            pairs = scene_pairs(ctx, scene, steps, skip)
-            for j in range(len(pairs["rhs"])):
-                for k in keys:
-                    buf[k][fill] = pairs[k][j]
-                fill += 1
-                seen += 1
-                if fill == window or seen == total:
-                    train_window(fill)
-                    fill = 0
-                if seen == total:
-                    break
-    if ctx is not None:
-        ctx.close()
+            while seen < total:
+                scene = upcoming.result()
+                scene_seed += 1
+                upcoming = pool.submit(scenes.random_scene, shape, scene_seed)
+                if scene.fluid_cells == 0:
+                    continue
+                if ctx is None:
+                    ctx = runtime.Context(lib, scene.shape, scene.dx, capacity=scene.particle_count)
+                pairs = scene_pairs(ctx, scene, steps, skip)
+                for j in range(len(pairs["rhs"])):
+                    for k in keys:
+                        buf[k][fill] = pairs[k][j]
+                    fill += 1
+                    seen += 1
+                    if fill == window or seen == total:
+                        train_window(fill)
+                        fill = 0
+                    if seen == total:
+                        break
+    finally:
+        if ctx is not None:
+            ctx.close()
     return losses
 
 

@@ -290,6 +290,27 @@ def test_training_step_follows_adam_and_reduces_loss(hip_lib):
 
 
 @pytest.mark.gpu
+def test_streaming_training_windows_and_checkpoints(hip_lib):
+    """The streaming trainer fills windows from consecutive seeded scenes,
+    trains on each and calls the checkpoint once per window; the same seed
+    gives the same parameters."""
+    from continuum_liquid_amd import learned
+
+    shape = (16, 16, 16)
+    runs = []
+    for _ in range(2):
+        net = runtime.Network(hip_lib, shape, 1 / 16, seed=3)
+        seen = []
+        losses = learned.train_streaming(net, hip_lib, shape, total=20, window=8, passes=1, steps=4, skip=1,
+                                         seed=5, checkpoint=lambda n, k: seen.append(k))
+        assert len(losses) == 3 and all(np.isfinite(losses))
+        assert seen == [8, 16, 20]
+        runs.append(net.get_params())
+    assert np.array_equal(runs[0], runs[1])
+    assert not np.array_equal(runs[0], runtime.Network(hip_lib, shape, 1 / 16, seed=3).get_params())
+
+
+@pytest.mark.gpu
 def test_attach_rejects_a_network_of_another_grid(hip_lib):
     scene = scenes.small_scene(16)
     ctx = make_context(hip_lib, scene)
