@@ -191,8 +191,11 @@ struct BottomArgs {
 };
 void launch_wcycle_bottom(const BottomArgs &a, hipStream_t s);
 int resident_bottom_fits(const Level &B, const Level &K);
+// rows / count: B's rows holding an interior fluid cell (launch_bottom_rows),
+// with every other row of B at p = +0; NULL: all of B
 void launch_bottom_resident(const Level &B, const Level &K, float delta, bool f64, int pre, int post, int coarse_iters,
-                            int coarse_calls, hipStream_t s);
+                            int coarse_calls, const int32_t *rows, const int32_t *count, hipStream_t s);
+void launch_bottom_rows(const Level &B, int32_t *rows, int32_t *count, hipStream_t s);
 // active rows (cells (i, j, *) holding an interior cell with mask > 0)
 void launch_row_flags(const Level &L, int32_t *flags, hipStream_t s);
 void launch_coarse_row_flags(const int32_t *fine, const Level &F, const Level &K, int32_t *flags, hipStream_t s);

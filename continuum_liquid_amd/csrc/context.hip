@@ -126,6 +126,12 @@ struct clq_ctx {
     int32_t *row_count = nullptr, *row_count_host = nullptr;
     int64_t nrows[3] = {0, 0, 0};
     bool rows_valid = false, use_rows = false;
+    // rows of the resident bottom level B (the second coarsest) holding an
+    // interior fluid cell, and their number, on the device (bottom_listed
+    // when they follow the current operator); used inside
+    // clq_pressure_solve, where B's other rows hold p = +0
+    int32_t *bottom_rows = nullptr, *bottom_count = nullptr;
+    bool bottom_listed = false;
     uint8_t *op0 = nullptr;  // packed level-0 operator (valid when op0_ok)
     bool op0_ok = false;
     // op0 and the level-0 row flags were written by the operator build from
@@ -679,7 +685,7 @@ int clq_build_operator(clq_ctx *c) {
     launch_build_finest_operator(c->ff, c->diag, c->wx, c->wy, c->wz, pack ? c->op0 : nullptr,
                                  pack ? c->row_flags[0] : nullptr, c->nx, c->ny, c->nz, c->stream);
     c->coarse_ops_valid = false;
-    c->rows_valid = false;  // the row flags now follow the new operator
+    c->rows_valid = c->bottom_listed = false;  // the row flags now follow the new operator
     c->level0_packed = pack;
     return check_launch("build_operator");
 }
@@ -688,7 +694,7 @@ static void ensure_coarse_operators(clq_ctx *c) {
     if (c->coarse_ops_valid) return;
     for (int l = 0; l + 1 < c->nlevels; ++l) launch_coarse_operator(c->lv[l], c->lv[l + 1], c->stream);
     c->coarse_ops_valid = true;
-    c->rows_valid = false;
+    c->rows_valid = c->bottom_listed = false;
 }
 
 // Active-row lists of levels 0 and 1 for the current operator (one host
@@ -715,6 +721,13 @@ static int ensure_row_buffers(clq_ctx *c) {
         HIP_TRY(hipHostMalloc((void **)&c->row_count_host, 16, hipHostMallocDefault));
         if (int rc = c->alloc(&q, (size_t)c->C)) return rc;
         c->op0 = (uint8_t *)q;
+        const int b = c->nlevels - 2;
+        if (b >= 1 && resident_bottom_fits(c->lv[b], c->lv[b + 1])) {
+            if (int rc = c->alloc(&q, (size_t)c->lv[b].nx * c->lv[b].ny * 4)) return rc;
+            c->bottom_rows = (int32_t *)q;
+            if (int rc = c->alloc(&q, 16)) return rc;
+            c->bottom_count = (int32_t *)q;
+        }
     }
     return CLQ_OK;
 }
@@ -736,6 +749,8 @@ static int ensure_active_rows(clq_ctx *c) {
         launch_exclusive_scan(c->row_flags[l], c->row_pos, r, c->scan_scratch, c->stream);
         launch_compact_rows(c->row_flags[l], c->row_pos, r, c->rows[l], c->row_count + l, c->stream);
     }
+    if (c->bottom_rows) launch_bottom_rows(c->lv[c->nlevels - 2], c->bottom_rows, c->bottom_count, c->stream);
+    c->bottom_listed = c->bottom_rows != nullptr;
     HIP_TRY(hipMemcpyAsync(c->row_count_host, c->row_count, 16, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     for (int l = 0; l < row_levels(c); ++l) c->nrows[l] = c->row_count_host[l];
@@ -842,8 +857,9 @@ static int bottom_level(const clq_ctx *c) {
 static void launch_bottom(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool f64) {
     if (level == c->nlevels - 2 && resident_bottom_fits(c->lv[level], c->lv[level + 1])) {
         const int calls = (vfrom <= 0 || level < vfrom) ? 2 : 1;
+        const bool listed = c->use_rows && c->bottom_listed;
         launch_bottom_resident(c->lv[level], c->lv[level + 1], (float)c->dx, f64, pre, post, pre + extra, calls,
-                               c->stream);
+                               listed ? c->bottom_rows : nullptr, listed ? c->bottom_count : nullptr, c->stream);
         return;
     }
     BottomArgs a;

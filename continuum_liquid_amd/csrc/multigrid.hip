@@ -1079,40 +1079,115 @@ void launch_wcycle_bottom(const BottomArgs &a, hipStream_t s) {
 // above (rbgs_cell, restrict_item, coarse_smooth_body, prolong_item), so the
 // result is bit-identical; between two coarse calls the FP64 iterate is
 // rounded to FP32, as the level array stores it.
+//
+// With a row list (bottom_rows_kernel), only B's rows that hold an interior
+// fluid cell are loaded, swept and stored.  A cell of any other row is
+// either not interior (the sweeps, the restriction and the prolongation
+// leave it as it is) or has mask <= 0 (the sweeps set it to +0), so when
+// every such row holds p = +0 on entry, which is what the restriction onto
+// B writes there (and what the row-list solve's zeroing leaves in the rows
+// it does not restrict onto), the pressure the kernel leaves is the same
+// with and without the list.  A single work-group reads all of B through
+// one CU, so B's bytes set the kernel's time: at 256^3 the rows with fluid
+// are about a quarter of B.
 constexpr int kResB = 6 * kCoarseThreads, kResK = kCoarseThreads;
 
 int resident_bottom_fits(const Level &B, const Level &K) { return B.C <= kResB && K.C <= kResK; }
 
+// rows[0 .. *count) = the rows (i, j) of B with an interior cell whose mask
+// is > 0, in increasing order (one work-group)
+__global__ __launch_bounds__(kCoarseThreads) void bottom_rows_kernel(Level B, int32_t *__restrict__ rows,
+                                                                     int32_t *__restrict__ count) {
+    __shared__ int wave_count[kCoarseThreads / 64];
+    __shared__ int base;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nrows = B.nx * B.ny;
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (int r0 = 0; r0 < nrows; r0 += kCoarseThreads) {
+        const int r = r0 + tid;
+        bool active = false;
+        if (r < nrows) {
+            const int i = r / B.ny, j = r - i * B.ny;
+            if (i > 0 && i < B.nx - 1 && j > 0 && j < B.ny - 1)
+                for (int k = 1; k < B.nz - 1; ++k) active |= B.mask[(int64_t)r * B.nz + k] > 0.0f;
+        }
+        const uint64_t bal = __ballot(active);
+        if (lane == 0) wave_count[wave] = (int)__popcll(bal);
+        __syncthreads();
+        int at = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
+        for (int w = 0; w < wave; ++w) at += wave_count[w];
+        if (active) rows[at] = r;
+        __syncthreads();
+        if (tid == 0)
+            for (int w = 0; w < kCoarseThreads / 64; ++w) base += wave_count[w];
+        __syncthreads();
+    }
+    if (tid == 0) *count = base;
+}
+
+void launch_bottom_rows(const Level &B, int32_t *rows, int32_t *count, hipStream_t s) {
+    hipLaunchKernelGGL(bottom_rows_kernel, dim3(1), dim3(kCoarseThreads), 0, s, B, rows, count);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B, Level K, float dd, T ddk,
                                                                         float inv_delta2, int pre, int post,
-                                                                        int coarse_iters, int coarse_calls) {
+                                                                        int coarse_iters, int coarse_calls,
+                                                                        const int32_t *__restrict__ brows,
+                                                                        const int32_t *__restrict__ bcount) {
     __shared__ float pB[kResB];
     __shared__ float rB[kResB];
     __shared__ T pK[kResK];
     const int tid = threadIdx.x;
     const int bnx = B.nx, bny = B.ny, bnz = B.nz, BC = (int)B.C;
     const int bsx = bny * bnz, bsy = bnz;
-    // level B: cell tid + s * 1024, flags: bit 0 parity, 1 zero (mask <= 0),
-    // 2 update (interior, fluid, d > 0), 3 fluid (interior, mask > 0)
-    float w[6][6], bv[6], dv[6];
-    int fl[6];
-    // every load unconditional (indices clamped into the arrays; the values
-    // of cells that do not use them are dropped) and issued for all six cells
-    // before any is used: one memory round trip instead of one per branch
-    float lp[6], lm[6], ld[6], lb[6], lw[6][6];
-    int li[6], lj[6], lk[6];
+    // level B: item t = tid + s * 1024 is cell t, or with the row list cell
+    // t % bnz of the listed row t / bnz; cell BC marks an item past the end.
+    // Slot s is live (block-uniform) when item s * 1024 is.  cf[s] holds the
+    // item's cell << 4 and its flags: bit 0 parity, 1 zero (mask <= 0), 2
+    // update (interior, fluid, d > 0), 3 fluid (interior, mask > 0)
+    const int nact = brows ? *bcount * bnz : BC;
+    int cf[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const int cell = tid + s * kCoarseThreads;
-        const int cc = cell < BC ? cell : BC - 1;
-        coords(cc, bny, bnz, li[s], lj[s], lk[s]);
-        const bool inner = interior(li[s], lj[s], lk[s], bnx, bny, bnz);
+        const int t = tid + s * kCoarseThreads;
+        if (brows) {
+            const int ar = t / bnz;
+            const int row = brows[min(ar, bnx * bny - 1)];  // in the list's bounds; used when t < nact
+            cf[s] = (t < nact ? row * bnz + (t - ar * bnz) : BC) << 4;
+        } else {
+            cf[s] = (t < BC ? t : BC) << 4;
+        }
+        // rows off the list hold +0 (B's pressure on entry there), and so
+        // do their residuals
+        if (t < BC) {
+            pB[t] = 0.0f;
+            rB[t] = 0.0f;
+        }
+    }
+    auto live = [&](int s) { return s * kCoarseThreads < nact; };
+    __syncthreads();
+    float w[6][6], bv[6], dv[6];
+    // every load of a live slot unconditional (indices clamped into the
+    // arrays; the values of cells that do not use them are dropped) and
+    // issued for all slots before any is used: one memory round trip
+    float lp[6], lm[6], ld[6], lb[6], lw[6][6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        lp[s] = lm[s] = ld[s] = lb[s] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) lw[s][q] = 0.0f;
+        if (!live(s)) continue;
+        const int cc = (cf[s] >> 4) < BC ? (cf[s] >> 4) : BC - 1;
+        int i, j, k;
+        coords(cc, bny, bnz, i, j, k);
+        const bool inner = interior(i, j, k, bnx, bny, bnz);
         const int cxm = inner ? cc - bsx : cc, cym = inner ? cc - bsy : cc, czm = inner ? cc - 1 : cc;
         lp[s] = B.p[cc];
         lm[s] = B.mask[cc];
         ld[s] = B.diag[cc];
-        lb[s] = load_b_flat(B.b, B.b16, B.bkind, B.bscale, cc);
+        lb[s] = load_b(B.b, B.b16, B.bkind, B.bscale, cc);
         lw[s][0] = B.wx[cxm];
         lw[s][1] = B.wx[cc];
         lw[s][2] = B.wy[cc];
@@ -1122,26 +1197,29 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
     }
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const int cell = tid + s * kCoarseThreads;
-        fl[s] = 0;
+        const int c = cf[s] >> 4;
+        int fl = 0;
         bv[s] = 0.0f;
         dv[s] = 1.0f;
 #pragma unroll
         for (int q = 0; q < 6; ++q) w[s][q] = 0.0f;
-        if (cell < BC) {
-            pB[cell] = lp[s];
-            fl[s] = (li[s] + lj[s] + lk[s]) & 1;
+        if (live(s) && c < BC) {
+            int i, j, k;
+            coords(c, bny, bnz, i, j, k);
+            pB[c] = lp[s];
+            fl = (i + j + k) & 1;
             if (lm[s] <= 0.0f) {
-                fl[s] |= 2;
-            } else if (interior(li[s], lj[s], lk[s], bnx, bny, bnz)) {
-                fl[s] |= 8;
+                fl |= 2;
+            } else if (interior(i, j, k, bnx, bny, bnz)) {
+                fl |= 8;
                 dv[s] = ld[s];
-                if (ld[s] > 0.0f) fl[s] |= 4;
+                if (ld[s] > 0.0f) fl |= 4;
                 bv[s] = lb[s];
 #pragma unroll
                 for (int q = 0; q < 6; ++q) w[s][q] = lw[s][q];
             }
         }
+        cf[s] |= fl;
     }
     // level K: cell tid
     const int knx = K.nx, kny = K.ny, knz = K.nz, KC = (int)K.C;
@@ -1175,11 +1253,14 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
             for (int parity = 0; parity < 2; ++parity) {
 #pragma unroll
                 for (int s = 0; s < 6; ++s) {
-                    const int c = tid + s * kCoarseThreads;
-                    if (c >= BC || (fl[s] & 1) != parity) continue;
-                    if (fl[s] & 2) {
+                    int c = cf[s] >> 4;
+                    // the neighbours' LDS addresses are formed here, not
+                    // kept in registers across the sweeps
+                    asm volatile("" : "+v"(c));
+                    if (!live(s) || c >= BC || (cf[s] & 1) != parity) continue;
+                    if (cf[s] & 2) {
                         pB[c] = 0.0f;
-                    } else if (fl[s] & 4) {  // wsum (multigrid.cl:7-23) and :227
+                    } else if (cf[s] & 4) {  // wsum (multigrid.cl:7-23) and :227
                         float t = w[s][0] * pB[c - bsx];
                         t = fmaf(w[s][1], pB[c + bsx], t);
                         t = fmaf(w[s][2], pB[c + bsy], t);
@@ -1198,10 +1279,10 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
     // leaves the child sums unchanged), then the sum over the eight children
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const int c = tid + s * kCoarseThreads;
-        if (c >= BC) continue;
+        const int c = cf[s] >> 4;
+        if (!live(s) || c >= BC) continue;
         float r = 0.0f;
-        if (fl[s] & 8) {
+        if (cf[s] & 8) {
             float t = w[s][0] * pB[c - bsx];
             t = fmaf(w[s][1], pB[c + bsx], t);
             t = fmaf(w[s][2], pB[c + bsy], t);
@@ -1250,8 +1331,8 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
     // prolongation (prolong_item)
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const int c = tid + s * kCoarseThreads;
-        if (c >= BC || !(fl[s] & 8)) continue;
+        const int c = cf[s] >> 4;
+        if (!live(s) || c >= BC || !(cf[s] & 8)) continue;
         int i, j, k;
         coords(c, bny, bnz, i, j, k);
         const int a = min((i - 1) / 2 + 1, knx - 2), b = min((j - 1) / 2 + 1, kny - 2), cc = min((k - 1) / 2 + 1, knz - 2);
@@ -1261,21 +1342,22 @@ __global__ __launch_bounds__(kCoarseThreads) void bottom_resident_kernel(Level B
     smooth_b(post);
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        const int c = tid + s * kCoarseThreads;
-        if (c < BC) B.p[c] = pB[c];
+        const int c = cf[s] >> 4;
+        if (live(s) && c < BC) B.p[c] = pB[c];
     }
     if (tid < KC) K.p[tid] = (float)pK[tid];
 }
 
 void launch_bottom_resident(const Level &B, const Level &K, float delta, bool f64, int pre, int post, int coarse_iters,
-                            int coarse_calls, hipStream_t s) {
+                            int coarse_calls, const int32_t *rows, const int32_t *count, hipStream_t s) {
     const float dd = delta * delta;
     if (f64)
         hipLaunchKernelGGL(bottom_resident_kernel<double>, dim3(1), dim3(kCoarseThreads), 0, s, B, K, dd,
-                           (double)delta * (double)delta, 1.0f / dd, pre, post, coarse_iters, coarse_calls);
+                           (double)delta * (double)delta, 1.0f / dd, pre, post, coarse_iters, coarse_calls, rows,
+                           count);
     else
         hipLaunchKernelGGL(bottom_resident_kernel<float>, dim3(1), dim3(kCoarseThreads), 0, s, B, K, dd, dd, 1.0f / dd,
-                           pre, post, coarse_iters, coarse_calls);
+                           pre, post, coarse_iters, coarse_calls, rows, count);
 }
 
 int coarse_f64_capacity() { return kCoarseCells; }

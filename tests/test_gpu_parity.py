@@ -316,8 +316,10 @@ DEEP_SCENES = {"96^3": lambda: scenes.small_scene(96, seed=3),
 @pytest.mark.parametrize("shape", list(DEEP_SCENES), ids=list(DEEP_SCENES))
 def test_deep_cycles_bit_exact(oracle_lib, hip_lib, shape, coarse_dtype, vcycle_from, smooth, weight):
     """Hierarchies of five and four levels: the row kernels on levels 0-1, the
-    dense colour kernels on level 2, the single-work-group bottom below, and
-    the transfers folded into the neighbouring half-sweeps, against the
+    dense colour kernels on level 2, the single-work-group bottom below (only
+    its rows with fluid loaded and swept; with vcycle_from = 0 the W-cycle
+    enters it a second time from a non-zero pressure), and the transfers
+    folded into the neighbouring half-sweeps, against the
     oracle's plain recursion, bit for bit; with weight != 1 every folded
     prolongation weighs the coarse pressure as the oracle's does."""
     scene = DEEP_SCENES[shape]()
