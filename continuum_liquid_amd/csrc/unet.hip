@@ -19,8 +19,9 @@
 // The network grid is the pressure grid rounded up to multiples of 4 in each
 // dimension (zero inputs in the padding).  Activations are BF16 with channels
 // fastest ([voxel][channel]); gradients are FP32.  Convolutions are implicit
-// GEMMs on the matrix cores (v_mfma_f32_16x16x16_bf16, FP32 accumulation): a
-// wave owns 64 voxels x C_out, the K loop runs over (tap, 16 input channels).
+// GEMMs on the matrix cores (v_mfma_f32_16x16x32_bf16 over pairs of taps, or
+// v_mfma_f32_16x16x16_bf16 over single taps; FP32 accumulation): a wave owns
+// 64 voxels x C_out, the K loop runs over (tap, 16 input channels).
 //   forward   Y[v][co] = relu(b[co] + sum_{tap,ci} X[v + d(tap)][ci] W[tap][ci][co])
 //   data grad G[u][ci] = sum_{tap,co} Z[u - d(tap)][co] W[tap][ci][co],
 //             Z = dL/dY masked by Y > 0, rounded to BF16 on load
@@ -43,6 +44,7 @@ namespace clq {
 namespace {
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
@@ -76,7 +78,11 @@ __device__ __forceinline__ void vox(const Dims &d, int64_t v, int &x, int &y, in
 // taps read their A fragments from LDS: 32 contiguous bytes per voxel, so a
 // wave's fragment read is one conflict-free 512-byte run.  B fragments (the
 // weights of one tap and 16 output channels) come from global memory and
-// serve all four M-tiles.
+// serve all four M-tiles.  Two taps share one v_mfma_f32_16x16x32_bf16 (K =
+// 16 channels of each), which halves the fragment reads and weight loads per
+// MAC; the 16-channel convolution over the upsampled concatenation keeps one
+// tap per v_mfma_f32_16x16x16_bf16, which measured faster for it (1.21
+// against 1.40 ms at 256^3; every other shape 7-42 % faster with pairs).
 constexpr int kBX = 4, kBY = 4, kBZ = 16;
 constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
@@ -151,24 +157,56 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(const void *__restrict__
             if (item < kHalo * 2) *reinterpret_cast<uint4 *>(tile + (item >> 1) * 16 + (item & 1) * 8) = staged[it];
         }
         __syncthreads();
-        for (int tap = 0; tap < 27; ++tap) {
-            int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-            if (MODE == 1) {
-                dx = -dx;
-                dy = -dy;
-                dz = -dz;
-            }
-            short4_t b[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t)
-                b[t] = *reinterpret_cast<const short4_t *>(wt + ((int64_t)tap * CO + t * 16 + m) * CI + c0 + kg * 4);
-            const int base = ((wave + 1 + dx) * kHY + 1 + dy) * kHZ + m + 1 + dz;
-#pragma unroll
-            for (int mt = 0; mt < kBY; ++mt) {
-                const short4_t a = *reinterpret_cast<const short4_t *>(tile + (base + mt * kHZ) * 16 + kg * 4);
+        if constexpr (CO == 16 && CA > 0) {  // forward only: one tap per MFMA, K = 16 channels
+            for (int tap = 0; tap < 27; ++tap) {
+                const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+                short4_t b[NT];
 #pragma unroll
                 for (int t = 0; t < NT; ++t)
-                    acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[mt][t], 0, 0, 0);
+                    b[t] = *reinterpret_cast<const short4_t *>(wt + ((int64_t)tap * CO + t * 16 + m) * CI + c0 +
+                                                               kg * 4);
+                const int base = ((wave + 1 + dx) * kHY + 1 + dy) * kHZ + m + 1 + dz;
+#pragma unroll
+                for (int mt = 0; mt < kBY; ++mt) {
+                    const short4_t a = *reinterpret_cast<const short4_t *>(tile + (base + mt * kHZ) * 16 + kg * 4);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[mt][t], 0, 0, 0);
+                }
+            }
+        } else {
+            // two taps per MFMA: K = (tap 2p, channels 0-15; tap 2p + 1,
+            // channels 0-15); lane group kg holds K 8 kg .. 8 kg + 7, i.e. tap
+            // 2p + kg / 2, channels 8 (kg % 2) .. 8 (kg % 2) + 7; the 28th tap
+            // is zero (its lanes read tap 26's fragment and drop it)
+            const int cpart = (kg & 1) * 8;
+            for (int tp = 0; tp < 14; ++tp) {
+                const int tap_raw = 2 * tp + (kg >> 1);
+                const bool live = tap_raw < 27;
+                const int tap = live ? tap_raw : 26;
+                int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+                if (MODE == 1) {
+                    dx = -dx;
+                    dy = -dy;
+                    dz = -dz;
+                }
+                uint4 b[NT];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    b[t] = *reinterpret_cast<const uint4 *>(wt + ((int64_t)tap * CO + t * 16 + m) * CI + c0 + cpart);
+                    if (!live) b[t] = make_uint4(0u, 0u, 0u, 0u);
+                }
+                const int base = ((wave + 1 + dx) * kHY + 1 + dy) * kHZ + m + 1 + dz;
+#pragma unroll
+                for (int mt = 0; mt < kBY; ++mt) {
+                    uint4 a = *reinterpret_cast<const uint4 *>(tile + (base + mt * kHZ) * 16 + cpart);
+                    if (!live) a = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        acc[mt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                                             __builtin_bit_cast(bf16x8_t, b[t]),
+                                                                             acc[mt][t], 0, 0, 0);
+                }
             }
         }
     }
