@@ -235,22 +235,27 @@ __global__ __launch_bounds__(256) void conv3_lds_kernel(const void *__restrict__
 
 // ---- weight gradient ------------------------------------------------------------------------
 // dW[tap][ci][co] = sum_v X[v + d(tap)][ci] Z[v][co] for the 16 input channels
-// c0 .. c0 + 15 of one launch: per tap the GEMM D (ci x co) += A (ci x 16
-// voxels) B (16 voxels x co) on v_mfma_f32_16x16x16_bf16, K = the 16 voxels
-// of one z-row of a 4 x 4 x 16 brick.  A persistent block walks the bricks
+// c0 .. c0 + 15 of one launch: per tap the GEMM D (ci x co) += A (ci x 32
+// voxels) B (32 voxels x co) on v_mfma_f32_16x16x32_bf16, K = the 32 voxels
+// of two z-rows of a 4 x 4 x 16 brick.  A persistent block walks the bricks
 // blockIdx.x, blockIdx.x + gridDim.x, ...; for each it stages the 6 x 6 x 18
 // input brick with its halo in LDS as [ci][voxel] (zeros off the grid) and
 // Z of the brick's 256 voxels as [co][voxel] (dL/dY masked by Y > 0, BF16),
-// and wave w accumulates the taps w, w + 4, ... for all CO output channels in
-// registers over all the block's bricks.  The block writes its partial
-// dW[tap][ci - c0][co] at the end; the partials are summed over the blocks
-// in a fixed order (reduce_wgrad_kernel).
+// every staging load of an item batch issued before its LDS stores, and
+// wave w accumulates the taps w, w + 4, ... for all CO output channels in
+// registers over all the block's bricks.  A lane's eight voxels of a tap
+// start anywhere in a channel's row (the tap shifts them by one voxel), so
+// they are read as the three aligned 8-byte words that hold them and
+// funnel-shifted into place: the unaligned 8-byte reads the compiler made of
+// them before stalled the LDS (3x slower at 128^3).  The block writes its
+// partial dW[tap][ci - c0][co] at the end; the partials are summed over the
+// blocks in a fixed order (reduce_wgrad_kernel).
 constexpr int kWX = 4, kWY = 4, kWZ = 16;        // brick of one step
 constexpr int kWHX = kWX + 2, kWHY = kWY + 2, kWHZ = kWZ + 2;
 constexpr int kWHalo = kWHX * kWHY * kWHZ;
 constexpr int kWBlocks = 512;                  // persistent blocks (at most)
-constexpr int kXRow = kWHalo + 2;               // LDS row of one input channel (odd dword count)
-constexpr int kZRow = kWX * kWY * kWZ + 4;     // LDS row of one output channel (8-byte aligned)
+constexpr int kXRow = kWHalo + 4;               // LDS row of one input channel (8-byte multiple, 6-bank skew)
+constexpr int kZRow = kWX * kWY * kWZ + 8;     // LDS row of one output channel (16-byte multiple, 4-bank skew)
 constexpr int kTapsPerWave = 7;                // ceil(27 / 4)
 
 inline int wgrad_bricks(const Dims &d) {
@@ -278,62 +283,108 @@ __global__ __launch_bounds__(256) void conv3_wgrad_kernel(const uint16_t *__rest
         const int bz = brick % nbz, bxy = brick / nbz, by = bxy % nby, bx = bxy / nby;
         const int x0 = bx * kWX, y0 = by * kWY, z0 = bz * kWZ;
         __syncthreads();  // the previous brick's fragments have been read
-        // input brick with halo, 8 channels per item, transposed to [ci][voxel]
-        for (int item = threadIdx.x; item < kWHalo * 2; item += 256) {
-            const int hv = item >> 1, half = item & 1;
-            const int hz = hv % kWHZ, hxy = hv / kWHZ, hy = hxy % kWHY, hx = hxy / kWHY;
-            const int gx = x0 - 1 + hx, gyy = y0 - 1 + hy, gz = z0 - 1 + hz;
-            uint4 val = make_uint4(0u, 0u, 0u, 0u);
-            if (gx >= 0 && gx < d.X && gyy >= 0 && gyy < d.Y && gz >= 0 && gz < d.Z)
-                val = *reinterpret_cast<const uint4 *>(x + (((int64_t)gx * d.Y + gyy) * d.Z + gz) * CI + c0 + half * 8);
-            const uint32_t w4[4] = {val.x, val.y, val.z, val.w};
+        // input brick with halo, 8 channels per item, transposed to [ci][voxel];
+        // every item's load is issued before the first LDS store
+        constexpr int kXIters = (kWHalo * 2 + 255) / 256;
+        uint4 xv[kXIters];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                xs[(half * 8 + 2 * q) * kXRow + hv] = (uint16_t)(w4[q] & 0xffffu);
-                xs[(half * 8 + 2 * q + 1) * kXRow + hv] = (uint16_t)(w4[q] >> 16);
+        for (int it = 0; it < kXIters; ++it) {
+            const int item = threadIdx.x + it * 256;
+            xv[it] = make_uint4(0u, 0u, 0u, 0u);
+            if (item < kWHalo * 2) {
+                const int hv = item >> 1, half = item & 1;
+                const int hz = hv % kWHZ, hxy = hv / kWHZ, hy = hxy % kWHY, hx = hxy / kWHY;
+                const int gx = x0 - 1 + hx, gyy = y0 - 1 + hy, gz = z0 - 1 + hz;
+                if (gx >= 0 && gx < d.X && gyy >= 0 && gyy < d.Y && gz >= 0 && gz < d.Z)
+                    xv[it] = *reinterpret_cast<const uint4 *>(x + (((int64_t)gx * d.Y + gyy) * d.Z + gz) * CI + c0 +
+                                                              half * 8);
             }
         }
-        // Z of the brick's voxels, 8 channels per item, transposed to [co][voxel]
-        for (int item = threadIdx.x; item < kWX * kWY * kWZ * (CO / 8); item += 256) {
+        // Z of the brick's voxels, 8 channels per item, transposed to
+        // [co][voxel]; loads issued in batches of up to four items
+        constexpr int kZIters = CO / 8;  // kWX * kWY * kWZ * (CO / 8) items over 256 threads
+        constexpr int kZBatch = kZIters < 4 ? kZIters : 4;
+        auto z_item = [&](int it, float4 &g0, float4 &g1, uint4 &yy) {
+            const int item = threadIdx.x + it * 256;
             const int bv = item / (CO / 8), cg = item - bv * (CO / 8);
             const int lz = bv % kWZ, lxy = bv / kWZ, ly = lxy % kWY, lx = lxy / kWY;
             const int gx = x0 + lx, gyy = y0 + ly, gz = z0 + lz;
-            uint16_t zv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            g0 = g1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            yy = make_uint4(0u, 0u, 0u, 0u);  // Y = +0: masked out
             if (gx < d.X && gyy < d.Y && gz < d.Z) {
                 const int64_t off = (((int64_t)gx * d.Y + gyy) * d.Z + gz) * CO + cg * 8;
-                const float4 g0 = *reinterpret_cast<const float4 *>(gy + off);
-                const float4 g1 = *reinterpret_cast<const float4 *>(gy + off + 4);
-                const uint4 yy = *reinterpret_cast<const uint4 *>(y + off);
-                const float gv[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
-                const uint32_t y4[4] = {yy.x, yy.y, yy.z, yy.w};
+                g0 = *reinterpret_cast<const float4 *>(gy + off);
+                g1 = *reinterpret_cast<const float4 *>(gy + off + 4);
+                yy = *reinterpret_cast<const uint4 *>(y + off);
+            }
+        };
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const uint16_t yq = (uint16_t)(q & 1 ? y4[q >> 1] >> 16 : y4[q >> 1] & 0xffffu);
-                    zv[q] = bf2f(yq) > 0.0f ? f2bf(gv[q]) : (uint16_t)0;
+        for (int it = 0; it < kXIters; ++it) {
+            const int item = threadIdx.x + it * 256;
+            if (item < kWHalo * 2) {
+                const int hv = item >> 1, half = item & 1;
+                const uint32_t w4[4] = {xv[it].x, xv[it].y, xv[it].z, xv[it].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    xs[(half * 8 + 2 * q) * kXRow + hv] = (uint16_t)(w4[q] & 0xffffu);
+                    xs[(half * 8 + 2 * q + 1) * kXRow + hv] = (uint16_t)(w4[q] >> 16);
                 }
             }
+        }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) zs[(cg * 8 + q) * kZRow + bv] = zv[q];
+        for (int i0 = 0; i0 < kZIters; i0 += kZBatch) {
+            float4 g0[kZBatch], g1[kZBatch];
+            uint4 yy[kZBatch];
+#pragma unroll
+            for (int q = 0; q < kZBatch; ++q) z_item(i0 + q, g0[q], g1[q], yy[q]);
+#pragma unroll
+            for (int q = 0; q < kZBatch; ++q) {
+                const int item = threadIdx.x + (i0 + q) * 256;
+                const int bv = item / (CO / 8), cg = item - bv * (CO / 8);
+                const float gv[8] = {g0[q].x, g0[q].y, g0[q].z, g0[q].w, g1[q].x, g1[q].y, g1[q].z, g1[q].w};
+                const uint32_t y4[4] = {yy[q].x, yy[q].y, yy[q].z, yy[q].w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const uint16_t ye = (uint16_t)(e & 1 ? y4[e >> 1] >> 16 : y4[e >> 1] & 0xffffu);
+                    zs[(cg * 8 + e) * kZRow + bv] = bf2f(ye) > 0.0f ? f2bf(gv[e]) : (uint16_t)0;
+                }
+            }
         }
         __syncthreads();
-        // z-row r = (lx, ly) of the brick: K = its 16 voxels
-        for (int r = 0; r < kWX * kWY; ++r) {
-            const int lx = r / kWY, ly = r - lx * kWY;
-            short4_t b[TO];
+        // z-rows 2 p and 2 p + 1 of the brick (r = 4 lx + ly): K = their 32
+        // voxels on v_mfma_f32_16x16x32_bf16; lane group kg holds K 8 kg ..
+        // 8 kg + 7, i.e. row 2 p + kg / 2, voxels 8 (kg % 2) .. 8 (kg % 2) + 7
+        for (int p = 0; p < kWX * kWY / 2; ++p) {
+            const int rr = 2 * p + (kg >> 1);
+            const int lx = rr / kWY, ly = rr - lx * kWY;
+            uint4 b[TO];
 #pragma unroll
             for (int t = 0; t < TO; ++t)
-                b[t] = *reinterpret_cast<const short4_t *>(zs + (t * 16 + m) * kZRow + r * kWZ + kg * 4);
+                b[t] = *reinterpret_cast<const uint4 *>(zs + (t * 16 + m) * kZRow + rr * kWZ + (kg & 1) * 8);
 #pragma unroll
             for (int tt = 0; tt < kTapsPerWave; ++tt) {
                 const int tap = wave + 4 * tt;
                 if (tap < 27) {  // wave-uniform
                     const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
-                    const int hv = ((lx + 1 + dx) * kWHY + (ly + 1 + dy)) * kWHZ + 1 + dz + kg * 4;
-                    const uint16_t *row = xs + m * kXRow + hv;
-                    const short4_t a = short4_t{(short)row[0], (short)row[1], (short)row[2], (short)row[3]};
+                    const int hv = ((lx + 1 + dx) * kWHY + (ly + 1 + dy)) * kWHZ + 1 + dz + (kg & 1) * 8;
+                    // the eight voxels hv .. hv + 7 of channel m from the three
+                    // aligned 8-byte words that hold them (an unaligned LDS
+                    // read stalls); k = hv % 4
+                    const int o = m * kXRow + hv;
+                    const uint2 *w2 = reinterpret_cast<const uint2 *>(xs + (o & ~3));
+                    const uint2 q0 = w2[0], q1 = w2[1], q2 = w2[2];
+                    const bool hi = (hv & 2) != 0, odd = (hv & 1) != 0;
+                    const uint32_t e0 = hi ? q0.y : q0.x, e1 = hi ? q1.x : q0.y, e2 = hi ? q1.y : q1.x,
+                                   e3 = hi ? q2.x : q1.y, e4 = hi ? q2.y : q2.x;
+                    const uint32_t r0 = odd ? __builtin_amdgcn_alignbit(e1, e0, 16) : e0;
+                    const uint32_t r1 = odd ? __builtin_amdgcn_alignbit(e2, e1, 16) : e1;
+                    const uint32_t r2 = odd ? __builtin_amdgcn_alignbit(e3, e2, 16) : e2;
+                    const uint32_t r3 = odd ? __builtin_amdgcn_alignbit(e4, e3, 16) : e3;
+                    const bf16x8_t a = __builtin_bit_cast(bf16x8_t, make_uint4(r0, r1, r2, r3));
 #pragma unroll
                     for (int t = 0; t < TO; ++t)
-                        acc[tt][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc[tt][t], 0, 0, 0);
+                        acc[tt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8_t, b[t]),
+                                                                             acc[tt][t], 0, 0, 0);
                 }
             }
         }
