@@ -40,12 +40,15 @@ namespace clq {
 // neighbour ((ox, oy) in -1..1, and not (+1, +1), which carries no weight)
 // walks it: lane l takes the particles of its bin one per step, computes the
 // tent weights for its own cell (oz = 0), the cell above (the bin is that
-// cell's oz = -1 neighbour) and the cell below (oz = +1, w only), and hands
-// the neighbours' weights and velocities over with DPP wavefront shifts;
-// each cell adds them with the same fmaf into the partial sum of that
-// neighbour index (n = 9(ox+1) + 3(oy+1) + oz+1, partial n % 8), as the
-// reference's 8-lane work-group does.  A lane past its part of the chunk
-// reads a sentinel particle far outside the grid, whose weights are 0.
+// cell's oz = -1 neighbour) and the cell below (oz = +1, w only), and adds
+// each with the same fmaf into the partial sum of that neighbour index (n =
+// 9(ox+1) + 3(oy+1) + oz+1, partial n % 8), as the reference's 8-lane
+// work-group does.  The cells above' and below's running sums of the walk's
+// partials are moved into the particles' lanes with DPP wavefront shifts
+// when the walk starts and moved back when it ends (two shifts per sum and
+// walk instead of two per term and step), so every sum receives its terms in
+// the same order.  A lane past its part of the chunk reads a sentinel
+// particle far outside the grid, whose weights are 0.
 //
 // The kernel is bound by the chain of staged items each block runs through
 // (one memory round trip and one barrier per item), so nothing on that chain
@@ -124,13 +127,15 @@ __device__ __forceinline__ void walk_step(const WalkParticle &q, float gx, float
     constexpr int L_DOWN = (base + 2) & 7;  // oz = +1 for cell ck - 1
     constexpr bool U = OY <= 0, V = OX <= 0, W = OX <= 0 && OY <= 0;
     const float ay = tent(q.p1, gy + 0.5f);
+    // vs/ws[L_UP] hold the cell above's sums and vs/ws[L_DOWN] the cell
+    // below's during the walk (walk moves them in and out)
     if constexpr (U) {
         const float uxy = tent(q.p0, gx + 1.0f) * ay;
         const float wo = uxy * tent(q.p2, gz_own + 0.5f);
         vs[L_OWN][0] = fmaf(wo, q.qu, vs[L_OWN][0]);
         ws[L_OWN][0] = ws[L_OWN][0] + wo;
-        const float wu = lane_up(uxy * tent(q.p2, gz_up + 0.5f));
-        vs[L_UP][0] = fmaf(wu, lane_up(q.qu), vs[L_UP][0]);
+        const float wu = uxy * tent(q.p2, gz_up + 0.5f);
+        vs[L_UP][0] = fmaf(wu, q.qu, vs[L_UP][0]);
         ws[L_UP][0] = ws[L_UP][0] + wu;
     }
     if constexpr (V) {
@@ -138,8 +143,8 @@ __device__ __forceinline__ void walk_step(const WalkParticle &q, float gx, float
         const float wo = vxy * tent(q.p2, gz_own + 0.5f);
         vs[L_OWN][1] = fmaf(wo, q.qv, vs[L_OWN][1]);
         ws[L_OWN][1] = ws[L_OWN][1] + wo;
-        const float wu = lane_up(vxy * tent(q.p2, gz_up + 0.5f));
-        vs[L_UP][1] = fmaf(wu, lane_up(q.qv), vs[L_UP][1]);
+        const float wu = vxy * tent(q.p2, gz_up + 0.5f);
+        vs[L_UP][1] = fmaf(wu, q.qv, vs[L_UP][1]);
         ws[L_UP][1] = ws[L_UP][1] + wu;
     }
     if constexpr (W) {
@@ -147,13 +152,39 @@ __device__ __forceinline__ void walk_step(const WalkParticle &q, float gx, float
         const float wo = wxy * tent(q.p2, gz_own + 1.0f);
         vs[L_OWN][2] = fmaf(wo, q.qw, vs[L_OWN][2]);
         ws[L_OWN][2] = ws[L_OWN][2] + wo;
-        const float wu = lane_up(wxy * tent(q.p2, gz_up + 1.0f));
-        vs[L_UP][2] = fmaf(wu, lane_up(q.qw), vs[L_UP][2]);
+        const float wu = wxy * tent(q.p2, gz_up + 1.0f);
+        vs[L_UP][2] = fmaf(wu, q.qw, vs[L_UP][2]);
         ws[L_UP][2] = ws[L_UP][2] + wu;
         // the cell below (ck - 1) takes it as its oz = +1 neighbour
-        const float wd = lane_down(wxy * tent(q.p2, gz_dn + 1.0f));
-        vs[L_DOWN][2] = fmaf(wd, lane_down(q.qw), vs[L_DOWN][2]);
+        const float wd = wxy * tent(q.p2, gz_dn + 1.0f);
+        vs[L_DOWN][2] = fmaf(wd, q.qw, vs[L_DOWN][2]);
         ws[L_DOWN][2] = ws[L_DOWN][2] + wd;
+    }
+}
+
+// Moves the running sums of the cell above (L_UP) into the lane below and
+// those of the cell below (L_DOWN) into the lane above (IN), or back (!IN),
+// so that during a walk each lane adds its particles' terms for the cells
+// above and below to registers of its own; a sum's terms still arrive in the
+// same order.  Lanes 0 and 63 hold the segment's halo cells, whose sums are
+// dropped, so the zeros DPP brings in at the wave's ends change no result.
+template <int OX, int OY, bool IN>
+__device__ __forceinline__ void move_neighbour_sums(float (&vs)[8][3], float (&ws)[8][3]) {
+    constexpr int base = (OX + 1) * 9 + (OY + 1) * 3;
+    constexpr int L_UP = (base + 0) & 7, L_DOWN = (base + 2) & 7;
+    constexpr bool U = OY <= 0, V = OX <= 0, W = OX <= 0 && OY <= 0;
+    auto up_sum = [](float x) { return IN ? lane_down(x) : lane_up(x); };
+    auto down_sum = [](float x) { return IN ? lane_up(x) : lane_down(x); };
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if ((c == 0 && U) || (c == 1 && V) || (c == 2 && W)) {
+            vs[L_UP][c] = up_sum(vs[L_UP][c]);
+            ws[L_UP][c] = up_sum(ws[L_UP][c]);
+        }
+    }
+    if constexpr (W) {
+        vs[L_DOWN][2] = down_sum(vs[L_DOWN][2]);
+        ws[L_DOWN][2] = down_sum(ws[L_DOWN][2]);
     }
 }
 
@@ -170,6 +201,7 @@ __device__ __forceinline__ void walk(const StageBuf &sb, int lo, int hi, int b0,
     // two steps per trip, each step's particle read while the previous step
     // is computed
     WalkParticle A = walk_load<U, V, W>(sb, slot_of(0));
+    move_neighbour_sums<OX, OY, true>(vs, ws);
     for (int s = 0; __any(s < n); s += 2) {
         const WalkParticle B = walk_load<U, V, W>(sb, slot_of(s + 1));
         __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the step
@@ -179,6 +211,7 @@ __device__ __forceinline__ void walk(const StageBuf &sb, int lo, int hi, int b0,
         __builtin_amdgcn_sched_barrier(0);
         walk_step<OX, OY>(B, gx, gy, gz_own, gz_up, gz_dn, vs, ws);
     }
+    move_neighbour_sums<OX, OY, false>(vs, ws);
 }
 
 __global__ __launch_bounds__(kThreads) void p2g_staged_kernel(const float *__restrict__ pos,
