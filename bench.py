@@ -8,8 +8,9 @@ residual target (FP64-checked), pressure correction, and the fused G2P/FLIP
 blend/advection/particle boundary pass.  The default workload is the 256^3
 dam break + falling sphere (dx = 1/256, 8 stratified-jitter FP32 particles
 per fluid cell, ~28.6M particles, dt = 1/240, FLIP 0.95, pressure solve by
-FP64 flexible CG preconditioned with one mixed-precision W(1, 1) cycle whose
-coarse-grid corrections are weighted by 1.6).
+FP64 flexible CG preconditioned with one mixed-precision (1, 1) cycle that
+runs its coarse-grid correction twice below level 1 and once elsewhere, the
+corrections weighted by 1.6).
 
 Prints one JSON line: ms/step (`value`), particle-steps/s, pressure-solve ms,
 a `roofline` object for the dominant kernel (algorithmic bytes per launch
@@ -108,6 +109,19 @@ def krylov_coarse_sweeps(shape) -> int:
     return 8 if coarsest_cells(shape) <= 1000 else 16
 
 
+def krylov_cycle_shape(shape) -> int:
+    """vcycle_from of the FCG preconditioner's cycle: the coarse-grid
+    correction twice below level 1 only and once below level 0 (-2) when the
+    last two levels run in the resident work-group (coarsest level of at most
+    1000 cells), else twice below levels 0 and 1 (2).  The first visits the
+    levels of 66^3 and below half as often per cycle: at 256^3 12.2 FCG
+    iterations instead of 9.0 but 6.71 against 7.00 ms per step (128^3 2.38
+    against 2.55, 64^3 1.19 against 1.47; no breakdown over the 100 steps of
+    each workload); at 512x256x256 (coarsest 18x10x10) it lost, 12.99 against
+    12.82 ms."""
+    return -2 if coarsest_cells(shape) <= 1000 else 2
+
+
 def solver_settings(name: str, tolerance: float) -> dict:
     """W(2, 2) cycles for the plain FP32 and mixed-refinement solves.  The
     plain FP32 baseline keeps the reference's 64 extra coarsest sweeps and the
@@ -117,7 +131,8 @@ def solver_settings(name: str, tolerance: float) -> dict:
     with the V-cycle below.  The FCG solves precondition with W(1, 1) cycles:
     at 256^3 they take 16.8 iterations per step against 14.2 with W(2, 2), and
     15.5 against 17.2 ms/step (sweep over smoothing 0-3, V/W level and
-    coarsest sweeps on MI355X).  Their cycles weigh every coarse-grid
+    coarsest sweeps on MI355X; the level shape is then chosen by
+    krylov_cycle_shape).  Their cycles weigh every coarse-grid
     correction by 1.6 (over-correction of the piecewise-constant
     prolongation): 9.0 FCG iterations per 256^3 step against 17.2 unweighted,
     no breakdown over 20-100 steps of the 64^3, 128^3, 256^3 and 512x256x256
@@ -250,6 +265,7 @@ def main() -> int:
     settings = solver_settings(args.solver, args.tolerance)
     if settings["mode"] in ("fcg", "learned"):
         settings["coarse_extra"] = krylov_coarse_sweeps(scene.shape)
+        settings["vcycle_from"] = krylov_cycle_shape(scene.shape)
     if args.vcycle_from is not None:
         settings["vcycle_from"] = args.vcycle_from
     if args.coarse_extra is not None:
@@ -355,7 +371,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{scene.name}, dx=1/{round(1 / scene.dx)}, {n} FP32 particles (8 per fluid cell, "
-                        f"stratified jitter, seed 0), dt=1/240, FLIP 0.95, solver {args.solver} with W({settings['pre_smooth']},{settings['post_smooth']}) cycles to "
+                        f"stratified jitter, seed 0), dt=1/240, FLIP 0.95, solver {args.solver} with ({settings['pre_smooth']},{settings['post_smooth']}) cycles, vcycle_from {settings.get('vcycle_from', 0)}, to "
                         f"{args.tolerance:g} relative residual",
             "grid": list(scene.shape),
             "particles": n,

@@ -831,6 +831,14 @@ static void prolongate_level(clq_ctx *c, int level) {
 // (wcycle_bottom_kernel) for the rest of the recursion below them.
 constexpr int64_t kBottomCells = 8192;
 
+// Whether a visit of `level` runs its coarse-grid correction twice (the W
+// shape) or once (V): vcycle_from 0 at every level, k > 0 on the levels below
+// k, -m < 0 on the levels whose bit is set in m.
+static bool twice_at(int vfrom, int level) {
+    if (vfrom < 0) return level < 31 && ((((uint32_t)(-(int64_t)vfrom)) >> level) & 1u) != 0;
+    return vfrom == 0 || level < vfrom;
+}
+
 static void bottom_ops(const clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool f64,
                        BottomArgs &a) {
     if (level == c->nlevels - 1) {
@@ -840,7 +848,7 @@ static void bottom_ops(const clq_ctx *c, int level, int pre, int post, int extra
     a.ops[a.nops++] = BottomOp{kOpSmooth, level, pre};
     a.ops[a.nops++] = BottomOp{kOpRestrict, level, 0};
     bottom_ops(c, level + 1, pre, post, extra, vfrom, f64, a);
-    if (vfrom <= 0 || level < vfrom) bottom_ops(c, level + 1, pre, post, extra, vfrom, f64, a);
+    if (twice_at(vfrom, level)) bottom_ops(c, level + 1, pre, post, extra, vfrom, f64, a);
     a.ops[a.nops++] = BottomOp{kOpProlong, level, 0};
     a.ops[a.nops++] = BottomOp{kOpSmooth, level, post};
 }
@@ -856,7 +864,7 @@ static int bottom_level(const clq_ctx *c) {
 
 static void launch_bottom(clq_ctx *c, int level, int pre, int post, int extra, int vfrom, bool f64) {
     if (level == c->nlevels - 2 && resident_bottom_fits(c->lv[level], c->lv[level + 1])) {
-        const int calls = (vfrom <= 0 || level < vfrom) ? 2 : 1;
+        const int calls = twice_at(vfrom, level) ? 2 : 1;
         const bool listed = c->use_rows && c->bottom_listed;
         launch_bottom_resident(c->lv[level], c->lv[level + 1], (float)c->dx, f64, pre, post, pre + extra, calls,
                                listed ? c->bottom_rows : nullptr, listed ? c->bottom_count : nullptr, c->stream);
@@ -912,7 +920,7 @@ static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfr
     }
     const bool red = visit_down(c, level, pre, done);
     w_cycle(c, level + 1, pre, post, extra, vfrom, red);
-    if (vfrom <= 0 || level < vfrom) w_cycle(c, level + 1, pre, post, extra, vfrom);
+    if (twice_at(vfrom, level)) w_cycle(c, level + 1, pre, post, extra, vfrom);
     visit_up(c, level, post);
 }
 
@@ -997,7 +1005,7 @@ static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, i
     }
     const bool red = visit_down(c, level, pre, done);
     w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64, red);
-    if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64);
+    if (twice_at(vfrom, level)) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64);
     visit_up(c, level, post);
 }
 
@@ -1458,9 +1466,8 @@ int clq_pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *
 }
 
 static int pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
-    if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0 ||
-        sp->vcycle_from < 0)
-        return fail(CLQ_ERR_INVALID, "negative sweep, cycle count or level");
+    if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0)
+        return fail(CLQ_ERR_INVALID, "negative sweep or cycle count");
     if (sp->mode < CLQ_SOLVE_FIXED_CYCLES || sp->mode > CLQ_SOLVE_FCG)
         return fail(CLQ_ERR_INVALID, "unknown solve mode %d", sp->mode);
     if (sp->mode == CLQ_SOLVE_LEARNED_FCG && !c->net)

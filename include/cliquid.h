@@ -127,7 +127,9 @@ typedef struct clq_solve_params {
     int32_t coarse_dtype;  /* modes 2-4: 0 = fp16, 1 = bf16 coarse b storage    */
     int32_t vcycle_from;   /* 0: W-cycle at every level (multigrid.py:53-187);
                               k > 0: the coarse-grid correction runs twice
-                              below levels < k and once from level k down  */
+                              below levels < k and once from level k down;
+                              -m < 0: twice below the levels whose bit is set
+                              in m (-2: level 1 only), once below the rest */
     float coarse_weight;   /* weight w of every coarse-grid correction of the
                               cycle, p += w * p_coarse[parent]; 0 selects 1,
                               the reference's plain sum (multigrid.cl:182-201);

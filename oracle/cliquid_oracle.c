@@ -839,6 +839,15 @@ int clq_coarse_smooth(clq_ctx *c, int32_t iterations) {
     return CLQ_OK;
 }
 
+/* Whether a visit of `level` runs its coarse-grid correction twice (the W
+ * shape, multigrid.py:53-187) or once (V): vcycle_from 0 at every level,
+ * k > 0 on the levels below k, -m < 0 on the levels whose bit is set in m.
+ * Same rule as the library's cycle. */
+static int twice_at(int vfrom, int level) {
+    if (vfrom < 0) return level < 31 && ((((uint32_t)(-(int64_t)vfrom)) >> level) & 1u) != 0;
+    return vfrom == 0 || level < vfrom;
+}
+
 /* w_cycle (multigrid.py:53-187): the operator of every coarse level is rebuilt
  * by each restriction, as in the reference. */
 static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfrom) {
@@ -849,7 +858,7 @@ static void w_cycle(clq_ctx *c, int level, int pre, int post, int extra, int vfr
     smooth_level(c, &c->lv[level], pre);
     clq_restrict(c, level);
     w_cycle(c, level + 1, pre, post, extra, vfrom);
-    if (vfrom <= 0 || level < vfrom) w_cycle(c, level + 1, pre, post, extra, vfrom);
+    if (twice_at(vfrom, level)) w_cycle(c, level + 1, pre, post, extra, vfrom);
     prolongate_weighted(c, level, c->coarse_weight);
     smooth_level(c, &c->lv[level], post);
 }
@@ -958,7 +967,7 @@ static void w_cycle_mixed(clq_ctx *c, int level, int pre, int post, int extra, i
     smooth_level(c, &c->lv[level], pre);
     clq_restrict(c, level);
     w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64);
-    if (vfrom <= 0 || level < vfrom) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64);
+    if (twice_at(vfrom, level)) w_cycle_mixed(c, level + 1, pre, post, extra, vfrom, f64);
     prolongate_weighted(c, level, c->coarse_weight);
     smooth_level(c, &c->lv[level], post);
 }
@@ -1241,9 +1250,8 @@ static long long active_cells(const clq_ctx *c) {
 }
 
 static int pressure_solve(clq_ctx *c, const clq_solve_params *sp, clq_solve_stats *st) {
-    if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0 ||
-        sp->vcycle_from < 0)
-        return fail(CLQ_ERR_INVALID, "negative sweep, cycle count or level");
+    if (sp->pre_smooth < 0 || sp->post_smooth < 0 || sp->coarse_extra < 0 || sp->max_cycles < 0)
+        return fail(CLQ_ERR_INVALID, "negative sweep or cycle count");
     if (sp->mode != CLQ_SOLVE_FIXED_CYCLES && sp->mode != CLQ_SOLVE_FP32_TOLERANCE &&
         sp->mode != CLQ_SOLVE_MIXED_REFINEMENT && sp->mode != CLQ_SOLVE_FCG)
         return fail(CLQ_ERR_UNSUPPORTED, "solve mode %d is not available in the oracle", sp->mode);

@@ -311,14 +311,17 @@ DEEP_SCENES = {"96^3": lambda: scenes.small_scene(96, seed=3),
 
 @pytest.mark.parametrize("coarse_dtype,vcycle_from,smooth,weight", [(0, 0, (2, 2), 1.0), (1, 1, (1, 1), 1.0),
                                                                      (0, 2, (1, 2), 1.0), (1, 0, (3, 1), 1.0),
-                                                                     (0, 2, (1, 1), 1.6), (1, 0, (2, 1), 1.3)],
-                         ids=["fp16-W22", "bf16-V1-11", "fp16-V2-12", "bf16-W31", "fp16-V2-11-w1.6", "bf16-W21-w1.3"])
+                                                                     (0, 2, (1, 1), 1.6), (1, 0, (2, 1), 1.3),
+                                                                     (0, -2, (1, 1), 1.6), (1, -2, (1, 2), 1.0)],
+                         ids=["fp16-W22", "bf16-V1-11", "fp16-V2-12", "bf16-W31", "fp16-V2-11-w1.6", "bf16-W21-w1.3",
+                              "fp16-Wlevel1-11-w1.6", "bf16-Wlevel1-12"])
 @pytest.mark.parametrize("shape", list(DEEP_SCENES), ids=list(DEEP_SCENES))
 def test_deep_cycles_bit_exact(oracle_lib, hip_lib, shape, coarse_dtype, vcycle_from, smooth, weight):
     """Hierarchies of five and four levels: the row kernels on levels 0-1, the
     dense colour kernels on level 2, the single-work-group bottom below (only
     its rows with fluid loaded and swept; with vcycle_from = 0 the W-cycle
-    enters it a second time from a non-zero pressure), and the transfers
+    enters it a second time from a non-zero pressure; vcycle_from = -2 runs
+    the coarse-grid correction twice below level 1 only), and the transfers
     folded into the neighbouring half-sweeps, against the
     oracle's plain recursion, bit for bit; with weight != 1 every folded
     prolongation weighs the coarse pressure as the oracle's does."""

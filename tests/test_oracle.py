@@ -511,6 +511,37 @@ def test_coarse_weight_cuts_fcg_iterations(oracle_lib):
     assert np.linalg.norm(b - A @ p64) <= 1.0000001e-8 * np.linalg.norm(b)
 
 
+def test_cycle_level_mask_matches_level_bound(oracle_lib):
+    """vcycle_from = -m runs the coarse-grid correction twice below the levels
+    whose bit is set in m: over three refinement cycles -1 and -3 give the
+    cycles of vcycle_from 1 and 2 bit for bit, -2 (twice below level 1 only,
+    once below level 0) a cycle of its own; FCG around that cycle, weighted by
+    1.6, reaches the tolerance, to the scipy FP64 solution."""
+    kw = dict(pre_smooth=1, post_smooth=1, coarse_extra=16)
+
+    def refine(vcycle_from):
+        ctx = _pressure_context(oracle_lib, 48, seed=1)
+        s = ctx.pressure_solve(runtime.SOLVE_MIXED_REFINEMENT, vcycle_from=vcycle_from, max_cycles=3,
+                               tolerance=1e-14, **kw)
+        assert s["cycles"] == 3
+        return ctx.get_field("pressure_fp64")
+
+    p = {v: refine(v) for v in (1, -1, 2, -3, -2)}
+    assert np.array_equal(p[-1], p[1]) and np.array_equal(p[-3], p[2])
+    assert not np.array_equal(p[-2], p[2]) and not np.array_equal(p[-2], p[1])
+    ctx = _pressure_context(oracle_lib, 48, seed=1)
+    assert len(ctx.level_shapes) >= 4  # levels 0-3: the masks differ in what they run
+    s = ctx.pressure_solve(runtime.SOLVE_FCG, vcycle_from=-2, max_cycles=60, tolerance=1e-8, coarse_weight=1.6, **kw)
+    assert s["relative_residual"] <= 1e-8
+    mask = ctx.get_field("fluid_fraction")
+    h2 = np.float64(F32(ctx.dx)) ** 2
+    A, fluid, _ = operator_matrix(*[ctx.get_field(n) for n in ("diagonal", "weight_x", "weight_y", "weight_z")],
+                                  mask, h2)
+    b = ctx.get_field("rhs").astype(np.float64)[fluid]
+    p64 = ctx.get_field("pressure_fp64")[fluid]
+    assert np.linalg.norm(b - A @ p64) <= 1.0000001e-8 * np.linalg.norm(b)
+
+
 def test_weighted_cycle_breakdown_falls_back_to_weight_one(oracle_lib):
     """With the V-cycle below level 1 a weight of 1.9 over-corrects the
     inexact coarse solves until the preconditioner is indefinite: a step
